@@ -1,0 +1,207 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X dense-product path (one JSON line on stdout).
+
+Headline workload: FP32 sgemm C = A*B, M = N = K = 8192, column-major,
+entries sign*U(1,2)*2^U(-8,8) from a seed, computed by the split-BF16 path
+(3 bf16 terms per operand, 6 cross products on BF16 MFMA, FP32
+accumulation).  One step = one such product with inputs resident in HBM.
+`value` is its TFLOP/s counting 2*M*N*K once.
+
+Beside it the same line reports the plain FP32-MFMA product on the same
+inputs (TFLOP/s and normwise error against an FP64 host product on sampled
+rows), the 2-term split, the FP64 dgemm drop-in (4096^3 and 16384x256x16384,
+C = 1.5*A*B + 0.5*C), the batched uneven products, and Eigen's CPU GEBP
+timed on this host's cores (`cpu_baseline`).
+
+    python bench.py [--gpus 1] [--steps 20] [--warmup 3]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK = {"bf16": 2500.0, "fp32": 157.3, "fp64": 78.6}  # dense MFMA TFLOP/s, MI355X
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def timed(fn, steps: int, warmup: int):
+    """Wall-clock ms per step and HIP-event ms per step on the launch stream."""
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    ev0.record()
+    for _ in range(steps):
+        fn()
+    ev1.record()
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) * 1e3 / steps
+    return wall, ev0.elapsed_time(ev1) / steps
+
+
+def fp64_rows_error(dA, dB, dC, m, n, k, rows, alpha=1.0, beta=0.0, C0=None, dtype=np.float32):
+    """Normwise error ||C - C_ref||_F / (K u ||A||_F ||B||_F) on sampled rows,
+    C_ref computed on the host in FP64 (numpy) from the same device inputs."""
+    A = dA.cpu().numpy().astype(np.float64).reshape((k, m)).T  # column-major m x k
+    B = dB.cpu().numpy().astype(np.float64).reshape((n, k)).T
+    Cg = dC.cpu().numpy().astype(np.float64).reshape((n, m)).T
+    ref = alpha * (A[rows] @ B)
+    if C0 is not None and beta != 0.0:
+        ref += beta * C0[rows]
+    u = float(np.finfo(dtype).eps) / 2
+    num = np.linalg.norm(Cg[rows] - ref)
+    # norms of the sampled rows of A scaled to the whole matrix
+    nA = np.linalg.norm(A[rows]) * np.sqrt(m / len(rows))
+    return float(num / (k * u * nA * np.linalg.norm(B)))
+
+
+def cpu_baseline(steps: int = 3):
+    """Eigen's CPU GEBP (built from the reference tree into oracle/_ref) on this host."""
+    try:
+        import oracle as O
+        if not O.ref_available():
+            return None
+        L = O.ref()
+        threads = L.ref_get_threads()
+        m = n = k = 8192
+        A = O.fill((m, k), np.float32, 1, O.WIDE, -8, 8)
+        B = O.fill((k, n), np.float32, 2, O.WIDE, -8, 8)
+        Cm = np.zeros((m, n), np.float32, order="F")
+        t0 = time.perf_counter()
+        O.eigen_gemm(False, False, 1.0, A, B, Cm)  # warm-up
+        first = time.perf_counter() - t0
+        reps = max(1, min(steps, int(20.0 // max(first, 1e-3))))
+        ts = []
+        for _ in range(reps):
+            Cm[:] = 0
+            t0 = time.perf_counter()
+            O.eigen_gemm(False, False, 1.0, A, B, Cm)
+            ts.append(time.perf_counter() - t0)
+        t = float(np.median(ts))
+        return {"value": round(2.0 * m * n * k / t / 1e12, 4), "unit": "TFLOP/s", "cores": int(threads),
+                "kind": "reference",
+                "sample": f"Eigen C.noalias() += A*B, FP32 8192^3 on host Maps, same input distribution, "
+                          f"OpenMP {threads} threads, {L.ref_simd().decode()}, median of {reps} after 1 warm-up "
+                          f"({os.cpu_count()} logical CPUs visible)"}
+    except Exception as e:  # the baseline is reported, never required
+        log("cpu baseline failed:", e)
+        return None
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-extra", action="store_true", help="headline only")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: the product path here has no multi-GPU mode "
+              "(sharding is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        return 2
+
+    import torch
+    import eigen_amd as E
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    E.lib()
+    m = n = k = 8192
+    flop = 2.0 * m * n * k
+    dA = torch.empty(m * k, dtype=torch.float32, device=dev)
+    dB = torch.empty(k * n, dtype=torch.float32, device=dev)
+    dC = torch.zeros(m * n, dtype=torch.float32, device=dev)
+    E.fill(E.F32, dA, m * k, 1, E.DIST_WIDE, -8, 8)
+    E.fill(E.F32, dB, k * n, 2, E.DIST_WIDE, -8, 8)
+    torch.cuda.synchronize()
+
+    def sgemm(prec):
+        return lambda: E.gemm(E.F32, E.OP_N, E.OP_N, m, n, k, 1.0, dA, m, dB, k, 0.0, dC, m, prec=prec)
+
+    rows = np.arange(0, m, m // 64)
+    res = {}
+    for name, prec in (("split3", E.PREC_SPLIT3), ("plain", E.PREC_PLAIN), ("split2", E.PREC_SPLIT2)):
+        if args.no_extra and name != "split3":
+            continue
+        steps = args.steps if name == "split3" else max(3, args.steps // 2)
+        wall, kern = timed(sgemm(prec), steps, args.warmup)
+        err = fp64_rows_error(dA, dB, dC, m, n, k, rows)
+        res[name] = {"ms": wall, "kernel_ms": kern, "tflops": flop / (wall * 1e-3) / 1e12, "normwise_err": err}
+        log(f"{name}: {res[name]}")
+
+    head = res["split3"]
+    out = {
+        "metric": "TFLOP/s+%peak: 8192^3 sgemm split-BF16 vs FP32 MFMA @equal err; batched GFLOP/s",
+        "value": round(head["tflops"], 3),
+        "unit": "TFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(head["ms"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (device splitmix64, sign*U(1,2)*2^U(-8,8))",
+        "config": {"workload": "sgemm C=A*B M=N=K=8192 column-major, split-BF16 3 terms / 6 products on BF16 "
+                               "MFMA, FP32 accumulate", "M": m, "N": n, "K": k,
+                   "flop_per_step": flop, "flop_counting": "2MNK once"},
+        # BF16 MFMA work actually issued: 6 products of 2MNK each
+        "roofline": {"bound": "mfma", "achieved": round(6 * flop / (head["kernel_ms"] * 1e-3) / 1e12, 2),
+                     "peak": PEAK["bf16"], "unit": "TFLOP/s",
+                     "frac": round(6 * flop / (head["kernel_ms"] * 1e-3) / 1e12 / PEAK["bf16"], 4),
+                     "traffic": None},
+        "cpu_baseline": None,
+        "detail": {"split3": res["split3"]},
+    }
+    if not args.no_extra:
+        out["detail"].update({k_: v for k_, v in res.items() if k_ != "split3"})
+        out["detail"]["split3_speedup_vs_plain"] = round(res["split3"]["tflops"] / res["plain"]["tflops"], 3)
+        out["detail"]["plain_fp32_frac_of_peak"] = round(res["plain"]["tflops"] / PEAK["fp32"], 4)
+        del dA, dB, dC
+        torch.cuda.empty_cache()
+        out["detail"].update(bench_dgemm(E, torch, dev, args))
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def bench_dgemm(E, torch, dev, args):
+    """FP64 dgemm drop-in: C = 1.5*A*B + 0.5*C, entries N(0,1)."""
+    out = {}
+    for name, (m, n, k) in (("dgemm_4096", (4096, 4096, 4096)), ("dgemm_tall", (16384, 256, 16384))):
+        dA = torch.empty(m * k, dtype=torch.float64, device=dev)
+        dB = torch.empty(k * n, dtype=torch.float64, device=dev)
+        dC = torch.empty(m * n, dtype=torch.float64, device=dev)
+        E.fill(E.F64, dA, m * k, 11, E.DIST_NORMAL, 0, 1)
+        E.fill(E.F64, dB, k * n, 12, E.DIST_NORMAL, 0, 1)
+        E.fill(E.F64, dC, m * n, 13, E.DIST_NORMAL, 0, 1)
+        fn = lambda: E.gemm(E.F64, E.OP_N, E.OP_N, m, n, k, 1.5, dA, m, dB, k, 0.5, dC, m)
+        wall, kern = timed(fn, max(3, args.steps // 2), args.warmup)
+        tf = 2.0 * m * n * k / (wall * 1e-3) / 1e12
+        out[name] = {"ms": wall, "kernel_ms": kern, "tflops": tf, "frac_of_fp64_peak": tf / PEAK["fp64"],
+                     "config": E.plan(E.F64, 0, 0, m, n, k)}
+        log(f"{name}: {out[name]}")
+        del dA, dB, dC
+    return out
+
+
+if __name__ == "__main__":
+    sys.exit(main())

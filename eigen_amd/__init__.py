@@ -1,0 +1,207 @@
+"""eigen_amd -- MI355X (gfx950) dense matrix products behind Eigen's interfaces.
+
+The product lives in two in-tree shared libraries built by ``eigen_amd.build``:
+
+* ``_lib/libeigen_amd.so``      -- the C-ABI of ``include/eigen_amd.h``
+* ``_lib/libeigen_amd_blas.so`` -- ``sgemm_``/``dgemm_`` (``include/eigen_amd_blas.h``)
+
+This module loads them with ctypes and offers thin helpers over device
+buffers (torch tensors on ``cuda`` are used only as device memory and
+streams).  Matrices are column-major: an m x n matrix with leading
+dimension ld occupies ``ld * n`` elements of a flat buffer.
+
+There is no CPU fallback: if the libraries are missing every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBDIR = os.path.join(HERE, "_lib")
+
+F32, F64 = 0, 1
+OP_N, OP_T, OP_C = 0, 1, 2
+PREC_AUTO, PREC_PLAIN, PREC_SPLIT2, PREC_SPLIT3 = 0, 1, 2, 3
+TUNE_HEURISTIC, TUNE_MLP, TUNE_TABLE = 0, 1, 2
+DIST_UNIFORM, DIST_NORMAL, DIST_WIDE, DIST_INT = 0, 1, 2, 3
+
+_lib = None
+_blas = None
+
+
+class EigenAmdError(RuntimeError):
+    def __init__(self, status: int, what: str = ""):
+        msg = f"{what}: status {status}"
+        if _lib is not None:
+            msg += f" ({_lib.eh_status_string(status).decode()}: {_lib.eh_last_error().decode()})"
+        super().__init__(msg)
+        self.status = status
+
+
+def lib_path() -> str:
+    return os.path.join(LIBDIR, "libeigen_amd.so")
+
+
+def blas_path() -> str:
+    return os.path.join(LIBDIR, "libeigen_amd_blas.so")
+
+
+def lib():
+    """ctypes handle of libeigen_amd.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        path = lib_path()
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} not built: run `python -m eigen_amd.build` (no CPU fallback exists)")
+        L = C.CDLL(path)
+        vp, i64, i32, u64, f64 = C.c_void_p, C.c_int64, C.c_int, C.c_uint64, C.c_double
+        L.eh_gemm.restype = i32
+        L.eh_gemm.argtypes = [i32, i32, i32, i64, i64, i64, vp, vp, i64, vp, i64, vp, vp, i64, i32, vp]
+        L.eh_gemm_config.restype = i32
+        L.eh_gemm_config.argtypes = [i32, i32, i32, i64, i64, i64, vp, vp, i64, vp, i64, vp, vp, i64, i32, i32, vp]
+        L.eh_num_configs.restype = i32
+        L.eh_config_info.restype = i32
+        L.eh_config_info.argtypes = [i32, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)]
+        L.eh_plan.restype = i32
+        L.eh_plan.argtypes = [i32, i32, i32, i64, i64, i64, C.POINTER(i32)]
+        L.eh_set_tune_mode.restype = i32
+        L.eh_set_tune_mode.argtypes = [i32]
+        L.eh_get_tune_mode.restype = i32
+        L.eh_set_sgemm_precision.restype = i32
+        L.eh_set_sgemm_precision.argtypes = [i32]
+        L.eh_get_sgemm_precision.restype = i32
+        L.eh_fill.restype = i32
+        L.eh_fill.argtypes = [i32, vp, i64, u64, i32, f64, f64, vp]
+        L.eh_split_bf16.restype = i32
+        L.eh_split_bf16.argtypes = [vp, i64, i32, vp, vp, vp, vp]
+        L.eh_synchronize.restype = i32
+        L.eh_synchronize.argtypes = [vp]
+        L.eh_status_string.restype = C.c_char_p
+        L.eh_status_string.argtypes = [i32]
+        L.eh_last_error.restype = C.c_char_p
+        L.eh_version.restype = C.c_char_p
+        _declare_extra(L)
+        _lib = L
+    return _lib
+
+
+def _declare_extra(L) -> None:
+    """Signatures of the grouped / tuner / MLP entry points (present in full builds)."""
+    vp, i64, i32, f64, f32 = C.c_void_p, C.c_int64, C.c_int, C.c_double, C.c_float
+    table = {
+        "eh_grouped_create": (i32, [vp, i64, C.POINTER(vp)]),
+        "eh_grouped_run": (i32, [vp, vp]),
+        "eh_grouped_graph_launch": (i32, [vp, vp]),
+        "eh_grouped_run_per_product": (i32, [vp, vp]),
+        "eh_grouped_destroy": (i32, [vp]),
+        "eh_grouped_info": (i32, [vp, C.POINTER(i64), C.POINTER(i64)]),
+        "eh_time_config": (i32, [i32, i32, i32, i64, i64, i64, vp, vp, vp, i32, i32, C.POINTER(f32)]),
+        "eh_tuner_load": (i32, [C.c_char_p]),
+        "eh_tuner_save": (i32, [C.c_char_p]),
+        "eh_tuner_count": (i64, []),
+        "eh_tuner_clear": (i32, []),
+        "eh_mlp_create": (i32, [i32, i32, i32, i64, C.POINTER(vp)]),
+        "eh_mlp_destroy": (i32, [vp]),
+        "eh_mlp_set_params": (i32, [vp, vp]),
+        "eh_mlp_get_params": (i32, [vp, vp]),
+        "eh_mlp_num_params": (i64, [vp]),
+        "eh_mlp_forward": (i32, [vp, vp, i64, vp, vp]),
+        "eh_mlp_backward": (i32, [vp, vp, vp, i64, vp, vp, vp]),
+        "eh_mlp_adam_step": (i32, [vp, vp, f32, f32, f32, f32, i32, vp]),
+        "eh_mlp_train": (i32, [vp, vp, vp, i64, i32, i32, f32, f32, i64, vp, vp]),
+        "eh_mlp_predict": (i32, [vp, vp, i64, vp, vp]),
+        "eh_planner_set_mlp": (i32, [vp, vp]),
+        "eh_mlp_features": (i32, [i32, i32, i32, i64, i64, i64, vp]),
+    }
+    for name, (res, args) in table.items():
+        if hasattr(L, name):
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+
+
+def blas():
+    """ctypes handle of libeigen_amd_blas.so."""
+    global _blas
+    if _blas is None:
+        lib()
+        path = blas_path()
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} not built")
+        _blas = C.CDLL(path)
+        _blas.eh_blas_set_stream.argtypes = [C.c_void_p]
+        _blas.eh_blas_get_stream.restype = C.c_void_p
+    return _blas
+
+
+def check(status: int, what: str = "eigen_amd call") -> None:
+    if status != 0:
+        raise EigenAmdError(status, what)
+
+
+def _scalar(dtype: int, v: float):
+    return C.c_float(v) if dtype == F32 else C.c_double(v)
+
+
+def _ptr(t) -> int | None:
+    if t is None:
+        return None
+    if isinstance(t, int):
+        return t
+    return t.data_ptr()
+
+
+def current_stream() -> int | None:
+    import torch
+    return torch.cuda.current_stream().cuda_stream if torch.cuda.is_available() else None
+
+
+def gemm(dtype: int, opa: int, opb: int, m: int, n: int, k: int, alpha: float, A, lda: int, B, ldb: int,
+         beta: float, Cm, ldc: int, prec: int = PREC_AUTO, config: int | None = None, stream=None) -> None:
+    """C = alpha*op(A)*op(B) + beta*C on device buffers (torch tensors or raw pointers)."""
+    L = lib()
+    a, b = _scalar(dtype, alpha), _scalar(dtype, beta)
+    s = current_stream() if stream is None else stream
+    if config is None:
+        st = L.eh_gemm(dtype, opa, opb, m, n, k, C.byref(a), _ptr(A), lda, _ptr(B), ldb, C.byref(b), _ptr(Cm), ldc,
+                       prec, s)
+    else:
+        st = L.eh_gemm_config(dtype, opa, opb, m, n, k, C.byref(a), _ptr(A), lda, _ptr(B), ldb, C.byref(b), _ptr(Cm),
+                              ldc, prec, config, s)
+    check(st, "eh_gemm")
+
+
+def fill(dtype: int, X, n: int, seed: int, dist: int = DIST_UNIFORM, p0: float = -1.0, p1: float = 1.0,
+         stream=None) -> None:
+    s = current_stream() if stream is None else stream
+    check(lib().eh_fill(dtype, _ptr(X), n, seed, dist, p0, p1, s), "eh_fill")
+
+
+def split_bf16(x, n: int, terms: int, t0, t1=None, t2=None, stream=None) -> None:
+    s = current_stream() if stream is None else stream
+    check(lib().eh_split_bf16(_ptr(x), n, terms, _ptr(t0), _ptr(t1), _ptr(t2), s), "eh_split_bf16")
+
+
+def num_configs() -> int:
+    return lib().eh_num_configs()
+
+
+def config_info(config: int):
+    tm, tn, sk = C.c_int(), C.c_int(), C.c_int()
+    check(lib().eh_config_info(config, C.byref(tm), C.byref(tn), C.byref(sk)), "eh_config_info")
+    return tm.value, tn.value, sk.value
+
+
+def plan(dtype: int, opa: int, opb: int, m: int, n: int, k: int) -> int:
+    c = C.c_int()
+    check(lib().eh_plan(dtype, opa, opb, m, n, k, C.byref(c)), "eh_plan")
+    return c.value
+
+
+def set_tune_mode(mode: int) -> None:
+    check(lib().eh_set_tune_mode(mode), "eh_set_tune_mode")
+
+
+def set_sgemm_precision(prec: int) -> None:
+    check(lib().eh_set_sgemm_precision(prec), "eh_set_sgemm_precision")

@@ -1,0 +1,64 @@
+"""Build the gfx950 libraries in-tree.
+
+    python -m eigen_amd.build            # compile everything (always rebuilds)
+
+Produces, next to this file:
+    _lib/libeigen_amd.so        the C-ABI (include/eigen_amd.h) and all kernels
+    _lib/libeigen_amd_blas.so   sgemm_/dgemm_/xerbla_ (include/eigen_amd_blas.h)
+"""
+from __future__ import annotations
+
+import concurrent.futures as cf
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+OBJ = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "_lib")
+
+ARCH = "gfx950"
+HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+# Packed FP32 instructions are disabled for all device code.
+HIPFLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
+            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+
+LIB_SOURCES = ["gemm_mfma.hip", "gemm_split.hip", "fill.hip", "grouped.hip", "mlp.hip",
+               "capi.cpp", "planner.cpp", "tuner.cpp"]
+BLAS_SOURCES = ["blas.cpp"]
+
+
+def _compile(src: str) -> str:
+    out = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
+    cmd = [HIPCC] + HIPFLAGS + ["-c", os.path.join(CSRC, src), "-o", out]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    return out
+
+
+def build(verbose: bool = False) -> None:
+    os.makedirs(OBJ, exist_ok=True)
+    os.makedirs(LIB, exist_ok=True)
+    sources = [s for s in LIB_SOURCES + BLAS_SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    jobs = min(len(sources), int(os.environ.get("MAX_JOBS", os.cpu_count() or 4)), 16)
+    with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
+        objs = dict(zip(sources, ex.map(_compile, sources)))
+    lib_objs = [objs[s] for s in LIB_SOURCES if s in objs]
+    libpath = os.path.join(LIB, "libeigen_amd.so")
+    subprocess.run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", libpath + ".tmp"] + lib_objs,
+                   check=True)
+    os.replace(libpath + ".tmp", libpath)
+    blaspath = os.path.join(LIB, "libeigen_amd_blas.so")
+    subprocess.run([HIPCC, "-shared", "-fPIC", "-o", blaspath + ".tmp", objs["blas.cpp"], "-L" + LIB,
+                    "-leigen_amd", "-Wl,-rpath,$ORIGIN", "-Wl,-Bsymbolic"], check=True)
+    os.replace(blaspath + ".tmp", blaspath)
+    if verbose:
+        print("built", libpath, blaspath)
+
+
+if __name__ == "__main__":
+    build(verbose=True)
+    sys.exit(0)

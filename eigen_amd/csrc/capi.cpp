@@ -1,0 +1,360 @@
+// capi.cpp -- the C-ABI: argument checks, planning, workspace, dispatch.
+//
+// Host-side counterpart of Eigen's GEMM driver
+// (Eigen/src/Core/products/GeneralMatrixMatrix.h:117-246 and :422-515): it
+// takes the product request, applies the BLAS quick-return rules
+// (blas/level3_impl.h:80-95, GeneralMatrixMatrix.h:130), chooses the tile /
+// split-K configuration (the role of computeProductBlockingSizes,
+// GeneralBlockPanelKernel.h:201-452) and launches the device kernels.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <utility>
+
+#include "../../include/eigen_amd.h"
+#include "internal.hpp"
+#include "kernels.hpp"
+
+namespace eh {
+
+thread_local std::string g_last_error;
+
+eh_status fail(eh_status s, const std::string& msg) {
+  g_last_error = msg;
+  return s;
+}
+
+eh_status hip_fail(hipError_t e, const char* what) {
+  return fail(EH_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// ---- configuration table --------------------------------------------------
+// config = tile * 4 + split index; tiles as in launch_gemm_mfma, split-K
+// factors {1, 2, 4, 8}.
+static const int kTileM[kNumTiles] = {64, 64, 128, 128, 128, 256};
+static const int kTileN[kNumTiles] = {64, 128, 64, 128, 256, 128};
+static const int kSplits[kNumSplits] = {1, 2, 4, 8};
+
+void config_decode(int config, int* tile, int* splits) {
+  *tile = config / kNumSplits;
+  *splits = kSplits[config % kNumSplits];
+}
+
+int config_tile_m(int config) { return kTileM[config / kNumSplits]; }
+int config_tile_n(int config) { return kTileN[config / kNumSplits]; }
+
+static int split_index(int s) {
+  for (int i = 0; i < kNumSplits; ++i)
+    if (kSplits[i] == s) return i;
+  return 0;
+}
+
+// The closed-form rule (the method without the learned selector): the
+// largest tile that still gives at least one wave of workgroups over the
+// 256 CUs, then split K while the grid stays under one wave and each split
+// keeps at least 512 of depth.
+int heuristic_config(int dtype, int64_t m, int64_t n, int64_t k) {
+  (void)dtype;
+  static const int order[] = {5, 3, 2, 1, 0};  // preferred tiles, largest first
+  int tile = 0;
+  for (int t : order) {
+    int64_t tiles = ((m + kTileM[t] - 1) / kTileM[t]) * ((n + kTileN[t] - 1) / kTileN[t]);
+    if (tiles >= 256) {
+      tile = t;
+      break;
+    }
+  }
+  int64_t tiles = ((m + kTileM[tile] - 1) / kTileM[tile]) * ((n + kTileN[tile] - 1) / kTileN[tile]);
+  int s = 1;
+  while (s < 8 && tiles * s < 256 && k / (2 * s) >= 512) s *= 2;
+  return tile * kNumSplits + split_index(s);
+}
+
+static std::mutex g_mutex;
+static eh_tune_mode g_mode = EH_TUNE_HEURISTIC;
+static int g_sgemm_prec = -1;  // -1: not yet read from the environment
+
+static eh_precision default_sgemm_precision() {
+  std::lock_guard<std::mutex> lk(g_mutex);
+  if (g_sgemm_prec < 0) {
+    g_sgemm_prec = EH_PREC_PLAIN;
+    if (const char* e = std::getenv("EIGEN_AMD_SGEMM_PRECISION")) {
+      if (!std::strcmp(e, "split3")) g_sgemm_prec = EH_PREC_SPLIT3;
+      if (!std::strcmp(e, "split2")) g_sgemm_prec = EH_PREC_SPLIT2;
+    }
+  }
+  return (eh_precision)g_sgemm_prec;
+}
+
+// ---- workspace --------------------------------------------------------------
+// One growable device buffer per (device, stream) for split-K partials.
+// Growing frees the old buffer with hipFree, which waits for the device, so
+// work still queued on it finishes first.
+struct Workspace {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+};
+static std::map<std::pair<int, hipStream_t>, Workspace> g_ws;
+
+eh_status workspace_get(hipStream_t stream, size_t bytes, void** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+  std::lock_guard<std::mutex> lk(g_mutex);
+  Workspace& w = g_ws[{dev, stream}];
+  if (w.bytes < bytes) {
+    if (w.ptr) {
+      e = hipFree(w.ptr);
+      if (e != hipSuccess) return hip_fail(e, "hipFree(workspace)");
+      w.ptr = nullptr;
+      w.bytes = 0;
+    }
+    size_t want = std::max(bytes, (size_t)1 << 20);
+    e = hipMalloc(&w.ptr, want);
+    if (e != hipSuccess) {
+      w.ptr = nullptr;
+      return fail(EH_ERR_ALLOC, std::string("hipMalloc(workspace): ") + hipGetErrorString(e));
+    }
+    w.bytes = want;
+  }
+  *out = w.ptr;
+  return EH_SUCCESS;
+}
+
+// ---- argument checks ------------------------------------------------------
+static bool valid_op(int op) { return op == EH_OP_N || op == EH_OP_T || op == EH_OP_C; }
+
+eh_status check_gemm_args(int dtype, int opa, int opb, int64_t m, int64_t n, int64_t k, const void* alpha,
+                          const void* A, int64_t lda, const void* B, int64_t ldb, const void* beta, const void* C,
+                          int64_t ldc) {
+  if (dtype != EH_F32 && dtype != EH_F64) return fail(EH_ERR_INVALID, "dtype must be EH_F32 or EH_F64");
+  if (!valid_op(opa)) return fail(EH_ERR_INVALID, "opa must be N, T or C");
+  if (!valid_op(opb)) return fail(EH_ERR_INVALID, "opb must be N, T or C");
+  if (m < 0 || n < 0 || k < 0) return fail(EH_ERR_INVALID, "m, n, k must be >= 0");
+  if (m > INT32_MAX || n > INT32_MAX || k > INT32_MAX) return fail(EH_ERR_UNSUPPORTED, "m, n, k must fit int32");
+  if (lda < std::max<int64_t>(1, opa == EH_OP_N ? m : k)) return fail(EH_ERR_INVALID, "lda too small");
+  if (ldb < std::max<int64_t>(1, opb == EH_OP_N ? k : n)) return fail(EH_ERR_INVALID, "ldb too small");
+  if (ldc < std::max<int64_t>(1, m)) return fail(EH_ERR_INVALID, "ldc too small");
+  if (!alpha || !beta) return fail(EH_ERR_INVALID, "alpha and beta must point to host scalars");
+  if (m > 0 && n > 0) {
+    if (!C) return fail(EH_ERR_INVALID, "C is NULL");
+    if (k > 0 && (!A || !B)) return fail(EH_ERR_INVALID, "A or B is NULL");
+  }
+  return EH_SUCCESS;
+}
+
+// ---- dispatch ---------------------------------------------------------------
+template <typename T>
+static eh_status run_gemm(int opa, int opb, int64_t m, int64_t n, int64_t k, T alpha, const T* A, int64_t lda,
+                          const T* B, int64_t ldb, T beta, T* C, int64_t ldc, eh_precision prec, int config,
+                          hipStream_t stream) {
+  if (m == 0 || n == 0) return EH_SUCCESS;
+  hipError_t e;
+  if (k == 0 || alpha == T(0)) {  // product term vanishes: A and B are not read
+    if (beta == T(1)) return EH_SUCCESS;
+    e = launch_scale_c<T>(C, (int)m, (int)n, ldc, beta, stream);
+    return e == hipSuccess ? EH_SUCCESS : hip_fail(e, "scale_c launch");
+  }
+  const bool ta = opa != EH_OP_N, tb = opb != EH_OP_N;
+  GemmParams<T> p{};
+  p.A = A;
+  p.B = B;
+  p.C = C;
+  p.lda = lda;
+  p.ldb = ldb;
+  p.ldc = ldc;
+  p.M = (int)m;
+  p.N = (int)n;
+  p.K = (int)k;
+  p.alpha = alpha;
+  p.beta = beta;
+  constexpr int EPC = 16 / sizeof(T);
+  // 16-byte loads along the k-contiguous operand need aligned rows.
+  p.vecA = ta && ((uintptr_t)A % 16 == 0) && (lda % EPC == 0);
+  p.vecB = !tb && ((uintptr_t)B % 16 == 0) && (ldb % EPC == 0);
+
+  int tile = 0, splits = 1, kslice = (int)(128 / sizeof(T));
+  bool split_prec = false;
+  int terms = 0;
+  if (std::is_same<T, float>::value) {
+    if (prec == EH_PREC_AUTO) prec = default_sgemm_precision();
+    if (prec == EH_PREC_SPLIT2 || prec == EH_PREC_SPLIT3) {
+      split_prec = true;
+      terms = prec == EH_PREC_SPLIT3 ? 3 : 2;
+    }
+  } else if (prec != EH_PREC_AUTO && prec != EH_PREC_PLAIN) {
+    return fail(EH_ERR_UNSUPPORTED, "FP64 products run on FP64 MFMA only (precision AUTO or PLAIN)");
+  }
+
+  if (split_prec) {
+    kslice = 32;
+    // 256x128 tiles once the grid fills the chip, 128x128 below that.
+    int64_t t256 = ((m + 255) / 256) * ((n + 127) / 128);
+    tile = t256 >= 256 ? 1 : 0;
+    int64_t tiles = tile ? t256 : ((m + 127) / 128) * ((n + 127) / 128);
+    if (config >= 0) {
+      config_decode(config, &tile, &splits);
+      tile = tile >= 5 ? 1 : 0;
+    } else {
+      while (splits < 8 && tiles * splits < 256 && k / (2 * splits) >= 512) splits *= 2;
+    }
+  } else {
+    if (config < 0) {
+      eh_status st = plan_config(std::is_same<T, float>::value ? EH_F32 : EH_F64, opa, opb, m, n, k, &config);
+      if (st != EH_SUCCESS) return st;
+    }
+    if (config < 0 || config >= kNumConfigs) return fail(EH_ERR_INVALID, "config out of range");
+    config_decode(config, &tile, &splits);
+  }
+
+  // depth per split: a whole number of k slices
+  int64_t per = (k + splits - 1) / splits;
+  per = (per + kslice - 1) / kslice * kslice;
+  p.k_per_split = (int)per;
+  splits = (int)((k + per - 1) / per);  // drop empty trailing splits
+  if (splits > 1) {
+    void* ws = nullptr;
+    eh_status st = workspace_get(stream, (size_t)splits * m * n * sizeof(T), &ws);
+    if (st != EH_SUCCESS) return st;
+    p.W = static_cast<T*>(ws);
+  }
+  if constexpr (std::is_same<T, float>::value) {
+    if (split_prec)
+      e = launch_gemm_split(p, terms, tile, ta, tb, splits, stream);
+    else
+      e = launch_gemm_mfma<T>(p, tile, ta, tb, splits, stream);
+  } else {
+    e = launch_gemm_mfma<T>(p, tile, ta, tb, splits, stream);
+  }
+  if (e != hipSuccess) return hip_fail(e, "gemm launch");
+  if (splits > 1) {
+    e = launch_splitk_reduce<T>(p, splits, stream);
+    if (e != hipSuccess) return hip_fail(e, "split-K reduce launch");
+  }
+  return EH_SUCCESS;
+}
+
+eh_status gemm_dispatch(int dtype, int opa, int opb, int64_t m, int64_t n, int64_t k, const void* alpha,
+                        const void* A, int64_t lda, const void* B, int64_t ldb, const void* beta, void* C,
+                        int64_t ldc, int prec, int config, void* stream) {
+  eh_status st = check_gemm_args(dtype, opa, opb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+  if (st != EH_SUCCESS) return st;
+  if (config >= kNumConfigs) return fail(EH_ERR_INVALID, "config out of range");
+  if (prec < EH_PREC_AUTO || prec > EH_PREC_SPLIT3) return fail(EH_ERR_INVALID, "bad precision");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (dtype == EH_F32)
+    return run_gemm<float>(opa, opb, m, n, k, *static_cast<const float*>(alpha), static_cast<const float*>(A), lda,
+                           static_cast<const float*>(B), ldb, *static_cast<const float*>(beta),
+                           static_cast<float*>(C), ldc, (eh_precision)prec, config, s);
+  return run_gemm<double>(opa, opb, m, n, k, *static_cast<const double*>(alpha), static_cast<const double*>(A), lda,
+                          static_cast<const double*>(B), ldb, *static_cast<const double*>(beta),
+                          static_cast<double*>(C), ldc, (eh_precision)prec, config, s);
+}
+
+eh_tune_mode tune_mode() {
+  std::lock_guard<std::mutex> lk(g_mutex);
+  return g_mode;
+}
+
+}  // namespace eh
+
+using namespace eh;
+
+extern "C" {
+
+eh_status eh_gemm(eh_dtype dtype, eh_op opa, eh_op opb, int64_t m, int64_t n, int64_t k, const void* alpha,
+                  const void* A, int64_t lda, const void* B, int64_t ldb, const void* beta, void* C, int64_t ldc,
+                  eh_precision prec, void* stream) {
+  return gemm_dispatch(dtype, opa, opb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, prec, -1, stream);
+}
+
+eh_status eh_gemm_config(eh_dtype dtype, eh_op opa, eh_op opb, int64_t m, int64_t n, int64_t k, const void* alpha,
+                         const void* A, int64_t lda, const void* B, int64_t ldb, const void* beta, void* C,
+                         int64_t ldc, eh_precision prec, int config, void* stream) {
+  return gemm_dispatch(dtype, opa, opb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, prec, config, stream);
+}
+
+int eh_num_configs(void) { return kNumConfigs; }
+
+eh_status eh_config_info(int config, int* tile_m, int* tile_n, int* split_k) {
+  if (config < 0 || config >= kNumConfigs) return fail(EH_ERR_INVALID, "config out of range");
+  int tile, s;
+  config_decode(config, &tile, &s);
+  if (tile_m) *tile_m = kTileM[tile];
+  if (tile_n) *tile_n = kTileN[tile];
+  if (split_k) *split_k = s;
+  return EH_SUCCESS;
+}
+
+eh_status eh_plan(eh_dtype dtype, eh_op opa, eh_op opb, int64_t m, int64_t n, int64_t k, int* config) {
+  if (!config) return fail(EH_ERR_INVALID, "config is NULL");
+  if (dtype != EH_F32 && dtype != EH_F64) return fail(EH_ERR_INVALID, "bad dtype");
+  if (m < 0 || n < 0 || k < 0) return fail(EH_ERR_INVALID, "m, n, k must be >= 0");
+  return plan_config(dtype, opa, opb, m, n, k, config);
+}
+
+eh_status eh_set_tune_mode(eh_tune_mode mode) {
+  if (mode != EH_TUNE_HEURISTIC && mode != EH_TUNE_MLP && mode != EH_TUNE_TABLE)
+    return fail(EH_ERR_INVALID, "bad tune mode");
+  std::lock_guard<std::mutex> lk(g_mutex);
+  g_mode = mode;
+  return EH_SUCCESS;
+}
+
+eh_tune_mode eh_get_tune_mode(void) { return tune_mode(); }
+
+eh_status eh_set_sgemm_precision(eh_precision prec) {
+  if (prec < EH_PREC_AUTO || prec > EH_PREC_SPLIT3) return fail(EH_ERR_INVALID, "bad precision");
+  std::lock_guard<std::mutex> lk(g_mutex);
+  g_sgemm_prec = prec == EH_PREC_AUTO ? (int)EH_PREC_PLAIN : (int)prec;
+  return EH_SUCCESS;
+}
+
+eh_precision eh_get_sgemm_precision(void) { return default_sgemm_precision(); }
+
+eh_status eh_fill(eh_dtype dtype, void* x, int64_t n, uint64_t seed, int dist, double p0, double p1, void* stream) {
+  if (dtype != EH_F32 && dtype != EH_F64) return fail(EH_ERR_INVALID, "bad dtype");
+  if (n < 0 || (n > 0 && !x)) return fail(EH_ERR_INVALID, "bad buffer");
+  if (dist < 0 || dist > 3) return fail(EH_ERR_INVALID, "bad distribution");
+  if (n == 0) return EH_SUCCESS;
+  hipError_t e = launch_fill(dtype, x, n, seed, dist, p0, p1, static_cast<hipStream_t>(stream));
+  return e == hipSuccess ? EH_SUCCESS : hip_fail(e, "fill launch");
+}
+
+eh_status eh_split_bf16(const float* x, int64_t n, int terms, uint16_t* t0, uint16_t* t1, uint16_t* t2,
+                        void* stream) {
+  if (terms < 1 || terms > 3) return fail(EH_ERR_INVALID, "terms must be 1, 2 or 3");
+  if (n < 0) return fail(EH_ERR_INVALID, "n < 0");
+  if (n > 0 && (!x || !t0 || (terms > 1 && !t1) || (terms > 2 && !t2))) return fail(EH_ERR_INVALID, "NULL buffer");
+  if (n == 0) return EH_SUCCESS;
+  hipError_t e = launch_split_bf16(x, n, terms, t0, t1, t2, static_cast<hipStream_t>(stream));
+  return e == hipSuccess ? EH_SUCCESS : hip_fail(e, "split launch");
+}
+
+eh_status eh_synchronize(void* stream) {
+  hipError_t e = hipStreamSynchronize(static_cast<hipStream_t>(stream));
+  return e == hipSuccess ? EH_SUCCESS : hip_fail(e, "hipStreamSynchronize");
+}
+
+const char* eh_status_string(eh_status s) {
+  switch (s) {
+    case EH_SUCCESS: return "success";
+    case EH_ERR_INVALID: return "invalid argument";
+    case EH_ERR_HIP: return "HIP runtime error";
+    case EH_ERR_UNSUPPORTED: return "unsupported";
+    case EH_ERR_ALLOC: return "allocation failed";
+  }
+  return "unknown status";
+}
+
+const char* eh_last_error(void) { return g_last_error.c_str(); }
+
+const char* eh_version(void) { return "eigen_amd 0.1 gfx950"; }
+
+}  // extern "C"

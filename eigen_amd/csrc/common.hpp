@@ -1,0 +1,84 @@
+// common.hpp -- shared device/host definitions for the gfx950 product kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace eh {
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
+
+// MFMA micro-tile traits.  The k index a lane feeds to one instruction is a
+// free permutation (k is summed), so kernels choose it to make each lane's
+// k values contiguous in LDS; only the output layout is fixed by hardware.
+template <typename T>
+struct Mfma;
+
+// v_mfma_f32_32x32x2_f32: A lane l -> (row l%32, k l/32); 16 accumulators.
+template <>
+struct Mfma<float> {
+  static constexpr int MI = 32, NI = 32;
+  static constexpr int KG = 2;  // k groups across lanes (lane / 32)
+  static constexpr int ACC = 16;
+  using acc_t = floatx16;
+  __device__ static inline void mma(float a, float b, acc_t& c) {
+    c = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, c, 0, 0, 0);
+  }
+  // row (within the 32x32 block) of accumulator r held by lane l; column is l%32
+  __device__ static inline int row(int r, int l) { return (r >> 2) * 8 + (l >> 5) * 4 + (r & 3); }
+  __device__ static inline int col(int l) { return l & 31; }
+};
+
+// v_mfma_f64_16x16x4_f64: A lane l -> (row l%16, k l/16); 4 accumulators.
+template <>
+struct Mfma<double> {
+  static constexpr int MI = 16, NI = 16;
+  static constexpr int KG = 4;
+  static constexpr int ACC = 4;
+  using acc_t = doublex4;
+  __device__ static inline void mma(double a, double b, acc_t& c) {
+    c = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  __device__ static inline int row(int r, int l) { return (l >> 4) * 4 + r; }
+  __device__ static inline int col(int l) { return l & 15; }
+};
+
+__device__ inline float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ inline double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+// XOR swizzle of 16-byte chunks inside an LDS row.  For 8 chunks per row
+// (128-byte rows) any 8 consecutive rows at the same logical chunk land on
+// distinct chunks modulo 8, and any 16 consecutive rows on distinct chunks
+// modulo 16, so ds_read_b128 / ds_write_b128 by consecutive lanes walking
+// rows do not collide on banks whether the LDS serves 128 or 256 bytes per
+// clock.
+__device__ __host__ inline int swz8(int row, int chunk) { return chunk ^ ((row ^ (row >> 3)) & 7); }
+// Same property for 4 chunks per row (64-byte rows).
+__device__ __host__ inline int swz4(int row, int chunk) { return chunk ^ (((row >> 1) ^ (row >> 3)) & 3); }
+
+// Map a linear workgroup id to a logical tile id so that each XCD (workgroups
+// are dealt round-robin over the 8 XCDs) owns one contiguous range of tiles,
+// which keeps the A row panels and B column panels it reuses in its own L2.
+__device__ inline int xcd_remap(int bid, int nblocks) {
+  constexpr int kXcd = 8;
+  if (nblocks % kXcd != 0) return bid;
+  return (bid % kXcd) * (nblocks / kXcd) + bid / kXcd;
+}
+
+// Grouped tile order: walk GM tile-rows at a time so neighbouring tiles share
+// B column panels in L2.
+__device__ inline void tile_coords(int logical, int ntm, int ntn, int gm, int& tm, int& tn) {
+  int group = gm * ntn;
+  int g = logical / group;
+  int first = g * gm;
+  int rows = ntm - first < gm ? ntm - first : gm;
+  int in = logical - g * group;
+  tm = first + in % rows;
+  tn = in / rows;
+}
+
+}  // namespace eh

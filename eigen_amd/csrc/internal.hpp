@@ -1,0 +1,36 @@
+// internal.hpp -- declarations shared by the host-side translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "../../include/eigen_amd.h"
+
+namespace eh {
+
+constexpr int kNumTiles = 6;
+constexpr int kNumSplits = 4;
+constexpr int kNumConfigs = kNumTiles * kNumSplits;  // 24 candidate configurations
+
+eh_status fail(eh_status s, const std::string& msg);
+eh_status hip_fail(hipError_t e, const char* what);
+
+void config_decode(int config, int* tile, int* splits);
+int config_tile_m(int config);
+int config_tile_n(int config);
+int heuristic_config(int dtype, int64_t m, int64_t n, int64_t k);
+eh_tune_mode tune_mode();
+
+// Planner (planner.cpp): configuration for a shape under the current mode.
+eh_status plan_config(int dtype, int opa, int opb, int64_t m, int64_t n, int64_t k, int* config);
+
+eh_status workspace_get(hipStream_t stream, size_t bytes, void** out);
+eh_status check_gemm_args(int dtype, int opa, int opb, int64_t m, int64_t n, int64_t k, const void* alpha,
+                          const void* A, int64_t lda, const void* B, int64_t ldb, const void* beta, const void* C,
+                          int64_t ldc);
+eh_status gemm_dispatch(int dtype, int opa, int opb, int64_t m, int64_t n, int64_t k, const void* alpha,
+                        const void* A, int64_t lda, const void* B, int64_t ldb, const void* beta, void* C,
+                        int64_t ldc, int prec, int config, void* stream);
+
+}  // namespace eh

@@ -1,0 +1,42 @@
+// kernels.hpp -- launch interfaces of the device kernels (host side).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace eh {
+
+// Parameters of one product C = alpha*op(A)*op(B) + beta*C (column-major).
+// W is the split-K partial buffer (splits x M x N, column-major with ld = M),
+// used only when the grid has more than one split.
+template <typename T>
+struct GemmParams {
+  const T* A;
+  const T* B;
+  T* C;
+  T* W;
+  int64_t lda, ldb, ldc;
+  int M, N, K;
+  int k_per_split;  // multiple of the kernel's k slice
+  T alpha, beta;
+  int vecA, vecB;  // 16-byte loads allowed along the k-contiguous operand
+};
+
+// Plain MFMA product: tile 0..5 = 64x64, 64x128, 128x64, 128x128, 128x256, 256x128.
+template <typename T>
+hipError_t launch_gemm_mfma(const GemmParams<T>& p, int tile, bool ta, bool tb, int splits, hipStream_t s);
+template <typename T>
+hipError_t launch_splitk_reduce(const GemmParams<T>& p, int splits, hipStream_t s);
+template <typename T>
+hipError_t launch_scale_c(T* C, int M, int N, int64_t ldc, T beta, hipStream_t s);
+
+// Split-BF16 FP32 product: tile 0 = 128x128, 1 = 256x128; terms 2 or 3.
+hipError_t launch_gemm_split(const GemmParams<float>& p, int terms, int tile, bool ta, bool tb, int splits,
+                             hipStream_t s);
+hipError_t launch_split_bf16(const float* x, int64_t n, int terms, unsigned short* t0, unsigned short* t1,
+                             unsigned short* t2, hipStream_t s);
+
+// Seeded device generators (same streams as the documented splitmix64 scheme).
+hipError_t launch_fill(int dtype, void* x, int64_t n, uint64_t seed, int dist, double p0, double p1,
+                       hipStream_t s);
+
+}  // namespace eh

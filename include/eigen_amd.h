@@ -1,0 +1,129 @@
+/*
+ * eigen_amd.h -- C-ABI of the MI355X (gfx950) dense matrix-product library.
+ *
+ * Plain pointers and sizes only.  Matrices are column-major; A, B, C are
+ * device pointers (hipMalloc'd or any device-accessible memory); alpha and
+ * beta point to ONE host scalar of the call's dtype; `stream` is a
+ * hipStream_t (NULL = the default stream).  Calls are asynchronous on the
+ * stream unless stated otherwise and return a status code, never throw.
+ *
+ * Reference interfaces replaced (paths inside the Eigen source tree):
+ *   eh_gemm / eh_gemm_config
+ *       Eigen::internal::general_matrix_matrix_product<...>::run
+ *       (Eigen/src/Core/products/GeneralMatrixMatrix.h:117-246) together with
+ *       gemm_blocking_space / computeProductBlockingSizes
+ *       (GeneralMatrixMatrix.h:282-416, GeneralBlockPanelKernel.h:201-452):
+ *       the whole GEBP product C = alpha*op(A)*op(B) + beta*C, where the
+ *       reference's beta handling lives one level up in ?gemm_
+ *       (blas/level3_impl.h:80-95) and is fused here.
+ *   eh_plan / eh_set_tune_mode
+ *       computeProductBlockingSizes + evaluateProductBlockingSizesHeuristic
+ *       (GeneralBlockPanelKernel.h:201-452): the blocking choice, here a
+ *       tile/split-K configuration picked by a heuristic, a trained MLP or an
+ *       exhaustive timing table.
+ *   eh_split_bf16
+ *       float -> bfloat16 RTNE (Eigen/src/Core/arch/Default/BFloat16.h:434)
+ *       applied term by term: the split used by the FP32-accurate BF16 path.
+ *   eh_grouped_*
+ *       no single reference function: Eigen runs many small products as a
+ *       host loop of general_matrix_matrix_product / lazy products
+ *       (Eigen/src/Core/ProductEvaluators.h:606-692); here one persistent
+ *       kernel walks a device work list.
+ *   eh_mlp_*
+ *       no reference counterpart (the learned replacement of the blocking
+ *       heuristic).
+ * The BLAS entry points sgemm_/dgemm_ are declared in eigen_amd_blas.h.
+ */
+#ifndef EIGEN_AMD_H
+#define EIGEN_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum eh_status {
+  EH_SUCCESS = 0,
+  EH_ERR_INVALID = 1,     /* bad argument (see eh_last_error) */
+  EH_ERR_HIP = 2,         /* a HIP runtime call failed */
+  EH_ERR_UNSUPPORTED = 3, /* valid request this build does not handle */
+  EH_ERR_ALLOC = 4        /* device allocation failed */
+} eh_status;
+
+typedef enum eh_dtype { EH_F32 = 0, EH_F64 = 1 } eh_dtype;
+
+/* op(X): N = X, T = X^T, C = X^H (== X^T for real scalars). */
+typedef enum eh_op { EH_OP_N = 0, EH_OP_T = 1, EH_OP_C = 2 } eh_op;
+
+/* Arithmetic of an FP32 product.  AUTO = the library default for FP32
+ * (eh_set_sgemm_precision, or EIGEN_AMD_SGEMM_PRECISION=plain|split2|split3
+ * in the environment; PLAIN when unset).  FP64 products always use FP64
+ * MFMA and accept AUTO or PLAIN only. */
+typedef enum eh_precision {
+  EH_PREC_AUTO = 0,
+  EH_PREC_PLAIN = 1,  /* FP32 MFMA (v_mfma_f32_32x32x2_f32) */
+  EH_PREC_SPLIT2 = 2, /* 2 bf16 terms, 3 cross products on BF16 MFMA */
+  EH_PREC_SPLIT3 = 3  /* 3 bf16 terms, 6 cross products on BF16 MFMA */
+} eh_precision;
+
+/* Tile / schedule selection for the plain kernels. */
+typedef enum eh_tune_mode {
+  EH_TUNE_HEURISTIC = 0, /* closed-form rule (the no-MLP method) */
+  EH_TUNE_MLP = 1,       /* trained selector (falls back to HEURISTIC without weights) */
+  EH_TUNE_TABLE = 2      /* exhaustive timing table (tunes unseen shapes on first use) */
+} eh_tune_mode;
+
+/* ---- products ------------------------------------------------------------ */
+
+/* C = alpha*op(A)*op(B) + beta*C.  m x n result, inner dimension k.
+ * beta == 0: C is written without being read (NaN/Inf in C are discarded).
+ * alpha == 0 or k == 0: A and B are not read; C = beta*C.
+ * m == 0 or n == 0: nothing happens. */
+eh_status eh_gemm(eh_dtype dtype, eh_op opa, eh_op opb, int64_t m, int64_t n, int64_t k, const void* alpha,
+                  const void* A, int64_t lda, const void* B, int64_t ldb, const void* beta, void* C, int64_t ldc,
+                  eh_precision prec, void* stream);
+
+/* Same, with an explicit configuration index in [0, eh_num_configs()) for
+ * the plain kernels (config < 0 = let the planner choose). */
+eh_status eh_gemm_config(eh_dtype dtype, eh_op opa, eh_op opb, int64_t m, int64_t n, int64_t k, const void* alpha,
+                         const void* A, int64_t lda, const void* B, int64_t ldb, const void* beta, void* C,
+                         int64_t ldc, eh_precision prec, int config, void* stream);
+
+/* ---- planning ------------------------------------------------------------ */
+
+int eh_num_configs(void);
+/* Tile (tile_m x tile_n) and split-K factor of configuration `config`. */
+eh_status eh_config_info(int config, int* tile_m, int* tile_n, int* split_k);
+/* Configuration the planner would use for this shape (synchronous). */
+eh_status eh_plan(eh_dtype dtype, eh_op opa, eh_op opb, int64_t m, int64_t n, int64_t k, int* config);
+eh_status eh_set_tune_mode(eh_tune_mode mode);
+eh_tune_mode eh_get_tune_mode(void);
+eh_status eh_set_sgemm_precision(eh_precision prec);
+eh_precision eh_get_sgemm_precision(void);
+
+/* ---- utilities ----------------------------------------------------------- */
+
+/* Seeded synthetic data on the device: element i from splitmix64 at
+ * (seed, 2i[, 2i+1]); dist 0 uniform[p0,p1], 1 normal(p0,p1),
+ * 2 sign*U(1,2)*2^U(p0,p1), 3 integers in [p0,p1]. */
+eh_status eh_fill(eh_dtype dtype, void* x, int64_t n, uint64_t seed, int dist, double p0, double p1, void* stream);
+
+/* The FP32 -> bf16 term split the split-precision products apply while
+ * staging: t0 = bf16(x), t1 = bf16(x - t0), t2 = bf16(x - t0 - t1)
+ * (t1/t2 may be NULL when terms < 2/3).  Device pointers. */
+eh_status eh_split_bf16(const float* x, int64_t n, int terms, uint16_t* t0, uint16_t* t1, uint16_t* t2,
+                        void* stream);
+
+eh_status eh_synchronize(void* stream);
+const char* eh_status_string(eh_status s);
+/* Message of the last failing call on this thread ("" if none). */
+const char* eh_last_error(void);
+const char* eh_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* EIGEN_AMD_H */

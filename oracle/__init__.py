@@ -1,0 +1,252 @@
+"""CPU oracle for the dense matrix-product path -- TEST INFRASTRUCTURE ONLY.
+
+Only tests/, the smoke check and the benchmark's CPU-baseline leg import
+this package, and only as the checker (or, for the baseline, as the timed
+Eigen CPU GEBP).  The eigen_amd product never imports it.
+
+Two layers:
+  * ``lib()``  -- the C restatement in oracle.c (always built by ``make``).
+  * ``ref()``  -- Eigen itself, compiled from the reference tree into
+    oracle/_ref by ``make ref`` (present only where that build ran; the .so
+    files travel with the tree).  ``blas_ref()`` is Eigen's BLAS library.
+
+The helpers below wrap both with numpy arrays (column-major, Fortran order).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REF_TREE = "/root/reference"
+
+UNIFORM, NORMAL, WIDE, INT = 0, 1, 2, 3
+
+_lib = None
+_ref = None
+_blas = None
+
+
+def build(force: bool = False, with_ref: bool | None = None) -> None:
+    """Compile the C restatement (and the Eigen reference when the tree exists)."""
+    args = ["make", "-C", HERE]
+    if force:
+        args.append("-B")
+    subprocess.run(args + ["all"], check=True, stdout=subprocess.DEVNULL)
+    if with_ref is None:
+        with_ref = os.path.isdir(os.path.join(REF_TREE, "Eigen"))
+    if with_ref:
+        subprocess.run(["make", "-C", HERE, "-j8", "ref"], check=True, stdout=subprocess.DEVNULL)
+
+
+def _cpu_has_avx512() -> bool:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("flags"):
+                    fl = line.split()
+                    return all(x in fl for x in ("avx512f", "avx512bw", "avx512dq", "avx512vl", "avx512cd"))
+    except OSError:
+        pass
+    return False
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        path = os.path.join(HERE, "_build", "liboracle.so")
+        if not os.path.exists(path):
+            build(with_ref=False)
+        L = C.CDLL(path)
+        vp, i64, u64, f64, i32 = C.c_void_p, C.c_int64, C.c_uint64, C.c_double, C.c_int
+        L.orc_splitmix64_at.restype = u64
+        L.orc_splitmix64_at.argtypes = [u64, u64]
+        L.orc_fill_f32.argtypes = [vp, i64, u64, i32, f64, f64]
+        L.orc_fill_f64.argtypes = [vp, i64, u64, i32, f64, f64]
+        L.orc_gemm_check.restype = i32
+        L.orc_gemm_check.argtypes = [C.c_char, C.c_char] + [i32] * 6
+        L.orc_gemm_gebp_f32.restype = i32
+        L.orc_gemm_gebp_f32.argtypes = [C.c_char, C.c_char, i32, i32, i32, C.c_float, vp, i32, vp, i32,
+                                        C.c_float, vp, i32, i32, i32, i32]
+        L.orc_gemm_gebp_f64.restype = i32
+        L.orc_gemm_gebp_f64.argtypes = [C.c_char, C.c_char, i32, i32, i32, f64, vp, i32, vp, i32,
+                                        f64, vp, i32, i32, i32, i32]
+        L.orc_gemm_ref.argtypes = [i32, i32, i32, i64, i64, i64, f64, vp, i64, vp, i64, f64, vp, i64,
+                                   vp, i64, vp, vp]
+        L.orc_bf16_rtne.restype = C.c_uint16
+        L.orc_bf16_rtne.argtypes = [C.c_float]
+        L.orc_bf16_to_f32.restype = C.c_float
+        L.orc_bf16_to_f32.argtypes = [C.c_uint16]
+        L.orc_bf16_split.argtypes = [vp, i64, i32, vp, vp, vp]
+        L.orc_bf16_rtne_n.argtypes = [vp, i64, vp]
+        L.orc_gemm_split_ref.argtypes = [i32, i32, i32, i64, i64, i64, f64, vp, i64, vp, i64, f64, vp, i64,
+                                         vp, i64, vp]
+        _lib = L
+    return _lib
+
+
+def ref_available() -> bool:
+    return os.path.exists(os.path.join(HERE, "_ref", "libeigen_ref_avx2.so"))
+
+
+def ref():
+    """Eigen's CPU product path compiled from the reference tree (AVX-512 build where the host has it)."""
+    global _ref
+    if _ref is None:
+        name = "libeigen_ref_avx512.so" if _cpu_has_avx512() else "libeigen_ref_avx2.so"
+        path = os.path.join(HERE, "_ref", name)
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} missing: run `make -C oracle ref` where the Eigen tree exists")
+        L = C.CDLL(path)
+        vp, lg, i32, i64 = C.c_void_p, C.c_long, C.c_int, C.c_int64
+        for suf, T in (("f32", C.c_float), ("f64", C.c_double)):
+            f = getattr(L, "ref_gemm_" + suf)
+            f.argtypes = [i32, i32, lg, lg, lg, T, vp, lg, vp, lg, vp, lg]
+            g = getattr(L, "ref_gemm_beta_" + suf)
+            g.argtypes = [i32, i32, lg, lg, lg, T, vp, lg, vp, lg, T, vp, lg]
+        L.ref_grouped.argtypes = [i64, vp, vp, vp, vp, vp, vp, vp, i32]
+        L.ref_bf16_rtne.restype = C.c_uint16
+        L.ref_bf16_rtne.argtypes = [C.c_float]
+        L.ref_bf16_rtne_n.argtypes = [vp, i64, vp]
+        L.ref_max_threads.restype = i32
+        L.ref_get_threads.restype = i32
+        L.ref_set_threads.argtypes = [i32]
+        L.ref_simd.restype = C.c_char_p
+        L.ref_blocking_f32.argtypes = [lg, lg, lg, vp, vp, vp]
+        _ref = L
+    return _ref
+
+
+def blas_ref():
+    """Eigen's own BLAS library (sgemm_/dgemm_) with a recording xerbla_."""
+    global _blas
+    if _blas is None:
+        path = os.path.join(HERE, "_ref", "libeigen_blas_ref.so")
+        if not os.path.exists(path):
+            raise FileNotFoundError(path)
+        L = C.CDLL(path)
+        L.ref_blas_last_info.restype = C.c_int
+        L.ref_blas_last_name.restype = C.c_char_p
+        _blas = L
+    return _blas
+
+
+# --------------------------------------------------------------------------
+# numpy helpers
+# --------------------------------------------------------------------------
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def fill(shape, dtype, seed: int, dist: int = UNIFORM, p0: float = -1.0, p1: float = 1.0) -> np.ndarray:
+    """Column-major array from the seeded splitmix64 stream (element i = column-major index i)."""
+    a = np.empty(shape, dtype=dtype, order="F")
+    n = a.size
+    flat = a.reshape(-1, order="F")
+    f = lib().orc_fill_f32 if a.dtype == np.float32 else lib().orc_fill_f64
+    f(_ptr(flat), n, seed, dist, p0, p1)
+    return a
+
+
+def gemm_check(ta: str, tb: str, m, n, k, lda, ldb, ldc) -> int:
+    return lib().orc_gemm_check(ta.encode(), tb.encode(), m, n, k, lda, ldb, ldc)
+
+
+def gemm_gebp(ta: str, tb: str, m, n, k, alpha, A, lda, B, ldb, beta, C_, ldc, kc=0, mc=0, nc=0) -> int:
+    """C restatement of Eigen's BLAS gemm (in place on C_, column-major numpy array)."""
+    f = lib().orc_gemm_gebp_f32 if C_.dtype == np.float32 else lib().orc_gemm_gebp_f64
+    return f(ta.encode(), tb.encode(), m, n, k, alpha, _ptr(A), lda, _ptr(B), ldb, beta, _ptr(C_), ldc,
+             kc, mc, nc)
+
+
+def gemm_ref(ta: bool, tb: bool, alpha, A, B, beta=0.0, C0=None, rows=None, m=None, n=None, k=None,
+             lda=None, ldb=None, ldc=None, with_abs=False):
+    """Long-double reference of alpha*op(A)*op(B) + beta*C0 on the selected rows.
+
+    A, B, C0 are column-major arrays; returns an (nrows, n) float64 array (and
+    the |alpha| sum |a||b| array when with_abs)."""
+    dtype = 0 if A.dtype == np.float32 else 1
+    if m is None:
+        m = A.shape[1] if ta else A.shape[0]
+    if k is None:
+        k = A.shape[0] if ta else A.shape[1]
+    if n is None:
+        n = B.shape[0] if tb else B.shape[1]
+    lda = lda or A.shape[0]
+    ldb = ldb or B.shape[0]
+    if C0 is None:
+        C0 = np.zeros((1, 1), dtype=A.dtype, order="F")
+        beta = 0.0
+    ldc = ldc or C0.shape[0]
+    if rows is None:
+        rows_arr = None
+        nrows = m
+    else:
+        rows_arr = np.ascontiguousarray(rows, dtype=np.int64)
+        nrows = rows_arr.size
+    out = np.empty((nrows, n), dtype=np.float64)
+    absout = np.empty((nrows, n), dtype=np.float64) if with_abs else None
+    lib().orc_gemm_ref(dtype, int(ta), int(tb), m, n, k, float(alpha), _ptr(A), lda, _ptr(B), ldb, float(beta),
+                       _ptr(C0), ldc, None if rows_arr is None else _ptr(rows_arr), nrows, _ptr(out),
+                       None if absout is None else _ptr(absout))
+    return (out, absout) if with_abs else out
+
+
+def bf16_rtne(x: np.ndarray) -> np.ndarray:
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    t0 = np.empty(x.shape, np.uint16)
+    lib().orc_bf16_rtne_n(_ptr(x), x.size, _ptr(t0))
+    return t0
+
+
+def bf16_split(x: np.ndarray, terms: int):
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    ts = [np.zeros(x.shape, np.uint16) for _ in range(3)]
+    lib().orc_bf16_split(_ptr(x), x.size, terms, _ptr(ts[0]), _ptr(ts[1]), _ptr(ts[2]))
+    return ts[:terms]
+
+
+def bf16_to_f32(h: np.ndarray) -> np.ndarray:
+    return (np.asarray(h, dtype=np.uint32) << 16).view(np.float32)
+
+
+def gemm_split_ref(terms: int, ta: bool, tb: bool, alpha, A, B, beta=0.0, C0=None, rows=None):
+    m = A.shape[1] if ta else A.shape[0]
+    k = A.shape[0] if ta else A.shape[1]
+    n = B.shape[0] if tb else B.shape[1]
+    if C0 is None:
+        C0 = np.zeros((1, 1), np.float32, order="F")
+        beta = 0.0
+    rows_arr = None if rows is None else np.ascontiguousarray(rows, dtype=np.int64)
+    nrows = m if rows is None else rows_arr.size
+    out = np.empty((nrows, n), np.float64)
+    lib().orc_gemm_split_ref(terms, int(ta), int(tb), m, n, k, float(alpha), _ptr(A), A.shape[0], _ptr(B),
+                             B.shape[0], float(beta), _ptr(C0), C0.shape[0],
+                             None if rows_arr is None else _ptr(rows_arr), nrows, _ptr(out))
+    return out
+
+
+def eigen_gemm(ta: bool, tb: bool, alpha, A, B, C_, threads: int | None = None):
+    """Eigen's own C.noalias() += alpha*op(A)*op(B) on host Maps (in place on C_)."""
+    L = ref()
+    if threads is not None:
+        L.ref_set_threads(threads)
+    m, n = C_.shape
+    k = A.shape[0] if ta else A.shape[1]
+    f = L.ref_gemm_f32 if C_.dtype == np.float32 else L.ref_gemm_f64
+    f(int(ta), int(tb), m, n, k, alpha, _ptr(A), A.shape[0], _ptr(B), B.shape[0], _ptr(C_), C_.shape[0])
+
+
+def normwise_error(Cgpu: np.ndarray, Cref: np.ndarray, k: int, u: float, normA: float, normB: float) -> float:
+    """||C - C_ref||_F / (K * u * ||A||_F * ||B||_F)."""
+    diff = np.linalg.norm(np.asarray(Cgpu, np.float64) - np.asarray(Cref, np.float64))
+    den = k * u * normA * normB
+    return float(diff / den) if den > 0 else float(diff)
+
+
+def higham_mary_bound(absprod: np.ndarray, k: int, u: float, lam: float = 5.0, nprod: int = 2) -> float:
+    """lambda * sqrt(k) * u * nprod * || |A||B| ||_F  (test/product_test_helpers.h:56-74 form)."""
+    return float(lam * np.sqrt(k) * u * nprod * np.linalg.norm(absprod))

@@ -1,0 +1,132 @@
+"""C-ABI checks that need no GPU: the libraries load, export every declared
+symbol, and reject invalid arguments before any device work is issued."""
+import ctypes as C
+import os
+import re
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import eigen_amd as E
+import oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared(header: str):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b((?:eh_|sgemm_|dgemm_|xerbla_)\w*)\s*\(", text)))
+
+
+def _exported(path: str):
+    out = subprocess.run(["nm", "-D", "--defined-only", path], capture_output=True, text=True, check=True).stdout
+    return {line.split()[-1] for line in out.splitlines() if len(line.split()) >= 3}
+
+
+def test_libraries_load():
+    assert E.lib().eh_version().decode().startswith("eigen_amd")
+    E.blas()
+
+
+def test_every_declared_symbol_is_exported():
+    main = _exported(E.lib_path())
+    blas = _exported(E.blas_path())
+    for name in _declared("eigen_amd.h"):
+        assert name in main, name
+    for name in _declared("eigen_amd_blas.h"):
+        assert name in blas, name
+    assert len(_declared("eigen_amd.h")) >= 15
+
+
+def test_config_table():
+    assert E.num_configs() == 24
+    seen = set()
+    for c in range(24):
+        tm, tn, sk = E.config_info(c)
+        assert tm in (64, 128, 256) and tn in (64, 128, 256) and sk in (1, 2, 4, 8)
+        seen.add((tm, tn, sk))
+    assert len(seen) == 24
+    with pytest.raises(E.EigenAmdError):
+        E.config_info(24)
+
+
+def test_heuristic_plan_is_valid():
+    E.set_tune_mode(E.TUNE_HEURISTIC)
+    for (m, n, k) in [(8192, 8192, 8192), (16384, 256, 16384), (64, 64, 64), (4096, 4096, 4096), (1, 1, 1)]:
+        c = E.plan(E.F64, E.OP_N, E.OP_N, m, n, k)
+        assert 0 <= c < 24
+    # a product whose tiles cannot fill the chip gets split K
+    assert E.config_info(E.plan(E.F64, E.OP_N, E.OP_N, 2048, 256, 32768))[2] > 1
+    assert E.config_info(E.plan(E.F64, E.OP_N, E.OP_N, 8192, 8192, 8192))[2] == 1
+
+
+def _call_gemm(dtype=E.F32, opa=E.OP_N, opb=E.OP_N, m=4, n=4, k=4, lda=4, ldb=4, ldc=4, alpha=True, beta=True,
+               A=1, B=1, Cp=1):
+    L = E.lib()
+    a = C.c_float(1) if dtype == E.F32 else C.c_double(1)
+    return L.eh_gemm(dtype, opa, opb, m, n, k, C.byref(a) if alpha else None, A, lda, B, ldb,
+                     C.byref(a) if beta else None, Cp, ldc, 0, None)
+
+
+@pytest.mark.parametrize("kw", [
+    dict(dtype=7), dict(opa=5), dict(opb=-1), dict(m=-1), dict(n=-2), dict(k=-3),
+    dict(lda=3), dict(opa=E.OP_T, m=8, lda=4, k=5), dict(ldb=3), dict(opb=E.OP_T, n=9, ldb=4), dict(ldc=3),
+    dict(alpha=False), dict(beta=False), dict(Cp=None), dict(A=None), dict(B=None),
+])
+def test_invalid_arguments_rejected(kw):
+    assert _call_gemm(**kw) == 1  # EH_ERR_INVALID, before any device work
+    assert E.lib().eh_last_error().decode() != ""
+
+
+def test_quick_returns_do_not_touch_device():
+    # m == 0 or n == 0: nothing happens, even with NULL buffers
+    assert _call_gemm(m=0, A=None, B=None, Cp=None, ldc=1, lda=1) == 0
+    assert _call_gemm(n=0, A=None, B=None, Cp=None) == 0
+
+
+def test_fp64_rejects_split_precision():
+    L = E.lib()
+    one = C.c_double(1)
+    st = L.eh_gemm(E.F64, 0, 0, 4, 4, 4, C.byref(one), 1, 4, 1, 4, C.byref(one), 1, 4, E.PREC_SPLIT3, None)
+    assert st == 3  # EH_ERR_UNSUPPORTED
+
+
+_XERBLA_SCRIPT = r"""
+import ctypes as C, sys
+sys.path.insert(0, {root!r})
+import eigen_amd as E
+L = E.blas()
+i = lambda v: C.byref(C.c_int(v))
+one = C.c_float(1.0)
+buf = (C.c_float * 4)()
+args = {args!r}
+ta, tb, m, n, k, lda, ldb, ldc = args
+L.sgemm_(ta.encode(), tb.encode(), i(m), i(n), i(k), C.byref(one), buf, i(lda), buf, i(ldb), C.byref(one), buf, i(ldc))
+sys.stdout.flush()
+"""
+
+INVALID = [a for a in [
+    ("X", "N", 2, 2, 2, 2, 2, 2), ("N", "Y", 2, 2, 2, 2, 2, 2), ("N", "N", -1, 2, 2, 2, 2, 2),
+    ("N", "N", 2, -1, 2, 2, 2, 2), ("N", "N", 2, 2, -1, 2, 2, 2), ("N", "N", 4, 2, 2, 3, 2, 4),
+    ("T", "N", 4, 2, 3, 2, 3, 4), ("N", "N", 2, 2, 5, 2, 4, 2), ("N", "T", 2, 6, 2, 2, 5, 2),
+    ("N", "N", 5, 2, 2, 5, 2, 4)]]
+
+
+@pytest.mark.parametrize("args", INVALID)
+def test_sgemm_error_exits_match_reference(args):
+    r = subprocess.run([sys.executable, "-c", _XERBLA_SCRIPT.format(root=ROOT, args=args)],
+                       capture_output=True, text=True, timeout=120)
+    want = O.gemm_check(*args)
+    assert want != 0
+    assert f"ERROR #{want}: SGEMM" in r.stdout, (r.stdout, r.stderr)
+
+
+def test_sgemm_zero_size_quick_return():
+    # m == 0: returns before touching any pointer (NULLs are fine)
+    L = E.blas()
+    i = lambda v: C.byref(C.c_int(v))
+    one = C.c_float(1.0)
+    L.sgemm_(b"N", b"N", i(0), i(3), i(3), C.byref(one), None, i(1), None, i(3), C.byref(one), None, i(1))

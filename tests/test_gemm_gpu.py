@@ -1,0 +1,239 @@
+"""Parity of the HIP products with the oracle (runs on an MI355X).
+
+Exact cases use small-integer inputs, for which every partial sum is exact
+in FP32/FP64, so the GPU result must equal the long-double oracle bit for
+bit whatever the summation order: this pins the MFMA operand/output
+layouts, the LDS staging, transposes, strides, edges and split-K.  Random
+cases are held to the normwise bound of test/product_test_helpers.h:56-74
+(lambda = 5, two products compared).
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import eigen_amd as E
+import oracle as O
+from helpers import empty_dev, from_dev, op_shape, to_dev, unit_roundoff
+
+pytestmark = pytest.mark.gpu
+
+DT = {np.float32: E.F32, np.float64: E.F64}
+
+
+def run_gemm(dtype, ta, tb, m, n, k, alpha, A, B, beta, C0, prec=E.PREC_AUTO, config=None, lda=None, ldb=None,
+             ldc=None):
+    dA, dB = to_dev(A), to_dev(B)
+    dC = to_dev(C0) if C0 is not None else empty_dev(m * n, dtype)
+    E.gemm(DT[dtype], int(ta), int(tb), m, n, k, alpha, dA, lda or A.shape[0], dB, ldb or B.shape[0], beta, dC,
+           ldc or m, prec=prec, config=config)
+    return from_dev(dC, (ldc or m, n))[:m]
+
+
+def int_operands(dtype, ta, tb, m, n, k, seed=1):
+    A = O.fill(op_shape(ta, m, k), dtype, seed, O.INT, -8, 8)
+    B = O.fill(op_shape(tb, k, n), dtype, seed + 1, O.INT, -8, 8)
+    C0 = O.fill((m, n), dtype, seed + 2, O.INT, -8, 8)
+    return A, B, C0
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("ta,tb", [(False, False), (True, False), (False, True), (True, True)])
+@pytest.mark.parametrize("shape", [(1, 1, 1), (7, 5, 3), (33, 31, 47), (64, 64, 64), (65, 129, 100),
+                                   (130, 70, 300), (257, 255, 33)])
+def test_exact_integer_products(gpu, dtype, ta, tb, shape):
+    m, n, k = shape
+    A, B, C0 = int_operands(dtype, ta, tb, m, n, k)
+    want = O.gemm_ref(ta, tb, 2.0, A, B, -1.0, C0)
+    got = run_gemm(dtype, ta, tb, m, n, k, 2.0, A, B, -1.0, C0)
+    assert np.array_equal(got.astype(np.float64), want)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("config", range(24))
+def test_every_config_exact(gpu, dtype, config):
+    m, n, k = 300, 260, 520
+    for ta, tb in [(False, False), (True, True)]:
+        A, B, C0 = int_operands(dtype, ta, tb, m, n, k, seed=config + 10)
+        want = O.gemm_ref(ta, tb, 1.0, A, B, 1.0, C0)
+        got = run_gemm(dtype, ta, tb, m, n, k, 1.0, A, B, 1.0, C0, prec=E.PREC_PLAIN, config=config)
+        assert np.array_equal(got.astype(np.float64), want), (config, ta, tb)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_padded_strides_and_unaligned_pointers(gpu, dtype):
+    import torch
+    m, n, k = 77, 45, 61
+    lda, ldb, ldc = 83, 70, 91
+    A = O.fill((lda, k), dtype, 3, O.INT, -8, 8)
+    B = O.fill((ldb, n), dtype, 4, O.INT, -8, 8)
+    C0 = O.fill((ldc, n), dtype, 5, O.INT, -8, 8)
+    want = O.gemm_ref(False, False, 1.0, A[:m], B[:k], 1.0, np.asfortranarray(C0[:m]))
+    got = run_gemm(dtype, False, False, m, n, k, 1.0, A, B, 1.0, C0, lda=lda, ldb=ldb, ldc=ldc)
+    assert np.array_equal(got.astype(np.float64), want)
+    # rows of ldc beyond m are untouched
+    full = run_gemm(dtype, False, False, m, n, k, 1.0, A, B, 1.0, C0, lda=lda, ldb=ldb, ldc=ldc)
+    del full
+    # pointers offset by one element (not 16-byte aligned), transposed operands
+    At = O.fill((k, m), dtype, 6, O.INT, -8, 8)
+    Bt = O.fill((n, k), dtype, 7, O.INT, -8, 8)
+    esz = np.dtype(dtype).itemsize
+    dA = torch.cat([torch.zeros(1, dtype=to_dev(At).dtype, device="cuda:0"), to_dev(At)])
+    dB = torch.cat([torch.zeros(1, dtype=dA.dtype, device="cuda:0"), to_dev(Bt)])
+    dC = to_dev(np.zeros((m, n), dtype, order="F"))
+    E.gemm(DT[dtype], E.OP_T, E.OP_T, m, n, k, 1.0, dA.data_ptr() + esz, k, dB.data_ptr() + esz, n, 0.0, dC, m)
+    assert np.array_equal(from_dev(dC, (m, n)).astype(np.float64), O.gemm_ref(True, True, 1.0, At, Bt))
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_blas_semantics_alpha_beta(gpu, dtype):
+    m, n, k = 40, 30, 20
+    C0 = O.fill((m, n), dtype, 1)
+    nanA = np.full((m, k), np.nan, dtype, order="F")
+    infB = np.full((k, n), np.inf, dtype, order="F")
+    # alpha == 0: A and B are not read; C = beta*C
+    got = run_gemm(dtype, False, False, m, n, k, 0.0, nanA, infB, 0.5, C0)
+    assert np.array_equal(got, (dtype(0.5) * C0))
+    got = run_gemm(dtype, False, False, m, n, k, -0.0, nanA, infB, 1.0, C0)
+    assert np.array_equal(got, C0)
+    # beta == 0: NaN in C is overwritten
+    A = O.fill((m, k), dtype, 2, O.INT, -8, 8)
+    B = O.fill((k, n), dtype, 3, O.INT, -8, 8)
+    nanC = np.full((m, n), np.nan, dtype, order="F")
+    got = run_gemm(dtype, False, False, m, n, k, 1.0, A, B, 0.0, nanC)
+    assert np.array_equal(got.astype(np.float64), O.gemm_ref(False, False, 1.0, A, B))
+    # k == 0: C = beta*C
+    got = run_gemm(dtype, False, False, m, n, 0, 1.0, A, B, 2.0, C0, lda=m, ldb=1)
+    assert np.array_equal(got, dtype(2.0) * C0)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("ta,tb", [(False, False), (True, False), (False, True), (True, True)])
+def test_random_within_bound(gpu, dtype, ta, tb):
+    m = n = k = 512  # the 512^3 parity configuration
+    A = O.fill(op_shape(ta, m, k), dtype, 21)
+    B = O.fill(op_shape(tb, k, n), dtype, 22)
+    ref, absref = O.gemm_ref(ta, tb, 1.0, A, B, with_abs=True)
+    got = run_gemm(dtype, ta, tb, m, n, k, 1.0, A, B, 0.0, None)
+    u = unit_roundoff(dtype)
+    err = np.linalg.norm(got.astype(np.float64) - ref)
+    assert err <= O.higham_mary_bound(absref, k, u)
+    nw = O.normwise_error(got, ref, k, u, np.linalg.norm(A), np.linalg.norm(B))
+    assert nw <= 1.0
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="Eigen reference build absent")
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_matches_eigen_cpu_gebp(gpu, dtype):
+    # C = A*B, 512^3, U[-1,1] from a seed, C zero-initialised, Eigen's C.noalias() = A*B on the CPU
+    m = n = k = 512
+    A = O.fill((m, k), dtype, 1)
+    B = O.fill((k, n), dtype, 2)
+    eig = np.zeros((m, n), dtype, order="F")
+    O.eigen_gemm(False, False, 1.0, A, B, eig)
+    got = run_gemm(dtype, False, False, m, n, k, 1.0, A, B, 0.0, None)
+    ref, absref = O.gemm_ref(False, False, 1.0, A, B, with_abs=True)
+    u = unit_roundoff(dtype)
+    bound = O.higham_mary_bound(absref, k, u)
+    assert np.linalg.norm(got.astype(np.float64) - eig) <= bound
+    assert np.linalg.norm(eig.astype(np.float64) - ref) <= bound
+
+
+def test_device_fill_matches_oracle(gpu):
+    n = 100003
+    for dtype, dist, p0, p1 in [(np.float64, O.UNIFORM, -1, 1), (np.float32, O.UNIFORM, -1, 1),
+                                (np.float64, O.INT, -8, 8), (np.float32, O.INT, -8, 8)]:
+        t = empty_dev(n, dtype)
+        E.fill(DT[dtype], t, n, 1234, dist, p0, p1)
+        got = from_dev(t, (n, 1))[:, 0]
+        want = O.fill((n, 1), dtype, 1234, dist, p0, p1)[:, 0]
+        assert np.array_equal(got, want)
+    # normal / wide use libm-style transcendental functions: equal to a few ulp
+    for dtype, dist, p0, p1 in [(np.float64, O.NORMAL, 0, 1), (np.float32, O.WIDE, -8, 8)]:
+        t = empty_dev(n, dtype)
+        E.fill(DT[dtype], t, n, 99, dist, p0, p1)
+        got = from_dev(t, (n, 1))[:, 0].astype(np.float64)
+        want = O.fill((n, 1), dtype, 99, dist, p0, p1)[:, 0].astype(np.float64)
+        assert np.allclose(got, want, rtol=8 * np.finfo(dtype).eps, atol=1e-12)
+
+
+def test_device_split_matches_oracle(gpu):
+    import torch
+    x = O.fill((300000, 1), np.float32, 5, O.WIDE, -40, 40)[:, 0]
+    special = np.float32([0.0, -0.0, 1.0, 3.4e38, -3.39e38, 3.3895e38, np.inf, -np.inf, 1e-40, -1e-45,
+                          0.33333, 65504.0])
+    x = np.concatenate([x, special])
+    n = x.size
+    dx = torch.from_numpy(x).cuda()
+    ts = [torch.empty(n, dtype=torch.int16, device="cuda:0") for _ in range(3)]
+    for terms in (1, 2, 3):
+        E.split_bf16(dx, n, terms, *ts[:terms])
+        torch.cuda.synchronize()
+        want = O.bf16_split(x, terms)
+        for t, w in zip(ts[:terms], want):
+            assert np.array_equal(t.cpu().numpy().view(np.uint16), w)
+    # NaN stays NaN in the leading term
+    dn = torch.tensor([float("nan")], device="cuda:0")
+    E.split_bf16(dn, 1, 3, *ts)
+    torch.cuda.synchronize()
+    assert (int(ts[0][0].item()) & 0x7FFF) > 0x7F80
+
+
+@pytest.mark.parametrize("ta,tb", [(False, False), (True, False), (False, True), (True, True)])
+@pytest.mark.parametrize("shape", [(1, 1, 1), (33, 65, 31), (200, 130, 257), (300, 280, 1100)])
+def test_split_product_vs_split_oracle(gpu, ta, tb, shape):
+    m, n, k = shape
+    A = O.fill(op_shape(ta, m, k), np.float32, 31, O.WIDE, -8, 8)
+    B = O.fill(op_shape(tb, k, n), np.float32, 32, O.WIDE, -8, 8)
+    C0 = O.fill((m, n), np.float32, 33)
+    for terms, prec in ((3, E.PREC_SPLIT3), (2, E.PREC_SPLIT2)):
+        want = O.gemm_split_ref(terms, ta, tb, 1.25, A, B, 0.5, C0)
+        _, absref = O.gemm_ref(ta, tb, 1.25, A, B, with_abs=True)
+        got = run_gemm(np.float32, ta, tb, m, n, k, 1.25, A, B, 0.5, C0, prec=prec)
+        # only FP32 accumulation separates the device result from the exact split sum
+        err = np.linalg.norm(got.astype(np.float64) - want)
+        assert err <= O.higham_mary_bound(absref, k, unit_roundoff(np.float32)) + 1e-30
+
+
+def test_split3_error_no_worse_than_plain(gpu):
+    # wide dynamic range entries sign*U(1,2)*2^U(-8,8), errors against the long-double oracle
+    m = n = k = 1024
+    A = O.fill((m, k), np.float32, 41, O.WIDE, -8, 8)
+    B = O.fill((k, n), np.float32, 42, O.WIDE, -8, 8)
+    rows = np.arange(0, m, 8)
+    ref = O.gemm_ref(False, False, 1.0, A, B, rows=rows)
+    u = unit_roundoff(np.float32)
+    nA, nB = np.linalg.norm(A), np.linalg.norm(B)
+    errs = {}
+    for name, prec in (("plain", E.PREC_PLAIN), ("split3", E.PREC_SPLIT3), ("split2", E.PREC_SPLIT2)):
+        got = run_gemm(np.float32, False, False, m, n, k, 1.0, A, B, 0.0, None, prec=prec)
+        errs[name] = O.normwise_error(got[rows], ref, k, u, nA, nB)
+    assert errs["split3"] <= errs["plain"] * 1.05, errs
+    assert errs["split2"] > errs["split3"], errs
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_blas_entry_device_and_host_pointers(gpu, dtype):
+    L = E.blas()
+    m, n, k = 70, 50, 90
+    A = O.fill((k, m), dtype, 51, O.INT, -8, 8)  # used as op(A) = A^T
+    B = O.fill((k, n), dtype, 52, O.INT, -8, 8)
+    C0 = O.fill((m, n), dtype, 53, O.INT, -8, 8)
+    want = O.gemm_ref(True, False, 1.5, A, B, 0.5, C0)
+    ct = C.c_float if dtype == np.float32 else C.c_double
+    fn = L.sgemm_ if dtype == np.float32 else L.dgemm_
+    i = lambda v: C.byref(C.c_int(v))
+    al, be = ct(1.5), ct(0.5)
+    # host pointers (staged through device buffers)
+    Ch = C0.copy(order="F")
+    fn(b"T", b"N", i(m), i(n), i(k), C.byref(al), A.ctypes.data_as(C.c_void_p), i(k),
+       B.ctypes.data_as(C.c_void_p), i(k), C.byref(be), Ch.ctypes.data_as(C.c_void_p), i(m))
+    assert np.array_equal(Ch.astype(np.float64), want)
+    # device pointers (runs in place)
+    import torch
+    torch.cuda.synchronize()
+    dA, dB, dC = to_dev(A), to_dev(B), to_dev(C0)
+    torch.cuda.synchronize()
+    fn(b"T", b"N", i(m), i(n), i(k), C.byref(al), C.c_void_p(dA.data_ptr()), i(k), C.c_void_p(dB.data_ptr()),
+       i(k), C.byref(be), C.c_void_p(dC.data_ptr()), i(m))
+    assert np.array_equal(from_dev(dC, (m, n)).astype(np.float64), want)
