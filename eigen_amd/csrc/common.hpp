@@ -33,7 +33,9 @@ struct Mfma<float> {
   __device__ static inline int col(int l) { return l & 31; }
 };
 
-// v_mfma_f64_16x16x4_f64: A lane l -> (row l%16, k l/16); 4 accumulators.
+// v_mfma_f64_16x16x4_f64: A lane l -> (row l%16, k l/16); 4 accumulators,
+// accumulator r of lane l holds row l/16 + 4r (rows interleaved across lanes,
+// unlike the FP32 16x16 layout).
 template <>
 struct Mfma<double> {
   static constexpr int MI = 16, NI = 16;
@@ -43,7 +45,7 @@ struct Mfma<double> {
   __device__ static inline void mma(double a, double b, acc_t& c) {
     c = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
-  __device__ static inline int row(int r, int l) { return (l >> 4) * 4 + r; }
+  __device__ static inline int row(int r, int l) { return (l >> 4) + 4 * r; }
   __device__ static inline int col(int l) { return l & 15; }
 };
 

@@ -68,12 +68,15 @@ def test_padded_strides_and_unaligned_pointers(gpu, dtype):
     A = O.fill((lda, k), dtype, 3, O.INT, -8, 8)
     B = O.fill((ldb, n), dtype, 4, O.INT, -8, 8)
     C0 = O.fill((ldc, n), dtype, 5, O.INT, -8, 8)
-    want = O.gemm_ref(False, False, 1.0, A[:m], B[:k], 1.0, np.asfortranarray(C0[:m]))
+    want = O.gemm_ref(False, False, 1.0, np.asfortranarray(A[:m]), np.asfortranarray(B[:k]), 1.0,
+                      np.asfortranarray(C0[:m]))
     got = run_gemm(dtype, False, False, m, n, k, 1.0, A, B, 1.0, C0, lda=lda, ldb=ldb, ldc=ldc)
     assert np.array_equal(got.astype(np.float64), want)
-    # rows of ldc beyond m are untouched
-    full = run_gemm(dtype, False, False, m, n, k, 1.0, A, B, 1.0, C0, lda=lda, ldb=ldb, ldc=ldc)
-    del full
+    # rows of C beyond m (inside ldc) are untouched
+    dA, dB, dC = to_dev(A), to_dev(B), to_dev(C0)
+    E.gemm(DT[dtype], E.OP_N, E.OP_N, m, n, k, 1.0, dA, lda, dB, ldb, 1.0, dC, ldc)
+    full = from_dev(dC, (ldc, n))
+    assert np.array_equal(full[m:], C0[m:])
     # pointers offset by one element (not 16-byte aligned), transposed operands
     At = O.fill((k, m), dtype, 6, O.INT, -8, 8)
     Bt = O.fill((n, k), dtype, 7, O.INT, -8, 8)
