@@ -18,7 +18,7 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIBDIR = os.path.join(HERE, "_lib")
+LIBDIR = os.environ.get("EIGEN_AMD_LIBDIR") or os.path.join(HERE, "_lib")
 
 F32, F64 = 0, 1
 OP_N, OP_T, OP_C = 0, 1, 2

@@ -2,7 +2,7 @@
 
     python -m eigen_amd.build            # compile everything (always rebuilds)
 
-Produces, next to this file:
+Produces, next to this file (or in $EIGEN_AMD_LIBDIR when set):
     _lib/libeigen_amd.so        the C-ABI (include/eigen_amd.h) and all kernels
     _lib/libeigen_amd_blas.so   sgemm_/dgemm_/xerbla_ (include/eigen_amd_blas.h)
 """
@@ -16,8 +16,8 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
-OBJ = os.path.join(HERE, "_build")
-LIB = os.path.join(HERE, "_lib")
+OBJ = os.environ.get("EIGEN_AMD_OBJDIR") or os.path.join(HERE, "_build")
+LIB = os.environ.get("EIGEN_AMD_LIBDIR") or os.path.join(HERE, "_lib")
 
 ARCH = "gfx950"
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
@@ -25,14 +25,19 @@ HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 HIPFLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
-LIB_SOURCES = ["gemm_mfma.hip", "gemm_split.hip", "fill.hip", "grouped.hip", "mlp.hip",
+LIB_SOURCES = ["gemm_f32.hip", "gemm_f64.hip", "gemm_split.hip", "fill.hip", "grouped.hip", "mlp.hip",
                "capi.cpp", "planner.cpp", "tuner.cpp"]
 BLAS_SOURCES = ["blas.cpp"]
+# Per-file code generation choices.  The FP64 kernels keep their
+# accumulators in the VGPR form of the MFMA: with AGPR accumulators the
+# 128x128 tile copied every accumulator through VGPRs on each k slice.
+FILE_FLAGS = {"gemm_f64.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
 
 
 def _compile(src: str) -> str:
     out = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
-    cmd = [HIPCC] + HIPFLAGS + ["-c", os.path.join(CSRC, src), "-o", out]
+    extra = FILE_FLAGS.get(src, []) + os.environ.get("EIGEN_AMD_HIPFLAGS", "").split()
+    cmd = [HIPCC] + HIPFLAGS + extra + ["-c", os.path.join(CSRC, src), "-o", out]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")

@@ -56,12 +56,14 @@ static int split_index(int s) {
 }
 
 // The closed-form rule (the method without the learned selector): the
-// largest tile that still gives at least one wave of workgroups over the
-// 256 CUs, then split K while the grid stays under one wave and each split
-// keeps at least 512 of depth.
+// largest preferred tile that still gives at least one wave of workgroups
+// over the 256 CUs, then split K while the grid stays under one wave and
+// each split keeps at least 512 of depth.
 int heuristic_config(int dtype, int64_t m, int64_t n, int64_t k) {
   (void)dtype;
-  static const int order[] = {5, 3, 2, 1, 0};  // preferred tiles, largest first
+  // 128x128 first: on gfx950 the 256-wide tiles run at occupancy 1 and
+  // measured slower than 128x128 for FP32 and FP64 alike.
+  static const int order[] = {3, 2, 1, 0};
   int tile = 0;
   for (int t : order) {
     int64_t tiles = ((m + kTileM[t] - 1) / kTileM[t]) * ((n + kTileN[t] - 1) / kTileN[t]);
@@ -150,18 +152,57 @@ eh_status check_gemm_args(int dtype, int opa, int opb, int64_t m, int64_t n, int
 }
 
 // ---- dispatch ---------------------------------------------------------------
+// Byte extents of the operands as the kernels address them, with the margin
+// the k tail and edge tiles may touch.  Kernels index operands with 32-bit
+// offsets from the base pointer, so a product whose operands exceed that is
+// cut into sub-products first.
+constexpr uint64_t kMaxExtent = 0xFFFFFFFFull;
+
 template <typename T>
-static eh_status run_gemm(int opa, int opb, int64_t m, int64_t n, int64_t k, T alpha, const T* A, int64_t lda,
-                          const T* B, int64_t ldb, T beta, T* C, int64_t ldc, eh_precision prec, int config,
-                          hipStream_t stream) {
-  if (m == 0 || n == 0) return EH_SUCCESS;
-  hipError_t e;
-  if (k == 0 || alpha == T(0)) {  // product term vanishes: A and B are not read
-    if (beta == T(1)) return EH_SUCCESS;
-    e = launch_scale_c<T>(C, (int)m, (int)n, ldc, beta, stream);
-    return e == hipSuccess ? EH_SUCCESS : hip_fail(e, "scale_c launch");
+static bool operand_fits(bool kcont, int64_t rows, int64_t k, int64_t ld) {
+  const uint64_t sz = sizeof(T);
+  if (kcont) return ((uint64_t)(rows - 1) * ld + k) * sz < kMaxExtent;      // rows of k values
+  return ((uint64_t)(k + 64) * ld + rows) * sz < kMaxExtent;                // k columns of rows
+}
+
+template <typename T>
+static eh_status launch_product(bool ta, bool tb, int64_t m, int64_t n, int64_t k, T alpha, const T* A,
+                                int64_t lda, const T* B, int64_t ldb, T beta, T* C, int64_t ldc, bool split_prec,
+                                int terms, int config, hipStream_t stream) {
+  const bool fa = operand_fits<T>(ta, m, k, lda), fb = operand_fits<T>(!tb, n, k, ldb);
+  if (!fa || !fb) {
+    // Halve the dimension that drives the oversized extent and recurse.
+    const bool cut_k = (!fa && !ta) || (!fb && tb);
+    const bool cut_m = !fa && ta;
+    if (cut_k && k > 1) {
+      const int64_t h = k / 2;
+      eh_status st = launch_product<T>(ta, tb, m, n, h, alpha, A, lda, B, ldb, beta, C, ldc, split_prec, terms,
+                                       config, stream);
+      if (st != EH_SUCCESS) return st;
+      const T* A2 = ta ? A + h : A + h * lda;
+      const T* B2 = tb ? B + h * ldb : B + h;
+      return launch_product<T>(ta, tb, m, n, k - h, alpha, A2, lda, B2, ldb, T(1), C, ldc, split_prec, terms,
+                               config, stream);
+    }
+    if (cut_m && m > 1) {
+      const int64_t h = m / 2;
+      eh_status st = launch_product<T>(ta, tb, h, n, k, alpha, A, lda, B, ldb, beta, C, ldc, split_prec, terms,
+                                       config, stream);
+      if (st != EH_SUCCESS) return st;
+      return launch_product<T>(ta, tb, m - h, n, k, alpha, ta ? A + h * lda : A + h, lda, B, ldb, beta, C + h, ldc,
+                               split_prec, terms, config, stream);
+    }
+    if (!fb && !tb && n > 1) {
+      const int64_t h = n / 2;
+      eh_status st = launch_product<T>(ta, tb, m, h, k, alpha, A, lda, B, ldb, beta, C, ldc, split_prec, terms,
+                                       config, stream);
+      if (st != EH_SUCCESS) return st;
+      return launch_product<T>(ta, tb, m, n - h, k, alpha, A, lda, B + h * ldb, ldb, beta, C + h * ldc, ldc,
+                               split_prec, terms, config, stream);
+    }
+    return fail(EH_ERR_UNSUPPORTED, "operand leading dimension too large for 32-bit addressing");
   }
-  const bool ta = opa != EH_OP_N, tb = opb != EH_OP_N;
+
   GemmParams<T> p{};
   p.A = A;
   p.B = B;
@@ -174,24 +215,15 @@ static eh_status run_gemm(int opa, int opb, int64_t m, int64_t n, int64_t k, T a
   p.K = (int)k;
   p.alpha = alpha;
   p.beta = beta;
+  p.bytesA = (uint32_t)((ta ? (uint64_t)(m - 1) * lda + k : (uint64_t)(k - 1) * lda + m) * sizeof(T));
+  p.bytesB = (uint32_t)((!tb ? (uint64_t)(n - 1) * ldb + k : (uint64_t)(k - 1) * ldb + n) * sizeof(T));
   constexpr int EPC = 16 / sizeof(T);
   // 16-byte loads along the k-contiguous operand need aligned rows.
   p.vecA = ta && ((uintptr_t)A % 16 == 0) && (lda % EPC == 0);
   p.vecB = !tb && ((uintptr_t)B % 16 == 0) && (ldb % EPC == 0);
 
   int tile = 0, splits = 1, kslice = (int)(128 / sizeof(T));
-  bool split_prec = false;
-  int terms = 0;
-  if (std::is_same<T, float>::value) {
-    if (prec == EH_PREC_AUTO) prec = default_sgemm_precision();
-    if (prec == EH_PREC_SPLIT2 || prec == EH_PREC_SPLIT3) {
-      split_prec = true;
-      terms = prec == EH_PREC_SPLIT3 ? 3 : 2;
-    }
-  } else if (prec != EH_PREC_AUTO && prec != EH_PREC_PLAIN) {
-    return fail(EH_ERR_UNSUPPORTED, "FP64 products run on FP64 MFMA only (precision AUTO or PLAIN)");
-  }
-
+  hipError_t e;
   if (split_prec) {
     kslice = 32;
     // 256x128 tiles once the grid fills the chip, 128x128 below that.
@@ -205,10 +237,6 @@ static eh_status run_gemm(int opa, int opb, int64_t m, int64_t n, int64_t k, T a
       while (splits < 8 && tiles * splits < 256 && k / (2 * splits) >= 512) splits *= 2;
     }
   } else {
-    if (config < 0) {
-      eh_status st = plan_config(std::is_same<T, float>::value ? EH_F32 : EH_F64, opa, opb, m, n, k, &config);
-      if (st != EH_SUCCESS) return st;
-    }
     if (config < 0 || config >= kNumConfigs) return fail(EH_ERR_INVALID, "config out of range");
     config_decode(config, &tile, &splits);
   }
@@ -230,6 +258,7 @@ static eh_status run_gemm(int opa, int opb, int64_t m, int64_t n, int64_t k, T a
     else
       e = launch_gemm_mfma<T>(p, tile, ta, tb, splits, stream);
   } else {
+    (void)terms;
     e = launch_gemm_mfma<T>(p, tile, ta, tb, splits, stream);
   }
   if (e != hipSuccess) return hip_fail(e, "gemm launch");
@@ -238,6 +267,36 @@ static eh_status run_gemm(int opa, int opb, int64_t m, int64_t n, int64_t k, T a
     if (e != hipSuccess) return hip_fail(e, "split-K reduce launch");
   }
   return EH_SUCCESS;
+}
+
+template <typename T>
+static eh_status run_gemm(int opa, int opb, int64_t m, int64_t n, int64_t k, T alpha, const T* A, int64_t lda,
+                          const T* B, int64_t ldb, T beta, T* C, int64_t ldc, eh_precision prec, int config,
+                          hipStream_t stream) {
+  if (m == 0 || n == 0) return EH_SUCCESS;
+  hipError_t e;
+  if (k == 0 || alpha == T(0)) {  // product term vanishes: A and B are not read
+    if (beta == T(1)) return EH_SUCCESS;
+    e = launch_scale_c<T>(C, (int)m, (int)n, ldc, beta, stream);
+    return e == hipSuccess ? EH_SUCCESS : hip_fail(e, "scale_c launch");
+  }
+  const bool ta = opa != EH_OP_N, tb = opb != EH_OP_N;
+  bool split_prec = false;
+  int terms = 0;
+  if (std::is_same<T, float>::value) {
+    if (prec == EH_PREC_AUTO) prec = default_sgemm_precision();
+    if (prec == EH_PREC_SPLIT2 || prec == EH_PREC_SPLIT3) {
+      split_prec = true;
+      terms = prec == EH_PREC_SPLIT3 ? 3 : 2;
+    }
+  } else if (prec != EH_PREC_AUTO && prec != EH_PREC_PLAIN) {
+    return fail(EH_ERR_UNSUPPORTED, "FP64 products run on FP64 MFMA only (precision AUTO or PLAIN)");
+  }
+  if (!split_prec && config < 0) {
+    eh_status st = plan_config(std::is_same<T, float>::value ? EH_F32 : EH_F64, opa, opb, m, n, k, &config);
+    if (st != EH_SUCCESS) return st;
+  }
+  return launch_product<T>(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, split_prec, terms, config, stream);
 }
 
 eh_status gemm_dispatch(int dtype, int opa, int opb, int64_t m, int64_t n, int64_t k, const void* alpha,

@@ -62,6 +62,24 @@ __device__ __host__ inline int swz8(int row, int chunk) { return chunk ^ ((row ^
 // Same property for 4 chunks per row (64-byte rows).
 __device__ __host__ inline int swz4(int row, int chunk) { return chunk ^ (((row >> 1) ^ (row >> 3)) & 3); }
 
+// Raw buffer resource over [base, base + bytes): loads past the end return 0
+// in hardware, which is how the product kernels handle matrix edges without
+// per-element branches.  Word 3 is the gfx9 default (32-bit data format).
+__device__ inline __amdgpu_buffer_rsrc_t make_rsrc(const void* base, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, 0x00020000);
+}
+__device__ inline u32x4 buf_load16(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+  return __builtin_amdgcn_raw_buffer_load_b128(r, (int)voff, (int)soff, 0);
+}
+__device__ inline float buf_load(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff, float*) {
+  return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
+}
+__device__ inline double buf_load(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff, double*) {
+  typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+  u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0);
+  return __hiloint2double((int)v.y, (int)v.x);
+}
+
 // Map a linear workgroup id to a logical tile id so that each XCD (workgroups
 // are dealt round-robin over the 8 XCDs) owns one contiguous range of tiles,
 // which keeps the A row panels and B column panels it reuses in its own L2.

@@ -1,0 +1,62 @@
+// gemm_f64.hip -- FP64 instantiations of the plain MFMA product and its launchers.
+#include "gemm_mfma.hpp"
+
+namespace eh {
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+template <typename T, int BM, int BN, bool TA, bool TB>
+static hipError_t launch_one(const GemmParams<T>& p, int splits, hipStream_t s) {
+  const int ntm = (p.M + BM - 1) / BM, ntn = (p.N + BN - 1) / BN;
+  dim3 grid(ntm * ntn, splits);
+  hipLaunchKernelGGL((gemm_mfma_kernel<T, BM, BN, 2, 2, TA, TB>), grid, dim3(256), 0, s, p);
+  return hipGetLastError();
+}
+
+template <typename T, int BM, int BN>
+static hipError_t launch_tile(const GemmParams<T>& p, bool ta, bool tb, int splits, hipStream_t s) {
+  if (!ta && !tb) return launch_one<T, BM, BN, false, false>(p, splits, s);
+  if (ta && !tb) return launch_one<T, BM, BN, true, false>(p, splits, s);
+  if (!ta && tb) return launch_one<T, BM, BN, false, true>(p, splits, s);
+  return launch_one<T, BM, BN, true, true>(p, splits, s);
+}
+
+template <typename T>
+hipError_t launch_gemm_mfma(const GemmParams<T>& p, int tile, bool ta, bool tb, int splits, hipStream_t s) {
+  switch (tile) {
+    case 0: return launch_tile<T, 64, 64>(p, ta, tb, splits, s);
+    case 1: return launch_tile<T, 64, 128>(p, ta, tb, splits, s);
+    case 2: return launch_tile<T, 128, 64>(p, ta, tb, splits, s);
+    case 3: return launch_tile<T, 128, 128>(p, ta, tb, splits, s);
+    case 4: return launch_tile<T, 128, 256>(p, ta, tb, splits, s);
+    case 5: return launch_tile<T, 256, 128>(p, ta, tb, splits, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template <typename T>
+hipError_t launch_splitk_reduce(const GemmParams<T>& p, int splits, hipStream_t s) {
+  const int64_t total = (int64_t)p.M * p.N;
+  int64_t blocks = (total + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(splitk_reduce_kernel<T>, dim3((unsigned)blocks), dim3(256), 0, s, p, splits);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_scale_c(T* C, int M, int N, int64_t ldc, T beta, hipStream_t s) {
+  const int64_t total = (int64_t)M * N;
+  int64_t blocks = (total + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(scale_c_kernel<T>, dim3((unsigned)blocks), dim3(256), 0, s, C, M, N, ldc, beta);
+  return hipGetLastError();
+}
+
+template hipError_t launch_gemm_mfma<double>(const GemmParams<double>&, int, bool, bool, int, hipStream_t);
+template hipError_t launch_splitk_reduce<double>(const GemmParams<double>&, int, hipStream_t);
+template hipError_t launch_scale_c<double>(double*, int, int, int64_t, double, hipStream_t);
+
+}  // namespace eh

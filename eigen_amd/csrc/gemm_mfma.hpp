@@ -1,4 +1,4 @@
-// gemm_mfma.hip -- plain FP32 / FP64 MFMA products for gfx950.
+// gemm_mfma.hpp -- kernel templates of the plain FP32 / FP64 MFMA products.
 //
 // One workgroup computes a BM x BN tile of C = alpha*op(A)*op(B) + beta*C
 // over a K range (the whole K, or one split of it).  This is the device form
@@ -13,6 +13,7 @@
 //     epilogue (beta == 0 never reads C, matching blas/level3_impl.h:82-87).
 // Transposed operands only change which global index is contiguous; the LDS
 // layout is the same for all four op combinations.
+#pragma once
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -52,68 +53,84 @@ __global__ void __launch_bounds__(64 * WGM * WGN) gemm_mfma_kernel(GemmParams<T>
   const int nkt = kend > kbeg ? (kend - kbeg + BK - 1) / BK : 0;
 
   u32x4 ra[A_CH], rb[B_CH];
+  constexpr uint32_t SZ = sizeof(T);
+  const __amdgpu_buffer_rsrc_t rA = make_rsrc(p.A, p.bytesA);
+  const __amdgpu_buffer_rsrc_t rB = make_rsrc(p.B, p.bytesB);
+
+  // Loop-invariant per-lane byte offsets of every staged chunk; the k advance
+  // goes into the wave-uniform soffset.  Rows / columns past M / N read zeros
+  // or in-range values that only reach outputs that are never stored.
+  uint32_t voA[A_CH], voB[B_CH];
+#pragma unroll
+  for (int j = 0; j < A_CH; ++j) {
+    const int q = tid + NT * j;
+    if constexpr (TA) {  // op(A)(i,k) = A[k + i*lda]: k contiguous
+      const int r = q / CH, c = q % CH;
+      voA[j] = (uint32_t)(((uint64_t)(m0 + r) * p.lda + c * EPC) * SZ);
+    } else {  // op(A)(i,k) = A[i + k*lda]: rows contiguous
+      voA[j] = (uint32_t)((m0 + q % BM) * SZ);
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < B_CH; ++j) {
+    const int q = tid + NT * j;
+    if constexpr (!TB) {  // op(B)(k,j) = B[k + j*ldb]: k contiguous
+      const int r = q / CH, c = q % CH;
+      voB[j] = (uint32_t)(((uint64_t)(n0 + r) * p.ldb + c * EPC) * SZ);
+    } else {  // op(B)(k,j) = B[j + k*ldb]
+      voB[j] = (uint32_t)((n0 + q % BN) * SZ);
+    }
+  }
+
+  // k-contiguous chunk: one 16-byte load (or EPC element loads when rows are
+  // not 16-byte aligned); on the final partial slice, k >= kend is zeroed.
+  auto load_kcont = [&](__amdgpu_buffer_rsrc_t rs, uint32_t vo, int k0, int c, bool vec, bool tail) -> u32x4 {
+    u32x4 v;
+    if (vec) {
+      v = buf_load16(rs, vo, (uint32_t)k0 * SZ);
+    } else {
+      T t[EPC];
+#pragma unroll
+      for (int e = 0; e < EPC; ++e) t[e] = buf_load(rs, vo + e * SZ, (uint32_t)k0 * SZ, (T*)nullptr);
+      v = *reinterpret_cast<u32x4*>(t);
+    }
+    if (tail) {
+      T* t = reinterpret_cast<T*>(&v);
+#pragma unroll
+      for (int e = 0; e < EPC; ++e)
+        if (k0 + c * EPC + e >= kend) t[e] = T(0);
+    }
+    return v;
+  };
+  // rows-contiguous chunk: EPC loads, one per k, coalesced across the wave;
+  // k >= K lies past the end of the operand and reads zero.
+  auto load_mncont = [&](__amdgpu_buffer_rsrc_t rs, uint32_t vo, int k0, int cu, int64_t ld) -> u32x4 {
+    T t[EPC];
+#pragma unroll
+    for (int e = 0; e < EPC; ++e)
+      t[e] = buf_load(rs, vo, (uint32_t)((uint64_t)(k0 + cu * EPC + e) * ld * SZ), (T*)nullptr);
+    return *reinterpret_cast<u32x4*>(t);
+  };
 
   // ---- global -> registers for one BK slice starting at k0 ----------------
-  auto load_a = [&](int k0) {
+  auto load_a = [&](int k0, bool tail) {
 #pragma unroll
     for (int j = 0; j < A_CH; ++j) {
       const int q = tid + NT * j;
-      u32x4 v = {0u, 0u, 0u, 0u};
-      if constexpr (TA) {  // op(A)(i,k) = A[k + i*lda]: k contiguous
-        const int r = q / CH, c = q % CH;
-        const int gi = m0 + r, gk = k0 + c * EPC;
-        if (gi < p.M) {
-          const T* src = p.A + (int64_t)gi * p.lda + gk;
-          if (p.vecA && gk + EPC <= kend) {
-            v = *reinterpret_cast<const u32x4*>(src);
-          } else {
-            T t[EPC];
-#pragma unroll
-            for (int e = 0; e < EPC; ++e) t[e] = (gk + e < kend) ? src[e] : T(0);
-            v = *reinterpret_cast<u32x4*>(t);
-          }
-        }
-      } else {  // op(A)(i,k) = A[i + k*lda]: rows contiguous
-        const int r = q % BM, c = q / BM;
-        const int gi = m0 + r, gk = k0 + c * EPC;
-        T t[EPC];
-#pragma unroll
-        for (int e = 0; e < EPC; ++e)
-          t[e] = (gi < p.M && gk + e < kend) ? p.A[gi + (int64_t)(gk + e) * p.lda] : T(0);
-        v = *reinterpret_cast<u32x4*>(t);
-      }
-      ra[j] = v;
+      if constexpr (TA)
+        ra[j] = load_kcont(rA, voA[j], k0, q % CH, p.vecA, tail);
+      else
+        ra[j] = load_mncont(rA, voA[j], k0, __builtin_amdgcn_readfirstlane(q / BM), p.lda);
     }
   };
-  auto load_b = [&](int k0) {
+  auto load_b = [&](int k0, bool tail) {
 #pragma unroll
     for (int j = 0; j < B_CH; ++j) {
       const int q = tid + NT * j;
-      u32x4 v = {0u, 0u, 0u, 0u};
-      if constexpr (!TB) {  // op(B)(k,j) = B[k + j*ldb]: k contiguous
-        const int r = q / CH, c = q % CH;
-        const int gj = n0 + r, gk = k0 + c * EPC;
-        if (gj < p.N) {
-          const T* src = p.B + (int64_t)gj * p.ldb + gk;
-          if (p.vecB && gk + EPC <= kend) {
-            v = *reinterpret_cast<const u32x4*>(src);
-          } else {
-            T t[EPC];
-#pragma unroll
-            for (int e = 0; e < EPC; ++e) t[e] = (gk + e < kend) ? src[e] : T(0);
-            v = *reinterpret_cast<u32x4*>(t);
-          }
-        }
-      } else {  // op(B)(k,j) = B[j + k*ldb]: columns of op(B) contiguous
-        const int r = q % BN, c = q / BN;
-        const int gj = n0 + r, gk = k0 + c * EPC;
-        T t[EPC];
-#pragma unroll
-        for (int e = 0; e < EPC; ++e)
-          t[e] = (gj < p.N && gk + e < kend) ? p.B[gj + (int64_t)(gk + e) * p.ldb] : T(0);
-        v = *reinterpret_cast<u32x4*>(t);
-      }
-      rb[j] = v;
+      if constexpr (!TB)
+        rb[j] = load_kcont(rB, voB[j], k0, q % CH, p.vecB, tail);
+      else
+        rb[j] = load_mncont(rB, voB[j], k0, __builtin_amdgcn_readfirstlane(q / BN), p.ldb);
     }
   };
   // ---- registers -> LDS buffer -------------------------------------------
@@ -146,15 +163,16 @@ __global__ void __launch_bounds__(64 * WGM * WGN) gemm_mfma_kernel(GemmParams<T>
   const int lg = lane / F::MI;  // k group of this lane
 
   if (nkt > 0) {
-    load_a(kbeg);
-    load_b(kbeg);
+    load_a(kbeg, kbeg + BK > kend);
+    load_b(kbeg, kbeg + BK > kend);
     store_tiles(0);
     __syncthreads();
     for (int kt = 0; kt < nkt; ++kt) {
       const int buf = kt & 1;
       if (kt + 1 < nkt) {
-        load_a(kbeg + (kt + 1) * BK);
-        load_b(kbeg + (kt + 1) * BK);
+        const int k1 = kbeg + (kt + 1) * BK;
+        load_a(k1, k1 + BK > kend);
+        load_b(k1, k1 + BK > kend);
       }
       const unsigned char* As = smem + buf * (BM + BN) * ROWB;
       const unsigned char* Bs = As + BM * ROWB;
@@ -239,64 +257,5 @@ __global__ void __launch_bounds__(256) scale_c_kernel(T* C, int M, int N, int64_
     *dst = (beta == T(0)) ? T(0) : beta * *dst;
   }
 }
-
-// ---------------------------------------------------------------------------
-// launchers
-// ---------------------------------------------------------------------------
-template <typename T, int BM, int BN, bool TA, bool TB>
-static hipError_t launch_one(const GemmParams<T>& p, int splits, hipStream_t s) {
-  const int ntm = (p.M + BM - 1) / BM, ntn = (p.N + BN - 1) / BN;
-  dim3 grid(ntm * ntn, splits);
-  hipLaunchKernelGGL((gemm_mfma_kernel<T, BM, BN, 2, 2, TA, TB>), grid, dim3(256), 0, s, p);
-  return hipGetLastError();
-}
-
-template <typename T, int BM, int BN>
-static hipError_t launch_tile(const GemmParams<T>& p, bool ta, bool tb, int splits, hipStream_t s) {
-  if (!ta && !tb) return launch_one<T, BM, BN, false, false>(p, splits, s);
-  if (ta && !tb) return launch_one<T, BM, BN, true, false>(p, splits, s);
-  if (!ta && tb) return launch_one<T, BM, BN, false, true>(p, splits, s);
-  return launch_one<T, BM, BN, true, true>(p, splits, s);
-}
-
-template <typename T>
-hipError_t launch_gemm_mfma(const GemmParams<T>& p, int tile, bool ta, bool tb, int splits, hipStream_t s) {
-  switch (tile) {
-    case 0: return launch_tile<T, 64, 64>(p, ta, tb, splits, s);
-    case 1: return launch_tile<T, 64, 128>(p, ta, tb, splits, s);
-    case 2: return launch_tile<T, 128, 64>(p, ta, tb, splits, s);
-    case 3: return launch_tile<T, 128, 128>(p, ta, tb, splits, s);
-    case 4: return launch_tile<T, 128, 256>(p, ta, tb, splits, s);
-    case 5: return launch_tile<T, 256, 128>(p, ta, tb, splits, s);
-    default: return hipErrorInvalidValue;
-  }
-}
-
-template <typename T>
-hipError_t launch_splitk_reduce(const GemmParams<T>& p, int splits, hipStream_t s) {
-  const int64_t total = (int64_t)p.M * p.N;
-  int64_t blocks = (total + 255) / 256;
-  if (blocks > 65536) blocks = 65536;
-  if (blocks < 1) blocks = 1;
-  hipLaunchKernelGGL(splitk_reduce_kernel<T>, dim3((unsigned)blocks), dim3(256), 0, s, p, splits);
-  return hipGetLastError();
-}
-
-template <typename T>
-hipError_t launch_scale_c(T* C, int M, int N, int64_t ldc, T beta, hipStream_t s) {
-  const int64_t total = (int64_t)M * N;
-  int64_t blocks = (total + 255) / 256;
-  if (blocks > 65536) blocks = 65536;
-  if (blocks < 1) blocks = 1;
-  hipLaunchKernelGGL(scale_c_kernel<T>, dim3((unsigned)blocks), dim3(256), 0, s, C, M, N, ldc, beta);
-  return hipGetLastError();
-}
-
-template hipError_t launch_gemm_mfma<float>(const GemmParams<float>&, int, bool, bool, int, hipStream_t);
-template hipError_t launch_gemm_mfma<double>(const GemmParams<double>&, int, bool, bool, int, hipStream_t);
-template hipError_t launch_splitk_reduce<float>(const GemmParams<float>&, int, hipStream_t);
-template hipError_t launch_splitk_reduce<double>(const GemmParams<double>&, int, hipStream_t);
-template hipError_t launch_scale_c<float>(float*, int, int, int64_t, float, hipStream_t);
-template hipError_t launch_scale_c<double>(double*, int, int, int64_t, double, hipStream_t);
 
 }  // namespace eh
