@@ -165,6 +165,60 @@ static bool operand_fits(bool kcont, int64_t rows, int64_t k, int64_t ld) {
   return ((uint64_t)(k + 64) * ld + rows) * sz < kMaxExtent;                // k columns of rows
 }
 
+// Split-BF16 FP32 product: pack both operands into bf16 planes (one pass
+// over each), then run the packed product kernel (and the split-K reduction
+// when K is cut).
+static eh_status launch_split_product(const GemmParams<float>& g, bool ta, bool tb, int terms, int config,
+                                      hipStream_t stream) {
+  const int m = g.M, n = g.N, k = g.K;
+  const int KS = split_kslices(k);
+  int64_t t256 = ((int64_t)(m + 255) / 256) * ((n + 127) / 128);
+  int tile = t256 >= 256 ? 1 : 0;
+  int64_t tiles = tile ? t256 : ((int64_t)(m + 127) / 128) * ((n + 127) / 128);
+  int splits = 1;
+  if (config >= 0) {
+    int ct = 0;
+    config_decode(config, &ct, &splits);
+    tile = ct >= 5 ? 1 : 0;
+  } else {
+    while (splits < 8 && tiles * splits < 256 && k / (2 * splits) >= 512) splits *= 2;
+  }
+  const int per = (KS + splits - 1) / splits;
+  splits = (KS + per - 1) / per;
+  const size_t bytesA = split_packed_bytes(m, k, terms), bytesB = split_packed_bytes(n, k, terms);
+  const size_t bytesW = splits > 1 ? (size_t)splits * m * n * sizeof(float) : 0;
+  void* ws = nullptr;
+  eh_status st = workspace_get(stream, bytesA + bytesB + bytesW, &ws);
+  if (st != EH_SUCCESS) return st;
+  unsigned char* base = static_cast<unsigned char*>(ws);
+  hipError_t e = launch_pack_split(g.A, g.lda, ta, m, k, terms, base, stream);
+  if (e == hipSuccess) e = launch_pack_split(g.B, g.ldb, !tb, n, k, terms, base + bytesA, stream);
+  if (e != hipSuccess) return hip_fail(e, "pack launch");
+  SplitParams sp{};
+  sp.Ap = base;
+  sp.Bp = base + bytesA;
+  sp.C = g.C;
+  sp.W = splits > 1 ? reinterpret_cast<float*>(base + bytesA + bytesB) : nullptr;
+  sp.ldc = g.ldc;
+  sp.M = m;
+  sp.N = n;
+  sp.KS = KS;
+  sp.Mp = split_rows_padded(m);
+  sp.Np = split_rows_padded(n);
+  sp.ks_per_split = per;
+  sp.alpha = g.alpha;
+  sp.beta = g.beta;
+  e = launch_gemm_split_packed(sp, terms, tile, splits, stream);
+  if (e != hipSuccess) return hip_fail(e, "split product launch");
+  if (splits > 1) {
+    GemmParams<float> r = g;
+    r.W = sp.W;
+    e = launch_splitk_reduce<float>(r, splits, stream);
+    if (e != hipSuccess) return hip_fail(e, "split-K reduce launch");
+  }
+  return EH_SUCCESS;
+}
+
 template <typename T>
 static eh_status launch_product(bool ta, bool tb, int64_t m, int64_t n, int64_t k, T alpha, const T* A,
                                 int64_t lda, const T* B, int64_t ldb, T beta, T* C, int64_t ldc, bool split_prec,
@@ -224,19 +278,10 @@ static eh_status launch_product(bool ta, bool tb, int64_t m, int64_t n, int64_t 
 
   int tile = 0, splits = 1, kslice = (int)(128 / sizeof(T));
   hipError_t e;
-  if (split_prec) {
-    kslice = 32;
-    // 256x128 tiles once the grid fills the chip, 128x128 below that.
-    int64_t t256 = ((m + 255) / 256) * ((n + 127) / 128);
-    tile = t256 >= 256 ? 1 : 0;
-    int64_t tiles = tile ? t256 : ((m + 127) / 128) * ((n + 127) / 128);
-    if (config >= 0) {
-      config_decode(config, &tile, &splits);
-      tile = tile >= 5 ? 1 : 0;
-    } else {
-      while (splits < 8 && tiles * splits < 256 && k / (2 * splits) >= 512) splits *= 2;
-    }
-  } else {
+  if constexpr (std::is_same<T, float>::value) {
+    if (split_prec) return launch_split_product(p, ta, tb, terms, config, stream);
+  }
+  {
     if (config < 0 || config >= kNumConfigs) return fail(EH_ERR_INVALID, "config out of range");
     config_decode(config, &tile, &splits);
   }
@@ -252,15 +297,7 @@ static eh_status launch_product(bool ta, bool tb, int64_t m, int64_t n, int64_t 
     if (st != EH_SUCCESS) return st;
     p.W = static_cast<T*>(ws);
   }
-  if constexpr (std::is_same<T, float>::value) {
-    if (split_prec)
-      e = launch_gemm_split(p, terms, tile, ta, tb, splits, stream);
-    else
-      e = launch_gemm_mfma<T>(p, tile, ta, tb, splits, stream);
-  } else {
-    (void)terms;
-    e = launch_gemm_mfma<T>(p, tile, ta, tb, splits, stream);
-  }
+  e = launch_gemm_mfma<T>(p, tile, ta, tb, splits, stream);
   if (e != hipSuccess) return hip_fail(e, "gemm launch");
   if (splits > 1) {
     e = launch_splitk_reduce<T>(p, splits, stream);

@@ -1,7 +1,7 @@
 // gemm_split.hip -- FP32-accurate sgemm on BF16 MFMA (v_mfma_f32_32x32x16_bf16).
 //
-// Every FP32 element x of op(A) and op(B) is split, while it is staged from
-// registers into LDS, into bf16 terms
+// Every FP32 element x of op(A) and op(B) is split once, by a packing pass,
+// into bf16 terms
 //     t0 = bf16(x),  t1 = bf16(x - t0),  t2 = bf16(x - t0 - t1)
 // (each residual is exact in FP32).  The product keeps the cross terms
 // t_p(A) * t_q(B) with p + q < TERMS -- 6 products for TERMS = 3 (hi*hi,
@@ -17,9 +17,10 @@
 // its truncated leading term instead, so the split stays exact for finite
 // inputs up to FLT_MAX.  Non-finite x gives t0 = bf16(x) and zero residuals.
 //
-// Tiling and staging mirror gemm_mfma.hip (the GEBP packers become LDS
-// staging, the micro-kernel becomes MFMA micro-tiles, alpha/beta is fused),
-// with the LDS holding TERMS planes of 64-byte bf16 rows per operand.
+// The packing pass plays the role of Eigen's gemm_pack_lhs/gemm_pack_rhs; the
+// product kernel streams the packed planes into LDS with direct
+// global-to-LDS loads, runs the cross products as MFMA micro-tiles and fuses
+// alpha/beta into the store.
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -61,25 +62,88 @@ __device__ inline void split8(const float (&x)[8], u16x8 (&out)[TERMS]) {
   }
 }
 
-template <int BM, int BN, int WGM, int WGN, bool TA, bool TB, int TERMS>
-__global__ void __launch_bounds__(64 * WGM * WGN) gemm_split_kernel(GemmParams<float> p) {
-  constexpr int NT = 64 * WGM * WGN;
-  constexpr int BK = 32;                 // k per slice
-  constexpr int ROWB = BK * 2;           // bytes of one bf16 plane row
-  constexpr int PC = BK / 8;             // 16-byte chunks per plane row (8 bf16 each) = 4
+// ---------------------------------------------------------------------------
+// Pack: op(X) (rows x K, FP32) -> TERMS bf16 planes laid out
+//     P[ks][t][r][0..31]   ks = k / 32, r < Rp (rows padded to 256), 64 B rows
+// with zeros past `rows` and past K.  This is gemm_pack_lhs / gemm_pack_rhs
+// (Eigen/src/Core/products/GeneralBlockPanelKernel.h:1893-2478) for the
+// split product: each element is split once here instead of once per tile
+// that stages it, and every k slice of a tile becomes one contiguous block.
+// One thread writes one 16-byte chunk (8 k values of one row) of every plane.
+// KCONT: op(X)(r,k) = X[k + r*ld]; otherwise op(X)(r,k) = X[r + k*ld].
+// ---------------------------------------------------------------------------
+template <int TERMS, bool KCONT>
+__global__ void __launch_bounds__(256) pack_split_kernel(const float* __restrict__ X, int64_t ld, int rows, int K,
+                                                         int Rp, int KS, u16x8* __restrict__ P) {
+  const int64_t total = (int64_t)KS * Rp * 4;
+  for (int64_t idx = blockIdx.x * 256ll + threadIdx.x; idx < total; idx += (int64_t)gridDim.x * 256) {
+    int ks, r, c;
+    if constexpr (KCONT) {  // chunk fastest: each row's 128 B of k are read together
+      c = (int)(idx & 3);
+      r = (int)((idx >> 2) % Rp);
+      ks = (int)((idx >> 2) / Rp);
+    } else {  // row fastest: one k column of 64 consecutive rows per wave load
+      r = (int)(idx % Rp);
+      c = (int)((idx / Rp) & 3);
+      ks = (int)((idx / Rp) >> 2);
+    }
+    const int k0 = ks * 32 + c * 8;
+    float v[8];
+    if (r < rows) {
+      if constexpr (KCONT) {
+        const float* src = X + (int64_t)r * ld + k0;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) v[e] = (k0 + e < K) ? src[e] : 0.0f;
+      } else {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) v[e] = (k0 + e < K) ? X[r + (int64_t)(k0 + e) * ld] : 0.0f;
+      }
+    } else {
+#pragma unroll
+      for (int e = 0; e < 8; ++e) v[e] = 0.0f;
+    }
+    u16x8 t[TERMS];
+    split8<TERMS>(v, t);
+#pragma unroll
+    for (int s = 0; s < TERMS; ++s) P[(((int64_t)ks * TERMS + s) * Rp + r) * 4 + c] = t[s];
+  }
+}
+
+__device__ inline void wait_vmcnt0() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+template <int N>
+__device__ inline void wait_vmcnt() {
+  // gfx9 encoding: vmcnt[3:0] | expcnt[6:4] | lgkmcnt[11:8] | vmcnt[5:4] << 14
+  static_assert(N >= 0 && N < 64, "vmcnt range");
+  __builtin_amdgcn_s_waitcnt((N & 15) | (7 << 4) | (15 << 8) | ((N >> 4) << 14));
+}
+
+// ---------------------------------------------------------------------------
+// Product over packed planes.  Each stage of LDS holds one 32-deep k slice:
+// TERMS planes of [BM rows of A | BN rows of B] x 64 B, XOR-swizzled by
+// 16-byte chunk.  Stages are filled by global_load_lds_dwordx4: every wave
+// instruction moves one 1 KB block (16 rows) straight from HBM into LDS,
+// each lane fetching the global chunk that belongs at its LDS position.
+// NSTAGE - 1 slices are in flight while one is consumed.
+// ---------------------------------------------------------------------------
+template <int BM, int BN, int WGM, int WGN, int TERMS, int NSTAGE>
+__global__ void __launch_bounds__(64 * WGM * WGN) gemm_split_packed_kernel(SplitParams p) {
+  constexpr int NW = WGM * WGN;
+  constexpr int ROWB = 64;
+  constexpr int PROWS = BM + BN;             // rows of one plane in a stage
+  constexpr int PLANE = PROWS * ROWB;
+  constexpr int STAGE = TERMS * PLANE;
+  constexpr int BLOCKS = TERMS * PROWS / 16;  // 1 KB blocks per stage
+  constexpr int LPW = BLOCKS / NW;            // direct loads per wave per stage
   constexpr int WTM = BM / WGM, WTN = BN / WGN;
   constexpr int RM = WTM / 32, RN = WTN / 32;
-  constexpr int A_PC = BM * PC / NT, B_PC = BN * PC / NT;
-  constexpr int PLANE = (BM + BN) * ROWB;  // bytes of one term plane (A rows then B rows)
-  constexpr int STAGE = TERMS * PLANE;
+  static_assert(BLOCKS % NW == 0, "blocks must divide over waves");
   static_assert(RM >= 1 && RN >= 1, "wave tile smaller than one MFMA block");
-  static_assert(BM * PC % NT == 0 && BN * PC % NT == 0, "tile not divisible by threads");
 
-  __shared__ __attribute__((aligned(16))) unsigned char smem[2 * STAGE];
+  __shared__ __attribute__((aligned(1024))) unsigned char smem[NSTAGE * STAGE];
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
-  const int wid = tid >> 6;
+  const int wid = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int wm = wid % WGM, wn = wid / WGM;
 
   const int ntm = (p.M + BM - 1) / BM, ntn = (p.N + BN - 1) / BN;
@@ -87,78 +151,35 @@ __global__ void __launch_bounds__(64 * WGM * WGN) gemm_split_kernel(GemmParams<f
   tile_coords(xcd_remap(blockIdx.x, gridDim.x), ntm, ntn, 8, tm, tn);
   const int m0 = tm * BM, n0 = tn * BN;
   const int z = blockIdx.y;
-  const int kbeg = z * p.k_per_split;
-  const int kend = min(p.K, kbeg + p.k_per_split);
-  const int nkt = kend > kbeg ? (kend - kbeg + BK - 1) / BK : 0;
+  const int ks0 = z * p.ks_per_split;
+  const int nkt = min(p.KS, ks0 + p.ks_per_split) - ks0;
 
-  float ra[A_PC][8], rb[B_PC][8];
-
-  // Load 8 consecutive k of one row of op(X) (X = A or B) into v.
-  auto load8 = [&](const float* X, int64_t ld, bool kcont, bool vec, int gi, int rows_valid, int gk,
-                   float (&v)[8]) {
-    if (kcont) {
-      if (gi < rows_valid) {
-        const float* src = X + (int64_t)gi * ld + gk;
-        if (vec && gk + 8 <= kend) {
-          floatx4 lo = *reinterpret_cast<const floatx4*>(src);
-          floatx4 hi = *reinterpret_cast<const floatx4*>(src + 4);
+  // Per-lane global byte offset (within a k slice) of the chunk each direct
+  // load of this wave fetches; the slice advance is added per stage.
+  const unsigned char* srcA = reinterpret_cast<const unsigned char*>(p.Ap);
+  const unsigned char* srcB = reinterpret_cast<const unsigned char*>(p.Bp);
+  const int64_t sliceA = (int64_t)TERMS * p.Mp * ROWB, sliceB = (int64_t)TERMS * p.Np * ROWB;
+  int64_t goff[LPW];
+  bool isA[LPW];
 #pragma unroll
-          for (int e = 0; e < 4; ++e) {
-            v[e] = lo[e];
-            v[e + 4] = hi[e];
-          }
-        } else {
+  for (int i = 0; i < LPW; ++i) {
+    const int b = wid + NW * i;              // block index within the stage
+    const int t = b / (PROWS / 16);          // plane
+    const int row = (b % (PROWS / 16)) * 16 + (lane >> 2);  // plane row
+    const int c = (lane & 3) ^ (swz4(row, 0));               // logical chunk at this LDS slot
+    isA[i] = row < BM;
+    goff[i] = isA[i] ? ((int64_t)t * p.Mp + m0 + row) * ROWB + c * 16
+                     : ((int64_t)t * p.Np + n0 + row - BM) * ROWB + c * 16;
+  }
+  auto issue = [&](int kt, int buf) {
+    const int64_t ks = ks0 + kt;
 #pragma unroll
-          for (int e = 0; e < 8; ++e) v[e] = (gk + e < kend) ? src[e] : 0.0f;
-        }
-      } else {
-#pragma unroll
-        for (int e = 0; e < 8; ++e) v[e] = 0.0f;
-      }
-    } else {
-#pragma unroll
-      for (int e = 0; e < 8; ++e)
-        v[e] = (gi < rows_valid && gk + e < kend) ? X[gi + (int64_t)(gk + e) * ld] : 0.0f;
-    }
-  };
-
-  auto load_tiles = [&](int k0) {
-#pragma unroll
-    for (int j = 0; j < A_PC; ++j) {
-      const int q = tid + NT * j;
-      const int r = TA ? q / PC : q % BM, c = TA ? q % PC : q / BM;
-      load8(p.A, p.lda, TA, p.vecA, m0 + r, p.M, k0 + c * 8, ra[j]);
-    }
-#pragma unroll
-    for (int j = 0; j < B_PC; ++j) {
-      const int q = tid + NT * j;
-      const int r = TB ? q % BN : q / PC, c = TB ? q / BN : q % PC;
-      load8(p.B, p.ldb, !TB, p.vecB, n0 + r, p.N, k0 + c * 8, rb[j]);
-    }
-  };
-
-  auto store_tiles = [&](int buf) {
-    unsigned char* base = smem + buf * STAGE;
-#pragma unroll
-    for (int j = 0; j < A_PC; ++j) {
-      const int q = tid + NT * j;
-      const int r = TA ? q / PC : q % BM, c = TA ? q % PC : q / BM;
-      u16x8 t[TERMS];
-      split8<TERMS>(ra[j], t);
-#pragma unroll
-      for (int s = 0; s < TERMS; ++s)
-        *reinterpret_cast<u16x8*>(base + s * PLANE + r * ROWB + swz4(r, c) * 16) = t[s];
-    }
-#pragma unroll
-    for (int j = 0; j < B_PC; ++j) {
-      const int q = tid + NT * j;
-      const int r = TB ? q % BN : q / PC, c = TB ? q / BN : q % PC;
-      u16x8 t[TERMS];
-      split8<TERMS>(rb[j], t);
-      const int rr = BM + r;
-#pragma unroll
-      for (int s = 0; s < TERMS; ++s)
-        *reinterpret_cast<u16x8*>(base + s * PLANE + rr * ROWB + swz4(rr, c) * 16) = t[s];
+    for (int i = 0; i < LPW; ++i) {
+      const int b = wid + NW * i;
+      const unsigned char* g = isA[i] ? srcA + ks * sliceA + goff[i] : srcB + ks * sliceB + goff[i];
+      __attribute__((address_space(3))) void* l =
+          (__attribute__((address_space(3))) void*)(smem + buf * STAGE + b * 1024);
+      __builtin_amdgcn_global_load_lds((const void*)g, l, 16, 0, 0);
     }
   };
 
@@ -173,49 +194,54 @@ __global__ void __launch_bounds__(64 * WGM * WGN) gemm_split_kernel(GemmParams<f
   const int li = lane & 31;
   const int lg = lane >> 5;
 
-  if (nkt > 0) {
-    load_tiles(kbeg);
-    store_tiles(0);
+#pragma unroll
+  for (int s = 0; s < NSTAGE - 1; ++s)
+    if (s < nkt) issue(s, s);
+
+  for (int kt = 0; kt < nkt; ++kt) {
+    if constexpr (NSTAGE > 2) {
+      if (kt + 1 < nkt)
+        wait_vmcnt<(NSTAGE - 2) * LPW>();
+      else
+        wait_vmcnt0();
+    } else {
+      wait_vmcnt0();
+    }
     __syncthreads();
-    for (int kt = 0; kt < nkt; ++kt) {
-      const int buf = kt & 1;
-      if (kt + 1 < nkt) load_tiles(kbeg + (kt + 1) * BK);
-      const unsigned char* base = smem + buf * STAGE;
+    if (kt + NSTAGE - 1 < nkt) issue(kt + NSTAGE - 1, (kt + NSTAGE - 1) % NSTAGE);
+    const unsigned char* base = smem + (kt % NSTAGE) * STAGE;
 #pragma unroll
-      for (int s = 0; s < 2; ++s) {  // two 16-deep MFMA steps per slice
-        const int c = 2 * s + lg;
-        bf16x8 a[TERMS][RM], b[TERMS][RN];
+    for (int s = 0; s < 2; ++s) {  // two 16-deep MFMA steps per slice
+      const int c = 2 * s + lg;
+      bf16x8 a[TERMS][RM], b[TERMS][RN];
 #pragma unroll
-        for (int t = 0; t < TERMS; ++t) {
+      for (int t = 0; t < TERMS; ++t) {
 #pragma unroll
-          for (int i = 0; i < RM; ++i) {
-            const int r = wm * WTM + i * 32 + li;
-            a[t][i] = *reinterpret_cast<const bf16x8*>(base + t * PLANE + r * ROWB + swz4(r, c) * 16);
-          }
-#pragma unroll
-          for (int j = 0; j < RN; ++j) {
-            const int r = BM + wn * WTN + j * 32 + li;
-            b[t][j] = *reinterpret_cast<const bf16x8*>(base + t * PLANE + r * ROWB + swz4(r, c) * 16);
-          }
+        for (int i = 0; i < RM; ++i) {
+          const int r = wm * WTM + i * 32 + li;
+          a[t][i] = *reinterpret_cast<const bf16x8*>(base + t * PLANE + r * ROWB + swz4(r, c) * 16);
         }
+#pragma unroll
+        for (int j = 0; j < RN; ++j) {
+          const int r = BM + wn * WTN + j * 32 + li;
+          b[t][j] = *reinterpret_cast<const bf16x8*>(base + t * PLANE + r * ROWB + swz4(r, c) * 16);
+        }
+      }
+      // cross products with p + q < TERMS, smallest first; consecutive MFMAs
+      // go to different accumulators
+      constexpr int NPROD = TERMS == 3 ? 6 : 3;
+      constexpr int PA[6] = {TERMS == 3 ? 2 : 1, TERMS == 3 ? 0 : 0, TERMS == 3 ? 1 : 0, 1, 0, 0};
+      constexpr int PB[6] = {TERMS == 3 ? 0 : 0, TERMS == 3 ? 2 : 1, TERMS == 3 ? 1 : 0, 0, 1, 0};
+#pragma unroll
+      for (int pq = 0; pq < NPROD; ++pq) {
+        const int pa = PA[pq];
+        const int pb = PB[pq];
 #pragma unroll
         for (int i = 0; i < RM; ++i)
 #pragma unroll
-          for (int j = 0; j < RN; ++j) {
-            if constexpr (TERMS == 3) {
-              acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[2][i], b[0][j], acc[i][j], 0, 0, 0);
-              acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[0][i], b[2][j], acc[i][j], 0, 0, 0);
-              acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[1][i], b[1][j], acc[i][j], 0, 0, 0);
-            }
-            if constexpr (TERMS >= 2) {
-              acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[1][i], b[0][j], acc[i][j], 0, 0, 0);
-              acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[0][i], b[1][j], acc[i][j], 0, 0, 0);
-            }
-            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[0][i], b[0][j], acc[i][j], 0, 0, 0);
-          }
+          for (int j = 0; j < RN; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[pa][i], b[pb][j], acc[i][j], 0, 0, 0);
       }
-      if (kt + 1 < nkt) store_tiles(buf ^ 1);
-      __syncthreads();
     }
   }
 
@@ -256,31 +282,46 @@ __global__ void __launch_bounds__(256) split_bf16_kernel(const float* x, int64_t
   }
 }
 
-template <int BM, int BN, int WGM, int WGN, int TERMS>
-static hipError_t launch_split_t(const GemmParams<float>& p, bool ta, bool tb, int splits, hipStream_t s) {
-  const int ntm = (p.M + BM - 1) / BM, ntn = (p.N + BN - 1) / BN;
-  dim3 grid(ntm * ntn, splits), block(64 * WGM * WGN);
-  if (!ta && !tb)
-    hipLaunchKernelGGL((gemm_split_kernel<BM, BN, WGM, WGN, false, false, TERMS>), grid, block, 0, s, p);
-  else if (ta && !tb)
-    hipLaunchKernelGGL((gemm_split_kernel<BM, BN, WGM, WGN, true, false, TERMS>), grid, block, 0, s, p);
-  else if (!ta && tb)
-    hipLaunchKernelGGL((gemm_split_kernel<BM, BN, WGM, WGN, false, true, TERMS>), grid, block, 0, s, p);
+template <int TERMS>
+static hipError_t launch_pack_t(const float* X, int64_t ld, bool kcont, int rows, int K, int Rp, int KS, void* P,
+                                hipStream_t s) {
+  const int64_t total = (int64_t)KS * Rp * 4;
+  int64_t blocks = (total + 255) / 256;
+  if (blocks > 32768) blocks = 32768;
+  if (kcont)
+    hipLaunchKernelGGL((pack_split_kernel<TERMS, true>), dim3((unsigned)blocks), dim3(256), 0, s, X, ld, rows, K, Rp,
+                       KS, static_cast<u16x8*>(P));
   else
-    hipLaunchKernelGGL((gemm_split_kernel<BM, BN, WGM, WGN, true, true, TERMS>), grid, block, 0, s, p);
+    hipLaunchKernelGGL((pack_split_kernel<TERMS, false>), dim3((unsigned)blocks), dim3(256), 0, s, X, ld, rows, K,
+                       Rp, KS, static_cast<u16x8*>(P));
   return hipGetLastError();
 }
 
-hipError_t launch_gemm_split(const GemmParams<float>& p, int terms, int tile, bool ta, bool tb, int splits,
+hipError_t launch_pack_split(const float* X, int64_t ld, bool kcont, int rows, int K, int terms, void* P,
                              hipStream_t s) {
-  // tile 0: 128x128 (4 waves of 64x64), 1: 256x128 (4 waves of 128x64)
+  const int Rp = split_rows_padded(rows), KS = split_kslices(K);
+  if (terms == 3) return launch_pack_t<3>(X, ld, kcont, rows, K, Rp, KS, P, s);
+  if (terms == 2) return launch_pack_t<2>(X, ld, kcont, rows, K, Rp, KS, P, s);
+  return hipErrorInvalidValue;
+}
+
+template <int BM, int BN, int TERMS, int NSTAGE>
+static hipError_t launch_packed_t(const SplitParams& p, int splits, hipStream_t s) {
+  const int ntm = (p.M + BM - 1) / BM, ntn = (p.N + BN - 1) / BN;
+  hipLaunchKernelGGL((gemm_split_packed_kernel<BM, BN, 2, 2, TERMS, NSTAGE>), dim3(ntm * ntn, splits), dim3(256), 0,
+                     s, p);
+  return hipGetLastError();
+}
+
+hipError_t launch_gemm_split_packed(const SplitParams& p, int terms, int tile, int splits, hipStream_t s) {
+  // tile 0: 128x128 (3 stages), 1: 256x128 (2 stages)
   if (terms == 3) {
-    if (tile == 0) return launch_split_t<128, 128, 2, 2, 3>(p, ta, tb, splits, s);
-    return launch_split_t<256, 128, 2, 2, 3>(p, ta, tb, splits, s);
+    if (tile == 0) return launch_packed_t<128, 128, 3, 3>(p, splits, s);
+    return launch_packed_t<256, 128, 3, 2>(p, splits, s);
   }
   if (terms == 2) {
-    if (tile == 0) return launch_split_t<128, 128, 2, 2, 2>(p, ta, tb, splits, s);
-    return launch_split_t<256, 128, 2, 2, 2>(p, ta, tb, splits, s);
+    if (tile == 0) return launch_packed_t<128, 128, 2, 3>(p, splits, s);
+    return launch_packed_t<256, 128, 2, 2>(p, splits, s);
   }
   return hipErrorInvalidValue;
 }

@@ -34,9 +34,28 @@ hipError_t launch_splitk_reduce(const GemmParams<T>& p, int splits, hipStream_t 
 template <typename T>
 hipError_t launch_scale_c(T* C, int M, int N, int64_t ldc, T beta, hipStream_t s);
 
-// Split-BF16 FP32 product: tile 0 = 128x128, 1 = 256x128; terms 2 or 3.
-hipError_t launch_gemm_split(const GemmParams<float>& p, int terms, int tile, bool ta, bool tb, int splits,
+// Split-BF16 FP32 product over packed operands.  Each operand is first packed
+// into bf16 planes P[ks][term][row][32] (rows padded to split_rows_padded,
+// ks = k / 32 slices) by launch_pack_split, then the product kernel streams
+// the planes into LDS.
+inline int split_rows_padded(int rows) { return (rows + 255) / 256 * 256; }
+inline int split_kslices(int K) { return (K + 31) / 32; }
+inline size_t split_packed_bytes(int rows, int K, int terms) {
+  return (size_t)split_kslices(K) * terms * split_rows_padded(rows) * 64;
+}
+struct SplitParams {
+  const void* Ap;  // packed op(A): M rows
+  const void* Bp;  // packed op(B)^T: N rows
+  float* C;
+  float* W;
+  int64_t ldc;
+  int M, N, KS, Mp, Np;
+  int ks_per_split;
+  float alpha, beta;
+};
+hipError_t launch_pack_split(const float* X, int64_t ld, bool kcont, int rows, int K, int terms, void* P,
                              hipStream_t s);
+hipError_t launch_gemm_split_packed(const SplitParams& p, int terms, int tile, int splits, hipStream_t s);
 hipError_t launch_split_bf16(const float* x, int64_t n, int terms, unsigned short* t0, unsigned short* t1,
                              unsigned short* t2, hipStream_t s);
 
