@@ -154,32 +154,41 @@ __global__ void __launch_bounds__(64 * WGM * WGN) gemm_split_packed_kernel(Split
   const int ks0 = z * p.ks_per_split;
   const int nkt = min(p.KS, ks0 + p.ks_per_split) - ks0;
 
-  // Per-lane global byte offset (within a k slice) of the chunk each direct
-  // load of this wave fetches; the slice advance is added per stage.
+  // Direct loads: block b of a stage covers 16 plane rows [blk0, blk0 + 16)
+  // of one plane; lane l fills LDS slot l of the block, i.e. plane row
+  // blk0 + l/4 at physical chunk l&3, which holds logical chunk
+  // (l&3) ^ s4(row).  With blk0 a multiple of 16 the swizzle term depends on
+  // blk0 only through bit 4, so the per-lane offset is computed per block
+  // once; the block bases are wave-uniform and advance by one slice per stage.
   const unsigned char* srcA = reinterpret_cast<const unsigned char*>(p.Ap);
   const unsigned char* srcB = reinterpret_cast<const unsigned char*>(p.Bp);
   const int64_t sliceA = (int64_t)TERMS * p.Mp * ROWB, sliceB = (int64_t)TERMS * p.Np * ROWB;
-  int64_t goff[LPW];
-  bool isA[LPW];
+  const unsigned char* gbase[LPW];
+  uint32_t loff[LPW];
+  int64_t gstride[LPW];
 #pragma unroll
   for (int i = 0; i < LPW; ++i) {
-    const int b = wid + NW * i;              // block index within the stage
-    const int t = b / (PROWS / 16);          // plane
-    const int row = (b % (PROWS / 16)) * 16 + (lane >> 2);  // plane row
-    const int c = (lane & 3) ^ (swz4(row, 0));               // logical chunk at this LDS slot
-    isA[i] = row < BM;
-    goff[i] = isA[i] ? ((int64_t)t * p.Mp + m0 + row) * ROWB + c * 16
-                     : ((int64_t)t * p.Np + n0 + row - BM) * ROWB + c * 16;
+    const int b = wid + NW * i;                // block index within the stage (wave-uniform)
+    const int t = b / (PROWS / 16);            // plane
+    const int blk0 = (b % (PROWS / 16)) * 16;  // first plane row of the block
+    const int row = blk0 + (lane >> 2);
+    const int c = (lane & 3) ^ swz4(row, 0);   // logical chunk stored at this LDS slot
+    loff[i] = (uint32_t)((lane >> 2) * ROWB + c * 16);
+    if (blk0 < BM) {
+      gbase[i] = srcA + (int64_t)ks0 * sliceA + ((int64_t)t * p.Mp + m0 + blk0) * ROWB;
+      gstride[i] = sliceA;
+    } else {
+      gbase[i] = srcB + (int64_t)ks0 * sliceB + ((int64_t)t * p.Np + n0 + blk0 - BM) * ROWB;
+      gstride[i] = sliceB;
+    }
   }
   auto issue = [&](int kt, int buf) {
-    const int64_t ks = ks0 + kt;
 #pragma unroll
     for (int i = 0; i < LPW; ++i) {
       const int b = wid + NW * i;
-      const unsigned char* g = isA[i] ? srcA + ks * sliceA + goff[i] : srcB + ks * sliceB + goff[i];
       __attribute__((address_space(3))) void* l =
           (__attribute__((address_space(3))) void*)(smem + buf * STAGE + b * 1024);
-      __builtin_amdgcn_global_load_lds((const void*)g, l, 16, 0, 0);
+      __builtin_amdgcn_global_load_lds((const void*)(gbase[i] + kt * gstride[i] + loff[i]), l, 16, 0, 0);
     }
   };
 
@@ -210,39 +219,44 @@ __global__ void __launch_bounds__(64 * WGM * WGN) gemm_split_packed_kernel(Split
     __syncthreads();
     if (kt + NSTAGE - 1 < nkt) issue(kt + NSTAGE - 1, (kt + NSTAGE - 1) % NSTAGE);
     const unsigned char* base = smem + (kt % NSTAGE) * STAGE;
+    // Fragments travel as 32-bit integer vectors (bf16 vectors crossing
+    // control flow are legalised element by element) and are bit-cast at the
+    // MFMA.  (Forcing all reads of both 16-deep steps ahead of the MFMAs
+    // measured slower than the compiler's interleaving.)
+    u32x4 a[2][TERMS][RM], b[2][TERMS][RN];
 #pragma unroll
-    for (int s = 0; s < 2; ++s) {  // two 16-deep MFMA steps per slice
+    for (int s = 0; s < 2; ++s) {
       const int c = 2 * s + lg;
-      bf16x8 a[TERMS][RM], b[TERMS][RN];
 #pragma unroll
       for (int t = 0; t < TERMS; ++t) {
 #pragma unroll
         for (int i = 0; i < RM; ++i) {
           const int r = wm * WTM + i * 32 + li;
-          a[t][i] = *reinterpret_cast<const bf16x8*>(base + t * PLANE + r * ROWB + swz4(r, c) * 16);
+          a[s][t][i] = *reinterpret_cast<const u32x4*>(base + t * PLANE + r * ROWB + swz4(r, c) * 16);
         }
 #pragma unroll
         for (int j = 0; j < RN; ++j) {
           const int r = BM + wn * WTN + j * 32 + li;
-          b[t][j] = *reinterpret_cast<const bf16x8*>(base + t * PLANE + r * ROWB + swz4(r, c) * 16);
+          b[s][t][j] = *reinterpret_cast<const u32x4*>(base + t * PLANE + r * ROWB + swz4(r, c) * 16);
         }
       }
-      // cross products with p + q < TERMS, smallest first; consecutive MFMAs
-      // go to different accumulators
-      constexpr int NPROD = TERMS == 3 ? 6 : 3;
-      constexpr int PA[6] = {TERMS == 3 ? 2 : 1, TERMS == 3 ? 0 : 0, TERMS == 3 ? 1 : 0, 1, 0, 0};
-      constexpr int PB[6] = {TERMS == 3 ? 0 : 0, TERMS == 3 ? 2 : 1, TERMS == 3 ? 1 : 0, 0, 1, 0};
+    }
+    // cross products with p + q < TERMS, smallest first; consecutive MFMAs go
+    // to different accumulators
+    constexpr int NPROD = TERMS == 3 ? 6 : 3;
+    constexpr int PA[6] = {TERMS == 3 ? 2 : 1, 0, TERMS == 3 ? 1 : 0, 1, 0, 0};
+    constexpr int PB[6] = {0, TERMS == 3 ? 2 : 1, TERMS == 3 ? 1 : 0, 0, 1, 0};
 #pragma unroll
-      for (int pq = 0; pq < NPROD; ++pq) {
-        const int pa = PA[pq];
-        const int pb = PB[pq];
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+      for (int pq = 0; pq < NPROD; ++pq)
 #pragma unroll
         for (int i = 0; i < RM; ++i)
 #pragma unroll
           for (int j = 0; j < RN; ++j)
-            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[pa][i], b[pb][j], acc[i][j], 0, 0, 0);
-      }
-    }
+            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a[s][PA[pq]][i]),
+                                                                __builtin_bit_cast(bf16x8, b[s][PB[pq]][j]),
+                                                                acc[i][j], 0, 0, 0);
   }
 
   const bool partial = gridDim.y > 1;
