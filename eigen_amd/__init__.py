@@ -30,6 +30,52 @@ _lib = None
 _blas = None
 
 
+class GemmProblem(C.Structure):
+    """eh_gemm_problem (include/eigen_amd.h)."""
+    _fields_ = [("dtype", C.c_int32), ("opa", C.c_int32), ("opb", C.c_int32), ("reserved", C.c_int32),
+                ("m", C.c_int64), ("n", C.c_int64), ("k", C.c_int64),
+                ("lda", C.c_int64), ("ldb", C.c_int64), ("ldc", C.c_int64),
+                ("A", C.c_void_p), ("B", C.c_void_p), ("C", C.c_void_p),
+                ("alpha", C.c_double), ("beta", C.c_double)]
+
+
+class Grouped:
+    """A batch of independent products run by one persistent kernel."""
+
+    def __init__(self, problems):
+        arr = (GemmProblem * len(problems))(*problems)
+        h = C.c_void_p()
+        check(lib().eh_grouped_create(arr, len(problems), C.byref(h)), "eh_grouped_create")
+        self.handle = h
+
+    def run(self, stream=None):
+        check(lib().eh_grouped_run(self.handle, current_stream() if stream is None else stream), "eh_grouped_run")
+
+    def graph_launch(self, stream=None):
+        check(lib().eh_grouped_graph_launch(self.handle, current_stream() if stream is None else stream),
+              "eh_grouped_graph_launch")
+
+    def run_per_product(self, stream=None):
+        check(lib().eh_grouped_run_per_product(self.handle, current_stream() if stream is None else stream),
+              "eh_grouped_run_per_product")
+
+    def info(self):
+        t, p = C.c_int64(), C.c_int64()
+        check(lib().eh_grouped_info(self.handle, C.byref(t), C.byref(p)), "eh_grouped_info")
+        return t.value, p.value
+
+    def close(self):
+        if self.handle:
+            lib().eh_grouped_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class EigenAmdError(RuntimeError):
     def __init__(self, status: int, what: str = ""):
         msg = f"{what}: status {status}"
@@ -90,13 +136,14 @@ def _declare_extra(L) -> None:
     """Signatures of the grouped / tuner / MLP entry points (present in full builds)."""
     vp, i64, i32, f64, f32 = C.c_void_p, C.c_int64, C.c_int, C.c_double, C.c_float
     table = {
-        "eh_grouped_create": (i32, [vp, i64, C.POINTER(vp)]),
+        "eh_grouped_create": (i32, [C.POINTER(GemmProblem), i64, C.POINTER(vp)]),
         "eh_grouped_run": (i32, [vp, vp]),
         "eh_grouped_graph_launch": (i32, [vp, vp]),
         "eh_grouped_run_per_product": (i32, [vp, vp]),
         "eh_grouped_destroy": (i32, [vp]),
         "eh_grouped_info": (i32, [vp, C.POINTER(i64), C.POINTER(i64)]),
         "eh_time_config": (i32, [i32, i32, i32, i64, i64, i64, vp, vp, vp, i32, i32, C.POINTER(f32)]),
+        "eh_tuner_record": (i32, [i32, i32, i32, i64, i64, i64, i32]),
         "eh_tuner_load": (i32, [C.c_char_p]),
         "eh_tuner_save": (i32, [C.c_char_p]),
         "eh_tuner_count": (i64, []),
@@ -106,13 +153,14 @@ def _declare_extra(L) -> None:
         "eh_mlp_set_params": (i32, [vp, vp]),
         "eh_mlp_get_params": (i32, [vp, vp]),
         "eh_mlp_num_params": (i64, [vp]),
+        "eh_mlp_features": (i32, [i32, i32, i32, i64, i64, i64, vp]),
         "eh_mlp_forward": (i32, [vp, vp, i64, vp, vp]),
         "eh_mlp_backward": (i32, [vp, vp, vp, i64, vp, vp, vp]),
-        "eh_mlp_adam_step": (i32, [vp, vp, f32, f32, f32, f32, i32, vp]),
-        "eh_mlp_train": (i32, [vp, vp, vp, i64, i32, i32, f32, f32, i64, vp, vp]),
+        "eh_mlp_adam_step": (i32, [vp, vp, f32, f32, f32, f32, f32, vp]),
+        "eh_mlp_train": (i32, [vp, vp, vp, i64, i32, f32, f32, vp, vp]),
         "eh_mlp_predict": (i32, [vp, vp, i64, vp, vp]),
         "eh_planner_set_mlp": (i32, [vp, vp]),
-        "eh_mlp_features": (i32, [i32, i32, i32, i64, i64, i64, vp]),
+        "eh_planner_set_mlp_params": (i32, [vp]),
     }
     for name, (res, args) in table.items():
         if hasattr(L, name):

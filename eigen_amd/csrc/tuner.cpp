@@ -1,0 +1,187 @@
+// tuner.cpp -- exhaustive autotuning of the plain products' configuration.
+//
+// Eigen's own answer to "which blocking for this shape" is a cache-model
+// heuristic (evaluateProductBlockingSizesHeuristic,
+// Eigen/src/Core/products/GeneralBlockPanelKernel.h:201-405) plus an offline
+// sweep tool (benchmarks/Tuning/bench_blocking_sizes.cpp).  Here every
+// candidate configuration is timed on the device with HIP events and the
+// fastest one is kept in a table keyed by (dtype, opa, opb, m, n, k), which
+// can be saved to / loaded from a text file.  The table is the judge the
+// learned selector is measured against (regret = t(choice)/t(best) - 1).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
+#include "../../include/eigen_amd.h"
+#include "internal.hpp"
+#include "kernels.hpp"
+
+namespace eh {
+
+using ShapeKey = std::tuple<int, int, int, int64_t, int64_t, int64_t>;
+static std::mutex g_tune_mutex;
+static std::map<ShapeKey, int> g_table;
+
+static ShapeKey key_of(int dtype, int opa, int opb, int64_t m, int64_t n, int64_t k) {
+  return ShapeKey{dtype, opa != EH_OP_N, opb != EH_OP_N, m, n, k};
+}
+
+// Median time of `reps` runs of configuration `config` (after one warm-up) on
+// the caller's buffers, C = A*B with beta = 0.
+static eh_status time_config(int dtype, int opa, int opb, int64_t m, int64_t n, int64_t k, const void* A,
+                             const void* B, void* C, int config, int reps, float* ms) {
+  hipStream_t s;
+  hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+  if (e != hipSuccess) return hip_fail(e, "tuner stream");
+  hipEvent_t e0, e1;
+  hipEventCreate(&e0);
+  hipEventCreate(&e1);
+  const double one = 1.0, zero = 0.0;
+  const float onef = 1.0f, zerof = 0.0f;
+  const void* al = dtype == EH_F32 ? (const void*)&onef : (const void*)&one;
+  const void* be = dtype == EH_F32 ? (const void*)&zerof : (const void*)&zero;
+  const int64_t lda = opa == EH_OP_N ? std::max<int64_t>(m, 1) : std::max<int64_t>(k, 1);
+  const int64_t ldb = opb == EH_OP_N ? std::max<int64_t>(k, 1) : std::max<int64_t>(n, 1);
+  std::vector<float> t;
+  eh_status st = EH_SUCCESS;
+  for (int r = 0; r <= reps && st == EH_SUCCESS; ++r) {
+    hipEventRecord(e0, s);
+    st = gemm_dispatch(dtype, opa, opb, m, n, k, al, A, lda, B, ldb, be, C, std::max<int64_t>(m, 1), EH_PREC_PLAIN,
+                       config, s);
+    hipEventRecord(e1, s);
+    if (st != EH_SUCCESS) break;
+    e = hipEventSynchronize(e1);
+    if (e != hipSuccess) {
+      st = hip_fail(e, "tuner sync");
+      break;
+    }
+    float v = 0.0f;
+    hipEventElapsedTime(&v, e0, e1);
+    if (r > 0) t.push_back(v);  // first run is the warm-up
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  hipStreamDestroy(s);
+  if (st != EH_SUCCESS) return st;
+  std::sort(t.begin(), t.end());
+  *ms = t.empty() ? 0.0f : t[t.size() / 2];
+  return EH_SUCCESS;
+}
+
+// Tune one shape on scratch buffers (used by the table mode on a miss).
+static eh_status tune_shape(int dtype, int opa, int opb, int64_t m, int64_t n, int64_t k, int* best) {
+  const size_t esz = dtype == EH_F32 ? 4 : 8;
+  void *A = nullptr, *B = nullptr, *C = nullptr;
+  hipError_t e = hipMalloc(&A, std::max<int64_t>(m * k, 1) * esz);
+  if (e == hipSuccess) e = hipMalloc(&B, std::max<int64_t>(k * n, 1) * esz);
+  if (e == hipSuccess) e = hipMalloc(&C, std::max<int64_t>(m * n, 1) * esz);
+  eh_status st = e == hipSuccess ? EH_SUCCESS : fail(EH_ERR_ALLOC, "tuner scratch");
+  if (st == EH_SUCCESS) st = eh_fill((eh_dtype)dtype, A, m * k, 1, 0, -1.0, 1.0, nullptr);
+  if (st == EH_SUCCESS) st = eh_fill((eh_dtype)dtype, B, k * n, 2, 0, -1.0, 1.0, nullptr);
+  float best_ms = 0.0f;
+  *best = -1;
+  for (int c = 0; c < kNumConfigs && st == EH_SUCCESS; ++c) {
+    float ms = 0.0f;
+    st = time_config(dtype, opa, opb, m, n, k, A, B, C, c, 2, &ms);
+    if (st == EH_SUCCESS && (*best < 0 || ms < best_ms)) {
+      best_ms = ms;
+      *best = c;
+    }
+  }
+  hipFree(A);
+  hipFree(B);
+  hipFree(C);
+  return st;
+}
+
+int table_plan(int dtype, int opa, int opb, int64_t m, int64_t n, int64_t k) {
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mutex);
+    auto it = g_table.find(key_of(dtype, opa, opb, m, n, k));
+    if (it != g_table.end()) return it->second;
+  }
+  int best = -1;
+  if (tune_shape(dtype, opa, opb, m, n, k, &best) != EH_SUCCESS || best < 0) return -1;
+  std::lock_guard<std::mutex> lk(g_tune_mutex);
+  g_table[key_of(dtype, opa, opb, m, n, k)] = best;
+  return best;
+}
+
+}  // namespace eh
+
+using namespace eh;
+
+extern "C" {
+
+eh_status eh_time_config(int dtype, int opa, int opb, int64_t m, int64_t n, int64_t k, const void* A, const void* B,
+                         void* C, int config, int reps, float* ms) {
+  if (!ms) return fail(EH_ERR_INVALID, "ms is NULL");
+  if (config < 0 || config >= kNumConfigs) return fail(EH_ERR_INVALID, "config out of range");
+  if (reps < 1) return fail(EH_ERR_INVALID, "reps must be >= 1");
+  return time_config(dtype, opa, opb, m, n, k, A, B, C, config, reps, ms);
+}
+
+eh_status eh_tuner_record(int dtype, int opa, int opb, int64_t m, int64_t n, int64_t k, int config) {
+  if (config < 0 || config >= kNumConfigs) return fail(EH_ERR_INVALID, "config out of range");
+  std::lock_guard<std::mutex> lk(g_tune_mutex);
+  g_table[key_of(dtype, opa, opb, m, n, k)] = config;
+  return EH_SUCCESS;
+}
+
+int64_t eh_tuner_count(void) {
+  std::lock_guard<std::mutex> lk(g_tune_mutex);
+  return (int64_t)g_table.size();
+}
+
+eh_status eh_tuner_clear(void) {
+  std::lock_guard<std::mutex> lk(g_tune_mutex);
+  g_table.clear();
+  return EH_SUCCESS;
+}
+
+// Text format, one shape per line: dtype opa opb m n k config
+eh_status eh_tuner_save(const char* path) {
+  if (!path) return fail(EH_ERR_INVALID, "path is NULL");
+  std::lock_guard<std::mutex> lk(g_tune_mutex);
+  FILE* f = std::fopen(path, "w");
+  if (!f) return fail(EH_ERR_INVALID, std::string("cannot write ") + path);
+  std::fprintf(f, "# eigen_amd tuning table v1: dtype opa opb m n k config\n");
+  for (const auto& kv : g_table)
+    std::fprintf(f, "%d %d %d %lld %lld %lld %d\n", std::get<0>(kv.first), std::get<1>(kv.first),
+                 std::get<2>(kv.first), (long long)std::get<3>(kv.first), (long long)std::get<4>(kv.first),
+                 (long long)std::get<5>(kv.first), kv.second);
+  std::fclose(f);
+  return EH_SUCCESS;
+}
+
+eh_status eh_tuner_load(const char* path) {
+  if (!path) return fail(EH_ERR_INVALID, "path is NULL");
+  FILE* f = std::fopen(path, "r");
+  if (!f) return fail(EH_ERR_INVALID, std::string("cannot read ") + path);
+  std::map<ShapeKey, int> loaded;
+  char line[256];
+  int lineno = 0;
+  while (std::fgets(line, sizeof line, f)) {
+    ++lineno;
+    if (line[0] == '#' || line[0] == '\n') continue;
+    int d, a, b, c;
+    long long m, n, k;
+    if (std::sscanf(line, "%d %d %d %lld %lld %lld %d", &d, &a, &b, &m, &n, &k, &c) != 7 || c < 0 ||
+        c >= kNumConfigs || (d != EH_F32 && d != EH_F64) || m < 0 || n < 0 || k < 0) {
+      std::fclose(f);
+      return fail(EH_ERR_INVALID, std::string("malformed tuning table line ") + std::to_string(lineno));
+    }
+    loaded[key_of(d, a, b, m, n, k)] = c;
+  }
+  std::fclose(f);
+  std::lock_guard<std::mutex> lk(g_tune_mutex);
+  for (const auto& kv : loaded) g_table[kv.first] = kv.second;
+  return EH_SUCCESS;
+}
+
+}  // extern "C"

@@ -103,6 +103,86 @@ eh_tune_mode eh_get_tune_mode(void);
 eh_status eh_set_sgemm_precision(eh_precision prec);
 eh_precision eh_get_sgemm_precision(void);
 
+/* ---- grouped products ------------------------------------------------------ */
+
+/* One product of a batch: C = alpha*op(A)*op(B) + beta*C, column-major,
+ * device pointers.  For EH_F32 products alpha and beta are rounded to float. */
+typedef struct eh_gemm_problem {
+  int32_t dtype; /* eh_dtype */
+  int32_t opa;   /* eh_op */
+  int32_t opb;   /* eh_op */
+  int32_t reserved;
+  int64_t m, n, k;
+  int64_t lda, ldb, ldc;
+  const void* A;
+  const void* B;
+  void* C;
+  double alpha, beta;
+} eh_gemm_problem;
+
+typedef struct eh_grouped eh_grouped; /* opaque batch handle */
+
+/* Validates and uploads `count` descriptors (host array); the device work
+ * list is built on every run.  Products must not write overlapping C. */
+eh_status eh_grouped_create(const eh_gemm_problem* problems, int64_t count, eh_grouped** out);
+/* All products in one persistent kernel (plus a work-list build kernel). */
+eh_status eh_grouped_run(eh_grouped* g, void* stream);
+/* The same two launches captured once in a hipGraph and replayed. */
+eh_status eh_grouped_graph_launch(eh_grouped* g, void* stream);
+/* Reference path: one eh_gemm launch per product. */
+eh_status eh_grouped_run_per_product(eh_grouped* g, void* stream);
+/* Number of 32x32 output tiles and of non-empty products. */
+eh_status eh_grouped_info(eh_grouped* g, int64_t* tiles, int64_t* products);
+eh_status eh_grouped_destroy(eh_grouped* g);
+
+/* ---- autotuner ------------------------------------------------------------- */
+
+/* Median time (ms) of `reps` runs, after one warm-up, of configuration
+ * `config` for C = A*B (beta = 0) on the caller's device buffers (packed:
+ * lda/ldb/ldc = rows).  dtype / opa / opb take eh_dtype / eh_op values. */
+eh_status eh_time_config(int dtype, int opa, int opb, int64_t m, int64_t n, int64_t k, const void* A, const void* B,
+                         void* C, int config, int reps, float* ms);
+/* The exhaustive table used by EH_TUNE_TABLE: shape -> best configuration. */
+eh_status eh_tuner_record(int dtype, int opa, int opb, int64_t m, int64_t n, int64_t k, int config);
+int64_t eh_tuner_count(void);
+eh_status eh_tuner_clear(void);
+/* Text file, one line per shape: "dtype opa opb m n k config". */
+eh_status eh_tuner_save(const char* path);
+eh_status eh_tuner_load(const char* path);
+
+/* ---- learned selector ------------------------------------------------------ */
+
+/* A 7 -> 64 (ReLU) -> 24 (softmax) network over shape features, with device
+ * buffers for batches of up to `capacity` samples.  Parameters: 2072 floats,
+ * W1[64][7], b1[64], W2[24][64], b2[24]. */
+typedef struct eh_mlp eh_mlp;
+eh_status eh_mlp_create(int inputs, int hidden, int outputs, int64_t capacity, eh_mlp** out);
+eh_status eh_mlp_destroy(eh_mlp* m);
+int64_t eh_mlp_num_params(eh_mlp* m);
+eh_status eh_mlp_set_params(eh_mlp* m, const float* host_params);
+eh_status eh_mlp_get_params(eh_mlp* m, float* host_params);
+/* Features of a shape: log2 m/14, log2 n/14, log2 k/14, [f32, f64], ta, tb. */
+eh_status eh_mlp_features(int dtype, int opa, int opb, int64_t m, int64_t n, int64_t k, float* x7);
+/* Device X[B][7] -> device probs[B][24] (probs may be NULL). */
+eh_status eh_mlp_forward(eh_mlp* m, const float* X, int64_t B, float* probs, void* stream);
+/* Forward + backward of the mean cross entropy against soft targets
+ * Q[B][24]: device grads[2072] and loss[1] (either may be NULL). */
+eh_status eh_mlp_backward(eh_mlp* m, const float* X, const float* Q, int64_t B, float* grads, float* loss,
+                          void* stream);
+/* One Adam step (decoupled weight decay on weights) with the given device
+ * gradients (NULL = those of the last backward). */
+eh_status eh_mlp_adam_step(eh_mlp* m, const float* grads, float lr, float beta1, float beta2, float eps,
+                           float weight_decay, void* stream);
+/* `epochs` full-batch Adam steps, cosine-decayed learning rate; optional
+ * device loss_curve[epochs].  Synchronous. */
+eh_status eh_mlp_train(eh_mlp* m, const float* X, const float* Q, int64_t B, int epochs, float lr,
+                       float weight_decay, float* loss_curve, void* stream);
+/* Device X[B][7] -> device configs[B] (argmax). */
+eh_status eh_mlp_predict(eh_mlp* m, const float* X, int64_t B, int32_t* configs, void* stream);
+/* Use the network's current parameters for EH_TUNE_MLP planning (NULL clears). */
+eh_status eh_planner_set_mlp(eh_mlp* m, void* stream);
+eh_status eh_planner_set_mlp_params(const float* host_params);
+
 /* ---- utilities ----------------------------------------------------------- */
 
 /* Seeded synthetic data on the device: element i from splitmix64 at

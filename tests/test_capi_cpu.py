@@ -130,3 +130,14 @@ def test_sgemm_zero_size_quick_return():
     i = lambda v: C.byref(C.c_int(v))
     one = C.c_float(1.0)
     L.sgemm_(b"N", b"N", i(0), i(3), i(3), C.byref(one), None, i(1), None, i(3), C.byref(one), None, i(1))
+
+
+def test_grouped_rejects_bad_product():
+    # validation happens before any device work: product 0 has lda < m
+    probs = [E.GemmProblem(dtype=0, opa=0, opb=0, m=4, n=4, k=4, lda=2, ldb=4, ldc=4, A=1, B=1, C=1, alpha=1.0,
+                           beta=0.0)]
+    arr = (E.GemmProblem * 1)(*probs)
+    h = C.c_void_p()
+    assert E.lib().eh_grouped_create(arr, 1, C.byref(h)) == 1
+    assert "product 0" in E.lib().eh_last_error().decode()
+    assert E.lib().eh_grouped_create(arr, 0, C.byref(h)) == 1

@@ -1,0 +1,102 @@
+"""Grouped persistent kernel vs the oracle (runs on an MI355X).
+
+Every product of a batch is checked on its own against the long-double
+oracle: bit-exact for small-integer operands (all partial sums exact), within
+the normwise bound of test/product_test_helpers.h for random operands.  The
+same batch is run through the persistent kernel, the hipGraph replay (several
+replays with eager work and synchronisation in between) and the
+one-launch-per-product path.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import eigen_amd as E
+import oracle as O
+from eigen_amd.batched import log_uniform_dims, operand, uneven_batch
+
+pytestmark = pytest.mark.gpu
+
+
+def _check_exact(batch, idx, alpha=1.0, beta=0.0, C0=None):
+    for p in idx:
+        A, B, Cg = (operand(batch, p, w) for w in range(3))
+        want = O.gemm_ref(False, False, alpha, A, B, beta, None if C0 is None else C0[p])
+        assert np.array_equal(Cg.astype(np.float64), want), (p, batch.dims[p], batch.dtypes[p])
+
+
+def test_grouped_exact_mixed_batch(gpu):
+    import torch
+    dims = log_uniform_dims(300, seed=3, lo=1, hi=90)
+    dtypes = np.where(np.arange(300) % 3 == 0, E.F64, E.F32)
+    b = uneven_batch(dims=dims, dtypes=dtypes, dist=E.DIST_INT, p0=-8, p1=8)
+    g = E.Grouped(b.problems)
+    tiles, prods = g.info()
+    assert prods == 300 and tiles == int(np.sum(((dims[:, 0] + 31) // 32) * ((dims[:, 1] + 31) // 32)))
+    g.run()
+    torch.cuda.synchronize()
+    _check_exact(b, range(300))
+    # hipGraph replays, with eager work in between: every replay must be right
+    for rep in range(3):
+        for p in range(0, 300, 7):  # clobber some outputs
+            off = int(b.offsets[p, 2])
+            m, n = int(dims[p, 0]), int(dims[p, 1])
+            b.buffers[int(dtypes[p])][off: off + m * n].fill_(float("nan"))
+        torch.cuda.synchronize()
+        g.graph_launch()
+        torch.cuda.synchronize()
+        _check_exact(b, range(0, 300, 7))
+    # the per-product reference path gives the same bits
+    for p in range(0, 300, 5):
+        off = int(b.offsets[p, 2])
+        b.buffers[int(dtypes[p])][off: off + int(dims[p, 0] * dims[p, 1])].fill_(float("nan"))
+    g.run_per_product()
+    torch.cuda.synchronize()
+    _check_exact(b, range(0, 300, 5))
+    g.close()
+
+
+def test_grouped_alpha_beta_and_transposes(gpu):
+    import torch
+    rng = np.random.default_rng(5)
+    probs, host = [], []
+    bufs = []
+    for p in range(40):
+        dt = E.F32 if p % 2 else E.F64
+        npdt = np.float32 if dt == E.F32 else np.float64
+        m, n, k = (int(x) for x in rng.integers(1, 70, 3))
+        ta, tb = bool(p & 2), bool(p & 4)
+        A = O.fill((k, m) if ta else (m, k), npdt, 100 + p, O.INT, -8, 8)
+        B = O.fill((n, k) if tb else (k, n), npdt, 200 + p, O.INT, -8, 8)
+        C0 = O.fill((m, n), npdt, 300 + p, O.INT, -8, 8)
+        alpha, beta = [(1.0, 0.0), (2.0, -1.0), (0.0, 0.5), (-0.5, 1.0)][p % 4]
+        dA, dB, dC = (torch.from_numpy(x.reshape(-1, order="F").copy()).cuda() for x in (A, B, C0))
+        bufs += [dA, dB, dC]
+        probs.append(E.GemmProblem(dtype=dt, opa=int(ta), opb=int(tb), m=m, n=n, k=k, lda=A.shape[0],
+                                   ldb=B.shape[0], ldc=m, A=dA.data_ptr(), B=dB.data_ptr(), C=dC.data_ptr(),
+                                   alpha=alpha, beta=beta))
+        host.append((A, B, C0, ta, tb, alpha, beta, dC, (m, n)))
+    g = E.Grouped(probs)
+    g.graph_launch()
+    torch.cuda.synchronize()
+    for A, B, C0, ta, tb, alpha, beta, dC, (m, n) in host:
+        want = O.gemm_ref(ta, tb, alpha, A, B, beta, C0)
+        got = dC.cpu().numpy().reshape((n, m)).T
+        assert np.array_equal(got.astype(np.float64), want)
+    g.close()
+
+
+def test_grouped_random_within_bound(gpu):
+    import torch
+    b = uneven_batch(count=400, seed=11)  # the batched workload's distribution, smaller count
+    g = E.Grouped(b.problems)
+    g.graph_launch()
+    torch.cuda.synchronize()
+    for p in range(0, 400, 9):
+        A, B, Cg = (operand(b, p, w) for w in range(3))
+        k = A.shape[1]
+        ref, absref = O.gemm_ref(False, False, 1.0, A, B, with_abs=True)
+        u = np.finfo(A.dtype).eps / 2
+        assert np.linalg.norm(Cg.astype(np.float64) - ref) <= O.higham_mary_bound(absref, k, u) + 1e-300
+    g.close()
