@@ -74,14 +74,15 @@ __global__ void mlp_output_kernel(const float* __restrict__ P, const float* __re
   if (o < kOut && Prob) Prob[b * kOut + o] = expf(z[o] - mx) / sum;
 }
 
-// dZ = (p - q) / B  and per-sample loss -sum q log p
+// dZ = (p - q) / B and the per-element loss terms -q log p / B (summed to the
+// batch mean by mlp_sum_kernel)
 __global__ void mlp_dz_kernel(const float* __restrict__ Prob, const float* __restrict__ Q, int64_t B,
                               float* __restrict__ dZ, float* __restrict__ loss_parts) {
   const int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (idx >= B * kOut) return;
   const float p = Prob[idx], q = Q[idx];
   dZ[idx] = (p - q) / (float)B;
-  loss_parts[idx] = q > 0.0f ? -q * logf(fmaxf(p, 1e-30f)) : 0.0f;
+  loss_parts[idx] = q > 0.0f ? -q * logf(fmaxf(p, 1e-30f)) / (float)B : 0.0f;
 }
 
 // dH = (W2^T dZ) * [h > 0]

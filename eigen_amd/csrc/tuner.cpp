@@ -33,11 +33,28 @@ static ShapeKey key_of(int dtype, int opa, int opb, int64_t m, int64_t n, int64_
 
 // Median time of `reps` runs of configuration `config` (after one warm-up) on
 // the caller's buffers, C = A*B with beta = 0.
+// One stream for all timing runs (the split-K workspace is cached per stream).
+static eh_status tuner_stream(hipStream_t* s) {
+  static hipStream_t stream = nullptr;
+  static std::mutex mu;
+  std::lock_guard<std::mutex> lk(mu);
+  if (!stream) {
+    hipError_t e = hipStreamCreateWithFlags(&stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+      stream = nullptr;
+      return hip_fail(e, "tuner stream");
+    }
+  }
+  *s = stream;
+  return EH_SUCCESS;
+}
+
 static eh_status time_config(int dtype, int opa, int opb, int64_t m, int64_t n, int64_t k, const void* A,
                              const void* B, void* C, int config, int reps, float* ms) {
   hipStream_t s;
-  hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
-  if (e != hipSuccess) return hip_fail(e, "tuner stream");
+  eh_status sst = tuner_stream(&s);
+  if (sst != EH_SUCCESS) return sst;
+  hipError_t e;
   hipEvent_t e0, e1;
   hipEventCreate(&e0);
   hipEventCreate(&e1);
@@ -66,7 +83,6 @@ static eh_status time_config(int dtype, int opa, int opb, int64_t m, int64_t n, 
   }
   hipEventDestroy(e0);
   hipEventDestroy(e1);
-  hipStreamDestroy(s);
   if (st != EH_SUCCESS) return st;
   std::sort(t.begin(), t.end());
   *ms = t.empty() ? 0.0f : t[t.size() / 2];

@@ -1,0 +1,164 @@
+"""The learned tile / schedule selector: training and evaluation.
+
+The network (7 -> 64 ReLU -> 24 softmax, FP32) lives in libeigen_amd and is
+trained on the device by the library's own HIP kernels (eh_mlp_train:
+forward, back-propagation, Adam).  This module prepares its data from the
+autotuner's timing table and measures it:
+
+* targets: soft labels q_c proportional to exp(-(t_c / t_best - 1) / tau)
+  over the 24 measured configurations of each shape;
+* regret of a choice c on a shape: t_c / t_best - 1 against the exhaustive
+  sweep (the judge);
+* the method without the network is the library's closed-form heuristic
+  (eh_plan in TUNE_HEURISTIC mode).
+
+    python -m eigen_amd.selector --db tune.csv --out selector.json
+"""
+from __future__ import annotations
+
+import argparse
+import csv
+import ctypes as C
+import json
+import sys
+
+import numpy as np
+
+from . import TUNE_HEURISTIC, check, lib, plan, set_tune_mode
+
+N_TRAIN = 4000
+
+
+def load_db(path: str):
+    """Returns (shapes (S,6) int64: dtype, ta, tb, m, n, k;  times (S,24) float64 ms)."""
+    rows = []
+    with open(path) as f:
+        r = csv.reader(f)
+        next(r)
+        for row in r:
+            if row:
+                rows.append(row)
+    rows.sort(key=lambda x: int(x[0]))
+    shapes = np.array([[int(v) for v in row[1:7]] for row in rows], dtype=np.int64)
+    times = np.array([[float(v) for v in row[7:]] for row in rows], dtype=np.float64)
+    return shapes, times
+
+
+def features(shapes: np.ndarray) -> np.ndarray:
+    L = lib()
+    X = np.empty((len(shapes), 7), dtype=np.float32)
+    for i, (d, ta, tb, m, n, k) in enumerate(shapes.tolist()):
+        check(L.eh_mlp_features(d, ta, tb, m, n, k, X[i].ctypes.data_as(C.c_void_p)), "eh_mlp_features")
+    return X
+
+
+def soft_targets(times: np.ndarray, tau: float = 0.05) -> np.ndarray:
+    rel = times / times.min(axis=1, keepdims=True) - 1.0
+    w = np.exp(-rel / tau)
+    return (w / w.sum(axis=1, keepdims=True)).astype(np.float32)
+
+
+def regret(choice: np.ndarray, times: np.ndarray) -> np.ndarray:
+    return times[np.arange(len(times)), choice] / times.min(axis=1) - 1.0
+
+
+def summary(r: np.ndarray) -> dict:
+    return {"mean": float(r.mean()), "median": float(np.median(r)), "p90": float(np.quantile(r, 0.9)),
+            "max": float(r.max()), "geomean_slowdown": float(np.exp(np.mean(np.log1p(r)))) - 1.0}
+
+
+def heuristic_choices(shapes: np.ndarray) -> np.ndarray:
+    set_tune_mode(TUNE_HEURISTIC)
+    return np.array([plan(d, ta, tb, m, n, k) for d, ta, tb, m, n, k in shapes.tolist()], dtype=np.int64)
+
+
+class Selector:
+    """A device-resident network (handle of eh_mlp_create)."""
+
+    def __init__(self, capacity: int = 8192):
+        h = C.c_void_p()
+        check(lib().eh_mlp_create(7, 64, 24, capacity, C.byref(h)), "eh_mlp_create")
+        self.h = h
+        self.n_params = int(lib().eh_mlp_num_params(h))
+
+    def params(self) -> np.ndarray:
+        P = np.empty(self.n_params, dtype=np.float32)
+        check(lib().eh_mlp_get_params(self.h, P.ctypes.data_as(C.c_void_p)), "eh_mlp_get_params")
+        return P
+
+    def set_params(self, P) -> None:
+        P = np.ascontiguousarray(P, dtype=np.float32)
+        check(lib().eh_mlp_set_params(self.h, P.ctypes.data_as(C.c_void_p)), "eh_mlp_set_params")
+
+    def train(self, X: np.ndarray, Q: np.ndarray, epochs: int = 3000, lr: float = 1e-2, wd: float = 1e-4):
+        import torch
+        dX = torch.from_numpy(np.ascontiguousarray(X, np.float32)).cuda()
+        dQ = torch.from_numpy(np.ascontiguousarray(Q, np.float32)).cuda()
+        curve = torch.empty(epochs, dtype=torch.float32, device="cuda")
+        s = torch.cuda.current_stream().cuda_stream
+        check(lib().eh_mlp_train(self.h, dX.data_ptr(), dQ.data_ptr(), len(X), epochs, lr, wd, curve.data_ptr(), s),
+              "eh_mlp_train")
+        torch.cuda.synchronize()
+        return curve.cpu().numpy()
+
+    def predict(self, X: np.ndarray) -> np.ndarray:
+        import torch
+        dX = torch.from_numpy(np.ascontiguousarray(X, np.float32)).cuda()
+        out = torch.empty(len(X), dtype=torch.int32, device="cuda")
+        s = torch.cuda.current_stream().cuda_stream
+        check(lib().eh_mlp_predict(self.h, dX.data_ptr(), len(X), out.data_ptr(), s), "eh_mlp_predict")
+        torch.cuda.synchronize()
+        return out.cpu().numpy().astype(np.int64)
+
+    def install(self) -> None:
+        """Make this network the planner's selector (TUNE_MLP mode)."""
+        import torch
+        check(lib().eh_planner_set_mlp(self.h, torch.cuda.current_stream().cuda_stream), "eh_planner_set_mlp")
+
+    def close(self) -> None:
+        if self.h:
+            lib().eh_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def train_and_evaluate(db: str, epochs: int = 3000, lr: float = 1e-2, wd: float = 1e-4, tau: float = 0.05):
+    shapes, times = load_db(db)
+    tr, te = slice(0, N_TRAIN), slice(N_TRAIN, len(shapes))
+    X = features(shapes)
+    Q = soft_targets(times, tau)
+    sel = Selector(capacity=len(shapes))
+    curve = sel.train(X[tr], Q[tr], epochs=epochs, lr=lr, wd=wd)
+    pred = sel.predict(X[te])
+    heur = heuristic_choices(shapes[te])
+    res = {"train_shapes": int(shapes[tr].shape[0]), "heldout_shapes": int(shapes[te].shape[0]),
+           "loss_first": float(curve[0]), "loss_last": float(curve[-1]),
+           "mlp_regret": summary(regret(pred, times[te])),
+           "heuristic_regret": summary(regret(heur, times[te])),
+           "mlp_train_regret": summary(regret(sel.predict(X[tr]), times[tr]))}
+    return sel, res
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--db", required=True)
+    ap.add_argument("--out", required=True)
+    ap.add_argument("--epochs", type=int, default=3000)
+    ap.add_argument("--lr", type=float, default=1e-2)
+    ap.add_argument("--wd", type=float, default=1e-4)
+    args = ap.parse_args(argv)
+    sel, res = train_and_evaluate(args.db, args.epochs, args.lr, args.wd)
+    res["params"] = [float(v) for v in sel.params()]
+    with open(args.out, "w") as f:
+        json.dump(res, f)
+    print(json.dumps({k: v for k, v in res.items() if k != "params"}), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

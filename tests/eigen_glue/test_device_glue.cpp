@@ -1,0 +1,125 @@
+// Products on device Maps through Eigen's own device seam:
+//   C.device(dev).noalias() {=, +=, -=} expr
+// Each case runs the same expression on host matrices with Eigen's CPU path
+// and compares bit for bit (small-integer operands).
+#include "../../include/eigen_amd_eigen.h"
+#include "common.hpp"
+
+using Eigen::Dynamic;
+using Eigen::Index;
+using Eigen::OuterStride;
+
+template <typename T, int Order = Eigen::ColMajor>
+using Mat = Eigen::Matrix<T, Dynamic, Dynamic, Order>;
+template <typename T, int Order = Eigen::ColMajor>
+using DMap = Eigen::Map<Mat<T, Order>, 0, OuterStride<>>;
+
+template <typename T, int Order>
+struct Dev {
+  Mat<T, Order> host;
+  Index ld;
+  DeviceBuffer<T> buf;
+  Dev(Index r, Index c, unsigned seed, Index pad = 0)
+      : host(r, c), ld((Order == Eigen::ColMajor ? r : c) + pad), buf((size_t)ld * (Order == Eigen::ColMajor ? c : r)) {
+    fill_int(host, seed);
+    std::vector<T> h(buf.n, T(-99));
+    for (Index j = 0; j < c; ++j)
+      for (Index i = 0; i < r; ++i) h[Order == Eigen::ColMajor ? i + j * ld : j + i * ld] = host(i, j);
+    buf.upload(h.data());
+  }
+  DMap<T, Order> map() { return DMap<T, Order>(buf.ptr, host.rows(), host.cols(), OuterStride<>(ld)); }
+  Mat<T, Order> fetch() const {
+    std::vector<T> h(buf.n);
+    buf.download(h.data());
+    Mat<T, Order> out(host.rows(), host.cols());
+    for (Index j = 0; j < host.cols(); ++j)
+      for (Index i = 0; i < host.rows(); ++i) out(i, j) = h[Order == Eigen::ColMajor ? i + j * ld : j + i * ld];
+    return out;
+  }
+};
+
+template <typename T>
+int run_all(const char* tname) {
+  int fails = 0;
+  eigen_amd::HipDevice dev;
+  char name[128];
+  const Index m = 70, n = 45, k = 33;
+  auto tag = [&](const char* what) {
+    std::snprintf(name, sizeof name, "%s/%s", tname, what);
+    return name;
+  };
+  {  // =
+    Dev<T, 0> A(m, k, 1), B(k, n, 2), C(m, n, 3);
+    auto c = C.map();
+    c.device(dev).noalias() = A.map() * B.map();
+    Mat<T> want = A.host * B.host;
+    fails += report(tag("assign"), C.fetch() == want && dev.status == EH_SUCCESS);
+  }
+  {  // += alpha * A * B
+    Dev<T, 0> A(m, k, 4), B(k, n, 5), C(m, n, 6);
+    auto c = C.map();
+    c.device(dev).noalias() += T(2) * A.map() * B.map();
+    Mat<T> want = C.host + T(2) * A.host * B.host;
+    fails += report(tag("add_scaled"), C.fetch() == want);
+  }
+  {  // -= A^T * B with A stored k x m, padded strides
+    Dev<T, 0> A(k, m, 7, 5), B(k, n, 8, 3), C(m, n, 9, 11);
+    auto c = C.map();
+    c.device(dev).noalias() -= A.map().transpose() * B.map();
+    Mat<T> want = C.host - A.host.transpose() * B.host;
+    fails += report(tag("sub_transposed_padded"), C.fetch() == want);
+  }
+  {  // = (A*3)^T * B^T * -1 style factors on both sides
+    Dev<T, 0> A(k, m, 10), B(n, k, 11), C(m, n, 12);
+    auto c = C.map();
+    c.device(dev).noalias() = (A.map() * T(3)).transpose() * (-B.map().transpose());
+    Mat<T> want = (A.host * T(3)).transpose() * (-B.host.transpose());
+    fails += report(tag("factors_and_negation"), C.fetch() == want);
+  }
+  {  // row-major operands and destination
+    Dev<T, Eigen::RowMajor> A(m, k, 13), C(m, n, 15);
+    Dev<T, 0> B(k, n, 14);
+    auto c = C.map();
+    c.device(dev).noalias() += A.map() * B.map();
+    Mat<T, Eigen::RowMajor> want = C.host + A.host * B.host;
+    fails += report(tag("row_major"), C.fetch() == want);
+  }
+  {  // blocks of device maps
+    Dev<T, 0> A(m, k, 16), B(k, n, 17), C(m, n, 18);
+    auto c = C.map();
+    c.block(3, 4, 40, 30).device(dev).noalias() = A.map().block(5, 2, 40, 20) * B.map().block(1, 6, 20, 30);
+    Mat<T> want = C.host;
+    want.block(3, 4, 40, 30) = A.host.block(5, 2, 40, 20) * B.host.block(1, 6, 20, 30);
+    fails += report(tag("blocks"), C.fetch() == want);
+  }
+  {  // matrix * vector and tiny products go to the device as well
+    Dev<T, 0> A(m, k, 19), x(k, 1, 20), y(m, 1, 21);
+    auto yv = y.map();
+    yv.device(dev).noalias() += A.map() * x.map();
+    Mat<T> want = y.host + A.host * x.host;
+    fails += report(tag("gemv_shape"), y.fetch() == want);
+    Dev<T, 0> a(2, 3, 22), b(3, 1, 23), cc(2, 1, 24);
+    auto cm = cc.map();
+    cm.device(dev).noalias() = a.map() * b.map();
+    Mat<T> want2 = a.host * b.host;
+    fails += report(tag("tiny"), cc.fetch() == want2);
+  }
+  {  // split-BF16 precision keeps integer products exact
+    if (std::is_same<T, float>::value) {
+      eigen_amd::HipDevice d3;
+      d3.precision = EH_PREC_SPLIT3;
+      Dev<T, 0> A(m, k, 25), B(k, n, 26), C(m, n, 27);
+      auto c = C.map();
+      c.device(d3).noalias() = A.map() * B.map();
+      Mat<T> want = A.host * B.host;
+      fails += report(tag("split3"), C.fetch() == want);
+    }
+  }
+  return fails;
+}
+
+int main() {
+  int fails = run_all<float>("float") + run_all<double>("double");
+  std::printf("%s (%d failures)\n", fails ? "SOME FAILED" : "ALL PASSED", fails);
+  return fails ? 1 : 0;
+}
