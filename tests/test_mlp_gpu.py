@@ -42,7 +42,8 @@ def test_forward_matches_oracle(gpu):
     s = Selector(capacity=64)
     s.set_params(P)
     probs = torch.empty((53, 24), dtype=torch.float32, device="cuda")
-    E.check(E.lib().eh_mlp_forward(s.h, _dev(X).data_ptr(), 53, probs.data_ptr(), None))
+    dX = _dev(X)  # kept alive until the kernels have run
+    E.check(E.lib().eh_mlp_forward(s.h, dX.data_ptr(), 53, probs.data_ptr(), None))
     torch.cuda.synchronize()
     want = M.forward(P.astype(np.float64), X.astype(np.float64))[2]
     assert np.max(np.abs(probs.cpu().numpy() - want)) < 2e-6
@@ -61,8 +62,8 @@ def test_backward_matches_oracle_and_finite_differences(gpu):
     s.set_params(P)
     g = torch.empty(M.N_PARAMS, dtype=torch.float32, device="cuda")
     loss = torch.empty(1, dtype=torch.float32, device="cuda")
-    E.check(E.lib().eh_mlp_backward(s.h, _dev(X).data_ptr(), _dev(Q).data_ptr(), 41, g.data_ptr(), loss.data_ptr(),
-                                    None))
+    dX, dQ = _dev(X), _dev(Q)  # kept alive until the kernels have run
+    E.check(E.lib().eh_mlp_backward(s.h, dX.data_ptr(), dQ.data_ptr(), 41, g.data_ptr(), loss.data_ptr(), None))
     torch.cuda.synchronize()
     P64, X64, Q64 = P.astype(np.float64), X.astype(np.float64), Q.astype(np.float64)
     want = M.backward(P64, X64, Q64)
@@ -81,7 +82,8 @@ def test_adam_step_matches_oracle(gpu):
     s = Selector(capacity=16)
     s.set_params(P)
     G = np.random.default_rng(4).normal(0, 1e-2, M.N_PARAMS).astype(np.float32)
-    E.check(E.lib().eh_mlp_adam_step(s.h, _dev(G).data_ptr(), 0.01, 0.9, 0.999, 1e-8, 1e-3, None))
+    dG = _dev(G)
+    E.check(E.lib().eh_mlp_adam_step(s.h, dG.data_ptr(), 0.01, 0.9, 0.999, 1e-8, 1e-3, None))
     torch.cuda.synchronize()
     want, _, _ = M.adam_step(P.astype(np.float64), G.astype(np.float64), np.zeros(M.N_PARAMS), np.zeros(M.N_PARAMS),
                              1, lr=0.01, wd=1e-3)
