@@ -58,7 +58,7 @@ def build(verbose: bool = False) -> None:
     os.replace(libpath + ".tmp", libpath)
     blaspath = os.path.join(LIB, "libeigen_amd_blas.so")
     subprocess.run([HIPCC, "-shared", "-fPIC", "-o", blaspath + ".tmp", objs["blas.cpp"], "-L" + LIB,
-                    "-leigen_amd", "-Wl,-rpath,$ORIGIN", "-Wl,-Bsymbolic"], check=True)
+                    "-leigen_amd", "-Wl,-rpath,$ORIGIN"], check=True)  # xerbla_ stays interposable
     os.replace(blaspath + ".tmp", blaspath)
     if verbose:
         print("built", libpath, blaspath)

@@ -40,6 +40,8 @@ def build(force: bool = False, with_ref: bool | None = None) -> None:
         with_ref = os.path.isdir(os.path.join(REF_TREE, "Eigen"))
     if with_ref:
         subprocess.run(["make", "-C", HERE, "-j8", "ref"], check=True, stdout=subprocess.DEVNULL)
+        if os.path.exists(os.path.join(HERE, "..", "eigen_amd", "_lib", "libeigen_amd_blas.so")):
+            subprocess.run(["make", "-C", HERE, "-B", "netlib"], check=True, stdout=subprocess.DEVNULL)
 
 
 def _cpu_has_avx512() -> bool:
