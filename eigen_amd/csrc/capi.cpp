@@ -172,14 +172,16 @@ static eh_status launch_split_product(const GemmParams<float>& g, bool ta, bool 
                                       hipStream_t stream) {
   const int m = g.M, n = g.N, k = g.K;
   const int KS = split_kslices(k);
+  // 256x128 tiles with 8 waves (two per SIMD) once the grid fills the chip,
+  // 128x128 below that
   int64_t t256 = ((int64_t)(m + 255) / 256) * ((n + 127) / 128);
-  int tile = t256 >= 256 ? 1 : 0;
+  int tile = t256 >= 256 ? 2 : 0;
   int64_t tiles = tile ? t256 : ((int64_t)(m + 127) / 128) * ((n + 127) / 128);
   int splits = 1;
   if (config >= 0) {
     int ct = 0;
     config_decode(config, &ct, &splits);
-    tile = ct >= 5 ? 1 : 0;
+    tile = ct >= 5 ? 1 : (ct == 4 ? 2 : 0);
   } else {
     while (splits < 8 && tiles * splits < 256 && k / (2 * splits) >= 512) splits *= 2;
   }

@@ -69,43 +69,43 @@ __device__ inline void split8(const float (&x)[8], u16x8 (&out)[TERMS]) {
 // (Eigen/src/Core/products/GeneralBlockPanelKernel.h:1893-2478) for the
 // split product: each element is split once here instead of once per tile
 // that stages it, and every k slice of a tile becomes one contiguous block.
-// One thread writes one 16-byte chunk (8 k values of one row) of every plane.
 // KCONT: op(X)(r,k) = X[k + r*ld]; otherwise op(X)(r,k) = X[r + k*ld].
 // ---------------------------------------------------------------------------
 template <int TERMS, bool KCONT>
 __global__ void __launch_bounds__(256) pack_split_kernel(const float* __restrict__ X, int64_t ld, int rows, int K,
                                                          int Rp, int KS, u16x8* __restrict__ P) {
-  const int64_t total = (int64_t)KS * Rp * 4;
-  for (int64_t idx = blockIdx.x * 256ll + threadIdx.x; idx < total; idx += (int64_t)gridDim.x * 256) {
-    int ks, r, c;
-    if constexpr (KCONT) {  // chunk fastest: each row's 128 B of k are read together
-      c = (int)(idx & 3);
-      r = (int)((idx >> 2) % Rp);
-      ks = (int)((idx >> 2) / Rp);
-    } else {  // row fastest: one k column of 64 consecutive rows per wave load
-      r = (int)(idx % Rp);
-      c = (int)((idx / Rp) & 3);
-      ks = (int)((idx / Rp) >> 2);
-    }
-    const int k0 = ks * 32 + c * 8;
+  // A workgroup moves a 64-row x 64-k tile through LDS: reads run along the
+  // contiguous index of X, writes along r (four 16-byte chunks per 64-byte
+  // row, 64 rows contiguous per plane and k slice).
+  constexpr int TR = 64, TK = 64;
+  __shared__ float tile[TR][TK + 1];
+  const int r0 = blockIdx.x * TR, k0 = blockIdx.y * TK;
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int i = 0; i < TR * TK / 256; ++i) {
+    const int q = tid + 256 * i;
+    const int r = KCONT ? q / TK : q % TR;
+    const int k = KCONT ? q % TK : q / TR;
+    const int gr = r0 + r, gk = k0 + k;
+    float v = 0.0f;
+    if (gr < rows && gk < K) v = KCONT ? X[(int64_t)gr * ld + gk] : X[gr + (int64_t)gk * ld];
+    tile[r][k] = v;
+  }
+  __syncthreads();
+  // 2 slices x 64 rows x 4 chunks = 512 chunks: chunk fastest, then row
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const int q = tid + 256 * i;
+    const int c = q & 3, r = (q >> 2) & (TR - 1), sl = q >> 8;
+    const int ks = k0 / 32 + sl;
+    if (ks >= KS) continue;
     float v[8];
-    if (r < rows) {
-      if constexpr (KCONT) {
-        const float* src = X + (int64_t)r * ld + k0;
 #pragma unroll
-        for (int e = 0; e < 8; ++e) v[e] = (k0 + e < K) ? src[e] : 0.0f;
-      } else {
-#pragma unroll
-        for (int e = 0; e < 8; ++e) v[e] = (k0 + e < K) ? X[r + (int64_t)(k0 + e) * ld] : 0.0f;
-      }
-    } else {
-#pragma unroll
-      for (int e = 0; e < 8; ++e) v[e] = 0.0f;
-    }
+    for (int e = 0; e < 8; ++e) v[e] = tile[r][sl * 32 + c * 8 + e];
     u16x8 t[TERMS];
     split8<TERMS>(v, t);
 #pragma unroll
-    for (int s = 0; s < TERMS; ++s) P[(((int64_t)ks * TERMS + s) * Rp + r) * 4 + c] = t[s];
+    for (int s2 = 0; s2 < TERMS; ++s2) P[(((int64_t)ks * TERMS + s2) * Rp + r0 + r) * 4 + c] = t[s2];
   }
 }
 
@@ -299,15 +299,13 @@ __global__ void __launch_bounds__(256) split_bf16_kernel(const float* x, int64_t
 template <int TERMS>
 static hipError_t launch_pack_t(const float* X, int64_t ld, bool kcont, int rows, int K, int Rp, int KS, void* P,
                                 hipStream_t s) {
-  const int64_t total = (int64_t)KS * Rp * 4;
-  int64_t blocks = (total + 255) / 256;
-  if (blocks > 32768) blocks = 32768;
+  dim3 grid(Rp / 64, (K + 63) / 64);
   if (kcont)
-    hipLaunchKernelGGL((pack_split_kernel<TERMS, true>), dim3((unsigned)blocks), dim3(256), 0, s, X, ld, rows, K, Rp,
-                       KS, static_cast<u16x8*>(P));
+    hipLaunchKernelGGL((pack_split_kernel<TERMS, true>), grid, dim3(256), 0, s, X, ld, rows, K, Rp, KS,
+                       static_cast<u16x8*>(P));
   else
-    hipLaunchKernelGGL((pack_split_kernel<TERMS, false>), dim3((unsigned)blocks), dim3(256), 0, s, X, ld, rows, K,
-                       Rp, KS, static_cast<u16x8*>(P));
+    hipLaunchKernelGGL((pack_split_kernel<TERMS, false>), grid, dim3(256), 0, s, X, ld, rows, K, Rp, KS,
+                       static_cast<u16x8*>(P));
   return hipGetLastError();
 }
 
@@ -319,23 +317,26 @@ hipError_t launch_pack_split(const float* X, int64_t ld, bool kcont, int rows, i
   return hipErrorInvalidValue;
 }
 
-template <int BM, int BN, int TERMS, int NSTAGE>
+template <int BM, int BN, int WGM, int WGN, int TERMS, int NSTAGE>
 static hipError_t launch_packed_t(const SplitParams& p, int splits, hipStream_t s) {
   const int ntm = (p.M + BM - 1) / BM, ntn = (p.N + BN - 1) / BN;
-  hipLaunchKernelGGL((gemm_split_packed_kernel<BM, BN, 2, 2, TERMS, NSTAGE>), dim3(ntm * ntn, splits), dim3(256), 0,
-                     s, p);
+  hipLaunchKernelGGL((gemm_split_packed_kernel<BM, BN, WGM, WGN, TERMS, NSTAGE>), dim3(ntm * ntn, splits),
+                     dim3(64 * WGM * WGN), 0, s, p);
   return hipGetLastError();
 }
 
 hipError_t launch_gemm_split_packed(const SplitParams& p, int terms, int tile, int splits, hipStream_t s) {
-  // tile 0: 128x128 (3 stages), 1: 256x128 (2 stages)
+  // tile 0: 128x128, 4 waves (3 stages); 1: 256x128, 4 waves of 128x64 (2 stages);
+  // 2: 256x128, 8 waves of 64x64 -- two waves per SIMD (2 stages)
   if (terms == 3) {
-    if (tile == 0) return launch_packed_t<128, 128, 3, 3>(p, splits, s);
-    return launch_packed_t<256, 128, 3, 2>(p, splits, s);
+    if (tile == 0) return launch_packed_t<128, 128, 2, 2, 3, 3>(p, splits, s);
+    if (tile == 2) return launch_packed_t<256, 128, 4, 2, 3, 2>(p, splits, s);
+    return launch_packed_t<256, 128, 2, 2, 3, 2>(p, splits, s);
   }
   if (terms == 2) {
-    if (tile == 0) return launch_packed_t<128, 128, 2, 3>(p, splits, s);
-    return launch_packed_t<256, 128, 2, 2>(p, splits, s);
+    if (tile == 0) return launch_packed_t<128, 128, 2, 2, 2, 3>(p, splits, s);
+    if (tile == 2) return launch_packed_t<256, 128, 4, 2, 2, 2>(p, splits, s);
+    return launch_packed_t<256, 128, 2, 2, 2, 2>(p, splits, s);
   }
   return hipErrorInvalidValue;
 }
