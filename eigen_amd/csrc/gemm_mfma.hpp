@@ -8,7 +8,8 @@
 //     every LDS row holding 128 contiguous bytes of k for one row of op(A) /
 //     one column of op(B) (an XOR-swizzled, bank-conflict-free packed layout);
 //   * the register-blocked micro-kernel (gebp_micro_panel_impl, :1352-1456)
-//     becomes a 2x2 grid of waves, each issuing RM x RN MFMA micro-tiles;
+//     becomes a WGM x WGN grid of waves (2x2, or 4x2 / 2x4 for two waves per
+//     SIMD), each issuing RM x RN MFMA micro-tiles;
 //   * the store `C += alpha*acc` (:1446-1455) becomes a fused alpha/beta
 //     epilogue (beta == 0 never reads C, matching blas/level3_impl.h:82-87).
 // Transposed operands only change which global index is contiguous; the LDS
