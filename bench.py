@@ -202,7 +202,8 @@ def bench_dgemm(E, torch, dev, args):
         wall, kern = timed(fn, max(3, args.steps // 2), args.warmup)
         tf = 2.0 * m * n * k / (wall * 1e-3) / 1e12
         out[name] = {"ms": wall, "kernel_ms": kern, "tflops": tf, "frac_of_fp64_peak": tf / PEAK["fp64"],
-                     "config": E.plan(E.F64, 0, 0, m, n, k)}
+                     "config": E.plan(E.F64, 0, 0, m, n, k), "config_tile_split": E.config_info(E.plan(E.F64, 0, 0, m, n, k)),
+                     "planner": ["heuristic", "mlp", "table"][E.lib().eh_get_tune_mode()]}
         log(f"{name}: {out[name]}")
         del dA, dB, dC
     return out

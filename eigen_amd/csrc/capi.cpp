@@ -78,8 +78,10 @@ int heuristic_config(int dtype, int64_t m, int64_t n, int64_t k) {
   return tile * kNumSplits + split_index(s);
 }
 
+bool mlp_has_default();
+
 static std::mutex g_mutex;
-static eh_tune_mode g_mode = EH_TUNE_HEURISTIC;
+static int g_mode = -1;  // -1: not yet chosen (EIGEN_AMD_TUNE, else MLP when compiled in)
 static int g_sgemm_prec = -1;  // -1: not yet read from the environment
 
 static eh_precision default_sgemm_precision() {
@@ -357,7 +359,15 @@ eh_status gemm_dispatch(int dtype, int opa, int opb, int64_t m, int64_t n, int64
 
 eh_tune_mode tune_mode() {
   std::lock_guard<std::mutex> lk(g_mutex);
-  return g_mode;
+  if (g_mode < 0) {
+    g_mode = mlp_has_default() ? EH_TUNE_MLP : EH_TUNE_HEURISTIC;
+    if (const char* e = std::getenv("EIGEN_AMD_TUNE")) {
+      if (!std::strcmp(e, "heuristic")) g_mode = EH_TUNE_HEURISTIC;
+      if (!std::strcmp(e, "mlp")) g_mode = EH_TUNE_MLP;
+      if (!std::strcmp(e, "table")) g_mode = EH_TUNE_TABLE;
+    }
+  }
+  return (eh_tune_mode)g_mode;
 }
 
 }  // namespace eh

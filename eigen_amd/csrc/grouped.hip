@@ -9,19 +9,19 @@
 //   1. eh_grouped_create uploads one descriptor per product (ordered by
 //      decreasing depth, longest work first);
 //   2. a build kernel computes each product's 32x32 output-tile count and an
-//      exclusive prefix sum over the products (the work list), and zeroes the
-//      tile counter;
-//   3. the persistent kernel starts one wave per SIMD slot; every wave takes
-//      the next tile index from an atomic counter, finds its product by binary
-//      search in the prefix sums and computes the 32x32 tile over the full
-//      depth with MFMA micro-tiles (v_mfma_f32_32x32x2_f32 for FP32, a 2x2
-//      block of v_mfma_f64_16x16x4_f64 for FP64), reading its operands straight
-//      from HBM (no LDS: one wave owns one tile), and fuses alpha/beta into the
-//      store.  Waves exit when the counter passes the last tile.
+//      exclusive prefix sum over the products (the work list);
+//   3. the persistent kernel starts 8 waves per SIMD; wave w takes tiles
+//      w, w + W, w + 2W, ... of the list (W waves in all: the list is sorted
+//      deepest first, so the round-robin deal balances like longest-first
+//      scheduling, without a shared counter to contend on), finds each tile's
+//      product by binary search in the prefix sums and computes the 32x32 tile
+//      over the full depth with MFMA micro-tiles (v_mfma_f32_32x32x2_f32 for
+//      FP32, a 2x2 block of v_mfma_f64_16x16x4_f64 for FP64), reading its
+//      operands straight from HBM (no LDS: one wave owns one tile), and fuses
+//      alpha/beta into the store.  A wave exits after its last tile.
 //
 // The two launches are captured once in a hipGraph and replayed
-// (eh_grouped_graph_launch).  The build kernel also resets the tile counter
-// (no memset), so the graph holds kernel nodes only.
+// (eh_grouped_graph_launch); the graph holds kernel nodes only.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -48,8 +48,7 @@ struct GroupedDesc {
 
 // Work list: tile_begin[p] = first tile index of product p (exclusive prefix
 // sum of tiles_m * tiles_n), tile_begin[count] = total tiles.
-__global__ void __launch_bounds__(1024) grouped_build_kernel(GroupedDesc* d, int64_t count, int64_t* tile_begin,
-                                                             unsigned int* counter) {
+__global__ void __launch_bounds__(1024) grouped_build_kernel(GroupedDesc* d, int64_t count, int64_t* tile_begin) {
   __shared__ int64_t partial[1024];
   const int tid = threadIdx.x;
   // each thread owns a contiguous run of products
@@ -77,7 +76,6 @@ __global__ void __launch_bounds__(1024) grouped_build_kernel(GroupedDesc* d, int
     run += (int64_t)d[p].tiles_m * d[p].tiles_n;
   }
   if (tid == 1023) tile_begin[count] = partial[1023];
-  if (tid == 0) *counter = 0u;
 }
 
 template <typename T>
@@ -169,28 +167,25 @@ __device__ void grouped_tile_f64(const GroupedDesc& g, int tm, int tn, int lane)
   }
 }
 
-// Persistent kernel: every wave loops over tiles taken from the counter.
+// Persistent kernel: wave w computes tiles w, w + W, w + 2W, ...
 __global__ void __launch_bounds__(256) grouped_gemm_kernel(const GroupedDesc* __restrict__ d, int64_t count,
-                                                           const int64_t* __restrict__ tile_begin,
-                                                           unsigned int* counter) {
+                                                           const int64_t* __restrict__ tile_begin) {
   const int lane = threadIdx.x & 63;
   const int64_t total = tile_begin[count];
-  while (true) {
-    unsigned int t = 0;
-    if (lane == 0) t = atomicAdd(counter, 1u);
-    t = __builtin_amdgcn_readfirstlane(t);
-    if ((int64_t)t >= total) break;
+  const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
+  const int64_t first = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  for (int64_t t = first; t < total; t += waves) {
     // product p with tile_begin[p] <= t < tile_begin[p+1]
     int64_t lo = 0, hi = count - 1;
     while (lo < hi) {
       const int64_t mid = (lo + hi + 1) >> 1;
-      if (tile_begin[mid] <= (int64_t)t)
+      if (tile_begin[mid] <= t)
         lo = mid;
       else
         hi = mid - 1;
     }
     const GroupedDesc g = d[lo];
-    const int local = (int)((int64_t)t - tile_begin[lo]);
+    const int local = (int)(t - tile_begin[lo]);
     const int tm = local % g.tiles_m, tn = local / g.tiles_m;
     if (g.dtype == EH_F32)
       grouped_tile_f32(g, tm, tn, lane);
@@ -207,7 +202,6 @@ struct eh_grouped {
   std::vector<eh_gemm_problem> host;  // as given (for the per-product path)
   GroupedDesc* d_desc = nullptr;
   int64_t* d_begin = nullptr;
-  unsigned int* d_counter = nullptr;
   int64_t count = 0;
   int64_t tiles = 0;
   hipGraphExec_t exec = nullptr;
@@ -222,13 +216,12 @@ static eh_status grouped_enqueue(eh_grouped* g, hipStream_t s) {
     if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0)
       cus = prop.multiProcessorCount;
   }
-  hipLaunchKernelGGL(grouped_build_kernel, dim3(1), dim3(1024), 0, s, g->d_desc, g->count, g->d_begin, g->d_counter);
+  hipLaunchKernelGGL(grouped_build_kernel, dim3(1), dim3(1024), 0, s, g->d_desc, g->count, g->d_begin);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return hip_fail(e, "grouped build launch");
   // 8 waves per SIMD: every wave is independent (no LDS, no barrier)
   const int64_t want = std::max<int64_t>(1, std::min<int64_t>((int64_t)cus * 8, (g->tiles + 3) / 4));
-  hipLaunchKernelGGL(grouped_gemm_kernel, dim3((unsigned)want), dim3(256), 0, s, g->d_desc, g->count, g->d_begin,
-                     g->d_counter);
+  hipLaunchKernelGGL(grouped_gemm_kernel, dim3((unsigned)want), dim3(256), 0, s, g->d_desc, g->count, g->d_begin);
   e = hipGetLastError();
   return e == hipSuccess ? EH_SUCCESS : hip_fail(e, "grouped kernel launch");
 }
@@ -267,7 +260,7 @@ eh_status eh_grouped_create(const eh_gemm_problem* problems, int64_t count, eh_g
     desc.push_back(d);
     tiles += ((q.m + 31) / 32) * ((q.n + 31) / 32);
   }
-  // longest products first: the atomic counter then hands out the deep tiles early
+  // longest products first: the round-robin deal then balances like longest-first scheduling
   std::stable_sort(desc.begin(), desc.end(), [](const GroupedDesc& x, const GroupedDesc& y) {
     return (int64_t)x.k * (x.dtype == EH_F64 ? 2 : 1) > (int64_t)y.k * (y.dtype == EH_F64 ? 2 : 1);
   });
@@ -281,7 +274,6 @@ eh_status eh_grouped_create(const eh_gemm_problem* problems, int64_t count, eh_g
   }
   hipError_t e = hipMalloc(&g->d_desc, sizeof(GroupedDesc) * g->count);
   if (e == hipSuccess) e = hipMalloc(&g->d_begin, sizeof(int64_t) * (g->count + 1));
-  if (e == hipSuccess) e = hipMalloc(&g->d_counter, sizeof(unsigned int));
   if (e == hipSuccess) e = hipMemcpy(g->d_desc, desc.data(), sizeof(GroupedDesc) * g->count, hipMemcpyHostToDevice);
   if (e != hipSuccess) {
     eh_grouped_destroy(g);
@@ -360,7 +352,6 @@ eh_status eh_grouped_destroy(eh_grouped* g) {
   if (g->graph) hipGraphDestroy(g->graph);
   if (g->d_desc) hipFree(g->d_desc);
   if (g->d_begin) hipFree(g->d_begin);
-  if (g->d_counter) hipFree(g->d_counter);
   delete g;
   return EH_SUCCESS;
 }

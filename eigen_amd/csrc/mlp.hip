@@ -208,10 +208,22 @@ static eh_status mlp_backward_dev(eh_mlp* m, const float* X, const float* Q, int
   return e == hipSuccess ? EH_SUCCESS : hip_fail(e, "mlp backward launch");
 }
 
+#if __has_include("selector_weights.inc")
+#include "selector_weights.inc"
+#define EH_HAVE_DEFAULT_SELECTOR 1
+#endif
+
 namespace eh {
-// The planner's network (host copy of the parameters); null = none set.
+// The planner's network (host copy of the parameters); empty = none set.
+// Starts as the compiled-in network trained on this device's timing table.
 static std::mutex g_mlp_mutex;
+#ifdef EH_HAVE_DEFAULT_SELECTOR
+static std::vector<float> g_planner_params(kDefaultSelectorParams, kDefaultSelectorParams + kNumParams);
+bool mlp_has_default() { return true; }
+#else
 static std::vector<float> g_planner_params;
+bool mlp_has_default() { return false; }
+#endif
 
 // Feature vector of a product shape (the network's input).
 void mlp_features(int dtype, int opa, int opb, int64_t m, int64_t n, int64_t k, float* x) {
@@ -414,6 +426,7 @@ eh_status eh_mlp_predict(eh_mlp* m, const float* X, int64_t B, int32_t* configs,
 
 // Install the network as the planner's selector (its current parameters are
 // copied; later training does not change the planner until set again).
+// NULL restores the compiled-in default network (if the build has one).
 eh_status eh_planner_set_mlp(eh_mlp* m, void* stream) {
   std::vector<float> p(kNumParams);
   if (m) {
@@ -422,21 +435,22 @@ eh_status eh_planner_set_mlp(eh_mlp* m, void* stream) {
     if (e == hipSuccess) e = hipStreamSynchronize(static_cast<hipStream_t>(stream));
     if (e != hipSuccess) return hip_fail(e, "mlp params to host");
   }
-  std::lock_guard<std::mutex> lk(g_mlp_mutex);
-  if (m)
-    g_planner_params.swap(p);
-  else
-    g_planner_params.clear();
-  return EH_SUCCESS;
+  return eh_planner_set_mlp_params(m ? p.data() : nullptr);
 }
 
-// Same from a host parameter array (no device work).
+// Same from a host parameter array (no device work); NULL restores the
+// compiled-in default.
 eh_status eh_planner_set_mlp_params(const float* host_params) {
   std::lock_guard<std::mutex> lk(g_mlp_mutex);
-  if (host_params)
+  if (host_params) {
     g_planner_params.assign(host_params, host_params + kNumParams);
-  else
+  } else {
+#ifdef EH_HAVE_DEFAULT_SELECTOR
+    g_planner_params.assign(kDefaultSelectorParams, kDefaultSelectorParams + kNumParams);
+#else
     g_planner_params.clear();
+#endif
+  }
   return EH_SUCCESS;
 }
 

@@ -20,6 +20,7 @@ import argparse
 import csv
 import ctypes as C
 import json
+import os
 import sys
 
 import numpy as np
@@ -68,8 +69,12 @@ def summary(r: np.ndarray) -> dict:
 
 
 def heuristic_choices(shapes: np.ndarray) -> np.ndarray:
+    prev = lib().eh_get_tune_mode()
     set_tune_mode(TUNE_HEURISTIC)
-    return np.array([plan(d, ta, tb, m, n, k) for d, ta, tb, m, n, k in shapes.tolist()], dtype=np.int64)
+    try:
+        return np.array([plan(d, ta, tb, m, n, k) for d, ta, tb, m, n, k in shapes.tolist()], dtype=np.int64)
+    finally:
+        set_tune_mode(prev)
 
 
 class Selector:
@@ -144,14 +149,35 @@ def train_and_evaluate(db: str, epochs: int = 3000, lr: float = 1e-2, wd: float 
     return sel, res
 
 
+def emit_inc(params, path: str, source: str) -> None:
+    """Write the parameters as the C array the library compiles in as its default selector."""
+    vals = [float(v) for v in params]
+    assert len(vals) == 2072
+    with open(path, "w") as f:
+        f.write("// Default tile / schedule selector parameters (7 -> 64 -> 24 network), generated by\n")
+        f.write(f"// `python -m eigen_amd.selector --emit-inc` from {source}.\n")
+        f.write("static const float kDefaultSelectorParams[2072] = {\n")
+        for i in range(0, len(vals), 6):
+            f.write("    " + ", ".join(f"{v:.9g}f" for v in vals[i:i + 6]) + ",\n")
+        f.write("};\n")
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
-    ap.add_argument("--db", required=True)
-    ap.add_argument("--out", required=True)
+    ap.add_argument("--db")
+    ap.add_argument("--out")
+    ap.add_argument("--emit-inc", nargs=2, metavar=("JSON", "INC"),
+                    help="write the parameters of a trained JSON as the library's compiled-in default")
     ap.add_argument("--epochs", type=int, default=3000)
     ap.add_argument("--lr", type=float, default=1e-2)
     ap.add_argument("--wd", type=float, default=1e-4)
     args = ap.parse_args(argv)
+    if args.emit_inc:
+        src, inc = args.emit_inc
+        emit_inc(json.load(open(src))["params"], inc, os.path.basename(src))
+        return 0
+    if not args.db or not args.out:
+        ap.error("--db and --out are required for training")
     sel, res = train_and_evaluate(args.db, args.epochs, args.lr, args.wd)
     res["params"] = [float(v) for v in sel.params()]
     with open(args.out, "w") as f:

@@ -68,7 +68,9 @@ typedef enum eh_precision {
   EH_PREC_SPLIT3 = 3  /* 3 bf16 terms, 6 cross products on BF16 MFMA */
 } eh_precision;
 
-/* Tile / schedule selection for the plain kernels. */
+/* Tile / schedule selection for the plain kernels.  Default: MLP when the
+ * library was built with a trained selector (eigen_amd/csrc/selector_weights.inc),
+ * else HEURISTIC; EIGEN_AMD_TUNE=heuristic|mlp|table overrides. */
 typedef enum eh_tune_mode {
   EH_TUNE_HEURISTIC = 0, /* closed-form rule (the no-MLP method) */
   EH_TUNE_MLP = 1,       /* trained selector (falls back to HEURISTIC without weights) */
@@ -179,7 +181,8 @@ eh_status eh_mlp_train(eh_mlp* m, const float* X, const float* Q, int64_t B, int
                        float weight_decay, float* loss_curve, void* stream);
 /* Device X[B][7] -> device configs[B] (argmax). */
 eh_status eh_mlp_predict(eh_mlp* m, const float* X, int64_t B, int32_t* configs, void* stream);
-/* Use the network's current parameters for EH_TUNE_MLP planning (NULL clears). */
+/* Use the network's current parameters for EH_TUNE_MLP planning (NULL
+ * restores the compiled-in default network, or none). */
 eh_status eh_planner_set_mlp(eh_mlp* m, void* stream);
 eh_status eh_planner_set_mlp_params(const float* host_params);
 

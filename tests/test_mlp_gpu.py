@@ -111,6 +111,7 @@ def test_planner_uses_installed_network(gpu):
     s = Selector(capacity=8)
     s.set_params(P)
     s.install()
+    prev = E.lib().eh_get_tune_mode()
     E.set_tune_mode(E.TUNE_MLP)
     try:
         shapes = [(0, 0, 0, 4096, 4096, 4096), (1, 1, 1, 300, 9000, 70), (0, 1, 0, 64, 64, 16384)]
@@ -119,6 +120,6 @@ def test_planner_uses_installed_network(gpu):
         host_pred = [E.plan(d, ta, tb, m, n, k) for d, ta, tb, m, n, k in shapes]
         assert list(dev_pred) == host_pred
     finally:
-        E.set_tune_mode(E.TUNE_HEURISTIC)
+        E.set_tune_mode(prev)
         E.check(E.lib().eh_planner_set_mlp(None, None))
         s.close()

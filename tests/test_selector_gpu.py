@@ -29,6 +29,7 @@ def test_selector_beats_heuristic_on_heldout(gpu):
     # the planner in MLP mode picks exactly what the device inference picks
     shapes, _ = S.load_db(DB)
     sel.install()
+    prev = E.lib().eh_get_tune_mode()
     E.set_tune_mode(E.TUNE_MLP)
     try:
         held = shapes[4000:4100]
@@ -36,6 +37,6 @@ def test_selector_beats_heuristic_on_heldout(gpu):
         host = np.array([E.plan(d, ta, tb, m, n, k) for d, ta, tb, m, n, k in held.tolist()])
         assert np.mean(dev == host) >= 0.99  # FP32 ties aside, host and device inference agree
     finally:
-        E.set_tune_mode(E.TUNE_HEURISTIC)
+        E.set_tune_mode(prev)
         E.check(E.lib().eh_planner_set_mlp(None, None))
         sel.close()
