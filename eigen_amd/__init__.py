@@ -100,6 +100,15 @@ def lib():
         path = lib_path()
         if not os.path.exists(path):
             raise RuntimeError(f"{path} not built: run `python -m eigen_amd.build` (no CPU fallback exists)")
+        # One HIP runtime per process: torch bundles its own libamdhip64 with
+        # the same soname as the system one this library links.  Loading torch
+        # first makes the library bind to that already-loaded runtime; loading
+        # the library first would leave torch with a second runtime that finds
+        # no device.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(path)
         vp, i64, i32, u64, f64 = C.c_void_p, C.c_int64, C.c_int, C.c_uint64, C.c_double
         L.eh_gemm.restype = i32
