@@ -70,6 +70,17 @@ def fp64_rows_error(dA, dB, dC, m, n, k, rows, alpha=1.0, beta=0.0, C0=None, dty
     return float(num / (k * u * nA * np.linalg.norm(B)))
 
 
+def measured_traffic(workload: str):
+    """HBM bytes per launch of the workload's dominant kernel, as measured by the
+    FETCH_SIZE / WRITE_SIZE counter runs summarised in profiles/traffic.json
+    (each counter scaled by its reading for a copy kernel of known size)."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
+            return json.load(f).get(workload)
+    except (OSError, ValueError):
+        return None
+
+
 def cpu_baseline(steps: int = 3):
     """Eigen's CPU GEBP (built from the reference tree into oracle/_ref) on this host."""
     try:
@@ -166,7 +177,7 @@ def main() -> int:
         "roofline": {"bound": "mfma", "achieved": round(6 * flop / (head["kernel_ms"] * 1e-3) / 1e12, 2),
                      "peak": PEAK["bf16"], "unit": "TFLOP/s",
                      "frac": round(6 * flop / (head["kernel_ms"] * 1e-3) / 1e12 / PEAK["bf16"], 4),
-                     "traffic": None},
+                     "traffic": measured_traffic("split3_8192")},
         "cpu_baseline": None,
         "detail": {"split3": res["split3"]},
     }

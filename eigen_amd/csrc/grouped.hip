@@ -26,6 +26,7 @@
 
 #include <algorithm>
 #include <mutex>
+#include <type_traits>
 #include <numeric>
 #include <vector>
 
@@ -85,25 +86,89 @@ __device__ inline T load_or_zero(const T* X, int64_t ld, bool trans, int r, int 
   return trans ? X[c + (int64_t)r * ld] : X[r + (int64_t)c * ld];
 }
 
-// One 32x32 FP32 tile of C = alpha*op(A)*op(B) + beta*C.
-__device__ void grouped_tile_f32(const GroupedDesc& g, int tm, int tn, int lane) {
+// Element (r, k) of a 32-row slab of an operand, zero outside rows x K.
+// KC: the operand is k-contiguous (X[k + r*ld]); otherwise rows are
+// contiguous (X[r + k*ld]).
+template <typename T, bool KC>
+__device__ inline T slab_elem(const T* X, int64_t ld, int r, int k, int rows, int K) {
+  if (r >= rows || k >= K) return T(0);
+  return KC ? X[k + (int64_t)r * ld] : X[r + (int64_t)k * ld];
+}
+
+// Per-wave LDS scratch for one 32 x 16 slab (row stride padded to keep the
+// 16-byte reads aligned): 32 * 20 floats or 32 * 18 doubles.
+constexpr int kSlabF32 = 32 * 20, kSlabF64 = 32 * 18;
+
+// FP32: lane (i, g) needs v[s] = op(X)(r0 + i, k0 + g*8 + s), s < 8.  Rows-
+// contiguous operands are read directly (32 consecutive rows per load);
+// k-contiguous ones are loaded 16 k x 4 rows per instruction (64-byte runs),
+// transposed through the wave's LDS slab and read back.
+template <bool KC>
+__device__ inline void slab_f32(const float* X, int64_t ld, int r0, int k0, int rows, int K, int lane,
+                                float* slab, float (&v)[8]) {
+  const int i = lane & 31, g = lane >> 5;
+  if constexpr (!KC) {
+#pragma unroll
+    for (int s = 0; s < 8; ++s) v[s] = slab_elem<float, false>(X, ld, r0 + i, k0 + g * 8 + s, rows, K);
+  } else {
+    const int kk = lane & 15, rr = lane >> 4;
+#pragma unroll
+    for (int t = 0; t < 8; ++t) slab[(rr + 4 * t) * 20 + kk] = slab_elem<float, true>(X, ld, r0 + rr + 4 * t, k0 + kk, rows, K);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+    for (int s = 0; s < 8; ++s) v[s] = slab[i * 20 + g * 8 + s];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();  // the slab is rewritten by the next chunk
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+}
+
+// FP64: lane (i, g) needs v[x][s] = op(X)(r0 + 16x + i, k0 + g*4 + s), s < 4.
+template <bool KC>
+__device__ inline void slab_f64(const double* X, int64_t ld, int r0, int k0, int rows, int K, int lane,
+                                double* slab, double (&v)[2][4]) {
+  const int i = lane & 15, g = lane >> 4;
+  if constexpr (!KC) {
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+      for (int s = 0; s < 4; ++s) v[x][s] = slab_elem<double, false>(X, ld, r0 + 16 * x + i, k0 + g * 4 + s, rows, K);
+  } else {
+    const int kk = lane & 15, rr = lane >> 4;
+#pragma unroll
+    for (int t = 0; t < 8; ++t)
+      slab[(rr + 4 * t) * 18 + kk] = slab_elem<double, true>(X, ld, r0 + rr + 4 * t, k0 + kk, rows, K);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+      for (int s = 0; s < 4; ++s) v[x][s] = slab[(16 * x + i) * 18 + g * 4 + s];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+}
+
+// One 32x32 FP32 tile of C = alpha*op(A)*op(B) + beta*C.  op(A) is read as a
+// 32-row slab over k (k-contiguous when A is transposed); op(B) as the
+// 32-row slab of op(B)^T (k-contiguous when B is not transposed).
+template <bool TA, bool TB>
+__device__ void grouped_tile_f32(const GroupedDesc& g, int tm, int tn, int lane, float* slab) {
   const float* A = static_cast<const float*>(g.A);
   const float* B = static_cast<const float*>(g.B);
   float* C = static_cast<float*>(g.C);
-  const int i = lane & 31, grp = lane >> 5;
-  const int row = tm * 32 + i, col = tn * 32 + i;
   floatx16 acc;
 #pragma unroll
   for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
   // lane (i, grp) feeds k = k0 + grp*8 + s to instruction s of each 16-deep step
   for (int k0 = 0; k0 < g.k; k0 += 16) {
     float a[8], b[8];
-#pragma unroll
-    for (int s = 0; s < 8; ++s) {
-      const int kk = k0 + grp * 8 + s;
-      a[s] = load_or_zero(A, g.lda, g.ta, row, kk, g.m, g.k);
-      b[s] = load_or_zero(B, g.ldb, !g.tb, col, kk, g.n, g.k);  // op(B)^T(j, k)
-    }
+    slab_f32<TA>(A, g.lda, tm * 32, k0, g.m, g.k, lane, slab, a);
+    slab_f32<!TB>(B, g.ldb, tn * 32, k0, g.n, g.k, lane, slab, b);
 #pragma unroll
     for (int s = 0; s < 8; ++s) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a[s], b[s], acc, 0, 0, 0);
   }
@@ -120,11 +185,11 @@ __device__ void grouped_tile_f32(const GroupedDesc& g, int tm, int tn, int lane)
 }
 
 // One 32x32 FP64 tile as a 2x2 block of 16x16 MFMA tiles.
-__device__ void grouped_tile_f64(const GroupedDesc& g, int tm, int tn, int lane) {
+template <bool TA, bool TB>
+__device__ void grouped_tile_f64(const GroupedDesc& g, int tm, int tn, int lane, double* slab) {
   const double* A = static_cast<const double*>(g.A);
   const double* B = static_cast<const double*>(g.B);
   double* C = static_cast<double*>(g.C);
-  const int i = lane & 15, grp = lane >> 4;
   doublex4 acc[2][2];
 #pragma unroll
   for (int x = 0; x < 2; ++x)
@@ -134,15 +199,8 @@ __device__ void grouped_tile_f64(const GroupedDesc& g, int tm, int tn, int lane)
       for (int r = 0; r < 4; ++r) acc[x][y][r] = 0.0;
   for (int k0 = 0; k0 < g.k; k0 += 16) {
     double a[2][4], b[2][4];
-#pragma unroll
-    for (int s = 0; s < 4; ++s) {
-      const int kk = k0 + grp * 4 + s;
-#pragma unroll
-      for (int x = 0; x < 2; ++x) {
-        a[x][s] = load_or_zero(A, g.lda, g.ta, tm * 32 + x * 16 + i, kk, g.m, g.k);
-        b[x][s] = load_or_zero(B, g.ldb, !g.tb, tn * 32 + x * 16 + i, kk, g.n, g.k);
-      }
-    }
+    slab_f64<TA>(A, g.lda, tm * 32, k0, g.m, g.k, lane, slab, a);
+    slab_f64<!TB>(B, g.ldb, tn * 32, k0, g.n, g.k, lane, slab, b);
 #pragma unroll
     for (int s = 0; s < 4; ++s)
 #pragma unroll
@@ -167,10 +225,25 @@ __device__ void grouped_tile_f64(const GroupedDesc& g, int tm, int tn, int lane)
   }
 }
 
+template <typename F>
+__device__ inline void dispatch_trans(int ta, int tb, F&& f) {
+  if (!ta && !tb)
+    f(std::integral_constant<bool, false>(), std::integral_constant<bool, false>());
+  else if (ta && !tb)
+    f(std::integral_constant<bool, true>(), std::integral_constant<bool, false>());
+  else if (!ta && tb)
+    f(std::integral_constant<bool, false>(), std::integral_constant<bool, true>());
+  else
+    f(std::integral_constant<bool, true>(), std::integral_constant<bool, true>());
+}
+
 // Persistent kernel: wave w computes tiles w, w + W, w + 2W, ...
 __global__ void __launch_bounds__(256) grouped_gemm_kernel(const GroupedDesc* __restrict__ d, int64_t count,
                                                            const int64_t* __restrict__ tile_begin) {
+  __shared__ __attribute__((aligned(16))) double lds[4 * kSlabF64];  // one slab per wave
   const int lane = threadIdx.x & 63;
+  double* slab64 = lds + (threadIdx.x >> 6) * kSlabF64;
+  float* slab32 = reinterpret_cast<float*>(slab64);
   const int64_t total = tile_begin[count];
   const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
   const int64_t first = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
@@ -188,9 +261,9 @@ __global__ void __launch_bounds__(256) grouped_gemm_kernel(const GroupedDesc* __
     const int local = (int)(t - tile_begin[lo]);
     const int tm = local % g.tiles_m, tn = local / g.tiles_m;
     if (g.dtype == EH_F32)
-      grouped_tile_f32(g, tm, tn, lane);
+      dispatch_trans(g.ta, g.tb, [&](auto ta, auto tb) { grouped_tile_f32<ta, tb>(g, tm, tn, lane, slab32); });
     else
-      grouped_tile_f64(g, tm, tn, lane);
+      dispatch_trans(g.ta, g.tb, [&](auto ta, auto tb) { grouped_tile_f64<ta, tb>(g, tm, tn, lane, slab64); });
   }
 }
 

@@ -67,14 +67,17 @@ def test_grouped_alpha_beta_and_transposes(gpu):
         npdt = np.float32 if dt == E.F32 else np.float64
         m, n, k = (int(x) for x in rng.integers(1, 70, 3))
         ta, tb = bool(p & 2), bool(p & 4)
-        A = O.fill((k, m) if ta else (m, k), npdt, 100 + p, O.INT, -8, 8)
-        B = O.fill((n, k) if tb else (k, n), npdt, 200 + p, O.INT, -8, 8)
+        # every other problem stores A and B with padded leading dimensions
+        pad = 3 if p % 3 == 0 else 0
+        Af = O.fill((k + pad, m) if ta else (m + pad, k), npdt, 100 + p, O.INT, -8, 8)
+        Bf = O.fill((n + pad, k) if tb else (k + pad, n), npdt, 200 + p, O.INT, -8, 8)
+        A, B = np.asfortranarray(Af[:Af.shape[0] - pad]), np.asfortranarray(Bf[:Bf.shape[0] - pad])
         C0 = O.fill((m, n), npdt, 300 + p, O.INT, -8, 8)
         alpha, beta = [(1.0, 0.0), (2.0, -1.0), (0.0, 0.5), (-0.5, 1.0)][p % 4]
-        dA, dB, dC = (torch.from_numpy(x.reshape(-1, order="F").copy()).cuda() for x in (A, B, C0))
+        dA, dB, dC = (torch.from_numpy(x.reshape(-1, order="F").copy()).cuda() for x in (Af, Bf, C0))
         bufs += [dA, dB, dC]
-        probs.append(E.GemmProblem(dtype=dt, opa=int(ta), opb=int(tb), m=m, n=n, k=k, lda=A.shape[0],
-                                   ldb=B.shape[0], ldc=m, A=dA.data_ptr(), B=dB.data_ptr(), C=dC.data_ptr(),
+        probs.append(E.GemmProblem(dtype=dt, opa=int(ta), opb=int(tb), m=m, n=n, k=k, lda=Af.shape[0],
+                                   ldb=Bf.shape[0], ldc=m, A=dA.data_ptr(), B=dB.data_ptr(), C=dC.data_ptr(),
                                    alpha=alpha, beta=beta))
         host.append((A, B, C0, ta, tb, alpha, beta, dC, (m, n)))
     g = E.Grouped(probs)
