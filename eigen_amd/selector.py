@@ -158,7 +158,7 @@ def emit_inc(params, path: str, source: str) -> None:
         f.write(f"// `python -m eigen_amd.selector --emit-inc` from {source}.\n")
         f.write("static const float kDefaultSelectorParams[2072] = {\n")
         for i in range(0, len(vals), 6):
-            f.write("    " + ", ".join(f"{v:.9g}f" for v in vals[i:i + 6]) + ",\n")
+            f.write("    " + ", ".join(f"{v:.9e}f" for v in vals[i:i + 6]) + ",\n")
         f.write("};\n")
 
 

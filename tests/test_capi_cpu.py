@@ -54,13 +54,42 @@ def test_config_table():
 
 
 def test_heuristic_plan_is_valid():
+    prev = E.lib().eh_get_tune_mode()
     E.set_tune_mode(E.TUNE_HEURISTIC)
-    for (m, n, k) in [(8192, 8192, 8192), (16384, 256, 16384), (64, 64, 64), (4096, 4096, 4096), (1, 1, 1)]:
-        c = E.plan(E.F64, E.OP_N, E.OP_N, m, n, k)
-        assert 0 <= c < 24
-    # a product whose tiles cannot fill the chip gets split K
-    assert E.config_info(E.plan(E.F64, E.OP_N, E.OP_N, 2048, 256, 32768))[2] > 1
-    assert E.config_info(E.plan(E.F64, E.OP_N, E.OP_N, 8192, 8192, 8192))[2] == 1
+    try:
+        for (m, n, k) in [(8192, 8192, 8192), (16384, 256, 16384), (64, 64, 64), (4096, 4096, 4096), (1, 1, 1)]:
+            c = E.plan(E.F64, E.OP_N, E.OP_N, m, n, k)
+            assert 0 <= c < 24
+        # a product whose tiles cannot fill the chip gets split K
+        assert E.config_info(E.plan(E.F64, E.OP_N, E.OP_N, 2048, 256, 32768))[2] > 1
+        assert E.config_info(E.plan(E.F64, E.OP_N, E.OP_N, 8192, 8192, 8192))[2] == 1
+    finally:
+        E.set_tune_mode(prev)
+
+
+def test_compiled_in_selector_is_the_trained_network():
+    """The planner's default network (compiled into the library) is the one
+    trained on the device's timing table: its host-side choices equal the
+    argmax of the FP64 restatement run on the saved parameters, and the saved
+    training run beat the heuristic on its held-out shapes."""
+    import json
+    import os
+    from oracle import mlp as M
+    from eigen_amd.selector import load_db
+    path = os.path.join(os.path.dirname(E.__file__), "data", "selector_gfx950.json")
+    res = json.load(open(path))
+    assert res["mlp_regret"]["mean"] < res["heuristic_regret"]["mean"]
+    P = np.array(res["params"], dtype=np.float64)
+    assert E.lib().eh_get_tune_mode() == E.TUNE_MLP  # the default when a network is compiled in
+    shapes, _ = load_db(os.path.join(os.path.dirname(E.__file__), "data", "tune_gfx950.csv"))
+    shapes = shapes[4000:4200]
+    X = np.array([M.features(*s) for s in shapes.tolist()])
+    _, Z, _ = M.forward(P, X)
+    host = np.array([E.plan(*s) for s in shapes.tolist()])
+    top2 = np.sort(Z, axis=1)[:, -2:]
+    clear = (top2[:, 1] - top2[:, 0]) > 1e-4  # FP32 host inference; skip near-ties
+    assert clear.sum() > 150
+    assert np.array_equal(host[clear], Z.argmax(axis=1)[clear])
 
 
 def _call_gemm(dtype=E.F32, opa=E.OP_N, opb=E.OP_N, m=4, n=4, k=4, lda=4, ldb=4, ldc=4, alpha=True, beta=True,
