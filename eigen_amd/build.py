@@ -31,7 +31,9 @@ BLAS_SOURCES = ["blas.cpp"]
 # Per-file code generation choices.  The FP64 kernels keep their
 # accumulators in the VGPR form of the MFMA: with AGPR accumulators the
 # 128x128 tile copied every accumulator through VGPRs on each k slice.
-FILE_FLAGS = {"gemm_f64.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
+FILE_FLAGS = {"gemm_f64.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"],
+              # the grouped kernel's accumulators in VGPRs: 112 registers, four waves per SIMD
+              "grouped.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
 
 
 def _compile(src: str) -> str:

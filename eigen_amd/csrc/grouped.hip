@@ -4,29 +4,36 @@
 // (Eigen/src/Core/GeneralProduct.h:26-150 chooses a coefficient-based or GEMM
 // path per product; Eigen/src/Core/ProductEvaluators.h:606-692 is the lazy
 // small-product kernel).  Here a whole batch of FP32 and FP64 products runs
-// in one persistent kernel that walks a device-side work list:
+// in one persistent kernel that walks a work list:
 //
-//   1. eh_grouped_create uploads one descriptor per product (ordered by
-//      decreasing depth, longest work first);
-//   2. a build kernel computes each product's 32x32 output-tile count and an
-//      exclusive prefix sum over the products (the work list);
-//   3. the persistent kernel starts 8 waves per SIMD; wave w takes tiles
-//      w, w + W, w + 2W, ... of the list (W waves in all: the list is sorted
-//      deepest first, so the round-robin deal balances like longest-first
-//      scheduling, without a shared counter to contend on), finds each tile's
-//      product by binary search in the prefix sums and computes the 32x32 tile
-//      over the full depth with MFMA micro-tiles (v_mfma_f32_32x32x2_f32 for
-//      FP32, a 2x2 block of v_mfma_f64_16x16x4_f64 for FP64), reading its
-//      operands straight from HBM (no LDS: one wave owns one tile), and fuses
-//      alpha/beta into the store.  A wave exits after its last tile.
+//   1. eh_grouped_create builds the work list once, on the host: one
+//      descriptor per product, the first 32x32 output tile of each product
+//      (prefix sums of tile counts), and a partition of the tile sequence
+//      into one contiguous run per resident wave, cut so that every run
+//      carries the same estimated cost (k steps per tile, FP64 steps counted
+//      twice, plus a per-tile overhead).  Runs of neighbouring waves on one
+//      XCD are neighbours in the tile sequence, so the tiles of one product
+//      share that XCD's L2.
+//   2. the kernel starts exactly the waves that fit on the chip at once;
+//      each wave reads its run bounds and walks its tiles in order, advancing
+//      a product cursor (all wave-uniform, scalar loads), and computes each
+//      32x32 tile over the full depth with MFMA (v_mfma_f32_32x32x2_f32 for
+//      FP32, a 2x2 block of v_mfma_f64_16x16x4_f64 for FP64).  Operands are
+//      read with raw buffer loads (reads past the end of an operand return
+//      zero, so no load can fault), double-buffered in registers so the next
+//      16-deep k step is in flight while the current one is multiplied; the
+//      output tile is produced transposed (C^T = op(B)^T op(A)^T) so that
+//      consecutive lanes store consecutive rows of the column-major C, with
+//      alpha/beta fused into the store.
+//   3. products whose operands span 2^31 bytes or more (32-bit buffer
+//      offsets) run after the kernel through the single-product path.
 //
-// The two launches are captured once in a hipGraph and replayed
-// (eh_grouped_graph_launch); the graph holds kernel nodes only.
+// eh_grouped_graph_launch captures these launches once in a hipGraph and
+// replays it; the graph holds kernel nodes only.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <mutex>
-#include <type_traits>
 #include <numeric>
 #include <vector>
 
@@ -40,231 +47,313 @@ struct GroupedDesc {
   const void* A;
   const void* B;
   void* C;
-  int64_t lda, ldb, ldc;
+  int64_t ldc;
   double alpha, beta;
+  uint32_t bytesA, bytesB;  // extents of the stored operands (buffer ranges)
+  uint32_t lda, ldb;        // clamped to 2^31 - 1: a leading dimension is only used when it is below that
   int32_t dtype, ta, tb;
   int32_t m, n, k;
-  int32_t tiles_m, tiles_n;
+  int32_t tiles_m;
 };
 
-// Work list: tile_begin[p] = first tile index of product p (exclusive prefix
-// sum of tiles_m * tiles_n), tile_begin[count] = total tiles.
-__global__ void __launch_bounds__(1024) grouped_build_kernel(GroupedDesc* d, int64_t count, int64_t* tile_begin) {
-  __shared__ int64_t partial[1024];
-  const int tid = threadIdx.x;
-  // each thread owns a contiguous run of products
-  const int64_t per = (count + 1023) / 1024;
-  const int64_t lo = std::min<int64_t>(count, tid * per), hi = std::min<int64_t>(count, lo + per);
-  int64_t sum = 0;
-  for (int64_t p = lo; p < hi; ++p) {
-    const int tm = (d[p].m + 31) / 32, tn = (d[p].n + 31) / 32;
-    d[p].tiles_m = tm;
-    d[p].tiles_n = tn;
-    sum += (int64_t)tm * tn;
-  }
-  partial[tid] = sum;
-  __syncthreads();
-  // inclusive scan of the 1024 partial sums (Hillis-Steele)
-  for (int off = 1; off < 1024; off <<= 1) {
-    const int64_t v = tid >= off ? partial[tid - off] : 0;
-    __syncthreads();
-    partial[tid] += v;
-    __syncthreads();
-  }
-  int64_t run = tid > 0 ? partial[tid - 1] : 0;
-  for (int64_t p = lo; p < hi; ++p) {
-    tile_begin[p] = run;
-    run += (int64_t)d[p].tiles_m * d[p].tiles_n;
-  }
-  if (tid == 1023) tile_begin[count] = partial[1023];
-}
+constexpr int kWavesPerGroup = 4;
+// k depth of one step: 16 for FP32 (8 MFMAs), 8 for FP64 (8 MFMAs of a 2x2
+// block), so both dtypes keep two steps of operands in flight per wave
+// within 128 registers (four waves per SIMD).
+constexpr int kStepF32 = 16, kStepF64 = 8;
 
-template <typename T>
-__device__ inline T load_or_zero(const T* X, int64_t ld, bool trans, int r, int c, int rows, int cols) {
-  // element (r, c) of op(X) (rows x cols); op(X) = X or X^T as stored column-major
-  if (r >= rows || c >= cols) return T(0);
-  return trans ? X[c + (int64_t)r * ld] : X[r + (int64_t)c * ld];
-}
+// ---- slab loads -------------------------------------------------------------
+// A slab is 32 rows (of op(A), or of op(B)^T) by 16 k.  KC: the operand is
+// k-contiguous in memory (X[k + r*ld]); otherwise its rows are contiguous
+// (X[r + k*ld]).  `vo` is the lane's byte offset at k0 = 0; elements with
+// r >= rows only reach outputs that are never stored, elements with k >= K
+// are zeroed by tail_* on the last step.
 
-// Element (r, k) of a 32-row slab of an operand, zero outside rows x K.
-// KC: the operand is k-contiguous (X[k + r*ld]); otherwise rows are
-// contiguous (X[r + k*ld]).
-template <typename T, bool KC>
-__device__ inline T slab_elem(const T* X, int64_t ld, int r, int k, int rows, int K) {
-  if (r >= rows || k >= K) return T(0);
-  return KC ? X[k + (int64_t)r * ld] : X[r + (int64_t)k * ld];
-}
-
-// Per-wave LDS scratch for one 32 x 16 slab (row stride padded to keep the
-// 16-byte reads aligned): 32 * 20 floats or 32 * 18 doubles.
-constexpr int kSlabF32 = 32 * 20, kSlabF64 = 32 * 18;
-
-// FP32: lane (i, g) needs v[s] = op(X)(r0 + i, k0 + g*8 + s), s < 8.  Rows-
-// contiguous operands are read directly (32 consecutive rows per load);
-// k-contiguous ones are loaded 16 k x 4 rows per instruction (64-byte runs),
-// transposed through the wave's LDS slab and read back.
+// FP32: lane (i = lane%32, g = lane/32) holds v[s] = X(r0 + i, k0 + 8g + s).
 template <bool KC>
-__device__ inline void slab_f32(const float* X, int64_t ld, int r0, int k0, int rows, int K, int lane,
-                                float* slab, float (&v)[8]) {
-  const int i = lane & 31, g = lane >> 5;
-  if constexpr (!KC) {
-#pragma unroll
-    for (int s = 0; s < 8; ++s) v[s] = slab_elem<float, false>(X, ld, r0 + i, k0 + g * 8 + s, rows, K);
-  } else {
-    const int kk = lane & 15, rr = lane >> 4;
-#pragma unroll
-    for (int t = 0; t < 8; ++t) slab[(rr + 4 * t) * 20 + kk] = slab_elem<float, true>(X, ld, r0 + rr + 4 * t, k0 + kk, rows, K);
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-#pragma unroll
-    for (int s = 0; s < 8; ++s) v[s] = slab[i * 20 + g * 8 + s];
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();  // the slab is rewritten by the next chunk
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  }
+__device__ inline uint32_t lane_offset_f32(int r0, int lane, uint32_t ld) {
+  const uint32_t r = (uint32_t)(r0 + (lane & 31)), g = (uint32_t)(lane >> 5);
+  return KC ? (r * ld + 8u * g) * 4u : (r + 8u * g * ld) * 4u;
 }
 
-// FP64: lane (i, g) needs v[x][s] = op(X)(r0 + 16x + i, k0 + g*4 + s), s < 4.
 template <bool KC>
-__device__ inline void slab_f64(const double* X, int64_t ld, int r0, int k0, int rows, int K, int lane,
-                                double* slab, double (&v)[2][4]) {
-  const int i = lane & 15, g = lane >> 4;
-  if constexpr (!KC) {
+__device__ inline void load_f32(__amdgpu_buffer_rsrc_t rs, uint32_t vo, int k0, uint32_t ld, float (&v)[8]) {
+  if constexpr (KC) {
+    const uint32_t so = (uint32_t)k0 * 4u;
+    const u32x4 x0 = buf_load16(rs, vo, so), x1 = buf_load16(rs, vo + 16u, so);
 #pragma unroll
-    for (int x = 0; x < 2; ++x)
-#pragma unroll
-      for (int s = 0; s < 4; ++s) v[x][s] = slab_elem<double, false>(X, ld, r0 + 16 * x + i, k0 + g * 4 + s, rows, K);
+    for (int s = 0; s < 4; ++s) {
+      v[s] = __uint_as_float(x0[s]);
+      v[s + 4] = __uint_as_float(x1[s]);
+    }
   } else {
-    const int kk = lane & 15, rr = lane >> 4;
 #pragma unroll
-    for (int t = 0; t < 8; ++t)
-      slab[(rr + 4 * t) * 18 + kk] = slab_elem<double, true>(X, ld, r0 + rr + 4 * t, k0 + kk, rows, K);
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-#pragma unroll
-    for (int x = 0; x < 2; ++x)
-#pragma unroll
-      for (int s = 0; s < 4; ++s) v[x][s] = slab[(16 * x + i) * 18 + g * 4 + s];
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (int s = 0; s < 8; ++s) v[s] = buf_load(rs, vo, (uint32_t)(k0 + s) * ld * 4u, (float*)nullptr);
   }
 }
 
-// One 32x32 FP32 tile of C = alpha*op(A)*op(B) + beta*C.  op(A) is read as a
-// 32-row slab over k (k-contiguous when A is transposed); op(B) as the
-// 32-row slab of op(B)^T (k-contiguous when B is not transposed).
-template <bool TA, bool TB>
-__device__ void grouped_tile_f32(const GroupedDesc& g, int tm, int tn, int lane, float* slab) {
-  const float* A = static_cast<const float*>(g.A);
-  const float* B = static_cast<const float*>(g.B);
-  float* C = static_cast<float*>(g.C);
-  floatx16 acc;
+__device__ inline void tail_f32(float (&v)[8], int k0, int K, int lane) {
+  const int kb = k0 + 8 * (lane >> 5);
 #pragma unroll
-  for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
-  // lane (i, grp) feeds k = k0 + grp*8 + s to instruction s of each 16-deep step
-  for (int k0 = 0; k0 < g.k; k0 += 16) {
-    float a[8], b[8];
-    slab_f32<TA>(A, g.lda, tm * 32, k0, g.m, g.k, lane, slab, a);
-    slab_f32<!TB>(B, g.ldb, tn * 32, k0, g.n, g.k, lane, slab, b);
+  for (int s = 0; s < 8; ++s)
+    if (kb + s >= K) v[s] = 0.0f;
+}
+
+// FP64: lane (i = lane%16, g = lane/16) holds v[x][s] = X(r0 + 16x + i, k0 + 2g + s).
+template <bool KC>
+__device__ inline uint32_t lane_offset_f64(int r0, int lane, uint32_t ld) {
+  const uint32_t r = (uint32_t)(r0 + (lane & 15)), g = (uint32_t)(lane >> 4);
+  return KC ? (r * ld + 2u * g) * 8u : (r + 2u * g * ld) * 8u;
+}
+
+template <bool KC>
+__device__ inline void load_f64(__amdgpu_buffer_rsrc_t rs, uint32_t vo, int k0, uint32_t ld, double (&v)[2][2]) {
 #pragma unroll
-    for (int s = 0; s < 8; ++s) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a[s], b[s], acc, 0, 0, 0);
-  }
-  const float alpha = (float)g.alpha, beta = (float)g.beta;
-  const int c = tn * 32 + Mfma<float>::col(lane);
-  if (c >= g.n) return;
+  for (int x = 0; x < 2; ++x) {
+    if constexpr (KC) {
+      const uint32_t so = (uint32_t)k0 * 8u + (uint32_t)x * 16u * ld * 8u;
+      const u32x4 q = buf_load16(rs, vo, so);
+      v[x][0] = __hiloint2double((int)q[1], (int)q[0]);
+      v[x][1] = __hiloint2double((int)q[3], (int)q[2]);
+    } else {
 #pragma unroll
-  for (int r = 0; r < 16; ++r) {
-    const int rr = tm * 32 + Mfma<float>::row(r, lane);
-    if (rr >= g.m) continue;
-    float* dst = C + rr + (int64_t)c * g.ldc;
-    *dst = beta == 0.0f ? alpha * acc[r] : __builtin_fmaf(alpha, acc[r], beta * *dst);
+      for (int s = 0; s < 2; ++s)
+        v[x][s] = buf_load(rs, vo + (uint32_t)x * 128u, (uint32_t)(k0 + s) * ld * 8u, (double*)nullptr);
+    }
   }
 }
 
-// One 32x32 FP64 tile as a 2x2 block of 16x16 MFMA tiles.
-template <bool TA, bool TB>
-__device__ void grouped_tile_f64(const GroupedDesc& g, int tm, int tn, int lane, double* slab) {
-  const double* A = static_cast<const double*>(g.A);
-  const double* B = static_cast<const double*>(g.B);
-  double* C = static_cast<double*>(g.C);
-  doublex4 acc[2][2];
+__device__ inline void tail_f64(double (&v)[2][2], int k0, int K, int lane) {
+  const int kb = k0 + 2 * (lane >> 4);
 #pragma unroll
   for (int x = 0; x < 2; ++x)
 #pragma unroll
-    for (int y = 0; y < 2; ++y)
+    for (int s = 0; s < 2; ++s)
+      if (kb + s >= K) v[x][s] = 0.0;
+}
+
+// ---- tiles ------------------------------------------------------------------
+// One 32x32 FP32 tile of C = alpha*op(A)*op(B) + beta*C, computed as C^T:
+// the MFMA's first operand is the op(B)^T slab (32 columns of C), its second
+// the op(A) slab (32 rows of C), so accumulator r of lane l holds
+// C(m0 + l%32, n0 + row(r, l)).
+template <bool TA, bool TB>
+__device__ void grouped_tile_f32(const GroupedDesc& g, int tm, int tn, int lane) {
+  const __amdgpu_buffer_rsrc_t rA = make_rsrc(g.A, g.bytesA), rB = make_rsrc(g.B, g.bytesB);
+  const uint32_t vA = lane_offset_f32<TA>(tm * 32, lane, g.lda);
+  const uint32_t vB = lane_offset_f32<!TB>(tn * 32, lane, g.ldb);
+  const int K = g.k;
+  floatx16 acc;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) acc[x][y][r] = 0.0;
-  for (int k0 = 0; k0 < g.k; k0 += 16) {
-    double a[2][4], b[2][4];
-    slab_f64<TA>(A, g.lda, tm * 32, k0, g.m, g.k, lane, slab, a);
-    slab_f64<!TB>(B, g.ldb, tn * 32, k0, g.n, g.k, lane, slab, b);
+  for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
+  if (K > 0) {
+    constexpr int kStep = kStepF32;
+    float a0[8], b0[8], a1[8], b1[8];
+    auto mma = [&](float (&a)[8], float (&b)[8]) {
 #pragma unroll
-    for (int s = 0; s < 4; ++s)
+      for (int s = 0; s < 8; ++s) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(b[s], a[s], acc, 0, 0, 0);
+    };
+    auto load = [&](int k0, float (&a)[8], float (&b)[8]) {
+      load_f32<TA>(rA, vA, k0, g.lda, a);
+      load_f32<!TB>(rB, vB, k0, g.ldb, b);
+    };
+    // Full steps in pairs with unconditional prefetch (a load past K stays
+    // inside the operand's buffer range or reads zero), so the loop has no
+    // branch and the wait before each step's MFMAs covers exactly that
+    // step's loads; the partial last step is masked after the loop.
+    const int nfull = K / kStep;
+    const bool tail = K % kStep != 0;
+    load(0, a0, b0);
+    int i = 0;
+    for (; i + 2 <= nfull; i += 2) {
+      load((i + 1) * kStep, a1, b1);
+      mma(a0, b0);
+      load((i + 2) * kStep, a0, b0);
+      mma(a1, b1);
+    }
+    if (i < nfull) {  // one full step left, then the partial one
+      if (tail) load((i + 1) * kStep, a1, b1);
+      mma(a0, b0);
+      if (tail) {
+        tail_f32(a1, (i + 1) * kStep, K, lane);
+        tail_f32(b1, (i + 1) * kStep, K, lane);
+        mma(a1, b1);
+      }
+    } else if (tail) {
+      tail_f32(a0, i * kStep, K, lane);
+      tail_f32(b0, i * kStep, K, lane);
+      mma(a0, b0);
+    }
+  }
+  const int m = tm * 32 + (lane & 31);
+  if (m >= g.m) return;
+  const float alpha = (float)g.alpha, beta = (float)g.beta;
+  float* C = static_cast<float*>(g.C) + m;
+  if (beta != 0.0f) {  // read every C value of the lane before the first store
+    float c[16];
 #pragma unroll
-      for (int x = 0; x < 2; ++x)
+    for (int r = 0; r < 16; ++r) {
+      const int n = tn * 32 + Mfma<float>::row(r, lane);
+      c[r] = n < g.n ? C[(int64_t)n * g.ldc] : 0.0f;
+    }
 #pragma unroll
-        for (int y = 0; y < 2; ++y)
-          acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[x][s], b[y][s], acc[x][y], 0, 0, 0);
+    for (int r = 0; r < 16; ++r) acc[r] = __builtin_fmaf(alpha, acc[r], beta * c[r]);
+  } else {
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[r] *= alpha;
   }
 #pragma unroll
-  for (int y = 0; y < 2; ++y) {
-    const int c = tn * 32 + y * 16 + Mfma<double>::col(lane);
-    if (c >= g.n) continue;
+  for (int r = 0; r < 16; ++r) {
+    const int n = tn * 32 + Mfma<float>::row(r, lane);
+    if (n < g.n) C[(int64_t)n * g.ldc] = acc[r];
+  }
+}
+
+// One 32x32 FP64 tile as a 2x2 block of 16x16 MFMA tiles, also as C^T:
+// acc[y][x] of lane l, entry r, holds C(m0 + 16x + l%16, n0 + 16y + l/16 + 4r).
+template <bool TA, bool TB>
+__device__ void grouped_tile_f64(const GroupedDesc& g, int tm, int tn, int lane) {
+  const __amdgpu_buffer_rsrc_t rA = make_rsrc(g.A, g.bytesA), rB = make_rsrc(g.B, g.bytesB);
+  const uint32_t vA = lane_offset_f64<TA>(tm * 32, lane, g.lda);
+  const uint32_t vB = lane_offset_f64<!TB>(tn * 32, lane, g.ldb);
+  const int K = g.k;
+  doublex4 acc[2][2];
+#pragma unroll
+  for (int y = 0; y < 2; ++y)
 #pragma unroll
     for (int x = 0; x < 2; ++x)
 #pragma unroll
+      for (int r = 0; r < 4; ++r) acc[y][x][r] = 0.0;
+  if (K > 0) {
+    constexpr int kStep = kStepF64;
+    double a0[2][2], b0[2][2], a1[2][2], b1[2][2];
+    auto mma = [&](double (&a)[2][2], double (&b)[2][2]) {
+#pragma unroll
+      for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+          for (int x = 0; x < 2; ++x)
+            acc[y][x] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[y][s], a[x][s], acc[y][x], 0, 0, 0);
+    };
+    auto load = [&](int k0, double (&a)[2][2], double (&b)[2][2]) {
+      load_f64<TA>(rA, vA, k0, g.lda, a);
+      load_f64<!TB>(rB, vB, k0, g.ldb, b);
+    };
+    const int nfull = K / kStep;
+    const bool tail = K % kStep != 0;
+    load(0, a0, b0);
+    int i = 0;
+    for (; i + 2 <= nfull; i += 2) {
+      load((i + 1) * kStep, a1, b1);
+      mma(a0, b0);
+      load((i + 2) * kStep, a0, b0);
+      mma(a1, b1);
+    }
+    if (i < nfull) {
+      if (tail) load((i + 1) * kStep, a1, b1);
+      mma(a0, b0);
+      if (tail) {
+        tail_f64(a1, (i + 1) * kStep, K, lane);
+        tail_f64(b1, (i + 1) * kStep, K, lane);
+        mma(a1, b1);
+      }
+    } else if (tail) {
+      tail_f64(a0, i * kStep, K, lane);
+      tail_f64(b0, i * kStep, K, lane);
+      mma(a0, b0);
+    }
+  }
+  double* C = static_cast<double*>(g.C);
+  if (g.beta != 0.0) {  // read every C value of the lane before the first store
+    double c[2][2][4];
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+      for (int y = 0; y < 2; ++y)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int m = tm * 32 + 16 * x + (lane & 15), n = tn * 32 + 16 * y + (lane >> 4) + 4 * r;
+          c[x][y][r] = (m < g.m && n < g.n) ? C[m + (int64_t)n * g.ldc] : 0.0;
+        }
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+      for (int y = 0; y < 2; ++y)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc[y][x][r] = __builtin_fma(g.alpha, acc[y][x][r], g.beta * c[x][y][r]);
+  } else {
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+      for (int y = 0; y < 2; ++y)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc[y][x][r] *= g.alpha;
+  }
+#pragma unroll
+  for (int x = 0; x < 2; ++x) {
+    const int m = tm * 32 + 16 * x + (lane & 15);
+    if (m >= g.m) continue;
+#pragma unroll
+    for (int y = 0; y < 2; ++y)
+#pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int rr = tm * 32 + x * 16 + Mfma<double>::row(r, lane);
-        if (rr >= g.m) continue;
-        double* dst = C + rr + (int64_t)c * g.ldc;
-        *dst = g.beta == 0.0 ? g.alpha * acc[x][y][r] : __builtin_fma(g.alpha, acc[x][y][r], g.beta * *dst);
+        const int n = tn * 32 + 16 * y + (lane >> 4) + 4 * r;
+        if (n < g.n) C[m + (int64_t)n * g.ldc] = acc[y][x][r];
       }
   }
 }
 
-template <typename F>
-__device__ inline void dispatch_trans(int ta, int tb, F&& f) {
-  if (!ta && !tb)
-    f(std::integral_constant<bool, false>(), std::integral_constant<bool, false>());
-  else if (ta && !tb)
-    f(std::integral_constant<bool, true>(), std::integral_constant<bool, false>());
-  else if (!ta && tb)
-    f(std::integral_constant<bool, false>(), std::integral_constant<bool, true>());
-  else
-    f(std::integral_constant<bool, true>(), std::integral_constant<bool, true>());
+// Persistent kernel: virtual wave v (XCD-major numbering of the resident
+// waves) computes tiles wave_tile[v] .. wave_tile[v+1]-1 of the work list.
+__global__ void __launch_bounds__(256) grouped_gemm_kernel(const GroupedDesc* __restrict__ d,
+                                                           const int64_t* __restrict__ tile_begin,
+                                                           const int64_t* __restrict__ wave_tile,
+                                                           const int32_t* __restrict__ wave_prod) {
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int v = xcd_remap(blockIdx.x, gridDim.x) * kWavesPerGroup + w;
+  const int64_t tend = wave_tile[v + 1];
+  int p = wave_prod[v];
+  for (int64_t t = wave_tile[v]; t < tend; ++t) {
+    while (tile_begin[p + 1] <= t) ++p;
+    const GroupedDesc g = d[p];
+    const int local = (int)(t - tile_begin[p]);
+    const int tm = local % g.tiles_m, tn = local / g.tiles_m;
+    if (g.dtype == EH_F32) {
+      if (!g.ta && !g.tb)
+        grouped_tile_f32<false, false>(g, tm, tn, lane);
+      else if (g.ta && !g.tb)
+        grouped_tile_f32<true, false>(g, tm, tn, lane);
+      else if (!g.ta && g.tb)
+        grouped_tile_f32<false, true>(g, tm, tn, lane);
+      else
+        grouped_tile_f32<true, true>(g, tm, tn, lane);
+    } else {
+      if (!g.ta && !g.tb)
+        grouped_tile_f64<false, false>(g, tm, tn, lane);
+      else if (g.ta && !g.tb)
+        grouped_tile_f64<true, false>(g, tm, tn, lane);
+      else if (!g.ta && g.tb)
+        grouped_tile_f64<false, true>(g, tm, tn, lane);
+      else
+        grouped_tile_f64<true, true>(g, tm, tn, lane);
+    }
+  }
 }
 
-// Persistent kernel: wave w computes tiles w, w + W, w + 2W, ...
-__global__ void __launch_bounds__(256) grouped_gemm_kernel(const GroupedDesc* __restrict__ d, int64_t count,
-                                                           const int64_t* __restrict__ tile_begin) {
-  __shared__ __attribute__((aligned(16))) double lds[4 * kSlabF64];  // one slab per wave
-  const int lane = threadIdx.x & 63;
-  double* slab64 = lds + (threadIdx.x >> 6) * kSlabF64;
-  float* slab32 = reinterpret_cast<float*>(slab64);
-  const int64_t total = tile_begin[count];
-  const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
-  const int64_t first = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  for (int64_t t = first; t < total; t += waves) {
-    // product p with tile_begin[p] <= t < tile_begin[p+1]
-    int64_t lo = 0, hi = count - 1;
-    while (lo < hi) {
-      const int64_t mid = (lo + hi + 1) >> 1;
-      if (tile_begin[mid] <= t)
-        lo = mid;
-      else
-        hi = mid - 1;
-    }
-    const GroupedDesc g = d[lo];
-    const int local = (int)(t - tile_begin[lo]);
-    const int tm = local % g.tiles_m, tn = local / g.tiles_m;
-    if (g.dtype == EH_F32)
-      dispatch_trans(g.ta, g.tb, [&](auto ta, auto tb) { grouped_tile_f32<ta, tb>(g, tm, tn, lane, slab32); });
-    else
-      dispatch_trans(g.ta, g.tb, [&](auto ta, auto tb) { grouped_tile_f64<ta, tb>(g, tm, tn, lane, slab64); });
-  }
+// Bytes spanned by a column-major operand of `rows` x `cols` (leading
+// dimension ld); 0 for an empty one.
+static uint64_t extent_bytes(int64_t rows, int64_t cols, int64_t ld, int es) {
+  if (rows <= 0 || cols <= 0) return 0;
+  return ((uint64_t)(cols - 1) * (uint64_t)ld + (uint64_t)rows) * (uint64_t)es;
+}
+
+// Estimated cost of one tile: its k steps (8 MFMAs of equal duration each,
+// for both dtypes) plus the store and the first load.
+static int64_t tile_cost(const GroupedDesc& d) {
+  const int step = d.dtype == EH_F64 ? kStepF64 : kStepF32;
+  return 2 + (d.k + step - 1) / step;
 }
 
 }  // namespace eh
@@ -272,31 +361,38 @@ __global__ void __launch_bounds__(256) grouped_gemm_kernel(const GroupedDesc* __
 using namespace eh;
 
 struct eh_grouped {
-  std::vector<eh_gemm_problem> host;  // as given (for the per-product path)
+  std::vector<eh_gemm_problem> host;   // as given (for the per-product path)
+  std::vector<eh_gemm_problem> large;  // products past 32-bit buffer offsets
   GroupedDesc* d_desc = nullptr;
   int64_t* d_begin = nullptr;
-  int64_t count = 0;
+  int64_t* d_wave_tile = nullptr;
+  int32_t* d_wave_prod = nullptr;
+  int64_t count = 0;   // products in the kernel's work list
   int64_t tiles = 0;
+  int64_t groups = 0;  // workgroups launched (kWavesPerGroup waves each)
   hipGraphExec_t exec = nullptr;
   hipGraph_t graph = nullptr;
   std::mutex mu;
 };
 
 static eh_status grouped_enqueue(eh_grouped* g, hipStream_t s) {
-  int dev = 0, cus = 256;
-  if (hipGetDevice(&dev) == hipSuccess) {
-    hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0)
-      cus = prop.multiProcessorCount;
+  if (g->count > 0) {
+    hipLaunchKernelGGL(grouped_gemm_kernel, dim3((unsigned)g->groups), dim3(64 * kWavesPerGroup), 0, s, g->d_desc,
+                       g->d_begin, g->d_wave_tile, g->d_wave_prod);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "grouped kernel launch");
   }
-  hipLaunchKernelGGL(grouped_build_kernel, dim3(1), dim3(1024), 0, s, g->d_desc, g->count, g->d_begin);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return hip_fail(e, "grouped build launch");
-  // 8 waves per SIMD: every wave is independent (no LDS, no barrier)
-  const int64_t want = std::max<int64_t>(1, std::min<int64_t>((int64_t)cus * 8, (g->tiles + 3) / 4));
-  hipLaunchKernelGGL(grouped_gemm_kernel, dim3((unsigned)want), dim3(256), 0, s, g->d_desc, g->count, g->d_begin);
-  e = hipGetLastError();
-  return e == hipSuccess ? EH_SUCCESS : hip_fail(e, "grouped kernel launch");
+  for (const eh_gemm_problem& q : g->large) {
+    // one 128x128 tile configuration without split-K: no workspace, so the
+    // launch can be captured in a graph
+    const float af = (float)q.alpha, bf = (float)q.beta;
+    const void* al = q.dtype == EH_F32 ? (const void*)&af : (const void*)&q.alpha;
+    const void* be = q.dtype == EH_F32 ? (const void*)&bf : (const void*)&q.beta;
+    eh_status st = gemm_dispatch(q.dtype, q.opa, q.opb, q.m, q.n, q.k, al, q.A, q.lda, q.B, q.ldb, be, q.C, q.ldc,
+                                 EH_PREC_PLAIN, 3 * kNumSplits, s);
+    if (st != EH_SUCCESS) return st;
+  }
+  return EH_SUCCESS;
 }
 
 extern "C" {
@@ -306,8 +402,9 @@ eh_status eh_grouped_create(const eh_gemm_problem* problems, int64_t count, eh_g
   *out = nullptr;
   if (count <= 0 || !problems) return fail(EH_ERR_INVALID, "need at least one product");
   std::vector<GroupedDesc> desc;
+  std::vector<eh_gemm_problem> large;
   desc.reserve(count);
-  int64_t tiles = 0;
+  constexpr uint64_t kMaxBytes = 0x7FFFFFFFull;  // buffer offsets of valid elements stay below 2^31
   for (int64_t p = 0; p < count; ++p) {
     const eh_gemm_problem& q = problems[p];
     const double one = 1.0;
@@ -315,39 +412,109 @@ eh_status eh_grouped_create(const eh_gemm_problem* problems, int64_t count, eh_g
                                    q.ldc);
     if (st != EH_SUCCESS) return fail(st, "product " + std::to_string(p) + ": " + eh_last_error());
     if (q.m == 0 || q.n == 0) continue;  // nothing to do
+    const bool ta = q.opa != EH_OP_N, tb = q.opb != EH_OP_N;
+    const int es = q.dtype == EH_F64 ? 8 : 4;
+    const bool reads = q.k > 0 && q.alpha != 0.0;  // alpha == 0: A and B are never read
+    const uint64_t ea = reads ? (ta ? extent_bytes(q.k, q.m, q.lda, es) : extent_bytes(q.m, q.k, q.lda, es)) : 0;
+    const uint64_t eb = reads ? (tb ? extent_bytes(q.n, q.k, q.ldb, es) : extent_bytes(q.k, q.n, q.ldb, es)) : 0;
+    const int64_t tm = (q.m + 31) / 32, tn = (q.n + 31) / 32;
+    if (ea > kMaxBytes || eb > kMaxBytes || q.m > (1 << 30) || q.n > (1 << 30) || tm * tn > (1ll << 30)) {
+      large.push_back(q);
+      continue;
+    }
     GroupedDesc d{};
     d.A = q.A;
     d.B = q.B;
     d.C = q.C;
-    d.lda = q.lda;
-    d.ldb = q.ldb;
     d.ldc = q.ldc;
-    d.alpha = (q.k == 0) ? 0.0 : q.alpha;
+    d.alpha = reads ? q.alpha : 0.0;
     d.beta = q.beta;
+    d.bytesA = (uint32_t)ea;
+    d.bytesB = (uint32_t)eb;
+    d.lda = (uint32_t)std::min<int64_t>(q.lda, kMaxBytes);
+    d.ldb = (uint32_t)std::min<int64_t>(q.ldb, kMaxBytes);
     d.dtype = q.dtype;
-    d.ta = q.opa != EH_OP_N;
-    d.tb = q.opb != EH_OP_N;
+    d.ta = ta;
+    d.tb = tb;
     d.m = (int32_t)q.m;
     d.n = (int32_t)q.n;
-    d.k = d.alpha == 0.0 ? 0 : (int32_t)q.k;  // alpha == 0: A and B are never read
+    d.k = reads ? (int32_t)q.k : 0;
+    d.tiles_m = (int32_t)tm;
     desc.push_back(d);
-    tiles += ((q.m + 31) / 32) * ((q.n + 31) / 32);
   }
-  // longest products first: the round-robin deal then balances like longest-first scheduling
+  // FP32 products first, then FP64; each group deepest first
   std::stable_sort(desc.begin(), desc.end(), [](const GroupedDesc& x, const GroupedDesc& y) {
-    return (int64_t)x.k * (x.dtype == EH_F64 ? 2 : 1) > (int64_t)y.k * (y.dtype == EH_F64 ? 2 : 1);
+    if (x.dtype != y.dtype) return x.dtype < y.dtype;
+    return x.k > y.k;
   });
   auto* g = new eh_grouped();
   g->host.assign(problems, problems + count);
+  g->large = std::move(large);
   g->count = (int64_t)desc.size();
-  g->tiles = tiles;
   if (g->count == 0) {
     *out = g;
     return EH_SUCCESS;
   }
-  hipError_t e = hipMalloc(&g->d_desc, sizeof(GroupedDesc) * g->count);
+
+  // work list: first tile of each product and cumulative cost
+  std::vector<int64_t> begin(g->count + 1);
+  std::vector<int64_t> cost_begin(g->count + 1);
+  begin[0] = cost_begin[0] = 0;
+  for (int64_t p = 0; p < g->count; ++p) {
+    const int64_t tiles = (int64_t)desc[p].tiles_m * ((desc[p].n + 31) / 32);
+    begin[p + 1] = begin[p] + tiles;
+    cost_begin[p + 1] = cost_begin[p] + tiles * tile_cost(desc[p]);
+  }
+  g->tiles = begin[g->count];
+
+  // resident waves: as many workgroups as fit on every CU at once
+  int dev = 0, cus = 256, per_cu = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess) {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0)
+      cus = prop.multiProcessorCount;
+  }
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, grouped_gemm_kernel, 64 * kWavesPerGroup, 0) !=
+          hipSuccess ||
+      per_cu < 1)
+    per_cu = 1;
+  int64_t groups = (int64_t)cus * per_cu;
+  groups = std::min<int64_t>(groups, (g->tiles + kWavesPerGroup - 1) / kWavesPerGroup);
+  if (groups > 8) groups = (groups + 7) / 8 * 8;  // whole XCD rows for the remap
+  g->groups = std::max<int64_t>(groups, 1);
+  const int64_t W = g->groups * kWavesPerGroup;
+
+  // partition the tile sequence into W runs of equal cost: run v starts at
+  // the first tile whose starting cost reaches v * total / W
+  std::vector<int64_t> wave_tile(W + 1);
+  std::vector<int32_t> wave_prod(W);
+  const int64_t total_cost = cost_begin[g->count];
+  int64_t p = 0;
+  for (int64_t v = 0; v < W; ++v) {
+    const int64_t target = (int64_t)((__int128)total_cost * v / W);
+    while (p < g->count - 1 && cost_begin[p + 1] <= target) ++p;
+    const int64_t c = tile_cost(desc[p]);
+    int64_t t = begin[p] + (target - cost_begin[p] + c - 1) / c;
+    t = std::min(t, begin[p + 1]);
+    if (v > 0) t = std::max(t, wave_tile[v - 1]);
+    wave_tile[v] = t;
+    // product holding tile t (the next product when t ends this one)
+    int64_t q = p;
+    while (q < g->count - 1 && begin[q + 1] <= t) ++q;
+    wave_prod[v] = (int32_t)q;
+  }
+  wave_tile[W] = g->tiles;
+
+  e = hipMalloc(&g->d_desc, sizeof(GroupedDesc) * g->count);
   if (e == hipSuccess) e = hipMalloc(&g->d_begin, sizeof(int64_t) * (g->count + 1));
+  if (e == hipSuccess) e = hipMalloc(&g->d_wave_tile, sizeof(int64_t) * (W + 1));
+  if (e == hipSuccess) e = hipMalloc(&g->d_wave_prod, sizeof(int32_t) * W);
   if (e == hipSuccess) e = hipMemcpy(g->d_desc, desc.data(), sizeof(GroupedDesc) * g->count, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(g->d_begin, begin.data(), sizeof(int64_t) * (g->count + 1), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(g->d_wave_tile, wave_tile.data(), sizeof(int64_t) * (W + 1), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(g->d_wave_prod, wave_prod.data(), sizeof(int32_t) * W, hipMemcpyHostToDevice);
   if (e != hipSuccess) {
     eh_grouped_destroy(g);
     return fail(EH_ERR_ALLOC, std::string("grouped setup: ") + hipGetErrorString(e));
@@ -358,13 +525,12 @@ eh_status eh_grouped_create(const eh_gemm_problem* problems, int64_t count, eh_g
 
 eh_status eh_grouped_run(eh_grouped* g, void* stream) {
   if (!g) return fail(EH_ERR_INVALID, "NULL handle");
-  if (g->count == 0) return EH_SUCCESS;
   return grouped_enqueue(g, static_cast<hipStream_t>(stream));
 }
 
 eh_status eh_grouped_graph_launch(eh_grouped* g, void* stream) {
   if (!g) return fail(EH_ERR_INVALID, "NULL handle");
-  if (g->count == 0) return EH_SUCCESS;
+  if (g->count == 0 && g->large.empty()) return EH_SUCCESS;
   hipStream_t s = static_cast<hipStream_t>(stream);
   std::lock_guard<std::mutex> lk(g->mu);
   if (!g->exec) {
@@ -374,18 +540,21 @@ eh_status eh_grouped_graph_launch(eh_grouped* g, void* stream) {
     if (e != hipSuccess) return hip_fail(e, "hipStreamCreate");
     e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) {
-      hipStreamDestroy(cs);
+      (void)hipStreamDestroy(cs);
       return hip_fail(e, "hipStreamBeginCapture");
     }
     eh_status st = grouped_enqueue(g, cs);
     hipGraph_t graph = nullptr;
     e = hipStreamEndCapture(cs, &graph);
-    hipStreamDestroy(cs);
-    if (st != EH_SUCCESS) return st;
+    (void)hipStreamDestroy(cs);
+    if (st != EH_SUCCESS) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return st;
+    }
     if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
     e = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0);
     if (e != hipSuccess) {
-      hipGraphDestroy(graph);
+      (void)hipGraphDestroy(graph);
       g->exec = nullptr;
       return hip_fail(e, "hipGraphInstantiate");
     }
@@ -415,16 +584,18 @@ eh_status eh_grouped_run_per_product(eh_grouped* g, void* stream) {
 eh_status eh_grouped_info(eh_grouped* g, int64_t* tiles, int64_t* products) {
   if (!g) return fail(EH_ERR_INVALID, "NULL handle");
   if (tiles) *tiles = g->tiles;
-  if (products) *products = g->count;
+  if (products) *products = g->count + (int64_t)g->large.size();
   return EH_SUCCESS;
 }
 
 eh_status eh_grouped_destroy(eh_grouped* g) {
   if (!g) return EH_SUCCESS;
-  if (g->exec) hipGraphExecDestroy(g->exec);
-  if (g->graph) hipGraphDestroy(g->graph);
-  if (g->d_desc) hipFree(g->d_desc);
-  if (g->d_begin) hipFree(g->d_begin);
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  if (g->d_desc) (void)hipFree(g->d_desc);
+  if (g->d_begin) (void)hipFree(g->d_begin);
+  if (g->d_wave_tile) (void)hipFree(g->d_wave_tile);
+  if (g->d_wave_prod) (void)hipFree(g->d_wave_prod);
   delete g;
   return EH_SUCCESS;
 }

@@ -1,0 +1,171 @@
+"""Short driver for profiler runs, and the summariser of their output.
+
+    python3 -m eigen_amd.profile_kernels run {split3,plain,dgemm,batched,copy} [reps]
+    python3 -m eigen_amd.profile_kernels summarize <rocprofv3 output root>
+
+`run` launches one workload a few times (the program under
+`rocprofv3 --kernel-trace --stats` or `rocprofv3 --pmc`):
+
+  split3   sgemm 8192^3, split-BF16 three terms (pack A, pack B, packed product)
+  plain    sgemm 8192^3 on FP32 MFMA
+  dgemm    dgemm 4096^3, C = 1.5*A*B + 0.5*C
+  batched  20,000 uneven products in one grouped launch (captured graph)
+  copy     a 1 GiB device-to-device copy: calibrates the HBM byte counters
+
+`summarize` reads directories named trace_<workload> (kernel-trace csv) and
+pmc_<workload>_<FETCH_SIZE|WRITE_SIZE> (counter csv) under the given root and
+prints per-kernel duration tables and counter readings as markdown, plus the
+HBM bytes per launch of each workload's dominant kernel: each counter
+reading scaled by bytes-moved / reading of the copy kernel (the copy reads
+and writes exactly 2^30 bytes).
+"""
+from __future__ import annotations
+
+import csv
+import glob
+import json
+import os
+import sys
+
+COPY_BYTES = 1 << 30
+# dominant kernel of each workload (substring of its name)
+DOMINANT = {"split3": "gemm_split_packed_kernel", "plain": "gemm_mfma_kernel<float", "dgemm": "gemm_mfma_kernel<double",
+            "batched": "grouped_gemm_kernel", "copy": "copyBuffer"}
+
+
+def run(what: str, reps: int) -> None:
+    import torch
+
+    import eigen_amd as E
+
+    E.lib()
+    if what in ("split3", "plain"):
+        n = 8192
+        A = torch.empty(n * n, dtype=torch.float32, device="cuda")
+        B = torch.empty(n * n, dtype=torch.float32, device="cuda")
+        C = torch.empty(n * n, dtype=torch.float32, device="cuda")
+        E.fill(E.F32, A, n * n, 1, E.DIST_WIDE, -8, 8)
+        E.fill(E.F32, B, n * n, 2, E.DIST_WIDE, -8, 8)
+        prec = E.PREC_SPLIT3 if what == "split3" else E.PREC_PLAIN
+        for _ in range(reps):
+            E.gemm(E.F32, 0, 0, n, n, n, 1.0, A, n, B, n, 0.0, C, n, prec=prec)
+    elif what == "dgemm":
+        n = 4096
+        A, B, C = (torch.empty(n * n, dtype=torch.float64, device="cuda") for _ in range(3))
+        for i, t in enumerate((A, B, C)):
+            E.fill(E.F64, t, n * n, 11 + i, E.DIST_NORMAL, 0, 1)
+        for _ in range(reps):
+            E.gemm(E.F64, 0, 0, n, n, n, 1.5, A, n, B, n, 0.5, C, n)
+    elif what == "batched":
+        from eigen_amd.batched import uneven_batch
+        b = uneven_batch(20000, seed=0)
+        g = E.Grouped(b.problems)
+        for _ in range(reps):
+            g.graph_launch()
+    elif what == "copy":
+        x = torch.empty(COPY_BYTES // 4, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(reps):
+            y.copy_(x)
+    else:
+        raise SystemExit(f"unknown workload {what}")
+    torch.cuda.synchronize()
+    print("done", what, flush=True)
+
+
+def _one(d: str, pattern: str):
+    f = sorted(glob.glob(os.path.join(d, "**", pattern), recursive=True))
+    return list(csv.DictReader(open(f[0]))) if f else None
+
+
+def kernel_table(d: str):
+    rows = _one(d, "*kernel_stats.csv")
+    if rows is None:
+        return None, {}
+    out = ["| kernel | calls | avg (us) | min (us) | max (us) | % of GPU time |", "|---|---|---|---|---|---|"]
+    avg = {}
+    for r in rows:
+        name = r["Name"]
+        avg[name] = float(r["AverageNs"]) / 1e3
+        short = name if len(name) <= 100 else name[:100] + "..."
+        out.append(f"| `{short}` | {r['Calls']} | {avg[name]:.1f} | {float(r['MinNs']) / 1e3:.1f} | "
+                   f"{float(r['MaxNs']) / 1e3:.1f} | {float(r['Percentage']):.1f} |")
+    return "\n".join(out), avg
+
+
+def counter_means(d: str, counter: str) -> dict:
+    """Mean over dispatches of the counter (summed over its instances) per kernel name, in KiB."""
+    rows = _one(d, "*counter_collection.csv") or []
+    per_dispatch = {}
+    for r in rows:
+        if r["Counter_Name"] != counter:
+            continue
+        key = (r["Dispatch_Id"], r["Kernel_Name"])
+        per_dispatch[key] = per_dispatch.get(key, 0.0) + float(r["Counter_Value"])
+    per_kernel = {}
+    for (_, name), v in per_dispatch.items():
+        per_kernel.setdefault(name, []).append(v)
+    return {n: sum(v) / len(v) for n, v in per_kernel.items()}
+
+
+def _dominant(values: dict, workload: str):
+    for name, v in values.items():
+        if DOMINANT[workload] in name:
+            return name, v
+    return None, None
+
+
+def summarize(root: str) -> dict:
+    traffic = {}
+    scale = {}
+    for c in ("FETCH_SIZE", "WRITE_SIZE"):
+        _, kib = _dominant(counter_means(os.path.join(root, f"pmc_copy_{c}"), c), "copy")
+        if kib:
+            scale[c] = COPY_BYTES / kib  # bytes per counter unit
+    print("## Counter calibration (1 GiB device copy)\n")
+    for c, s in scale.items():
+        print(f"- {c}: {COPY_BYTES / s:.1f} KiB read by the counter for {COPY_BYTES} bytes -> x{s / 1024:.4f} KiB")
+    print()
+    for d in sorted(glob.glob(os.path.join(root, "trace_*"))):
+        table, _ = kernel_table(d)
+        if table:
+            print(f"## {os.path.basename(d)}\n\n{table}\n")
+    workloads = sorted({os.path.basename(p).split("_")[1] for p in glob.glob(os.path.join(root, "pmc_*"))})
+    for w in workloads:
+        if w == "copy":
+            continue
+        got = {}
+        for c in ("FETCH_SIZE", "WRITE_SIZE"):
+            name, kib = _dominant(counter_means(os.path.join(root, f"pmc_{w}_{c}"), c), w)
+            if kib is not None and c in scale:
+                got[c] = (name, kib, kib * scale[c])
+        if len(got) == 2:
+            name = got["FETCH_SIZE"][0]
+            traffic[w] = {"kernel": name, "fetch_bytes": round(got["FETCH_SIZE"][2]),
+                          "write_bytes": round(got["WRITE_SIZE"][2]),
+                          "bytes": round(got["FETCH_SIZE"][2] + got["WRITE_SIZE"][2]),
+                          "fetch_kib_raw": round(got["FETCH_SIZE"][1], 1), "write_kib_raw": round(got["WRITE_SIZE"][1], 1)}
+            print(f"## HBM traffic per launch: {w}\n\n`{name[:100]}`: FETCH_SIZE {got['FETCH_SIZE'][1]:.1f} KiB raw -> "
+                  f"{got['FETCH_SIZE'][2]:.4g} B; WRITE_SIZE {got['WRITE_SIZE'][1]:.1f} KiB raw -> "
+                  f"{got['WRITE_SIZE'][2]:.4g} B; total {traffic[w]['bytes']:.4g} B\n")
+    return traffic
+
+
+def main(argv=None) -> int:
+    argv = sys.argv[1:] if argv is None else argv
+    if len(argv) >= 2 and argv[0] == "run":
+        run(argv[1], int(argv[2]) if len(argv) > 2 else 5)
+        return 0
+    if len(argv) >= 2 and argv[0] == "summarize":
+        traffic = summarize(argv[1])
+        if len(argv) > 2:
+            with open(argv[2], "w") as f:
+                json.dump(traffic, f, indent=1)
+                f.write("\n")
+        return 0
+    print(__doc__)
+    return 2
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -103,3 +103,63 @@ def test_grouped_random_within_bound(gpu):
         u = np.finfo(A.dtype).eps / 2
         assert np.linalg.norm(Cg.astype(np.float64) - ref) <= O.higham_mary_bound(absref, k, u) + 1e-300
     g.close()
+
+
+def _dev(x):
+    import torch
+    return torch.from_numpy(np.asarray(x).reshape(-1, order="F").copy()).cuda()
+
+
+def test_grouped_single_small_product(gpu):
+    """One product of one tile: a grid of one workgroup, most waves idle."""
+    import torch
+    A = O.fill((5, 3), np.float64, 1, O.INT, -8, 8)
+    B = O.fill((3, 7), np.float64, 2, O.INT, -8, 8)
+    dA, dB, dC = _dev(A), _dev(B), torch.zeros(35, dtype=torch.float64, device="cuda")
+    g = E.Grouped([E.GemmProblem(dtype=E.F64, opa=0, opb=0, m=5, n=7, k=3, lda=5, ldb=3, ldc=5, A=dA.data_ptr(),
+                                 B=dB.data_ptr(), C=dC.data_ptr(), alpha=2.0, beta=0.0)])
+    g.graph_launch()
+    torch.cuda.synchronize()
+    got = dC.cpu().numpy().reshape((7, 5)).T
+    assert np.array_equal(got, O.gemm_ref(False, False, 2.0, A, B))
+    g.close()
+
+
+def test_grouped_operand_past_31_bit_offsets(gpu):
+    """A product whose A spans more than 2^31 bytes (op(A) = A^T with its three
+    stored columns 2^28 floats apart) runs through the single-product path
+    after the kernel, in the same launch and the same captured graph, next to
+    ordinary products."""
+    import torch
+    m, n, k, lda = 3, 5, 2, (1 << 28) + 16
+    bigA = torch.zeros((m - 1) * lda + k, dtype=torch.float32, device="cuda")
+    At = O.fill((k, m), np.float32, 3, O.INT, -8, 8)  # stored k x m, op(A) = At^T
+    for i in range(m):
+        bigA[i * lda:i * lda + k] = torch.from_numpy(At[:, i].copy()).cuda()
+    B = O.fill((k, n), np.float32, 4, O.INT, -8, 8)
+    dB, dC = _dev(B), torch.zeros(m * n, dtype=torch.float32, device="cuda")
+    small = []
+    host = []
+    for p in range(6):
+        As = O.fill((9, 20), np.float32, 10 + p, O.INT, -8, 8)
+        Bs = O.fill((20, 11), np.float32, 20 + p, O.INT, -8, 8)
+        dAs, dBs, dCs = _dev(As), _dev(Bs), torch.zeros(99, dtype=torch.float32, device="cuda")
+        host.append((As, Bs, dAs, dBs, dCs))
+        small.append(E.GemmProblem(dtype=E.F32, opa=0, opb=0, m=9, n=11, k=20, lda=9, ldb=20, ldc=9,
+                                   A=dAs.data_ptr(), B=dBs.data_ptr(), C=dCs.data_ptr(), alpha=1.0, beta=0.0))
+    big = E.GemmProblem(dtype=E.F32, opa=1, opb=0, m=m, n=n, k=k, lda=lda, ldb=k, ldc=m, A=bigA.data_ptr(),
+                        B=dB.data_ptr(), C=dC.data_ptr(), alpha=1.0, beta=0.0)
+    g = E.Grouped(small[:3] + [big] + small[3:])
+    assert g.info() == (6, 7)  # six one-tile products in the kernel, seven products in all
+    for _ in range(2):
+        dC.zero_()
+        g.graph_launch()
+        torch.cuda.synchronize()
+        got = dC.cpu().numpy().reshape((n, m)).T
+        assert np.array_equal(got.astype(np.float64), O.gemm_ref(True, False, 1.0, At, B))
+        for As, Bs, _, _, dCs in host:
+            assert np.array_equal(dCs.cpu().numpy().reshape((11, 9)).T.astype(np.float64),
+                                  O.gemm_ref(False, False, 1.0, As, Bs))
+    g.close()
+    del bigA
+    torch.cuda.empty_cache()
