@@ -275,10 +275,6 @@ static eh_status launch_product(bool ta, bool tb, int64_t m, int64_t n, int64_t 
   p.beta = beta;
   p.bytesA = (uint32_t)((ta ? (uint64_t)(m - 1) * lda + k : (uint64_t)(k - 1) * lda + m) * sizeof(T));
   p.bytesB = (uint32_t)((!tb ? (uint64_t)(n - 1) * ldb + k : (uint64_t)(k - 1) * ldb + n) * sizeof(T));
-  constexpr int EPC = 16 / sizeof(T);
-  // 16-byte loads along the k-contiguous operand need aligned rows.
-  p.vecA = ta && ((uintptr_t)A % 16 == 0) && (lda % EPC == 0);
-  p.vecB = !tb && ((uintptr_t)B % 16 == 0) && (ldb % EPC == 0);
 
   int tile = 0, splits = 1, kslice = (int)(128 / sizeof(T));
   hipError_t e;

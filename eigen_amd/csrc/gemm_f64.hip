@@ -6,32 +6,34 @@ namespace eh {
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-template <typename T, int BM, int BN, int WGM, int WGN, bool TA, bool TB>
+template <typename T, int BM, int BN, int WGM, int WGN, int RS, bool TA, bool TB>
 static hipError_t launch_one(const GemmParams<T>& p, int splits, hipStream_t s) {
   const int ntm = (p.M + BM - 1) / BM, ntn = (p.N + BN - 1) / BN;
   dim3 grid(ntm * ntn, splits);
-  hipLaunchKernelGGL((gemm_mfma_kernel<T, BM, BN, WGM, WGN, TA, TB>), grid, dim3(64 * WGM * WGN), 0, s, p);
+  hipLaunchKernelGGL((gemm_mfma_kernel<T, BM, BN, WGM, WGN, TA, TB, RS>), grid, dim3(64 * WGM * WGN), 0, s, p);
   return hipGetLastError();
 }
 
-template <typename T, int BM, int BN, int WGM = 2, int WGN = 2>
+// RS: register stages of the global loads (2 where the registers fit
+// without lowering occupancy)
+template <typename T, int BM, int BN, int WGM, int WGN, int RS>
 static hipError_t launch_tile(const GemmParams<T>& p, bool ta, bool tb, int splits, hipStream_t s) {
-  if (!ta && !tb) return launch_one<T, BM, BN, WGM, WGN, false, false>(p, splits, s);
-  if (ta && !tb) return launch_one<T, BM, BN, WGM, WGN, true, false>(p, splits, s);
-  if (!ta && tb) return launch_one<T, BM, BN, WGM, WGN, false, true>(p, splits, s);
-  return launch_one<T, BM, BN, WGM, WGN, true, true>(p, splits, s);
+  if (!ta && !tb) return launch_one<T, BM, BN, WGM, WGN, RS, false, false>(p, splits, s);
+  if (ta && !tb) return launch_one<T, BM, BN, WGM, WGN, RS, true, false>(p, splits, s);
+  if (!ta && tb) return launch_one<T, BM, BN, WGM, WGN, RS, false, true>(p, splits, s);
+  return launch_one<T, BM, BN, WGM, WGN, RS, true, true>(p, splits, s);
 }
 
 template <typename T>
 hipError_t launch_gemm_mfma(const GemmParams<T>& p, int tile, bool ta, bool tb, int splits, hipStream_t s) {
   switch (tile) {
-    case 0: return launch_tile<T, 64, 64>(p, ta, tb, splits, s);
-    case 1: return launch_tile<T, 64, 128>(p, ta, tb, splits, s);
-    case 2: return launch_tile<T, 128, 64>(p, ta, tb, splits, s);
-    case 3: return launch_tile<T, 128, 128>(p, ta, tb, splits, s);
+    case 0: return launch_tile<T, 64, 64, 2, 2, 2>(p, ta, tb, splits, s);
+    case 1: return launch_tile<T, 64, 128, 2, 2, 1>(p, ta, tb, splits, s);
+    case 2: return launch_tile<T, 128, 64, 2, 2, 1>(p, ta, tb, splits, s);
+    case 3: return launch_tile<T, 128, 128, 2, 2, 1>(p, ta, tb, splits, s);
     // 8 waves (two per SIMD), wave tiles of 64x64
-    case 4: return launch_tile<T, 128, 256, 2, 4>(p, ta, tb, splits, s);
-    case 5: return launch_tile<T, 256, 128, 4, 2>(p, ta, tb, splits, s);
+    case 4: return launch_tile<T, 128, 256, 2, 4, 2>(p, ta, tb, splits, s);
+    case 5: return launch_tile<T, 256, 128, 4, 2, 2>(p, ta, tb, splits, s);
     default: return hipErrorInvalidValue;
   }
 }

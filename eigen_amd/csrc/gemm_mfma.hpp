@@ -20,7 +20,7 @@
 
 namespace eh {
 
-template <typename T, int BM, int BN, int WGM, int WGN, bool TA, bool TB>
+template <typename T, int BM, int BN, int WGM, int WGN, bool TA, bool TB, int RS>
 __global__ void __launch_bounds__(64 * WGM * WGN) gemm_mfma_kernel(GemmParams<T> p) {
   using F = Mfma<T>;
   constexpr int NT = 64 * WGM * WGN;
@@ -36,6 +36,7 @@ __global__ void __launch_bounds__(64 * WGM * WGN) gemm_mfma_kernel(GemmParams<T>
   static_assert(RM >= 1 && RN >= 1, "wave tile smaller than one MFMA block");
   static_assert(BM * CH % NT == 0 && BN * CH % NT == 0, "tile not divisible by threads");
   static_assert(CPL * EPC == KPL, "lane k range must be whole chunks");
+  static_assert(RS == 1 || RS == 2, "one or two register stages");
 
   __shared__ __attribute__((aligned(16))) unsigned char smem[2 * (BM + BN) * ROWB];
 
@@ -53,10 +54,14 @@ __global__ void __launch_bounds__(64 * WGM * WGN) gemm_mfma_kernel(GemmParams<T>
   const int kend = min(p.K, kbeg + p.k_per_split);
   const int nkt = kend > kbeg ? (kend - kbeg + BK - 1) / BK : 0;
 
-  u32x4 ra[A_CH], rb[B_CH];
   constexpr uint32_t SZ = sizeof(T);
-  const __amdgpu_buffer_rsrc_t rA = make_rsrc(p.A, p.bytesA);
-  const __amdgpu_buffer_rsrc_t rB = make_rsrc(p.B, p.bytesB);
+  // A rows-contiguous operand (k is its column index) gets a buffer range
+  // that ends at column kend, so every element with k >= kend reads zero;
+  // k-contiguous operands are masked in registers instead.
+  const uint32_t bytesA = TA ? p.bytesA : (uint32_t)min((uint64_t)p.bytesA, (uint64_t)kend * p.lda * SZ);
+  const uint32_t bytesB = !TB ? p.bytesB : (uint32_t)min((uint64_t)p.bytesB, (uint64_t)kend * p.ldb * SZ);
+  const __amdgpu_buffer_rsrc_t rA = make_rsrc(p.A, bytesA);
+  const __amdgpu_buffer_rsrc_t rB = make_rsrc(p.B, bytesB);
 
   // Loop-invariant per-lane byte offsets of every staged chunk; the k advance
   // goes into the wave-uniform soffset.  Rows / columns past M / N read zeros
@@ -83,28 +88,15 @@ __global__ void __launch_bounds__(64 * WGM * WGN) gemm_mfma_kernel(GemmParams<T>
     }
   }
 
-  // k-contiguous chunk: one 16-byte load (or EPC element loads when rows are
-  // not 16-byte aligned); on the final partial slice, k >= kend is zeroed.
-  auto load_kcont = [&](__amdgpu_buffer_rsrc_t rs, uint32_t vo, int k0, int c, bool vec, bool tail) -> u32x4 {
-    u32x4 v;
-    if (vec) {
-      v = buf_load16(rs, vo, (uint32_t)k0 * SZ);
-    } else {
-      T t[EPC];
-#pragma unroll
-      for (int e = 0; e < EPC; ++e) t[e] = buf_load(rs, vo + e * SZ, (uint32_t)k0 * SZ, (T*)nullptr);
-      v = *reinterpret_cast<u32x4*>(t);
-    }
-    if (tail) {
-      T* t = reinterpret_cast<T*>(&v);
-#pragma unroll
-      for (int e = 0; e < EPC; ++e)
-        if (k0 + c * EPC + e >= kend) t[e] = T(0);
-    }
-    return v;
+  // One BK slice of both operands in registers.  No load depends on data or
+  // on the slice index, so the code between loads and their use has no
+  // branches and each wait covers exactly the loads it needs.
+  struct Stage {
+    u32x4 a[A_CH], b[B_CH];
   };
-  // rows-contiguous chunk: EPC loads, one per k, coalesced across the wave;
-  // k >= K lies past the end of the operand and reads zero.
+  // k-contiguous chunk: one 16-byte load (buffer loads need 4-byte alignment
+  // only); rows-contiguous chunk: EPC loads, one per k, coalesced across the
+  // wave.
   auto load_mncont = [&](__amdgpu_buffer_rsrc_t rs, uint32_t vo, int k0, int cu, int64_t ld) -> u32x4 {
     T t[EPC];
 #pragma unroll
@@ -112,43 +104,51 @@ __global__ void __launch_bounds__(64 * WGM * WGN) gemm_mfma_kernel(GemmParams<T>
       t[e] = buf_load(rs, vo, (uint32_t)((uint64_t)(k0 + cu * EPC + e) * ld * SZ), (T*)nullptr);
     return *reinterpret_cast<u32x4*>(t);
   };
-
-  // ---- global -> registers for one BK slice starting at k0 ----------------
-  auto load_a = [&](int k0, bool tail) {
+  // slice s (clamped to the last one: a prefetch past the end re-reads it)
+  auto load = [&](Stage& st, int s) {
+    const int k0 = kbeg + min(s, nkt - 1) * BK;
 #pragma unroll
     for (int j = 0; j < A_CH; ++j) {
       const int q = tid + NT * j;
       if constexpr (TA)
-        ra[j] = load_kcont(rA, voA[j], k0, q % CH, p.vecA, tail);
+        st.a[j] = buf_load16(rA, voA[j], (uint32_t)k0 * SZ);
       else
-        ra[j] = load_mncont(rA, voA[j], k0, __builtin_amdgcn_readfirstlane(q / BM), p.lda);
+        st.a[j] = load_mncont(rA, voA[j], k0, __builtin_amdgcn_readfirstlane(q / BM), p.lda);
     }
-  };
-  auto load_b = [&](int k0, bool tail) {
 #pragma unroll
     for (int j = 0; j < B_CH; ++j) {
       const int q = tid + NT * j;
       if constexpr (!TB)
-        rb[j] = load_kcont(rB, voB[j], k0, q % CH, p.vecB, tail);
+        st.b[j] = buf_load16(rB, voB[j], (uint32_t)k0 * SZ);
       else
-        rb[j] = load_mncont(rB, voB[j], k0, __builtin_amdgcn_readfirstlane(q / BN), p.ldb);
+        st.b[j] = load_mncont(rB, voB[j], k0, __builtin_amdgcn_readfirstlane(q / BN), p.ldb);
     }
   };
+  // k-contiguous chunk c of the slice at k0: zero the elements at k >= kend
+  // (selects, no branch; a no-op except on the last slice)
+  auto mask_kcont = [&](u32x4& v, int k0, int c) {
+    T* t = reinterpret_cast<T*>(&v);
+#pragma unroll
+    for (int e = 0; e < EPC; ++e) t[e] = (k0 + c * EPC + e < kend) ? t[e] : T(0);
+  };
   // ---- registers -> LDS buffer -------------------------------------------
-  auto store_tiles = [&](int buf) {
+  auto store = [&](Stage& st, int s, int buf) {
+    const int k0 = kbeg + min(s, nkt - 1) * BK;
     unsigned char* As = smem + buf * (BM + BN) * ROWB;
     unsigned char* Bs = As + BM * ROWB;
 #pragma unroll
     for (int j = 0; j < A_CH; ++j) {
       const int q = tid + NT * j;
       const int r = TA ? q / CH : q % BM, c = TA ? q % CH : q / BM;
-      *reinterpret_cast<u32x4*>(As + r * ROWB + swz8(r, c) * 16) = ra[j];
+      if constexpr (TA) mask_kcont(st.a[j], k0, c);
+      *reinterpret_cast<u32x4*>(As + r * ROWB + swz8(r, c) * 16) = st.a[j];
     }
 #pragma unroll
     for (int j = 0; j < B_CH; ++j) {
       const int q = tid + NT * j;
       const int r = TB ? q % BN : q / CH, c = TB ? q / BN : q % CH;
-      *reinterpret_cast<u32x4*>(Bs + r * ROWB + swz8(r, c) * 16) = rb[j];
+      if constexpr (!TB) mask_kcont(st.b[j], k0, c);
+      *reinterpret_cast<u32x4*>(Bs + r * ROWB + swz8(r, c) * 16) = st.b[j];
     }
   };
 
@@ -163,49 +163,74 @@ __global__ void __launch_bounds__(64 * WGM * WGN) gemm_mfma_kernel(GemmParams<T>
   const int li = lane % F::MI;  // row of the MFMA operand this lane feeds
   const int lg = lane / F::MI;  // k group of this lane
 
+  // ---- MFMAs over the slice in LDS buffer `buf` ---------------------------
+  auto compute = [&](int buf) {
+    const unsigned char* As = smem + buf * (BM + BN) * ROWB;
+    const unsigned char* Bs = As + BM * ROWB;
+#pragma unroll
+    for (int cc = 0; cc < CPL; ++cc) {
+      const int c = lg * CPL + cc;
+      T a[RM][EPC], b[RN][EPC];
+#pragma unroll
+      for (int i = 0; i < RM; ++i) {
+        const int r = wm * WTM + i * F::MI + li;
+        u32x4 v = *reinterpret_cast<const u32x4*>(As + r * ROWB + swz8(r, c) * 16);
+        const T* t = reinterpret_cast<const T*>(&v);
+#pragma unroll
+        for (int e = 0; e < EPC; ++e) a[i][e] = t[e];
+      }
+#pragma unroll
+      for (int j = 0; j < RN; ++j) {
+        const int r = wn * WTN + j * F::NI + li;
+        u32x4 v = *reinterpret_cast<const u32x4*>(Bs + r * ROWB + swz8(r, c) * 16);
+        const T* t = reinterpret_cast<const T*>(&v);
+#pragma unroll
+        for (int e = 0; e < EPC; ++e) b[j][e] = t[e];
+      }
+#pragma unroll
+      for (int e = 0; e < EPC; ++e)
+#pragma unroll
+        for (int i = 0; i < RM; ++i)
+#pragma unroll
+          for (int j = 0; j < RN; ++j) F::mma(a[i][e], b[j][e], acc[i][j]);
+    }
+  };
+
   if (nkt > 0) {
-    load_a(kbeg, kbeg + BK > kend);
-    load_b(kbeg, kbeg + BK > kend);
-    store_tiles(0);
-    __syncthreads();
-    for (int kt = 0; kt < nkt; ++kt) {
-      const int buf = kt & 1;
-      if (kt + 1 < nkt) {
-        const int k1 = kbeg + (kt + 1) * BK;
-        load_a(k1, k1 + BK > kend);
-        load_b(k1, k1 + BK > kend);
-      }
-      const unsigned char* As = smem + buf * (BM + BN) * ROWB;
-      const unsigned char* Bs = As + BM * ROWB;
-#pragma unroll
-      for (int cc = 0; cc < CPL; ++cc) {
-        const int c = lg * CPL + cc;
-        T a[RM][EPC], b[RN][EPC];
-#pragma unroll
-        for (int i = 0; i < RM; ++i) {
-          const int r = wm * WTM + i * F::MI + li;
-          u32x4 v = *reinterpret_cast<const u32x4*>(As + r * ROWB + swz8(r, c) * 16);
-          const T* t = reinterpret_cast<const T*>(&v);
-#pragma unroll
-          for (int e = 0; e < EPC; ++e) a[i][e] = t[e];
-        }
-#pragma unroll
-        for (int j = 0; j < RN; ++j) {
-          const int r = wn * WTN + j * F::NI + li;
-          u32x4 v = *reinterpret_cast<const u32x4*>(Bs + r * ROWB + swz8(r, c) * 16);
-          const T* t = reinterpret_cast<const T*>(&v);
-#pragma unroll
-          for (int e = 0; e < EPC; ++e) b[j][e] = t[e];
-        }
-#pragma unroll
-        for (int e = 0; e < EPC; ++e)
-#pragma unroll
-          for (int i = 0; i < RM; ++i)
-#pragma unroll
-            for (int j = 0; j < RN; ++j) F::mma(a[i][e], b[j][e], acc[i][j]);
-      }
-      if (kt + 1 < nkt) store_tiles(buf ^ 1);
+    if constexpr (RS == 2) {
+      // Two register stages: the loads of slice s + 2 are issued while slice
+      // s is multiplied, so every slice has two slices of MFMA time to arrive.
+      Stage r0, r1;
+      load(r0, 0);
+      store(r0, 0, 0);
+      load(r1, 1);
+      load(r0, 2);
       __syncthreads();
+      // at the top of each pair: LDS buffer 0 holds slice kt, r1 slice
+      // kt + 1, r0 slice kt + 2
+      int kt = 0;
+      for (; kt + 2 <= nkt; kt += 2) {
+        compute(0);
+        store(r1, kt + 1, 1);
+        __syncthreads();
+        load(r1, kt + 3);
+        compute(1);
+        store(r0, kt + 2, 0);
+        __syncthreads();
+        load(r0, kt + 4);
+      }
+      if (kt < nkt) compute(0);  // odd slice count: the last slice is in buffer 0
+    } else {
+      Stage r0;
+      load(r0, 0);
+      store(r0, 0, 0);
+      __syncthreads();
+      for (int kt = 0; kt < nkt; ++kt) {
+        load(r0, kt + 1);
+        compute(kt & 1);
+        store(r0, kt + 1, (kt + 1) & 1);
+        __syncthreads();
+      }
     }
   }
 

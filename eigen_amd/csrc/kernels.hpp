@@ -18,7 +18,6 @@ struct GemmParams {
   int M, N, K;
   int k_per_split;  // multiple of the kernel's k slice
   T alpha, beta;
-  int vecA, vecB;  // 16-byte loads allowed along the k-contiguous operand
   // Byte extent of A and B from their base pointers (last element + 1); the
   // kernels read them through raw buffer resources of exactly this size, so
   // both must stay below 2^32 with a margin of one tile (the host splits
