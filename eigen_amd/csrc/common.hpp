@@ -59,8 +59,16 @@ __device__ inline double fma_t(double a, double b, double c) { return __builtin_
 // rows do not collide on banks whether the LDS serves 128 or 256 bytes per
 // clock.
 __device__ __host__ inline int swz8(int row, int chunk) { return chunk ^ ((row ^ (row >> 3)) & 7); }
-// Same property for 4 chunks per row (64-byte rows).
-__device__ __host__ inline int swz4(int row, int chunk) { return chunk ^ (((row >> 1) ^ (row >> 3)) & 3); }
+// 64-byte rows (4 chunks) are stored in pairs: rows 2R and 2R + 1 share the
+// 128-byte LDS row R, as logical chunks 0-3 and 4-7 under swz8.  32 lanes
+// reading 32 consecutive rows at one chunk then touch every bank group of
+// each 128-byte phase exactly once (a plain per-row XOR over 4 chunks costs
+// the same reads four extra LDS cycles each, measured with
+// SQ_LDS_BANK_CONFLICT on gfx950).
+__device__ __host__ inline int lds_pair_offset(int row, int chunk) {
+  const int R = row >> 1;
+  return R * 128 + swz8(R, ((row & 1) << 2) | chunk) * 16;
+}
 
 // Raw buffer resource over [base, base + bytes): loads past the end return 0
 // in hardware, which is how the product kernels handle matrix edges without

@@ -155,11 +155,12 @@ __global__ void __launch_bounds__(64 * WGM * WGN) gemm_split_packed_kernel(Split
   const int nkt = min(p.KS, ks0 + p.ks_per_split) - ks0;
 
   // Direct loads: block b of a stage covers 16 plane rows [blk0, blk0 + 16)
-  // of one plane; lane l fills LDS slot l of the block, i.e. plane row
-  // blk0 + l/4 at physical chunk l&3, which holds logical chunk
-  // (l&3) ^ s4(row).  With blk0 a multiple of 16 the swizzle term depends on
-  // blk0 only through bit 4, so the per-lane offset is computed per block
-  // once; the block bases are wave-uniform and advance by one slice per stage.
+  // of one plane, i.e. 8 LDS rows of 128 bytes (plane rows 2R and 2R + 1
+  // side by side, see lds_pair_offset); lane l fills LDS slot l of the block:
+  // LDS row blk0/2 + l/8, physical chunk l&7, which holds logical chunk
+  // V = (l&7) ^ swizzle(row), i.e. plane row 2R + V/4, 16-byte chunk V&3.
+  // The per-lane global offset is computed per block once; the block bases
+  // are wave-uniform and advance by one slice per stage.
   const unsigned char* srcA = reinterpret_cast<const unsigned char*>(p.Ap);
   const unsigned char* srcB = reinterpret_cast<const unsigned char*>(p.Bp);
   const int64_t sliceA = (int64_t)TERMS * p.Mp * ROWB, sliceB = (int64_t)TERMS * p.Np * ROWB;
@@ -171,9 +172,9 @@ __global__ void __launch_bounds__(64 * WGM * WGN) gemm_split_packed_kernel(Split
     const int b = wid + NW * i;                // block index within the stage (wave-uniform)
     const int t = b / (PROWS / 16);            // plane
     const int blk0 = (b % (PROWS / 16)) * 16;  // first plane row of the block
-    const int row = blk0 + (lane >> 2);
-    const int c = (lane & 3) ^ swz4(row, 0);   // logical chunk stored at this LDS slot
-    loff[i] = (uint32_t)((lane >> 2) * ROWB + c * 16);
+    const int R = (blk0 >> 1) + (lane >> 3);   // LDS row (pair of plane rows)
+    const int V = swz8(R, lane & 7);           // logical chunk stored at this LDS slot
+    loff[i] = (uint32_t)((2 * (lane >> 3) + (V >> 2)) * ROWB + (V & 3) * 16);
     if (blk0 < BM) {
       gbase[i] = srcA + (int64_t)ks0 * sliceA + ((int64_t)t * p.Mp + m0 + blk0) * ROWB;
       gstride[i] = sliceA;
@@ -232,12 +233,12 @@ __global__ void __launch_bounds__(64 * WGM * WGN) gemm_split_packed_kernel(Split
 #pragma unroll
         for (int i = 0; i < RM; ++i) {
           const int r = wm * WTM + i * 32 + li;
-          a[s][t][i] = *reinterpret_cast<const u32x4*>(base + t * PLANE + r * ROWB + swz4(r, c) * 16);
+          a[s][t][i] = *reinterpret_cast<const u32x4*>(base + t * PLANE + lds_pair_offset(r, c));
         }
 #pragma unroll
         for (int j = 0; j < RN; ++j) {
           const int r = BM + wn * WTN + j * 32 + li;
-          b[s][t][j] = *reinterpret_cast<const u32x4*>(base + t * PLANE + r * ROWB + swz4(r, c) * 16);
+          b[s][t][j] = *reinterpret_cast<const u32x4*>(base + t * PLANE + lds_pair_offset(r, c));
         }
       }
     }
