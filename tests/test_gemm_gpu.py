@@ -182,9 +182,15 @@ def test_device_split_matches_oracle(gpu):
     assert (int(ts[0][0].item()) & 0x7FFF) > 0x7F80
 
 
+# configurations of the split path: planner default, 128x128 (3 stages),
+# 256x128 with 8 waves, 256x128 with 4 waves, and that with split K
+SPLIT_CONFIGS = [None, 12, 16, 20, 21]
+
+
+@pytest.mark.parametrize("config", SPLIT_CONFIGS)
 @pytest.mark.parametrize("ta,tb", [(False, False), (True, False), (False, True), (True, True)])
 @pytest.mark.parametrize("shape", [(1, 1, 1), (33, 65, 31), (200, 130, 257), (300, 280, 1100)])
-def test_split_product_vs_split_oracle(gpu, ta, tb, shape):
+def test_split_product_vs_split_oracle(gpu, ta, tb, shape, config):
     m, n, k = shape
     A = O.fill(op_shape(ta, m, k), np.float32, 31, O.WIDE, -8, 8)
     B = O.fill(op_shape(tb, k, n), np.float32, 32, O.WIDE, -8, 8)
@@ -192,7 +198,7 @@ def test_split_product_vs_split_oracle(gpu, ta, tb, shape):
     for terms, prec in ((3, E.PREC_SPLIT3), (2, E.PREC_SPLIT2)):
         want = O.gemm_split_ref(terms, ta, tb, 1.25, A, B, 0.5, C0)
         _, absref = O.gemm_ref(ta, tb, 1.25, A, B, with_abs=True)
-        got = run_gemm(np.float32, ta, tb, m, n, k, 1.25, A, B, 0.5, C0, prec=prec)
+        got = run_gemm(np.float32, ta, tb, m, n, k, 1.25, A, B, 0.5, C0, prec=prec, config=config)
         # only FP32 accumulation separates the device result from the exact split sum
         err = np.linalg.norm(got.astype(np.float64) - want)
         assert err <= O.higham_mary_bound(absref, k, unit_roundoff(np.float32)) + 1e-30
