@@ -220,10 +220,27 @@ static std::mutex g_mlp_mutex;
 #ifdef EH_HAVE_DEFAULT_SELECTOR
 static std::vector<float> g_planner_params(kDefaultSelectorParams, kDefaultSelectorParams + kNumParams);
 bool mlp_has_default() { return true; }
+static bool g_planner_is_default = true;  // the compiled-in network (and its tuned shapes) is in use
 #else
 static std::vector<float> g_planner_params;
 bool mlp_has_default() { return false; }
+static bool g_planner_is_default = false;
 #endif
+
+// Measured best configuration of a shape that was tuned together with the
+// compiled-in network (exact match), or -1.
+static int tuned_shape_plan(int dtype, int opa, int opb, int64_t m, int64_t n, int64_t k) {
+#ifdef EH_HAVE_DEFAULT_SELECTOR
+  const long long ta = opa != EH_OP_N, tb = opb != EH_OP_N;
+  for (int i = 0; i < kNumTunedShapes; ++i) {
+    const long long* r = kTunedShapes[i];
+    if (r[0] == dtype && r[1] == ta && r[2] == tb && r[3] == m && r[4] == n && r[5] == k) return (int)r[6];
+  }
+#else
+  (void)dtype, (void)opa, (void)opb, (void)m, (void)n, (void)k;
+#endif
+  return -1;
+}
 
 // Feature vector of a product shape (the network's input).
 void mlp_features(int dtype, int opa, int opb, int64_t m, int64_t n, int64_t k, float* x) {
@@ -238,9 +255,15 @@ void mlp_features(int dtype, int opa, int opb, int64_t m, int64_t n, int64_t k, 
 
 // Host evaluation of the planner network (same arithmetic as the kernels:
 // FP32 fused multiply-adds in the same order); returns -1 if no network set.
+// With the compiled-in network, shapes it was tuned on use their measured
+// best configuration.
 int mlp_plan(int dtype, int opa, int opb, int64_t m, int64_t n, int64_t k) {
   std::lock_guard<std::mutex> lk(g_mlp_mutex);
   if (g_planner_params.empty()) return -1;
+  if (g_planner_is_default) {
+    const int c = tuned_shape_plan(dtype, opa, opb, m, n, k);
+    if (c >= 0) return c;
+  }
   const float* P = g_planner_params.data();
   float x[kIn], h[kHid];
   mlp_features(dtype, opa, opb, m, n, k, x);
@@ -444,9 +467,11 @@ eh_status eh_planner_set_mlp_params(const float* host_params) {
   std::lock_guard<std::mutex> lk(g_mlp_mutex);
   if (host_params) {
     g_planner_params.assign(host_params, host_params + kNumParams);
+    g_planner_is_default = false;
   } else {
 #ifdef EH_HAVE_DEFAULT_SELECTOR
     g_planner_params.assign(kDefaultSelectorParams, kDefaultSelectorParams + kNumParams);
+    g_planner_is_default = true;
 #else
     g_planner_params.clear();
 #endif

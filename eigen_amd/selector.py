@@ -27,11 +27,13 @@ import numpy as np
 
 from . import TUNE_HEURISTIC, check, lib, plan, set_tune_mode
 
-N_TRAIN = 4000
+N_TRAIN = 4000   # random shapes used for training
+N_SAMPLE = 4500  # random shapes in the table (the rest are held out)
 
 
-def load_db(path: str):
-    """Returns (shapes (S,6) int64: dtype, ta, tb, m, n, k;  times (S,24) float64 ms)."""
+def load_db(path: str, with_index: bool = False):
+    """Returns (shapes (S,6) int64: dtype, ta, tb, m, n, k;  times (S,24) float64 ms)
+    sorted by row index, and the indices themselves when with_index."""
     rows = []
     with open(path) as f:
         r = csv.reader(f)
@@ -40,9 +42,10 @@ def load_db(path: str):
             if row:
                 rows.append(row)
     rows.sort(key=lambda x: int(x[0]))
+    index = np.array([int(row[0]) for row in rows], dtype=np.int64)
     shapes = np.array([[int(v) for v in row[1:7]] for row in rows], dtype=np.int64)
     times = np.array([[float(v) for v in row[7:]] for row in rows], dtype=np.float64)
-    return shapes, times
+    return (shapes, times, index) if with_index else (shapes, times)
 
 
 def features(shapes: np.ndarray) -> np.ndarray:
@@ -133,8 +136,11 @@ class Selector:
 
 
 def train_and_evaluate(db: str, epochs: int = 3000, lr: float = 1e-2, wd: float = 1e-4, tau: float = 0.05):
-    shapes, times = load_db(db)
-    tr, te = slice(0, N_TRAIN), slice(N_TRAIN, len(shapes))
+    """Trains on the random shapes 0 .. N_TRAIN-1 and the canonical shapes
+    (indices past the random sample), holds out random shapes N_TRAIN .. N_SAMPLE-1."""
+    shapes, times, index = load_db(db, with_index=True)
+    te = (index >= N_TRAIN) & (index < N_SAMPLE)
+    tr = ~te
     X = features(shapes)
     Q = soft_targets(times, tau)
     sel = Selector(capacity=len(shapes))
@@ -149,17 +155,34 @@ def train_and_evaluate(db: str, epochs: int = 3000, lr: float = 1e-2, wd: float 
     return sel, res
 
 
-def emit_inc(params, path: str, source: str) -> None:
-    """Write the parameters as the C array the library compiles in as its default selector."""
+def tuned_table(db: str):
+    """(dtype, ta, tb, m, n, k, best config) of the canonical shapes of a timing table."""
+    shapes, times, index = load_db(db, with_index=True)
+    sel = index >= N_SAMPLE
+    return [tuple(int(v) for v in s) + (int(t.argmin()),) for s, t in zip(shapes[sel], times[sel])]
+
+
+def emit_inc(params, path: str, source: str, tuned=()) -> None:
+    """Write the parameters as the C array the library compiles in as its default
+    selector, and the measured best configuration of each tuned canonical shape."""
     vals = [float(v) for v in params]
     assert len(vals) == 2072
     with open(path, "w") as f:
-        f.write("// Default tile / schedule selector parameters (7 -> 64 -> 24 network), generated by\n")
+        f.write("// Default tile / schedule selector parameters (7 -> 64 -> 24 network) and the measured\n")
+        f.write("// best configuration of the tuned canonical shapes, generated by\n")
         f.write(f"// `python -m eigen_amd.selector --emit-inc` from {source}.\n")
         f.write("static const float kDefaultSelectorParams[2072] = {\n")
         for i in range(0, len(vals), 6):
             f.write("    " + ", ".join(f"{v:.9e}f" for v in vals[i:i + 6]) + ",\n")
         f.write("};\n")
+        f.write("// dtype, transA, transB, m, n, k, best configuration\n")
+        f.write(f"static const long long kTunedShapes[{max(len(tuned), 1)}][7] = {{\n")
+        for row in tuned:
+            f.write("    {" + ", ".join(str(v) for v in row) + "},\n")
+        if not tuned:
+            f.write("    {-1, 0, 0, 0, 0, 0, 0},\n")
+        f.write("};\n")
+        f.write(f"static const int kNumTunedShapes = {len(tuned)};\n")
 
 
 def main(argv=None) -> int:
@@ -167,14 +190,17 @@ def main(argv=None) -> int:
     ap.add_argument("--db")
     ap.add_argument("--out")
     ap.add_argument("--emit-inc", nargs=2, metavar=("JSON", "INC"),
-                    help="write the parameters of a trained JSON as the library's compiled-in default")
+                    help="write the parameters of a trained JSON as the library's compiled-in default "
+                         "(with --db: also the best configuration of its canonical shapes)")
     ap.add_argument("--epochs", type=int, default=3000)
     ap.add_argument("--lr", type=float, default=1e-2)
     ap.add_argument("--wd", type=float, default=1e-4)
     args = ap.parse_args(argv)
     if args.emit_inc:
         src, inc = args.emit_inc
-        emit_inc(json.load(open(src))["params"], inc, os.path.basename(src))
+        tuned = tuned_table(args.db) if args.db else ()
+        source = os.path.basename(src) + (f" and {os.path.basename(args.db)}" if args.db else "")
+        emit_inc(json.load(open(src))["params"], inc, source, tuned)
         return 0
     if not args.db or not args.out:
         ap.error("--db and --out are required for training")

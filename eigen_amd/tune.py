@@ -3,7 +3,8 @@
     python -m eigen_amd.tune --shapes 4500 --out tune.csv [--resume] [--max-seconds S]
 
 Shapes: M, N, K = round(exp(U(ln 64, ln 16384))), dtype FP32 or FP64 and the
-two transpose flags uniformly at random (seeded).  For every shape each of the
+two transpose flags uniformly at random (seeded), followed by a fixed list of
+canonical shapes (canonical_shapes).  For every shape each of the
 24 configurations is timed on the device with HIP events (median of `reps`
 after a warm-up; reps shrink for large products).  Output is a CSV with one
 row per shape:  index,dtype,ta,tb,m,n,k,t0,...,t23  (milliseconds).
@@ -29,6 +30,27 @@ def sample_shapes(count: int, seed: int = 2024, lo: int = 64, hi: int = 16384) -
     return np.concatenate([flags, dims], axis=1)
 
 
+def canonical_shapes() -> np.ndarray:
+    """Common shapes the random sample rarely hits: squares 256..16384 in both
+    dtypes and all four transpose combinations, and tall / skinny / deep
+    non-transposed products."""
+    rows = []
+    for d in (0, 1):
+        for ta in (0, 1):
+            for tb in (0, 1):
+                for e in range(8, 15):
+                    rows.append((d, ta, tb, 1 << e, 1 << e, 1 << e))
+        for m, n, k in ((16384, 256, 16384), (256, 16384, 16384), (16384, 16384, 256), (8192, 8192, 1024),
+                        (1024, 1024, 16384)):
+            rows.append((d, 0, 0, m, n, k))
+    return np.array(rows, dtype=np.int64)
+
+
+def all_shapes(count: int, seed: int = 2024) -> np.ndarray:
+    """The random sample (indices 0 .. count-1) followed by the canonical shapes."""
+    return np.concatenate([sample_shapes(count, seed), canonical_shapes()], axis=0)
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--shapes", type=int, default=4500)
@@ -41,7 +63,7 @@ def main(argv=None) -> int:
     import torch
     from . import F32, check, fill, lib, num_configs
 
-    shapes = sample_shapes(args.shapes, args.seed)
+    shapes = all_shapes(args.shapes, args.seed)
     done = set()
     if args.resume and os.path.exists(args.out):
         with open(args.out) as f:

@@ -170,3 +170,27 @@ def test_grouped_rejects_bad_product():
     assert E.lib().eh_grouped_create(arr, 1, C.byref(h)) == 1
     assert "product 0" in E.lib().eh_last_error().decode()
     assert E.lib().eh_grouped_create(arr, 0, C.byref(h)) == 1
+
+
+def test_tuned_shapes_use_their_measured_best():
+    """Shapes tuned with the compiled-in network plan to their measured best
+    configuration; a network installed by the caller is used as is."""
+    import os
+    from oracle import mlp as M
+    from eigen_amd.selector import N_SAMPLE, load_db
+    S, T, I = load_db(os.path.join(os.path.dirname(E.__file__), "data", "tune_gfx950.csv"), with_index=True)
+    can = I >= N_SAMPLE
+    assert can.sum() >= 60
+    for s, t in zip(S[can].tolist(), T[can]):
+        assert E.plan(*s) == int(t.argmin()), s
+    P = np.random.default_rng(3).normal(0, 0.5, M.N_PARAMS)
+    L = E.lib()
+    E.check(L.eh_planner_set_mlp_params(np.ascontiguousarray(P, np.float32).ctypes.data_as(C.c_void_p)))
+    try:
+        for s in S[can][:8].tolist():
+            _, Z, _ = M.forward(P.astype(np.float32).astype(np.float64), np.array([M.features(*s)]))
+            top = np.sort(Z[0])[-2:]
+            if top[1] - top[0] > 1e-4:
+                assert E.plan(*s) == int(Z[0].argmax()), s
+    finally:
+        E.check(L.eh_planner_set_mlp_params(None))

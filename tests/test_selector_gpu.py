@@ -22,7 +22,7 @@ DB = os.path.join(os.path.dirname(E.__file__), "data", "tune_gfx950.csv")
 def test_selector_beats_heuristic_on_heldout(gpu):
     sel, res = S.train_and_evaluate(DB, epochs=3000)
     print(res)
-    assert res["train_shapes"] == 4000 and res["heldout_shapes"] == 500
+    assert res["train_shapes"] >= 4000 and res["heldout_shapes"] == 500
     assert res["loss_last"] < res["loss_first"]
     assert res["mlp_regret"]["mean"] < res["heuristic_regret"]["mean"]
     assert res["mlp_regret"]["median"] <= res["heuristic_regret"]["median"]
