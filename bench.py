@@ -193,6 +193,8 @@ def main() -> int:
         out["cpu_baseline"] = cpu_baseline()
         if not args.no_extra:
             out["detail"]["batched"]["cpu_eigen"] = cpu_baseline_batched(out["detail"]["batched"].pop("_batch"))
+            out["detail"]["dgemm_4096"]["cpu_eigen"] = cpu_baseline_dgemm(4096, 4096, 4096)
+            out["detail"]["dgemm_tall"]["cpu_eigen"] = cpu_baseline_dgemm(16384, 256, 16384)
     elif not args.no_extra:
         out["detail"]["batched"].pop("_batch", None)
     print(json.dumps(out), flush=True)
@@ -238,6 +240,34 @@ def bench_batched(E, torch, args):
     g.close()
     res["_batch"] = b
     return res
+
+
+def cpu_baseline_dgemm(m: int, n: int, k: int):
+    """Eigen's CPU dgemm semantics C = 1.5*A*B + 0.5*C (C *= beta, then
+    C.noalias() += alpha*A*B, as blas/level3_impl.h does) on host Maps of the
+    same shape and distribution, all host cores."""
+    try:
+        import oracle as O
+        if not O.ref_available():
+            return None
+        L = O.ref()
+        A = O.fill((m, k), np.float64, 11, O.NORMAL, 0, 1)
+        B = O.fill((k, n), np.float64, 12, O.NORMAL, 0, 1)
+        C0 = O.fill((m, n), np.float64, 13, O.NORMAL, 0, 1)
+        ts = []
+        for r in range(4):  # one warm-up, median of three
+            Cm = np.asfortranarray(C0.copy())
+            t0 = time.perf_counter()
+            L.ref_gemm_beta_f64(0, 0, m, n, k, 1.5, O._ptr(A), m, O._ptr(B), k, 0.5, O._ptr(Cm), m)
+            if r:
+                ts.append(time.perf_counter() - t0)
+        t = float(np.median(ts))
+        return {"tflops": round(2.0 * m * n * k / t / 1e12, 4), "ms": round(t * 1e3, 2),
+                "threads": int(L.ref_get_threads()), "kind": "reference",
+                "sample": "Eigen C = 1.5*A*B + 0.5*C (dgemm semantics), FP64 N(0,1), median of 3 after 1 warm-up"}
+    except Exception as e:
+        log("cpu dgemm baseline failed:", e)
+        return None
 
 
 def cpu_baseline_batched(b):

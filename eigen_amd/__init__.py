@@ -25,6 +25,7 @@ OP_N, OP_T, OP_C = 0, 1, 2
 PREC_AUTO, PREC_PLAIN, PREC_SPLIT2, PREC_SPLIT3 = 0, 1, 2, 3
 TUNE_HEURISTIC, TUNE_MLP, TUNE_TABLE = 0, 1, 2
 DIST_UNIFORM, DIST_NORMAL, DIST_WIDE, DIST_INT = 0, 1, 2, 3
+SUCCESS, ERR_INVALID, ERR_HIP, ERR_UNSUPPORTED, ERR_ALLOC = 0, 1, 2, 3, 4
 
 _lib = None
 _blas = None
@@ -115,6 +116,8 @@ def lib():
         L.eh_gemm.argtypes = [i32, i32, i32, i64, i64, i64, vp, vp, i64, vp, i64, vp, vp, i64, i32, vp]
         L.eh_gemm_config.restype = i32
         L.eh_gemm_config.argtypes = [i32, i32, i32, i64, i64, i64, vp, vp, i64, vp, i64, vp, vp, i64, i32, i32, vp]
+        L.eh_gemm_strided.restype = i32
+        L.eh_gemm_strided.argtypes = [i32, i32, i32, i64, i64, i64, vp, vp, i64, vp, i64, vp, vp, i64, i64, i32, vp]
         L.eh_num_configs.restype = i32
         L.eh_config_info.restype = i32
         L.eh_config_info.argtypes = [i32, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)]
@@ -227,6 +230,15 @@ def gemm(dtype: int, opa: int, opb: int, m: int, n: int, k: int, alpha: float, A
         st = L.eh_gemm_config(dtype, opa, opb, m, n, k, C.byref(a), _ptr(A), lda, _ptr(B), ldb, C.byref(b), _ptr(Cm),
                               ldc, prec, config, s)
     check(st, "eh_gemm")
+
+
+def gemm_strided(dtype: int, opa: int, opb: int, m: int, n: int, k: int, alpha: float, A, lda: int, B, ldb: int,
+                 beta: float, Cm, incc: int, ldc: int, prec: int = PREC_AUTO, stream=None) -> None:
+    """Same as gemm with C(i, j) at C[i*incc + j*ldc] (eh_gemm_strided)."""
+    a, b = _scalar(dtype, alpha), _scalar(dtype, beta)
+    s = current_stream() if stream is None else stream
+    check(lib().eh_gemm_strided(dtype, opa, opb, m, n, k, C.byref(a), _ptr(A), lda, _ptr(B), ldb, C.byref(b),
+                                _ptr(Cm), incc, ldc, prec, s), "eh_gemm_strided")
 
 
 def fill(dtype: int, X, n: int, seed: int, dist: int = DIST_UNIFORM, p0: float = -1.0, p1: float = 1.0,

@@ -384,6 +384,48 @@ eh_status eh_gemm_config(eh_dtype dtype, eh_op opa, eh_op opb, int64_t m, int64_
   return gemm_dispatch(dtype, opa, opb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, prec, config, stream);
 }
 
+eh_status eh_gemm_strided(eh_dtype dtype, eh_op opa, eh_op opb, int64_t m, int64_t n, int64_t k, const void* alpha,
+                          const void* A, int64_t lda, const void* B, int64_t ldb, const void* beta, void* C,
+                          int64_t incc, int64_t ldc, eh_precision prec, void* stream) {
+  if (incc < 1) return fail(EH_ERR_INVALID, "incc must be >= 1");
+  if (incc == 1) return eh_gemm(dtype, opa, opb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, prec, stream);
+  eh_status st =
+      check_gemm_args(dtype, opa, opb, m, n, k, alpha, A, lda, B, ldb, beta, C, std::max<int64_t>(1, m));
+  if (st != EH_SUCCESS) return st;
+  if (ldc < 1 || (m > 0 && ldc < (m - 1) * incc + 1)) return fail(EH_ERR_INVALID, "ldc too small for incc");
+  if (m == 0 || n == 0) return EH_SUCCESS;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const bool f64 = dtype == EH_F64;
+  const size_t es = f64 ? sizeof(double) : sizeof(float);
+  const bool vanishes = k == 0 || (f64 ? *static_cast<const double*>(alpha) == 0.0
+                                      : *static_cast<const float*>(alpha) == 0.0f);
+  void* W = nullptr;
+  if (!vanishes) {
+    // unscaled product into a stream-ordered temporary
+    hipError_t e = hipMallocAsync(&W, (size_t)m * n * es, s);
+    if (e != hipSuccess) return fail(EH_ERR_ALLOC, std::string("temporary for the strided result: ") +
+                                                       hipGetErrorString(e));
+    const double one64 = 1.0, zero64 = 0.0;
+    const float one32 = 1.0f, zero32 = 0.0f;
+    st = gemm_dispatch(dtype, opa, opb, m, n, k, f64 ? (const void*)&one64 : (const void*)&one32, A, lda, B, ldb,
+                       f64 ? (const void*)&zero64 : (const void*)&zero32, W, m, prec, -1, stream);
+    if (st != EH_SUCCESS) {
+      (void)hipFreeAsync(W, s);
+      return st;
+    }
+  }
+  hipError_t e = f64 ? launch_strided_combine<double>(static_cast<const double*>(W), (int)m, (int)n,
+                                                      static_cast<double*>(C), incc, ldc,
+                                                      *static_cast<const double*>(alpha),
+                                                      *static_cast<const double*>(beta), s)
+                     : launch_strided_combine<float>(static_cast<const float*>(W), (int)m, (int)n,
+                                                     static_cast<float*>(C), incc, ldc,
+                                                     *static_cast<const float*>(alpha),
+                                                     *static_cast<const float*>(beta), s);
+  if (W) (void)hipFreeAsync(W, s);
+  return e == hipSuccess ? EH_SUCCESS : hip_fail(e, "strided combine launch");
+}
+
 int eh_num_configs(void) { return kNumConfigs; }
 
 eh_status eh_config_info(int config, int* tile_m, int* tile_n, int* split_k) {

@@ -62,4 +62,17 @@ template hipError_t launch_gemm_mfma<double>(const GemmParams<double>&, int, boo
 template hipError_t launch_splitk_reduce<double>(const GemmParams<double>&, int, hipStream_t);
 template hipError_t launch_scale_c<double>(double*, int, int, int64_t, double, hipStream_t);
 
+template <typename T>
+hipError_t launch_strided_combine(const T* W, int M, int N, T* C, int64_t incc, int64_t ldc, T alpha, T beta,
+                                  hipStream_t s) {
+  const int64_t total = (int64_t)M * N;
+  int64_t blocks = (total + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(strided_combine_kernel<T>, dim3((unsigned)blocks), dim3(256), 0, s, W, M, N, C, incc, ldc, alpha,
+                     beta);
+  return hipGetLastError();
+}
+template hipError_t launch_strided_combine<double>(const double*, int, int, double*, int64_t, int64_t, double, double, hipStream_t);
+
 }  // namespace eh

@@ -284,4 +284,25 @@ __global__ void __launch_bounds__(256) scale_c_kernel(T* C, int M, int N, int64_
   }
 }
 
+// Result with an inner stride: C(i,j) = C[i*incc + j*ldc] gets
+// alpha*W(i,j) + beta*C(i,j) (W = the unscaled product, m x n contiguous), in
+// the epilogue's arithmetic; W == nullptr means the product term vanishes
+// (C = beta*C).  beta == 0 never reads C.
+template <typename T>
+__global__ void __launch_bounds__(256) strided_combine_kernel(const T* W, int M, int N, T* C, int64_t incc,
+                                                              int64_t ldc, T alpha, T beta) {
+  const int64_t total = (int64_t)M * N;
+  for (int64_t idx = blockIdx.x * 256ll + threadIdx.x; idx < total; idx += (int64_t)gridDim.x * 256) {
+    const int row = (int)(idx % M);
+    const int64_t col = idx / M;
+    T* dst = C + row * incc + col * ldc;
+    if (W) {
+      const T v = W[idx];
+      *dst = (beta == T(0)) ? alpha * v : fma_t(alpha, v, beta * *dst);
+    } else {
+      *dst = (beta == T(0)) ? T(0) : beta * *dst;
+    }
+  }
+}
+
 }  // namespace eh

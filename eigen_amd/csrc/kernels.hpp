@@ -32,6 +32,9 @@ template <typename T>
 hipError_t launch_splitk_reduce(const GemmParams<T>& p, int splits, hipStream_t s);
 template <typename T>
 hipError_t launch_scale_c(T* C, int M, int N, int64_t ldc, T beta, hipStream_t s);
+template <typename T>
+hipError_t launch_strided_combine(const T* W, int M, int N, T* C, int64_t incc, int64_t ldc, T alpha, T beta,
+                                  hipStream_t s);
 
 // Split-BF16 FP32 product over packed operands.  Each operand is first packed
 // into bf16 planes P[ks][term][row][32] (rows padded to split_rows_padded,

@@ -16,6 +16,9 @@
  *       the whole GEBP product C = alpha*op(A)*op(B) + beta*C, where the
  *       reference's beta handling lives one level up in ?gemm_
  *       (blas/level3_impl.h:80-95) and is fused here.
+ *   eh_gemm_strided
+ *       the same driver with a result increment (resIncr,
+ *       GeneralMatrixMatrix.h:98-137) for destinations with an inner stride.
  *   eh_plan / eh_set_tune_mode
  *       computeProductBlockingSizes + evaluateProductBlockingSizesHeuristic
  *       (GeneralBlockPanelKernel.h:201-452): the blocking choice, here a
@@ -92,6 +95,20 @@ eh_status eh_gemm(eh_dtype dtype, eh_op opa, eh_op opb, int64_t m, int64_t n, in
 eh_status eh_gemm_config(eh_dtype dtype, eh_op opa, eh_op opb, int64_t m, int64_t n, int64_t k, const void* alpha,
                          const void* A, int64_t lda, const void* B, int64_t ldb, const void* beta, void* C,
                          int64_t ldc, eh_precision prec, int config, void* stream);
+
+/* C(i, j) at C[i*incc + j*ldc]: a destination whose consecutive rows are
+ * incc elements apart (incc >= 1, ldc >= (m-1)*incc + 1).  This is the
+ * result increment of Eigen's GEMM driver (resIncr of
+ * general_matrix_matrix_product<...>::run, GeneralMatrixMatrix.h:98-137,
+ * addressed through blas_data_mapper<..., Incr>, BlasUtil.h:266-443), which
+ * Eigen uses for destinations with a non-unit inner stride
+ * (test/product.h:345-357).  incc == 1 is eh_gemm.  Otherwise the product
+ * op(A)*op(B) is formed in a stream-ordered temporary and combined into C
+ * with the same fused alpha/beta arithmetic as eh_gemm's epilogue, so the
+ * result is bit-identical to eh_gemm on a contiguous copy of C. */
+eh_status eh_gemm_strided(eh_dtype dtype, eh_op opa, eh_op opb, int64_t m, int64_t n, int64_t k, const void* alpha,
+                          const void* A, int64_t lda, const void* B, int64_t ldb, const void* beta, void* C,
+                          int64_t incc, int64_t ldc, eh_precision prec, void* stream);
 
 /* ---- planning ------------------------------------------------------------ */
 

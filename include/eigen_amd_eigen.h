@@ -25,10 +25,12 @@
 // Assignment kinds: `=` is C = alpha*op(A)*op(B) (beta = 0, C not read),
 // `+=` beta = 1, `-=` beta = 1 with alpha negated (GeneralMatrixMatrix.h:
 // 454, 465, 472).  A row-major destination is computed as the transposed
-// product (GeneralMatrixMatrix.h:98-115).  Only float and double, unit inner
-// stride, and direct-access operands (Matrix / Map / Block / Ref and their
-// transposes, possibly scaled or negated) are accepted; anything else is a
-// compile-time error.  Errors from the device call are stored in
+// product (GeneralMatrixMatrix.h:98-115); a destination with an inner stride
+// (Eigen's resIncr, GeneralMatrixMatrix.h:117-137) goes through
+// eh_gemm_strided.  Only float and double, and direct-access operands with
+// unit inner stride (Matrix / Map / Block / Ref and their transposes,
+// possibly scaled or negated) are accepted; anything else is a compile-time
+// error.  Errors from the device call are stored in
 // HipDevice::status (and reported through eigen_assert).
 #ifndef EIGEN_AMD_EIGEN_H
 #define EIGEN_AMD_EIGEN_H
@@ -65,6 +67,12 @@ struct AssignmentWithDevice<DstXprType, Product<Lhs, Rhs, Options>, Functor, eig
                 "eigen_amd device products support float and double");
   static_assert(std::is_same<typename DstXprType::Scalar, Scalar>::value, "mixed scalar types are not supported");
   static_assert(!LhsBlasTraits::NeedToConjugate && !RhsBlasTraits::NeedToConjugate, "real scalars only");
+  // Eigen would evaluate an operand without unit inner stride into a host
+  // temporary (blas_traits::DirectLinearAccessType), i.e. read device memory
+  // on the host: such operands must be copied to a contiguous device buffer
+  // by the caller.
+  static_assert(bool(LhsBlasTraits::HasUsableDirectAccess) && bool(RhsBlasTraits::HasUsableDirectAccess),
+                "device product operands need direct access with unit inner stride");
 
   static constexpr bool kAssign = std::is_same<Functor, assign_op<Scalar, Scalar>>::value;
   static constexpr bool kAdd = std::is_same<Functor, add_assign_op<Scalar, Scalar>>::value;
@@ -85,7 +93,7 @@ struct AssignmentWithDevice<DstXprType, Product<Lhs, Rhs, Options>, Functor, eig
     Scalar alpha = combine_scalar_factors(Scalar(1), a_lhs, a_rhs);
     if (kSub) alpha = -alpha;
     const Scalar beta = kAssign ? Scalar(0) : Scalar(1);
-    eigen_assert(lhs.innerStride() == 1 && rhs.innerStride() == 1 && dst.innerStride() == 1);
+    eigen_assert(lhs.innerStride() == 1 && rhs.innerStride() == 1);
 
     // Column-major view of each operand: a row-major m x k matrix with outer
     // stride s is the column-major k x m matrix X^T with leading dimension s.
@@ -110,8 +118,15 @@ struct AssignmentWithDevice<DstXprType, Product<Lhs, Rhs, Options>, Functor, eig
     int64_t ldc = dst.outerStride() < 1 ? 1 : dst.outerStride();
     if (m == 0 || n == 0) return;
     const eh_dtype dt = std::is_same<Scalar, float>::value ? EH_F32 : EH_F64;
-    dev.status = eh_gemm(dt, opa, opb, m, n, k, &alpha, A, lda, B, ldb, &beta, dst.data(), ldc,
-                         dt == EH_F32 ? dev.precision : EH_PREC_AUTO, dev.stream);
+    const eh_precision prec = dt == EH_F32 ? dev.precision : EH_PREC_AUTO;
+    // a destination with an inner stride keeps it in either storage order:
+    // element (i, j) of the column-major view is at i*inner + j*outer
+    const int64_t incc = dst.innerStride();
+    if (incc == 1)
+      dev.status = eh_gemm(dt, opa, opb, m, n, k, &alpha, A, lda, B, ldb, &beta, dst.data(), ldc, prec, dev.stream);
+    else
+      dev.status = eh_gemm_strided(dt, opa, opb, m, n, k, &alpha, A, lda, B, ldb, &beta, dst.data(), incc, ldc, prec,
+                                   dev.stream);
     if (dev.status == EH_SUCCESS && dev.synchronize) dev.status = eh_synchronize(dev.stream);
     eigen_assert(dev.status == EH_SUCCESS && "eigen_amd device product failed (see eh_last_error)");
   }

@@ -104,6 +104,52 @@ int run_all(const char* tname) {
     Mat<T> want2 = a.host * b.host;
     fails += report(tag("tiny"), cc.fetch() == want2);
   }
+  {  // destination with an inner stride (Stride<Dynamic, 2>), after test/product.h:345-357:
+     // row-major square map over every other element of a 2r x 2r buffer
+    const Index r = 37;
+    using RowSq = Mat<T, Eigen::RowMajor>;
+    using SMap = Eigen::Map<RowSq, 0, Eigen::Stride<Dynamic, 2>>;
+    Mat<T> m1(r, r), m2(r, r), init(r, r);
+    fill_int(m1, 28);
+    fill_int(m2, 29);
+    fill_int(init, 30);
+    DeviceBuffer<T> d1(r * r), d2(r * r), buf(4 * r * r);
+    d1.upload(m1.data());
+    d2.upload(m2.data());
+    const Eigen::Map<const Mat<T>> M1(d1.ptr, r, r), M2(d2.ptr, r, r);
+    std::vector<T> hb(4 * r * r);
+    auto put = [&](const Mat<T>& v) {  // the strided elements get v, the gaps -99
+      std::fill(hb.begin(), hb.end(), T(-99));
+      for (Index i = 0; i < r; ++i)
+        for (Index j = 0; j < r; ++j) hb[i * 2 * r + j * 2] = v(i, j);
+      buf.upload(hb.data());
+    };
+    auto get = [&](bool& gaps_ok) {
+      buf.download(hb.data());
+      Mat<T> out(r, r);
+      gaps_ok = true;
+      for (size_t q = 0; q < hb.size(); ++q) {
+        const Index i = (Index)q / (2 * r), j = ((Index)q % (2 * r)) / 2;
+        if ((Index)q % (2 * r) % 2 == 0 && i < r && j < r)
+          out(i, j) = hb[q];
+        else if (hb[q] != T(-99))
+          gaps_ok = false;
+      }
+      return out;
+    };
+    SMap map1(buf.ptr, r, r, Eigen::Stride<Dynamic, 2>(2 * r, 2));
+    const Mat<T> prod = m1 * m2.transpose();
+    bool gaps = false;
+    put(init);
+    map1.device(dev).noalias() = M1 * M2.transpose();
+    fails += report(tag("inner_stride_assign"), get(gaps) == prod && gaps && dev.status == EH_SUCCESS);
+    put(init);
+    map1.device(dev).noalias() += M1 * M2.transpose();
+    fails += report(tag("inner_stride_add"), get(gaps) == Mat<T>(init + prod) && gaps);
+    put(init);
+    map1.device(dev).noalias() -= T(2) * M1 * M2.transpose();
+    fails += report(tag("inner_stride_sub_scaled"), get(gaps) == Mat<T>(init - T(2) * prod) && gaps);
+  }
   {  // split-BF16 precision keeps integer products exact
     if (std::is_same<T, float>::value) {
       eigen_amd::HipDevice d3;

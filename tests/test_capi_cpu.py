@@ -194,3 +194,13 @@ def test_tuned_shapes_use_their_measured_best():
                 assert E.plan(*s) == int(Z[0].argmax()), s
     finally:
         E.check(L.eh_planner_set_mlp_params(None))
+
+
+def test_strided_destination_argument_checks():
+    L = E.lib()
+    one = C.c_float(1.0)
+    args = lambda incc, ldc: (E.F32, 0, 0, 4, 3, 2, C.byref(one), 1, 4, 1, 2, C.byref(one), 1, incc, ldc, 0, None)
+    assert L.eh_gemm_strided(*args(0, 4)) == E.ERR_INVALID          # incc < 1
+    assert L.eh_gemm_strided(*args(2, 6)) == E.ERR_INVALID          # columns would overlap: ldc < (m-1)*incc + 1
+    assert L.eh_gemm_strided(*args(1, 3)) == E.ERR_INVALID          # incc == 1 is eh_gemm: ldc >= m
+    assert "ldc" in E.lib().eh_last_error().decode()

@@ -246,3 +246,35 @@ def test_blas_entry_device_and_host_pointers(gpu, dtype):
     fn(b"T", b"N", i(m), i(n), i(k), C.byref(al), C.c_void_p(dA.data_ptr()), i(k), C.c_void_p(dB.data_ptr()),
        i(k), C.byref(be), C.c_void_p(dC.data_ptr()), i(m))
     assert np.array_equal(from_dev(dC, (m, n)).astype(np.float64), want)
+
+
+@pytest.mark.parametrize("dtype,prec", [(np.float32, E.PREC_PLAIN), (np.float32, E.PREC_SPLIT3),
+                                        (np.float64, E.PREC_AUTO)])
+@pytest.mark.parametrize("alpha,beta", [(1.5, 0.5), (2.0, 0.0), (0.0, 0.5), (0.0, 0.0)])
+def test_strided_destination_matches_contiguous(gpu, dtype, prec, alpha, beta):
+    """eh_gemm_strided (result increment incc) gives, bit for bit, what eh_gemm
+    gives on a contiguous C, touches only the strided elements, and with
+    beta == 0 never reads C."""
+    import torch
+    m, n, k, incc = 130, 70, 200, 3
+    ldc = m * incc + 5
+    A = O.fill((m, k), dtype, 61, O.WIDE, -4, 4)
+    B = O.fill((k, n), dtype, 62, O.WIDE, -4, 4)
+    C0 = O.fill((m, n), dtype, 63)
+    if beta == 0.0:
+        C0 = np.full((m, n), np.nan, dtype=dtype)  # must not be read
+    want = run_gemm(dtype, False, False, m, n, k, alpha, A, B, beta, C0, prec=prec)
+    sentinel = dtype(-99.0)
+    big = np.full(ldc * n, sentinel, dtype=dtype)
+    rows, cols = np.meshgrid(np.arange(m), np.arange(n), indexing="ij")
+    big[(rows * incc + cols * ldc).ravel()] = C0.ravel()
+    dbig = torch.from_numpy(big).cuda()
+    dA, dB = to_dev(A), to_dev(B)
+    E.gemm_strided(DT[dtype], 0, 0, m, n, k, alpha, dA, m, dB, k, beta, dbig, incc, ldc, prec=prec)
+    torch.cuda.synchronize()
+    out = dbig.cpu().numpy()
+    got = out[(rows * incc + cols * ldc).ravel()].reshape(m, n)
+    assert np.array_equal(got, want, equal_nan=False)
+    mask = np.ones(ldc * n, dtype=bool)
+    mask[(rows * incc + cols * ldc).ravel()] = False
+    assert np.all(out[mask] == sentinel)
