@@ -73,11 +73,13 @@ def fp64_rows_error(dA, dB, dC, m, n, k, rows, alpha=1.0, beta=0.0, C0=None, dty
 def measured_traffic(workload: str):
     """HBM bytes per launch of the workload's dominant kernel, as measured by the
     FETCH_SIZE / WRITE_SIZE counter runs summarised in profiles/traffic.json
-    (each counter scaled by its reading for a copy kernel of known size)."""
+    (each counter scaled by its reading for a copy kernel of known size; written
+    by `python -m eigen_amd.profile_kernels summarize`)."""
     try:
         with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
-            return json.load(f).get(workload)
-    except (OSError, ValueError):
+            entry = json.load(f).get(workload)
+        return int(entry["bytes"]) if entry else None
+    except (OSError, ValueError, KeyError, TypeError):
         return None
 
 
@@ -177,7 +179,7 @@ def main() -> int:
         "roofline": {"bound": "mfma", "achieved": round(6 * flop / (head["kernel_ms"] * 1e-3) / 1e12, 2),
                      "peak": PEAK["bf16"], "unit": "TFLOP/s",
                      "frac": round(6 * flop / (head["kernel_ms"] * 1e-3) / 1e12 / PEAK["bf16"], 4),
-                     "traffic": measured_traffic("split3_8192")},
+                     "traffic": measured_traffic("split3")},
         "cpu_baseline": None,
         "detail": {"split3": res["split3"]},
     }
