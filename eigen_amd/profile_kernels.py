@@ -12,8 +12,9 @@
   batched  20,000 uneven products in one grouped launch (captured graph)
   copy     a 1 GiB device-to-device copy: calibrates the HBM byte counters
 
-`summarize` reads directories named trace_<workload> (kernel-trace csv) and
-pmc_<workload>_<FETCH_SIZE|WRITE_SIZE> (counter csv) under the given root and
+`summarize` reads directories named trace_<workload> (kernel-trace csv),
+pmc_<workload>_<FETCH_SIZE|WRITE_SIZE> and util_<workload> (counter csv of
+one pass over UTIL_COUNTERS) under the given root and
 prints per-kernel duration tables and counter readings as markdown, plus the
 HBM bytes per launch of each workload's dominant kernel: each counter
 reading scaled by bytes-moved / reading of the copy kernel (the copy reads
@@ -108,6 +109,35 @@ def counter_means(d: str, counter: str) -> dict:
     return {n: sum(v) / len(v) for n, v in per_kernel.items()}
 
 
+UTIL_COUNTERS = ("SQ_VALU_MFMA_BUSY_CYCLES GRBM_GUI_ACTIVE SQ_WAVE_CYCLES SQ_WAIT_ANY SQ_WAIT_INST_ANY "
+                 "SQ_ACTIVE_INST_ANY SQ_BUSY_CYCLES")
+XCDS, SIMDS = 8, 1024  # MI355X: GRBM_GUI_ACTIVE is summed over the 8 XCDs; 256 CUs x 4 SIMDs
+
+
+def utilisation(d: str, workload: str):
+    """Clock and MFMA utilisation of the dominant kernel from one UTIL_COUNTERS pass:
+    clock = GRBM_GUI_ACTIVE / XCDS / duration, MFMA busy = SQ_VALU_MFMA_BUSY_CYCLES /
+    (GRBM_GUI_ACTIVE / XCDS * SIMDS), and the wave-cycle fractions spent waiting."""
+    rows = _one(d, "*counter_collection.csv") or []
+    per, dur = {}, {}
+    for r in rows:
+        if DOMINANT[workload] not in r["Kernel_Name"]:
+            continue
+        per.setdefault(r["Dispatch_Id"], {})[r["Counter_Name"]] = float(r["Counter_Value"])
+        dur[r["Dispatch_Id"]] = int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
+    if not per:
+        return None
+    out = []
+    for k, v in per.items():
+        gui = v["GRBM_GUI_ACTIVE"] / XCDS
+        out.append({"duration_us": dur[k] / 1e3, "clock_mhz": gui / dur[k] * 1e3,
+                    "mfma_busy": v["SQ_VALU_MFMA_BUSY_CYCLES"] / (gui * SIMDS),
+                    "wait_any": v["SQ_WAIT_ANY"] / v["SQ_WAVE_CYCLES"],
+                    "wait_inst_any": v["SQ_WAIT_INST_ANY"] / v["SQ_WAVE_CYCLES"],
+                    "active_inst_any": v["SQ_ACTIVE_INST_ANY"] / v["SQ_WAVE_CYCLES"]})
+    return {key: sum(o[key] for o in out) / len(out) for key in out[0]}
+
+
 def _dominant(values: dict, workload: str):
     for name, v in values.items():
         if DOMINANT[workload] in name:
@@ -148,6 +178,16 @@ def summarize(root: str) -> dict:
             print(f"## HBM traffic per launch: {w}\n\n`{name[:100]}`: FETCH_SIZE {got['FETCH_SIZE'][1]:.1f} KiB raw -> "
                   f"{got['FETCH_SIZE'][2]:.4g} B; WRITE_SIZE {got['WRITE_SIZE'][1]:.1f} KiB raw -> "
                   f"{got['WRITE_SIZE'][2]:.4g} B; total {traffic[w]['bytes']:.4g} B\n")
+    for d in sorted(glob.glob(os.path.join(root, "util_*"))):
+        w = os.path.basename(d)[5:]
+        u = utilisation(d, w)
+        if u:
+            print(f"## Clock and MFMA utilisation: {w}\n")
+            print("| duration (us) | clock (MHz) | MFMA busy | waves waiting (waitcnt/barrier) | waves waiting to issue | "
+                  "waves issuing |")
+            print("|---|---|---|---|---|---|")
+            print(f"| {u['duration_us']:.0f} | {u['clock_mhz']:.0f} | {u['mfma_busy']:.1%} | {u['wait_any']:.1%} | "
+                  f"{u['wait_inst_any']:.1%} | {u['active_inst_any']:.1%} |\n")
     return traffic
 
 
