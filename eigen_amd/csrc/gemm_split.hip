@@ -81,15 +81,27 @@ __global__ void __launch_bounds__(256) pack_split_kernel(const float* __restrict
   __shared__ float tile[TR][TK + 1];
   const int r0 = blockIdx.x * TR, k0 = blockIdx.y * TK;
   const int tid = threadIdx.x;
+  // 16-byte loads of 4 consecutive elements along the contiguous index (a
+  // buffer load needs 4-byte alignment only); reads past the operand return
+  // zero, elements past `rows` or K are zeroed.
+  const uint64_t extent = KCONT ? ((uint64_t)(rows - 1) * ld + K) * 4 : ((uint64_t)(K - 1) * ld + rows) * 4;
+  const int64_t base_el = KCONT ? (int64_t)r0 * ld + k0 : r0 + (int64_t)k0 * ld;
+  const uint64_t left = extent > (uint64_t)base_el * 4 ? extent - (uint64_t)base_el * 4 : 0;
+  const __amdgpu_buffer_rsrc_t rs =
+      make_rsrc(X + base_el, (uint32_t)(left < 0xFFFFFFF0ull ? left : 0xFFFFFFF0ull));
 #pragma unroll
-  for (int i = 0; i < TR * TK / 256; ++i) {
-    const int q = tid + 256 * i;
-    const int r = KCONT ? q / TK : q % TR;
-    const int k = KCONT ? q % TK : q / TR;
-    const int gr = r0 + r, gk = k0 + k;
-    float v = 0.0f;
-    if (gr < rows && gk < K) v = KCONT ? X[(int64_t)gr * ld + gk] : X[gr + (int64_t)gk * ld];
-    tile[r][k] = v;
+  for (int i = 0; i < TR * TK / 1024; ++i) {
+    const int q = tid + 256 * i;  // one 4-element group
+    const int r = KCONT ? q / (TK / 4) : (q % (TR / 4)) * 4;
+    const int k = KCONT ? (q % (TK / 4)) * 4 : q / (TR / 4);
+    const uint32_t off = (uint32_t)((KCONT ? (int64_t)r * ld + k : r + (int64_t)k * ld) * 4);
+    const u32x4 v = buf_load16(rs, off, 0);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int rr = KCONT ? r : r + e, kk = KCONT ? k + e : k;
+      const bool ok = r0 + rr < rows && k0 + kk < K;
+      tile[rr][kk] = ok ? __uint_as_float(v[e]) : 0.0f;
+    }
   }
   __syncthreads();
   // 2 slices x 64 rows x 4 chunks = 512 chunks: chunk fastest, then row
