@@ -190,6 +190,7 @@ def main() -> int:
         del dA, dB, dC
         torch.cuda.empty_cache()
         out["detail"].update(bench_dgemm(E, torch, dev, args))
+        out["detail"]["gemv_16384"] = bench_gemv(E, torch, dev, args)
         out["detail"]["batched"] = bench_batched(E, torch, args)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
@@ -221,6 +222,29 @@ def bench_dgemm(E, torch, dev, args):
                      "planner": ["heuristic", "mlp", "table"][E.lib().eh_get_tune_mode()]}
         log(f"{name}: {out[name]}")
         del dA, dB, dC
+    return out
+
+
+def bench_gemv(E, torch, dev, args):
+    """Matrix-vector products (the n == 1 path): y = A*x and y = A^T*x with A
+    16384 x 16384, FP32 and FP64, U[-1,1]; algorithmic bytes = the matrix read
+    once (the vectors are 0.01% of it)."""
+    out = {}
+    n = 16384
+    for dtype, name, tdt, es in ((E.F32, "f32", torch.float32, 4), (E.F64, "f64", torch.float64, 8)):
+        A = torch.empty(n * n, dtype=tdt, device=dev)
+        x = torch.empty(n, dtype=tdt, device=dev)
+        y = torch.zeros(n, dtype=tdt, device=dev)
+        E.fill(dtype, A, n * n, 21, E.DIST_UNIFORM, -1, 1)
+        E.fill(dtype, x, n, 22, E.DIST_UNIFORM, -1, 1)
+        for ta in (0, 1):
+            fn = lambda: E.gemm(dtype, ta, 0, n, 1, n, 1.0, A, n, x, n, 0.0, y, n)
+            wall, kern = timed(fn, max(10, args.steps), args.warmup)
+            gbs = n * n * es / (kern * 1e-3) / 1e9
+            out[f"{name}_{'T' if ta else 'N'}"] = {"ms": wall, "kernel_ms": kern, "gbs": gbs,
+                                                   "frac_of_hbm": gbs / HBM_PEAK_GBS}
+        del A, x, y
+    log(f"gemv: {out}")
     return out
 
 

@@ -306,6 +306,50 @@ static eh_status launch_product(bool ta, bool tb, int64_t m, int64_t n, int64_t 
   return EH_SUCCESS;
 }
 
+// A product with one output column (n == 1) or row (m == 1) is a
+// matrix-vector product: C^T = op(B)^T op(A)^T turns the row case into the
+// column case.  Returns false (nothing launched) when the matrix spans 4 GB
+// or more; the tiled path then handles it.
+template <typename T>
+static bool vector_product(int opa, int opb, int64_t m, int64_t n, int64_t k, T alpha, const T* A, int64_t lda,
+                           const T* B, int64_t ldb, T beta, T* C, int64_t ldc, hipStream_t stream, eh_status* st) {
+  bool trans;
+  int rows;
+  const T *mat, *x;
+  int64_t ldm, incx, incy;
+  if (n == 1) {  // y = op(A) * op(B)(:, 0)
+    trans = opa != EH_OP_N;
+    rows = (int)m;
+    mat = A;
+    ldm = lda;
+    x = B;
+    incx = opb == EH_OP_N ? 1 : ldb;
+    incy = 1;
+  } else {  // m == 1: y^T = op(B)^T * op(A)(0, :)^T, y stored along C's row
+    trans = opb == EH_OP_N;
+    rows = (int)n;
+    mat = B;
+    ldm = ldb;
+    x = A;
+    incx = opa == EH_OP_N ? lda : 1;
+    incy = ldc;
+  }
+  const uint64_t extent =
+      ((trans ? (uint64_t)(rows - 1) * ldm + k : (uint64_t)(k - 1) * ldm + rows) + 64) * sizeof(T);
+  if (extent >= kMaxExtent) return false;
+  const GemvPlan g = gemv_plan(trans, rows, (int)k, (int)sizeof(T));
+  void* ws = nullptr;
+  const size_t wsb = gemv_workspace_bytes(g, rows, (int)sizeof(T));
+  if (wsb > 0) {
+    *st = workspace_get(stream, wsb, &ws);
+    if (*st != EH_SUCCESS) return true;
+  }
+  hipError_t e = launch_gemv<T>(g, trans, rows, (int)k, alpha, mat, ldm, (uint32_t)(extent - 64 * sizeof(T)), x, incx,
+                                beta, C, incy, ws, stream);
+  *st = e == hipSuccess ? EH_SUCCESS : hip_fail(e, "gemv launch");
+  return true;
+}
+
 template <typename T>
 static eh_status run_gemm(int opa, int opb, int64_t m, int64_t n, int64_t k, T alpha, const T* A, int64_t lda,
                           const T* B, int64_t ldb, T beta, T* C, int64_t ldc, eh_precision prec, int config,
@@ -328,6 +372,12 @@ static eh_status run_gemm(int opa, int opb, int64_t m, int64_t n, int64_t k, T a
     }
   } else if (prec != EH_PREC_AUTO && prec != EH_PREC_PLAIN) {
     return fail(EH_ERR_UNSUPPORTED, "FP64 products run on FP64 MFMA only (precision AUTO or PLAIN)");
+  }
+  if (!split_prec && config < 0 && (n == 1 || m == 1)) {
+    // HBM-bound matrix-vector product (a split precision keeps its own
+    // arithmetic and runs the split product)
+    eh_status st = EH_SUCCESS;
+    if (vector_product<T>(opa, opb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, stream, &st)) return st;
   }
   if (!split_prec && config < 0) {
     eh_status st = plan_config(std::is_same<T, float>::value ? EH_F32 : EH_F64, opa, opb, m, n, k, &config);

@@ -36,6 +36,20 @@ template <typename T>
 hipError_t launch_strided_combine(const T* W, int M, int N, T* C, int64_t incc, int64_t ldc, T alpha, T beta,
                                   hipStream_t s);
 
+// Matrix-vector product y = alpha*op(A)*x + beta*y, op(A) m x k: trans
+// means op(A)(i,k) = A[k + i*lda], otherwise A[i + k*lda].  The plan fixes
+// the grid (row blocks x K splits); with more than one split the launcher
+// needs gemv_workspace_bytes of workspace for the partial sums.
+struct GemvPlan {
+  int blocks, splits, per;
+};
+GemvPlan gemv_plan(bool trans, int m, int k, int es);
+size_t gemv_workspace_bytes(const GemvPlan& g, int m, int es);
+template <typename T>
+hipError_t launch_gemv(const GemvPlan& g, bool trans, int m, int k, T alpha, const T* A, int64_t lda,
+                       uint32_t bytesA, const T* x, int64_t incx, T beta, T* y, int64_t incy, void* workspace,
+                       hipStream_t s);
+
 // Split-BF16 FP32 product over packed operands.  Each operand is first packed
 // into bf16 planes P[ks][term][row][32] (rows padded to split_rows_padded,
 // ks = k / 32 slices) by launch_pack_split, then the product kernel streams

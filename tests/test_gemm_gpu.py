@@ -278,3 +278,40 @@ def test_strided_destination_matches_contiguous(gpu, dtype, prec, alpha, beta):
     mask = np.ones(ldc * n, dtype=bool)
     mask[(rows * incc + cols * ldc).ravel()] = False
     assert np.all(out[mask] == sentinel)
+
+
+def _padded(a: np.ndarray, pad: int) -> np.ndarray:
+    """a stored with `pad` extra rows (leading dimension a.shape[0] + pad)."""
+    full = np.zeros((a.shape[0] + pad, a.shape[1]), dtype=a.dtype, order="F")
+    full[:a.shape[0]] = a
+    return full
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("ta,tb", [(False, False), (True, False), (False, True), (True, True)])
+@pytest.mark.parametrize("shape", [(300, 1, 257), (1, 300, 257), (1, 1, 1000), (5000, 1, 3000), (1, 4100, 2000),
+                                   (17, 1, 20000), (2, 1, 3)])
+def test_vector_products_exact(gpu, dtype, ta, tb, shape):
+    """Products with one output column or row take the matrix-vector kernels
+    (split K included); small-integer operands make every order exact."""
+    m, n, k = shape
+    A, B, C0 = int_operands(dtype, ta, tb, m, n, k, seed=70)
+    Ap, Bp, Cp = _padded(A, 3), _padded(B, 5), _padded(C0, 2)
+    for alpha, beta in ((1.5, 0.5), (-2.0, 0.0)):
+        want = O.gemm_ref(ta, tb, alpha, A, B, beta, C0)
+        got = run_gemm(dtype, ta, tb, m, n, k, alpha, Ap, Bp, beta, Cp, lda=Ap.shape[0], ldb=Bp.shape[0],
+                       ldc=Cp.shape[0])
+        assert np.array_equal(got.astype(np.float64), want), (alpha, beta)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("ta", [False, True])
+def test_vector_product_random_within_bound(gpu, dtype, ta):
+    m, k = 16384, 16384
+    A = O.fill((k, m) if ta else (m, k), dtype, 81, O.UNIFORM, -1, 1)
+    x = O.fill((k, 1), dtype, 82, O.UNIFORM, -1, 1)
+    got = run_gemm(dtype, ta, False, m, 1, k, 1.0, A, x, 0.0, None, prec=E.PREC_PLAIN)
+    rows = np.arange(0, m, 64)
+    ref, absref = O.gemm_ref(ta, False, 1.0, A, x, rows=rows, with_abs=True)
+    err = np.linalg.norm(got[rows].astype(np.float64) - ref)
+    assert err <= O.higham_mary_bound(absref, k, unit_roundoff(dtype)) + 1e-300
