@@ -26,6 +26,7 @@ PREC_AUTO, PREC_PLAIN, PREC_SPLIT2, PREC_SPLIT3 = 0, 1, 2, 3
 TUNE_HEURISTIC, TUNE_MLP, TUNE_TABLE = 0, 1, 2
 DIST_UNIFORM, DIST_NORMAL, DIST_WIDE, DIST_INT = 0, 1, 2, 3
 SUCCESS, ERR_INVALID, ERR_HIP, ERR_UNSUPPORTED, ERR_ALLOC = 0, 1, 2, 3, 4
+UPPER, LOWER = 0, 1
 
 _lib = None
 _blas = None
@@ -116,6 +117,8 @@ def lib():
         L.eh_gemm.argtypes = [i32, i32, i32, i64, i64, i64, vp, vp, i64, vp, i64, vp, vp, i64, i32, vp]
         L.eh_gemm_config.restype = i32
         L.eh_gemm_config.argtypes = [i32, i32, i32, i64, i64, i64, vp, vp, i64, vp, i64, vp, vp, i64, i32, i32, vp]
+        L.eh_syrk.restype = i32
+        L.eh_syrk.argtypes = [i32, i32, i32, i64, i64, vp, vp, i64, vp, vp, i64, vp]
         L.eh_gemm_strided.restype = i32
         L.eh_gemm_strided.argtypes = [i32, i32, i32, i64, i64, i64, vp, vp, i64, vp, i64, vp, vp, i64, i64, i32, vp]
         L.eh_num_configs.restype = i32
@@ -230,6 +233,14 @@ def gemm(dtype: int, opa: int, opb: int, m: int, n: int, k: int, alpha: float, A
         st = L.eh_gemm_config(dtype, opa, opb, m, n, k, C.byref(a), _ptr(A), lda, _ptr(B), ldb, C.byref(b), _ptr(Cm),
                               ldc, prec, config, s)
     check(st, "eh_gemm")
+
+
+def syrk(dtype: int, uplo: int, trans: int, n: int, k: int, alpha: float, A, lda: int, beta: float, Cm, ldc: int,
+         stream=None) -> None:
+    """uplo triangle of C = alpha*op(A)*op(A)^T + beta*C (eh_syrk)."""
+    a, b = _scalar(dtype, alpha), _scalar(dtype, beta)
+    s = current_stream() if stream is None else stream
+    check(lib().eh_syrk(dtype, uplo, trans, n, k, C.byref(a), _ptr(A), lda, C.byref(b), _ptr(Cm), ldc, s), "eh_syrk")
 
 
 def gemm_strided(dtype: int, opa: int, opb: int, m: int, n: int, k: int, alpha: float, A, lda: int, B, ldb: int,

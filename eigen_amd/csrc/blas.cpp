@@ -1,5 +1,6 @@
-// blas.cpp -- sgemm_ / dgemm_ with the semantics of Eigen's BLAS
-// (blas/level3_impl.h:12-96) on top of the device product.
+// blas.cpp -- sgemm_ / dgemm_ (and ssyrk_ / dsyrk_) with the semantics of
+// Eigen's BLAS (blas/level3_impl.h:12-96, 594-689) on top of the device
+// product.
 //
 // The reference validates in a fixed order and reports through xerbla_
 // (:61-78), returns early for m == 0 or n == 0 (:80), pre-scales C by beta
@@ -151,6 +152,69 @@ void gemm_impl(const char* name, const char* opa, const char* opb, const int* m,
   if (hipStreamSynchronize(s) != hipSuccess) std::fprintf(stderr, "eigen_amd BLAS: stream sync failed\n");
 }
 
+// ?syrk_ with the semantics of blas/level3_impl.h:594-689: validation order
+// and info codes 1 (uplo), 2 (trans), 3 (n), 4 (k), 7 (lda), 10 (ldc); only
+// the uplo triangle of C is read or written.
+template <typename T>
+void syrk_impl(const char* name, const char* uplo, const char* op, const int* n, const int* k, const T* palpha,
+               const T* a, const int* lda, const T* pbeta, T* c, const int* ldc) {
+  int info = 0;
+  const int ul = (*uplo == 'U' || *uplo == 'u') ? EH_UPPER : (*uplo == 'L' || *uplo == 'l') ? EH_LOWER : -1;
+  const int o = op_code(*op);
+  if (ul < 0)
+    info = 1;
+  else if (o == OP_BAD)
+    info = 2;
+  else if (*n < 0)
+    info = 3;
+  else if (*k < 0)
+    info = 4;
+  else if (*lda < std::max(1, o == OP_N ? *n : *k))
+    info = 7;
+  else if (*ldc < std::max(1, *n))
+    info = 10;
+  if (info) {
+    xerbla_(name, &info, 6);
+    return;
+  }
+  const T alpha = *palpha, beta = *pbeta;
+  const bool need_a = *k > 0 && alpha != T(0);
+  if (*n == 0 || (!need_a && beta == T(1))) return;
+
+  const eh_dtype dt = sizeof(T) == 4 ? EH_F32 : EH_F64;
+  const int arows = o == OP_N ? *n : *k, acols = o == OP_N ? *k : *n;
+  const bool dev_a = !need_a || is_device_ptr(a);
+  const bool dev_c = is_device_ptr(c);
+  hipStream_t s = g_stream;
+  std::unique_lock<std::mutex> lk(g_stage_mutex, std::defer_lock);
+  const T* da = a;
+  T* dc = c;
+  int64_t dlda = *lda, dldc = *ldc;
+  if (!dev_a || !dev_c) lk.lock();
+  if (!dev_a) {
+    if (!g_sa.ensure((size_t)std::max(arows, 1) * std::max(acols, 1) * sizeof(T)) ||
+        hipMemcpy2DAsync(g_sa.ptr, arows * sizeof(T), a, (size_t)(*lda) * sizeof(T), arows * sizeof(T), acols,
+                         hipMemcpyHostToDevice, s) != hipSuccess)
+      return (void)std::fprintf(stderr, "eigen_amd BLAS: staging A failed\n");
+    da = static_cast<const T*>(g_sa.ptr);
+    dlda = std::max(arows, 1);
+  }
+  if (!dev_c) {  // the whole of C travels, so the other triangle comes back unchanged
+    if (!g_sc.ensure((size_t)(*n) * (*n) * sizeof(T)) ||
+        hipMemcpy2DAsync(g_sc.ptr, (size_t)(*n) * sizeof(T), c, (size_t)(*ldc) * sizeof(T), (size_t)(*n) * sizeof(T),
+                         *n, hipMemcpyHostToDevice, s) != hipSuccess)
+      return (void)std::fprintf(stderr, "eigen_amd BLAS: staging C failed\n");
+    dc = static_cast<T*>(g_sc.ptr);
+    dldc = *n;
+  }
+  eh_status st = eh_syrk(dt, (eh_uplo)ul, (eh_op)o, *n, *k, &alpha, da, dlda, &beta, dc, dldc, s);
+  if (st != EH_SUCCESS) return report(name);
+  if (!dev_c && hipMemcpy2DAsync(c, (size_t)(*ldc) * sizeof(T), dc, (size_t)(*n) * sizeof(T),
+                                 (size_t)(*n) * sizeof(T), *n, hipMemcpyDeviceToHost, s) != hipSuccess)
+    std::fprintf(stderr, "eigen_amd BLAS: copying C back failed\n");
+  if (hipStreamSynchronize(s) != hipSuccess) std::fprintf(stderr, "eigen_amd BLAS: stream sync failed\n");
+}
+
 }  // namespace
 
 extern "C" {
@@ -169,6 +233,16 @@ void dgemm_(const char* transa, const char* transb, const int* m, const int* n, 
             const double* a, const int* lda, const double* b, const int* ldb, const double* beta, double* c,
             const int* ldc) {
   gemm_impl<double>("DGEMM ", transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
+}
+
+void ssyrk_(const char* uplo, const char* trans, const int* n, const int* k, const float* alpha, const float* a,
+            const int* lda, const float* beta, float* c, const int* ldc) {
+  syrk_impl<float>("SSYRK ", uplo, trans, n, k, alpha, a, lda, beta, c, ldc);
+}
+
+void dsyrk_(const char* uplo, const char* trans, const int* n, const int* k, const double* alpha, const double* a,
+            const int* lda, const double* beta, double* c, const int* ldc) {
+  syrk_impl<double>("DSYRK ", uplo, trans, n, k, alpha, a, lda, beta, c, ldc);
 }
 
 void eh_blas_set_stream(void* stream) { g_stream = static_cast<hipStream_t>(stream); }

@@ -226,7 +226,7 @@ static eh_status launch_split_product(const GemmParams<float>& g, bool ta, bool 
 template <typename T>
 static eh_status launch_product(bool ta, bool tb, int64_t m, int64_t n, int64_t k, T alpha, const T* A,
                                 int64_t lda, const T* B, int64_t ldb, T beta, T* C, int64_t ldc, bool split_prec,
-                                int terms, int config, hipStream_t stream) {
+                                int terms, int config, hipStream_t stream, int tri = 0) {
   const bool fa = operand_fits<T>(ta, m, k, lda), fb = operand_fits<T>(!tb, n, k, ldb);
   if (!fa || !fb) {
     // Halve the dimension that drives the oversized extent and recurse.
@@ -235,13 +235,14 @@ static eh_status launch_product(bool ta, bool tb, int64_t m, int64_t n, int64_t 
     if (cut_k && k > 1) {
       const int64_t h = k / 2;
       eh_status st = launch_product<T>(ta, tb, m, n, h, alpha, A, lda, B, ldb, beta, C, ldc, split_prec, terms,
-                                       config, stream);
+                                       config, stream, tri);
       if (st != EH_SUCCESS) return st;
       const T* A2 = ta ? A + h : A + h * lda;
       const T* B2 = tb ? B + h * ldb : B + h;
       return launch_product<T>(ta, tb, m, n, k - h, alpha, A2, lda, B2, ldb, T(1), C, ldc, split_prec, terms,
-                               config, stream);
+                               config, stream, tri);
     }
+    if (tri != 0) return fail(EH_ERR_UNSUPPORTED, "triangular update with operands past 32-bit addressing");
     if (cut_m && m > 1) {
       const int64_t h = m / 2;
       eh_status st = launch_product<T>(ta, tb, h, n, k, alpha, A, lda, B, ldb, beta, C, ldc, split_prec, terms,
@@ -273,6 +274,7 @@ static eh_status launch_product(bool ta, bool tb, int64_t m, int64_t n, int64_t 
   p.K = (int)k;
   p.alpha = alpha;
   p.beta = beta;
+  p.tri = tri;
   p.bytesA = (uint32_t)((ta ? (uint64_t)(m - 1) * lda + k : (uint64_t)(k - 1) * lda + m) * sizeof(T));
   p.bytesB = (uint32_t)((!tb ? (uint64_t)(n - 1) * ldb + k : (uint64_t)(k - 1) * ldb + n) * sizeof(T));
 
@@ -474,6 +476,40 @@ eh_status eh_gemm_strided(eh_dtype dtype, eh_op opa, eh_op opb, int64_t m, int64
                                                      *static_cast<const float*>(beta), s);
   if (W) (void)hipFreeAsync(W, s);
   return e == hipSuccess ? EH_SUCCESS : hip_fail(e, "strided combine launch");
+}
+
+eh_status eh_syrk(eh_dtype dtype, eh_uplo uplo, eh_op trans, int64_t n, int64_t k, const void* alpha, const void* A,
+                  int64_t lda, const void* beta, void* C, int64_t ldc, void* stream) {
+  if (dtype != EH_F32 && dtype != EH_F64) return fail(EH_ERR_INVALID, "dtype must be EH_F32 or EH_F64");
+  if (uplo != EH_UPPER && uplo != EH_LOWER) return fail(EH_ERR_INVALID, "uplo must be EH_UPPER or EH_LOWER");
+  if (!valid_op(trans)) return fail(EH_ERR_INVALID, "trans must be N, T or C");
+  if (n < 0 || k < 0) return fail(EH_ERR_INVALID, "n, k must be >= 0");
+  if (n > INT32_MAX || k > INT32_MAX) return fail(EH_ERR_UNSUPPORTED, "n, k must fit int32");
+  if (lda < std::max<int64_t>(1, trans == EH_OP_N ? n : k)) return fail(EH_ERR_INVALID, "lda too small");
+  if (ldc < std::max<int64_t>(1, n)) return fail(EH_ERR_INVALID, "ldc too small");
+  if (!alpha || !beta) return fail(EH_ERR_INVALID, "alpha and beta must point to host scalars");
+  if (n == 0) return EH_SUCCESS;
+  if (!C || (k > 0 && !A)) return fail(EH_ERR_INVALID, "A or C is NULL");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int tri = uplo == EH_LOWER ? 1 : 2;
+  const bool tr = trans != EH_OP_N;  // op(A) = A^T: C = alpha*A^T*A + beta*C
+  auto run = [&](auto zero) -> eh_status {
+    using T = decltype(zero);
+    const T al = *static_cast<const T*>(alpha), be = *static_cast<const T*>(beta);
+    T* Cp = static_cast<T*>(C);
+    if (k == 0 || al == T(0)) {  // A is not read
+      if (be == T(1)) return EH_SUCCESS;
+      hipError_t e = launch_scale_c<T>(Cp, (int)n, (int)n, ldc, be, s, tri);
+      return e == hipSuccess ? EH_SUCCESS : hip_fail(e, "scale_c launch");
+    }
+    const int opa = tr ? EH_OP_T : EH_OP_N, opb = tr ? EH_OP_N : EH_OP_T;
+    int config = -1;
+    eh_status st = plan_config(dtype, opa, opb, n, n, k, &config);
+    if (st != EH_SUCCESS) return st;
+    const T* Ap = static_cast<const T*>(A);
+    return launch_product<T>(tr, !tr, n, n, k, al, Ap, lda, Ap, lda, be, Cp, ldc, false, 0, config, s, tri);
+  };
+  return dtype == EH_F64 ? run(0.0) : run(0.0f);
 }
 
 int eh_num_configs(void) { return kNumConfigs; }

@@ -49,18 +49,18 @@ hipError_t launch_splitk_reduce(const GemmParams<T>& p, int splits, hipStream_t 
 }
 
 template <typename T>
-hipError_t launch_scale_c(T* C, int M, int N, int64_t ldc, T beta, hipStream_t s) {
+hipError_t launch_scale_c(T* C, int M, int N, int64_t ldc, T beta, hipStream_t s, int tri) {
   const int64_t total = (int64_t)M * N;
   int64_t blocks = (total + 255) / 256;
   if (blocks > 65536) blocks = 65536;
   if (blocks < 1) blocks = 1;
-  hipLaunchKernelGGL(scale_c_kernel<T>, dim3((unsigned)blocks), dim3(256), 0, s, C, M, N, ldc, beta);
+  hipLaunchKernelGGL(scale_c_kernel<T>, dim3((unsigned)blocks), dim3(256), 0, s, C, M, N, ldc, beta, tri);
   return hipGetLastError();
 }
 
 template hipError_t launch_gemm_mfma<float>(const GemmParams<float>&, int, bool, bool, int, hipStream_t);
 template hipError_t launch_splitk_reduce<float>(const GemmParams<float>&, int, hipStream_t);
-template hipError_t launch_scale_c<float>(float*, int, int, int64_t, float, hipStream_t);
+template hipError_t launch_scale_c<float>(float*, int, int, int64_t, float, hipStream_t, int);
 
 template <typename T>
 hipError_t launch_strided_combine(const T* W, int M, int N, T* C, int64_t incc, int64_t ldc, T alpha, T beta,

@@ -49,6 +49,8 @@ __global__ void __launch_bounds__(64 * WGM * WGN) gemm_mfma_kernel(GemmParams<T>
   int tm, tn;
   tile_coords(xcd_remap(blockIdx.x, gridDim.x), ntm, ntn, 8, tm, tn);
   const int m0 = tm * BM, n0 = tn * BN;
+  // a tile with no element in the updated triangle has nothing to do
+  if ((p.tri == 1 && m0 + BM <= n0) || (p.tri == 2 && n0 + BN <= m0)) return;
   const int z = blockIdx.y;
   const int kbeg = z * p.k_per_split;
   const int kend = min(p.K, kbeg + p.k_per_split);
@@ -245,7 +247,7 @@ __global__ void __launch_bounds__(64 * WGM * WGN) gemm_mfma_kernel(GemmParams<T>
 #pragma unroll
       for (int r = 0; r < F::ACC; ++r) {
         const int row = m0 + wm * WTM + i * F::MI + F::row(r, lane);
-        if (row >= p.M) continue;
+        if (row >= p.M || (p.tri == 1 && row < col) || (p.tri == 2 && row > col)) continue;
         if (partial) {
           p.W[(int64_t)z * p.M * p.N + row + (int64_t)col * p.M] = acc[i][j][r];
         } else {
@@ -262,23 +264,26 @@ template <typename T>
 __global__ void __launch_bounds__(256) splitk_reduce_kernel(GemmParams<T> p, int splits) {
   const int64_t total = (int64_t)p.M * p.N;
   for (int64_t idx = blockIdx.x * 256ll + threadIdx.x; idx < total; idx += (int64_t)gridDim.x * 256) {
-    T s = T(0);
-    for (int z = 0; z < splits; ++z) s += p.W[(int64_t)z * total + idx];
     const int row = (int)(idx % p.M);
     const int64_t col = idx / p.M;
+    if ((p.tri == 1 && row < col) || (p.tri == 2 && row > col)) continue;
+    T s = T(0);
+    for (int z = 0; z < splits; ++z) s += p.W[(int64_t)z * total + idx];
     T* dst = p.C + row + col * p.ldc;
     *dst = (p.beta == T(0)) ? p.alpha * s : fma_t(p.alpha, s, p.beta * *dst);
   }
 }
 
-// C = beta*C (beta==0 writes zeros without reading C).  Used when alpha==0 or
-// k==0: the product term vanishes and A, B are not read.
+// C = beta*C on the triangle `tri` (as GemmParams::tri) of C (beta==0 writes
+// zeros without reading C).  Used when alpha==0 or k==0: the product term
+// vanishes and A, B are not read.
 template <typename T>
-__global__ void __launch_bounds__(256) scale_c_kernel(T* C, int M, int N, int64_t ldc, T beta) {
+__global__ void __launch_bounds__(256) scale_c_kernel(T* C, int M, int N, int64_t ldc, T beta, int tri) {
   const int64_t total = (int64_t)M * N;
   for (int64_t idx = blockIdx.x * 256ll + threadIdx.x; idx < total; idx += (int64_t)gridDim.x * 256) {
     const int row = (int)(idx % M);
     const int64_t col = idx / M;
+    if ((tri == 1 && row < col) || (tri == 2 && row > col)) continue;
     T* dst = C + row + col * ldc;
     *dst = (beta == T(0)) ? T(0) : beta * *dst;
   }

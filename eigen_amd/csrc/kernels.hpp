@@ -23,6 +23,9 @@ struct GemmParams {
   // both must stay below 2^32 with a margin of one tile (the host splits
   // larger products).
   uint32_t bytesA, bytesB;
+  // Triangle of C the product updates: 0 all of it, 1 the lower (row >= col),
+  // 2 the upper (row <= col); the rest of C is neither read nor written.
+  int tri;
 };
 
 // Plain MFMA product: tile 0..5 = 64x64, 64x128, 128x64, 128x128, 128x256, 256x128.
@@ -31,7 +34,7 @@ hipError_t launch_gemm_mfma(const GemmParams<T>& p, int tile, bool ta, bool tb, 
 template <typename T>
 hipError_t launch_splitk_reduce(const GemmParams<T>& p, int splits, hipStream_t s);
 template <typename T>
-hipError_t launch_scale_c(T* C, int M, int N, int64_t ldc, T beta, hipStream_t s);
+hipError_t launch_scale_c(T* C, int M, int N, int64_t ldc, T beta, hipStream_t s, int tri = 0);
 template <typename T>
 hipError_t launch_strided_combine(const T* W, int M, int N, T* C, int64_t incc, int64_t ldc, T alpha, T beta,
                                   hipStream_t s);

@@ -19,6 +19,9 @@
  *   eh_gemm_strided
  *       the same driver with a result increment (resIncr,
  *       GeneralMatrixMatrix.h:98-137) for destinations with an inner stride.
+ *   eh_syrk
+ *       ?syrk_ (blas/level3_impl.h:594-689) /
+ *       general_matrix_matrix_triangular_product (GeneralMatrixMatrixTriangular.h).
  *   eh_plan / eh_set_tune_mode
  *       computeProductBlockingSizes + evaluateProductBlockingSizesHeuristic
  *       (GeneralBlockPanelKernel.h:201-452): the blocking choice, here a
@@ -109,6 +112,24 @@ eh_status eh_gemm_config(eh_dtype dtype, eh_op opa, eh_op opb, int64_t m, int64_
 eh_status eh_gemm_strided(eh_dtype dtype, eh_op opa, eh_op opb, int64_t m, int64_t n, int64_t k, const void* alpha,
                           const void* A, int64_t lda, const void* B, int64_t ldb, const void* beta, void* C,
                           int64_t incc, int64_t ldc, eh_precision prec, void* stream);
+
+/* Triangle of a symmetric result. */
+typedef enum { EH_UPPER = 0, EH_LOWER = 1 } eh_uplo;
+
+/* Symmetric rank-k update of one triangle of C (n x n):
+ *   trans = N:  C = alpha*A*A^T + beta*C   (A n x k)
+ *   trans = T:  C = alpha*A^T*A + beta*C   (A k x n)
+ * Only the `uplo` triangle of C (diagonal included) is read or written.
+ * Replaces ?syrk_ (blas/level3_impl.h:594-689) and the rank update it runs,
+ * general_matrix_matrix_triangular_product
+ * (Eigen/src/Core/products/GeneralMatrixMatrixTriangular.h:38-330), which
+ * Eigen's selfadjointView<>().rankUpdate() and triangularView<>() += A*A^T
+ * reach, and which GeneralMatrixMatrixTriangular_BLAS.h:80-106 redirects to
+ * ?syrk_ under EIGEN_USE_BLAS.  Runs the plain FP32 / FP64 MFMA kernels with
+ * the tiles outside the triangle skipped and the epilogue masked; alpha == 0
+ * or k == 0 scales the triangle without reading A. */
+eh_status eh_syrk(eh_dtype dtype, eh_uplo uplo, eh_op trans, int64_t n, int64_t k, const void* alpha, const void* A,
+                  int64_t lda, const void* beta, void* C, int64_t ldc, void* stream);
 
 /* ---- planning ------------------------------------------------------------ */
 

@@ -37,6 +37,16 @@ void sgemm_(const char* transa, const char* transb, const int* m, const int* n, 
 void dgemm_(const char* transa, const char* transb, const int* m, const int* n, const int* k, const double* alpha,
             const double* a, const int* lda, const double* b, const int* ldb, const double* beta, double* c,
             const int* ldc);
+/* Symmetric rank-k update of the uplo triangle of C: C = alpha*op(A)*op(A)^T + beta*C
+ * (trans 'N': A n x k; 'T'/'C': A k x n).  Same symbols, arguments, validation
+ * order and info codes (1 uplo, 2 trans, 3 n, 4 k, 7 lda, 10 ldc) as Eigen's
+ * ?syrk_ (blas/level3_impl.h:594-689), which Eigen's rank updates reach under
+ * EIGEN_USE_BLAS (GeneralMatrixMatrixTriangular_BLAS.h:80-106).  Device or
+ * host pointers, as for ?gemm_. */
+void ssyrk_(const char* uplo, const char* trans, const int* n, const int* k, const float* alpha, const float* a,
+            const int* lda, const float* beta, float* c, const int* ldc);
+void dsyrk_(const char* uplo, const char* trans, const int* n, const int* k, const double* alpha, const double* a,
+            const int* lda, const double* beta, double* c, const int* ldc);
 /* Error hook (blas/xerbla.cpp:18): weak default prints the routine and info. */
 void xerbla_(const char* name, int* info, size_t len);
 

@@ -171,6 +171,11 @@ def gemm_ref(ta: bool, tb: bool, alpha, A, B, beta=0.0, C0=None, rows=None, m=No
     A, B, C0 are column-major arrays; returns an (nrows, n) float64 array (and
     the |alpha| sum |a||b| array when with_abs)."""
     dtype = 0 if A.dtype == np.float32 else 1
+    # the leading dimension defaults to shape[0]: the arrays must then be
+    # column-major (a row-major array would be read transposed)
+    for name, x, ld in (("A", A, lda), ("B", B, ldb), ("C0", C0, ldc)):
+        if x is not None and ld is None and x.ndim == 2 and not x.flags.f_contiguous:
+            raise ValueError(f"gemm_ref: {name} must be column-major (Fortran order)")
     if m is None:
         m = A.shape[1] if ta else A.shape[0]
     if k is None:

@@ -1,7 +1,8 @@
 /* netlib_stubs.c -- TEST INFRASTRUCTURE ONLY.
  * The netlib level-3 testers (blas/testing/sblat3.f, dblat3.f) reference
- * every level-3 routine; the GEMM-only runs here switch the others off in
- * their input deck, so these entry points must exist but are never called. */
+ * every level-3 routine; the runs here switch on only the routines the
+ * library provides (GEMM, SYRK), so these entry points must exist but are
+ * never called. */
 #include <stdio.h>
 #include <stdlib.h>
 
@@ -11,12 +12,10 @@
     abort();                                                            \
   }
 NEVER(ssymm_)
-NEVER(ssyrk_)
 NEVER(ssyr2k_)
 NEVER(strmm_)
 NEVER(strsm_)
 NEVER(dsymm_)
-NEVER(dsyrk_)
 NEVER(dsyr2k_)
 NEVER(dtrmm_)
 NEVER(dtrsm_)

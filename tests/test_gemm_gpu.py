@@ -315,3 +315,30 @@ def test_vector_product_random_within_bound(gpu, dtype, ta):
     ref, absref = O.gemm_ref(ta, False, 1.0, A, x, rows=rows, with_abs=True)
     err = np.linalg.norm(got[rows].astype(np.float64) - ref)
     assert err <= O.higham_mary_bound(absref, k, unit_roundoff(dtype)) + 1e-300
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("uplo", [E.UPPER, E.LOWER])
+@pytest.mark.parametrize("trans", [False, True])
+@pytest.mark.parametrize("n,k", [(1, 1), (65, 47), (130, 300), (300, 1), (257, 2100), (256, 20000)])
+def test_syrk_triangle_exact(gpu, dtype, uplo, trans, n, k):
+    """eh_syrk updates exactly the uplo triangle with alpha*op(A)*op(A)^T + beta*C
+    (small-integer operands: exact) and leaves the other triangle untouched."""
+    A = O.fill((k, n) if trans else (n, k), dtype, 90 + n, O.INT, -8, 8)
+    Ap = _padded(A, 3)
+    C0 = O.fill((n, n), dtype, 91, O.INT, -8, 8)
+    iu = np.triu_indices(n) if uplo == E.UPPER else np.tril_indices(n)
+    other = np.ones((n, n), dtype=bool)
+    other[iu] = False
+    for alpha, beta in ((1.5, 0.5), (2.0, 0.0), (0.0, -1.0)):
+        Cin = C0.copy(order="F")
+        if beta == 0.0:
+            Cin[iu] = np.nan  # the updated triangle must not be read
+        Cp = _padded(Cin, 2)
+        dA, dC = to_dev(Ap), to_dev(Cp)
+        E.syrk(DT[dtype], uplo, int(trans), n, k, alpha, dA, Ap.shape[0], beta, dC, Cp.shape[0])
+        got = from_dev(dC, (n + 2, n))[:n]
+        full = O.gemm_ref(trans, not trans, alpha, A, A, beta, Cin) if beta != 0.0 else \
+            O.gemm_ref(trans, not trans, alpha, A, A)
+        assert np.array_equal(got[iu].astype(np.float64), full[iu]), (alpha, beta)
+        assert np.array_equal(got[other], Cin[other], equal_nan=True), (alpha, beta)

@@ -2,12 +2,13 @@
 reference, compiled into oracle/_ref and linked against our sgemm_/dgemm_).
 
 The input deck mirrors the reference's sblat3.dat / dblat3.dat (N in
-{0,1,2,3,5,9}, alpha in {0,1,0.7}, beta in {0,1,1.3}, threshold 16) with only
-the GEMM routine switched on.  The tester supplies its own XERBLA, which
+{0,1,2,3,5,9}, alpha in {0,1,0.7}, beta in {0,1,1.3}, threshold 16) with the
+routines the library provides switched on: GEMM and SYRK.  The tester supplies its own XERBLA, which
 overrides our weak xerbla_, to check the error exits; the computational tests
 compare every call with its own column-by-column recomputation (DMMCH).
 The error-exit run uses N = 0 only and needs no GPU."""
 import os
+import re
 import subprocess
 import tempfile
 
@@ -16,7 +17,10 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 REF = os.path.join(ROOT, "oracle", "_ref")
 
-OTHERS = {"s": ["SSYMM ", "STRMM ", "STRSM ", "SSYRK ", "SSYR2K"], "d": ["DSYMM ", "DTRMM ", "DTRSM ", "DSYRK ", "DSYR2K"]}
+# routines in the order of the reference decks (sblat3.dat / dblat3.dat) and
+# whether the library provides them
+ROUTINES = [("GEMM ", True), ("SYMM ", False), ("TRMM ", False), ("TRSM ", False), ("SYRK ", True),
+            ("SYR2K", False)]
 
 
 def deck(prec: str, ns, error_exits=True) -> str:
@@ -29,9 +33,8 @@ def deck(prec: str, ns, error_exits=True) -> str:
              "16.0     THRESHOLD VALUE OF TEST RATIO",
              f"{len(ns)}                 NUMBER OF VALUES OF N", " ".join(str(n) for n in ns) + "       VALUES OF N",
              "3                 NUMBER OF VALUES OF ALPHA", "0.0 1.0 0.7       VALUES OF ALPHA",
-             "3                 NUMBER OF VALUES OF BETA", "0.0 1.0 1.3       VALUES OF BETA",
-             f"{up}GEMM  T PUT F FOR NO TEST. SAME COLUMNS."]
-    lines += [f"{name} F PUT F FOR NO TEST. SAME COLUMNS." for name in OTHERS[prec]]
+             "3                 NUMBER OF VALUES OF BETA", "0.0 1.0 1.3       VALUES OF BETA"]
+    lines += [f"{up}{name} {'T' if on else 'F'} PUT F FOR NO TEST. SAME COLUMNS." for name, on in ROUTINES]
     return "\n".join(lines) + "\n"
 
 
@@ -51,9 +54,11 @@ def run_tester(prec: str, ns, error_exits=True) -> str:
 @pytest.mark.parametrize("prec", ["s", "d"])
 def test_error_exits(prec):
     out = run_tester(prec, [0])
-    name = f"{prec.upper()}GEMM"
-    assert f"{name}  PASSED THE TESTS OF ERROR-EXITS" in out, out
-    assert f"{name}  PASSED THE COMPUTATIONAL TESTS (    81 CALLS)" in out, out  # N = 0: quick returns only
+    for name in ("GEMM", "SYRK"):
+        assert f"{prec.upper()}{name}  PASSED THE TESTS OF ERROR-EXITS" in out, out
+        # N = 0: quick returns only
+        assert re.search(rf"{prec.upper()}{name}  PASSED THE COMPUTATIONAL TESTS \(\s*\d+ CALLS\)", out), out
+    assert f"{prec.upper()}GEMM  PASSED THE COMPUTATIONAL TESTS (    81 CALLS)" in out, out
     assert "FAILED" not in out and "SUSPECT" not in out, out
 
 
@@ -61,7 +66,9 @@ def test_error_exits(prec):
 @pytest.mark.parametrize("prec", ["s", "d"])
 def test_computational(gpu, prec):
     out = run_tester(prec, [0, 1, 2, 3, 5, 9])
-    name = f"{prec.upper()}GEMM"
-    assert f"{name}  PASSED THE TESTS OF ERROR-EXITS" in out, out
-    assert f"{name}  PASSED THE COMPUTATIONAL TESTS ( 17496 CALLS)" in out, out
+    for name in ("GEMM", "SYRK"):
+        assert f"{prec.upper()}{name}  PASSED THE TESTS OF ERROR-EXITS" in out, out
+    assert f"{prec.upper()}GEMM  PASSED THE COMPUTATIONAL TESTS ( 17496 CALLS)" in out, out
+    m = re.search(rf"{prec.upper()}SYRK  PASSED THE COMPUTATIONAL TESTS \(\s*(\d+) CALLS\)", out)
+    assert m and int(m.group(1)) > 1000, out
     assert "FAILED" not in out and "SUSPECT" not in out, out
