@@ -27,6 +27,8 @@ TUNE_HEURISTIC, TUNE_MLP, TUNE_TABLE = 0, 1, 2
 DIST_UNIFORM, DIST_NORMAL, DIST_WIDE, DIST_INT = 0, 1, 2, 3
 SUCCESS, ERR_INVALID, ERR_HIP, ERR_UNSUPPORTED, ERR_ALLOC = 0, 1, 2, 3, 4
 UPPER, LOWER = 0, 1
+LEFT, RIGHT = 0, 1
+NON_UNIT, UNIT = 0, 1
 
 _lib = None
 _blas = None
@@ -119,6 +121,15 @@ def lib():
         L.eh_gemm_config.argtypes = [i32, i32, i32, i64, i64, i64, vp, vp, i64, vp, i64, vp, vp, i64, i32, i32, vp]
         L.eh_syrk.restype = i32
         L.eh_syrk.argtypes = [i32, i32, i32, i64, i64, vp, vp, i64, vp, vp, i64, vp]
+        L.eh_gemmtr.restype = i32
+        L.eh_gemmtr.argtypes = [i32, i32, i32, i32, i64, i64, vp, vp, i64, vp, i64, vp, vp, i64, vp]
+        L.eh_syr2k.restype = i32
+        L.eh_syr2k.argtypes = [i32, i32, i32, i64, i64, vp, vp, i64, vp, i64, vp, vp, i64, vp]
+        L.eh_symm.restype = i32
+        L.eh_symm.argtypes = [i32, i32, i32, i64, i64, vp, vp, i64, vp, i64, vp, vp, i64, vp]
+        for f in (L.eh_trmm, L.eh_trsm):
+            f.restype = i32
+            f.argtypes = [i32, i32, i32, i32, i32, i64, i64, vp, vp, i64, vp, i64, vp]
         L.eh_gemm_strided.restype = i32
         L.eh_gemm_strided.argtypes = [i32, i32, i32, i64, i64, i64, vp, vp, i64, vp, i64, vp, vp, i64, i64, i32, vp]
         L.eh_num_configs.restype = i32
@@ -241,6 +252,49 @@ def syrk(dtype: int, uplo: int, trans: int, n: int, k: int, alpha: float, A, lda
     a, b = _scalar(dtype, alpha), _scalar(dtype, beta)
     s = current_stream() if stream is None else stream
     check(lib().eh_syrk(dtype, uplo, trans, n, k, C.byref(a), _ptr(A), lda, C.byref(b), _ptr(Cm), ldc, s), "eh_syrk")
+
+
+def gemmtr(dtype: int, uplo: int, opa: int, opb: int, n: int, k: int, alpha: float, A, lda: int, B, ldb: int,
+           beta: float, Cm, ldc: int, stream=None) -> None:
+    """uplo triangle of C = alpha*op(A)*op(B) + beta*C (eh_gemmtr)."""
+    a, b = _scalar(dtype, alpha), _scalar(dtype, beta)
+    s = current_stream() if stream is None else stream
+    check(lib().eh_gemmtr(dtype, uplo, opa, opb, n, k, C.byref(a), _ptr(A), lda, _ptr(B), ldb, C.byref(b), _ptr(Cm),
+                          ldc, s), "eh_gemmtr")
+
+
+def syr2k(dtype: int, uplo: int, trans: int, n: int, k: int, alpha: float, A, lda: int, B, ldb: int, beta: float,
+          Cm, ldc: int, stream=None) -> None:
+    """uplo triangle of C = alpha*(op(A)*op(B)^T + op(B)*op(A)^T) + beta*C (eh_syr2k)."""
+    a, b = _scalar(dtype, alpha), _scalar(dtype, beta)
+    s = current_stream() if stream is None else stream
+    check(lib().eh_syr2k(dtype, uplo, trans, n, k, C.byref(a), _ptr(A), lda, _ptr(B), ldb, C.byref(b), _ptr(Cm), ldc,
+                         s), "eh_syr2k")
+
+
+def symm(dtype: int, side: int, uplo: int, m: int, n: int, alpha: float, A, lda: int, B, ldb: int, beta: float, Cm,
+         ldc: int, stream=None) -> None:
+    """C = alpha*S*B + beta*C (LEFT) or alpha*B*S + beta*C (RIGHT), S symmetric in A's uplo triangle (eh_symm)."""
+    a, b = _scalar(dtype, alpha), _scalar(dtype, beta)
+    s = current_stream() if stream is None else stream
+    check(lib().eh_symm(dtype, side, uplo, m, n, C.byref(a), _ptr(A), lda, _ptr(B), ldb, C.byref(b), _ptr(Cm), ldc,
+                        s), "eh_symm")
+
+
+def trmm(dtype: int, side: int, uplo: int, transa: int, diag: int, m: int, n: int, alpha: float, A, lda: int, B,
+         ldb: int, stream=None) -> None:
+    """B = alpha*op(T)*B (LEFT) or alpha*B*op(T) (RIGHT), T the uplo triangle of A (eh_trmm)."""
+    a = _scalar(dtype, alpha)
+    s = current_stream() if stream is None else stream
+    check(lib().eh_trmm(dtype, side, uplo, transa, diag, m, n, C.byref(a), _ptr(A), lda, _ptr(B), ldb, s), "eh_trmm")
+
+
+def trsm(dtype: int, side: int, uplo: int, transa: int, diag: int, m: int, n: int, alpha: float, A, lda: int, B,
+         ldb: int, stream=None) -> None:
+    """Solve op(T)*X = alpha*B (LEFT) or X*op(T) = alpha*B (RIGHT) in place in B (eh_trsm)."""
+    a = _scalar(dtype, alpha)
+    s = current_stream() if stream is None else stream
+    check(lib().eh_trsm(dtype, side, uplo, transa, diag, m, n, C.byref(a), _ptr(A), lda, _ptr(B), ldb, s), "eh_trsm")
 
 
 def gemm_strided(dtype: int, opa: int, opb: int, m: int, n: int, k: int, alpha: float, A, lda: int, B, ldb: int,

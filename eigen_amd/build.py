@@ -25,7 +25,8 @@ HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 HIPFLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
-LIB_SOURCES = ["gemm_f32.hip", "gemm_f64.hip", "gemm_split.hip", "gemv.hip", "fill.hip", "grouped.hip", "mlp.hip",
+LIB_SOURCES = ["gemm_f32.hip", "gemm_f64.hip", "gemm_split.hip", "gemv.hip", "level3_kernels.hip", "fill.hip", "grouped.hip",
+               "mlp.hip", "level3.cpp",
                "capi.cpp", "planner.cpp", "tuner.cpp"]
 BLAS_SOURCES = ["blas.cpp"]
 # Per-file code generation choices.  The FP64 kernels keep their

@@ -1,6 +1,6 @@
-// blas.cpp -- sgemm_ / dgemm_ (and ssyrk_ / dsyrk_) with the semantics of
-// Eigen's BLAS (blas/level3_impl.h:12-96, 594-689) on top of the device
-// product.
+// blas.cpp -- sgemm_ / dgemm_ and the other real level-3 routines (?gemmtr_,
+// ?syrk_, ?syr2k_, ?symm_, ?trmm_, ?trsm_) with the semantics of Eigen's BLAS
+// (blas/level3_impl.h:12-758) on top of the device product.
 //
 // The reference validates in a fixed order and reports through xerbla_
 // (:61-78), returns early for m == 0 or n == 0 (:80), pre-scales C by beta
@@ -215,6 +215,222 @@ void syrk_impl(const char* name, const char* uplo, const char* op, const int* n,
   if (hipStreamSynchronize(s) != hipSuccess) std::fprintf(stderr, "eigen_amd BLAS: stream sync failed\n");
 }
 
+// ---- the other level-3 routines ---------------------------------------------
+int side_code(char x) { return (x == 'L' || x == 'l') ? EH_LEFT : (x == 'R' || x == 'r') ? EH_RIGHT : -1; }
+int uplo_code(char x) { return (x == 'U' || x == 'u') ? EH_UPPER : (x == 'L' || x == 'l') ? EH_LOWER : -1; }
+int diag_code(char x) { return (x == 'N' || x == 'n') ? EH_NON_UNIT : (x == 'U' || x == 'u') ? EH_UNIT : -1; }
+
+// Device view of one rows x cols matrix argument: the pointer itself when it
+// is device memory, else a packed copy (ld = rows) in `st`, filled from the
+// host when `in`; copy_back returns a staged copy to the host.
+// Arguments the routine does not read (alpha == 0) keep the caller's
+// pointer and leading dimension.
+template <typename T>
+struct DevMat {
+  DevMat(const T* p_, int ld_) : p(const_cast<T*>(p_)), ld(ld_) {}
+  T* p;
+  int64_t ld;
+  bool staged = false;
+  int rows = 0, cols = 0;
+  int64_t host_ld = 0;
+};
+
+template <typename T>
+bool dev_view(Stage& st, const T* host, int rows, int cols, int ld, bool in, hipStream_t s, DevMat<T>* d) {
+  d->rows = rows;
+  d->cols = cols;
+  d->host_ld = ld;
+  if (rows <= 0 || cols <= 0 || is_device_ptr(host)) {
+    d->p = const_cast<T*>(host);
+    d->ld = ld;
+    return true;
+  }
+  if (!st.ensure((size_t)rows * cols * sizeof(T))) return false;
+  d->p = static_cast<T*>(st.ptr);
+  d->ld = rows;
+  d->staged = true;
+  return !in || hipMemcpy2DAsync(st.ptr, rows * sizeof(T), host, (size_t)ld * sizeof(T), rows * sizeof(T), cols,
+                                 hipMemcpyHostToDevice, s) == hipSuccess;
+}
+
+template <typename T>
+bool copy_back(const DevMat<T>& d, T* host, hipStream_t s) {
+  return !d.staged || hipMemcpy2DAsync(host, (size_t)d.host_ld * sizeof(T), d.p, d.ld * sizeof(T),
+                                       d.rows * sizeof(T), d.cols, hipMemcpyDeviceToHost, s) == hipSuccess;
+}
+
+void finish(hipStream_t s) {
+  if (hipStreamSynchronize(s) != hipSuccess) std::fprintf(stderr, "eigen_amd BLAS: stream sync failed\n");
+}
+
+constexpr eh_dtype dtype_of(size_t es) { return es == 4 ? EH_F32 : EH_F64; }
+
+// ?gemmtr_ with the checks of blas/level3_impl.h:181-197: info 1 uplo,
+// 2 transa, 3 transb, 4 n, 5 k, 8 lda, 10 ldb, 13 ldc.
+template <typename T>
+void gemmtr_impl(const char* name, const char* uplo, const char* opa, const char* opb, const int* n, const int* k,
+                 const T* palpha, const T* a, const int* lda, const T* b, const int* ldb, const T* pbeta, T* c,
+                 const int* ldc) {
+  int info = 0;
+  const int ul = uplo_code(*uplo), oa = op_code(*opa), ob = op_code(*opb);
+  if (ul < 0)
+    info = 1;
+  else if (oa == OP_BAD)
+    info = 2;
+  else if (ob == OP_BAD)
+    info = 3;
+  else if (*n < 0)
+    info = 4;
+  else if (*k < 0)
+    info = 5;
+  else if (*lda < std::max(1, oa == OP_N ? *n : *k))
+    info = 8;
+  else if (*ldb < std::max(1, ob == OP_N ? *k : *n))
+    info = 10;
+  else if (*ldc < std::max(1, *n))
+    info = 13;
+  if (info) return xerbla_(name, &info, 7);
+  if (*n == 0) return;
+  const T alpha = *palpha, beta = *pbeta;
+  const bool need_ab = *k > 0 && alpha != T(0);
+  if (!need_ab && beta == T(1)) return;
+  hipStream_t s = g_stream;
+  std::lock_guard<std::mutex> lk(g_stage_mutex);
+  DevMat<T> A(a, *lda), B(b, *ldb), C(c, *ldc);
+  if ((need_ab && (!dev_view(g_sa, a, oa == OP_N ? *n : *k, oa == OP_N ? *k : *n, *lda, true, s, &A) ||
+                   !dev_view(g_sb, b, ob == OP_N ? *k : *n, ob == OP_N ? *n : *k, *ldb, true, s, &B))) ||
+      !dev_view(g_sc, c, *n, *n, *ldc, true, s, &C))  // all of C travels: the other triangle returns unchanged
+    return (void)std::fprintf(stderr, "eigen_amd BLAS: staging failed\n");
+  if (eh_gemmtr(dtype_of(sizeof(T)), (eh_uplo)ul, (eh_op)oa, (eh_op)ob, *n, *k, &alpha, A.p, A.ld, B.p, B.ld, &beta,
+                C.p, C.ld, s) != EH_SUCCESS)
+    return report(name);
+  if (!copy_back(C, c, s)) std::fprintf(stderr, "eigen_amd BLAS: copying C back failed\n");
+  finish(s);
+}
+
+// ?syr2k_ with the checks of blas/level3_impl.h:702-716: info 1 uplo,
+// 2 trans, 3 n, 4 k, 7 lda, 9 ldb, 12 ldc.
+template <typename T>
+void syr2k_impl(const char* name, const char* uplo, const char* op, const int* n, const int* k, const T* palpha,
+                const T* a, const int* lda, const T* b, const int* ldb, const T* pbeta, T* c, const int* ldc) {
+  int info = 0;
+  const int ul = uplo_code(*uplo), o = op_code(*op);
+  const int rows = o == OP_N ? *n : *k, cols = o == OP_N ? *k : *n;
+  if (ul < 0)
+    info = 1;
+  else if (o == OP_BAD)
+    info = 2;
+  else if (*n < 0)
+    info = 3;
+  else if (*k < 0)
+    info = 4;
+  else if (*lda < std::max(1, rows))
+    info = 7;
+  else if (*ldb < std::max(1, rows))
+    info = 9;
+  else if (*ldc < std::max(1, *n))
+    info = 12;
+  if (info) return xerbla_(name, &info, 6);
+  if (*n == 0) return;
+  const T alpha = *palpha, beta = *pbeta;
+  const bool need_ab = *k > 0 && alpha != T(0);
+  if (!need_ab && beta == T(1)) return;
+  hipStream_t s = g_stream;
+  std::lock_guard<std::mutex> lk(g_stage_mutex);
+  DevMat<T> A(a, *lda), B(b, *ldb), C(c, *ldc);
+  if ((need_ab && (!dev_view(g_sa, a, rows, cols, *lda, true, s, &A) ||
+                   !dev_view(g_sb, b, rows, cols, *ldb, true, s, &B))) ||
+      !dev_view(g_sc, c, *n, *n, *ldc, true, s, &C))
+    return (void)std::fprintf(stderr, "eigen_amd BLAS: staging failed\n");
+  if (eh_syr2k(dtype_of(sizeof(T)), (eh_uplo)ul, (eh_op)o, *n, *k, &alpha, A.p, A.ld, B.p, B.ld, &beta, C.p, C.ld, s) !=
+      EH_SUCCESS)
+    return report(name);
+  if (!copy_back(C, c, s)) std::fprintf(stderr, "eigen_amd BLAS: copying C back failed\n");
+  finish(s);
+}
+
+// ?symm_ with the checks of blas/level3_impl.h:514-532: info 1 side,
+// 2 uplo, 3 m, 4 n, 7 lda, 9 ldb, 12 ldc.
+template <typename T>
+void symm_impl(const char* name, const char* side, const char* uplo, const int* m, const int* n, const T* palpha,
+               const T* a, const int* lda, const T* b, const int* ldb, const T* pbeta, T* c, const int* ldc) {
+  int info = 0;
+  const int sd = side_code(*side), ul = uplo_code(*uplo);
+  const int ka = sd == EH_LEFT ? *m : *n;
+  if (sd < 0)
+    info = 1;
+  else if (ul < 0)
+    info = 2;
+  else if (*m < 0)
+    info = 3;
+  else if (*n < 0)
+    info = 4;
+  else if (*lda < std::max(1, ka))
+    info = 7;
+  else if (*ldb < std::max(1, *m))
+    info = 9;
+  else if (*ldc < std::max(1, *m))
+    info = 12;
+  if (info) return xerbla_(name, &info, 6);
+  if (*m == 0 || *n == 0) return;
+  const T alpha = *palpha, beta = *pbeta;
+  if (alpha == T(0) && beta == T(1)) return;
+  hipStream_t s = g_stream;
+  std::lock_guard<std::mutex> lk(g_stage_mutex);
+  DevMat<T> A(a, *lda), B(b, *ldb), C(c, *ldc);
+  const bool need_ab = alpha != T(0);
+  if ((need_ab && (!dev_view(g_sa, a, ka, ka, *lda, true, s, &A) ||
+                   !dev_view(g_sb, b, *m, *n, *ldb, true, s, &B))) ||
+      !dev_view(g_sc, c, *m, *n, *ldc, beta != T(0), s, &C))
+    return (void)std::fprintf(stderr, "eigen_amd BLAS: staging failed\n");
+  if (eh_symm(dtype_of(sizeof(T)), (eh_side)sd, (eh_uplo)ul, *m, *n, &alpha, A.p, A.ld, B.p, B.ld, &beta, C.p, C.ld,
+              s) != EH_SUCCESS)
+    return report(name);
+  if (!copy_back(C, c, s)) std::fprintf(stderr, "eigen_amd BLAS: copying C back failed\n");
+  finish(s);
+}
+
+// ?trmm_ / ?trsm_ with the checks of blas/level3_impl.h:339-357 and
+// :478-496: info 1 side, 2 uplo, 3 transa, 4 diag, 5 m, 6 n, 9 lda, 11 ldb.
+template <typename T, bool SOLVE>
+void tri_impl(const char* name, const char* side, const char* uplo, const char* opa, const char* diag, const int* m,
+              const int* n, const T* palpha, const T* a, const int* lda, T* b, const int* ldb) {
+  int info = 0;
+  const int sd = side_code(*side), ul = uplo_code(*uplo), o = op_code(*opa), dg = diag_code(*diag);
+  const int ka = sd == EH_LEFT ? *m : *n;
+  if (sd < 0)
+    info = 1;
+  else if (ul < 0)
+    info = 2;
+  else if (o == OP_BAD)
+    info = 3;
+  else if (dg < 0)
+    info = 4;
+  else if (*m < 0)
+    info = 5;
+  else if (*n < 0)
+    info = 6;
+  else if (*lda < std::max(1, ka))
+    info = 9;
+  else if (*ldb < std::max(1, *m))
+    info = 11;
+  if (info) return xerbla_(name, &info, 6);
+  if (*m == 0 || *n == 0) return;
+  const T alpha = *palpha;
+  hipStream_t s = g_stream;
+  std::lock_guard<std::mutex> lk(g_stage_mutex);
+  DevMat<T> A(a, *lda), B(b, *ldb);
+  if ((alpha != T(0) && !dev_view(g_sa, a, ka, ka, *lda, true, s, &A)) ||
+      !dev_view(g_sb, b, *m, *n, *ldb, alpha != T(0), s, &B))
+    return (void)std::fprintf(stderr, "eigen_amd BLAS: staging failed\n");
+  auto fn = SOLVE ? eh_trsm : eh_trmm;
+  if (fn(dtype_of(sizeof(T)), (eh_side)sd, (eh_uplo)ul, (eh_op)o, (eh_diag)dg, *m, *n, &alpha, A.p, A.ld, B.p, B.ld,
+         s) != EH_SUCCESS)
+    return report(name);
+  if (!copy_back(B, b, s)) std::fprintf(stderr, "eigen_amd BLAS: copying B back failed\n");
+  finish(s);
+}
+
 }  // namespace
 
 extern "C" {
@@ -243,6 +459,58 @@ void ssyrk_(const char* uplo, const char* trans, const int* n, const int* k, con
 void dsyrk_(const char* uplo, const char* trans, const int* n, const int* k, const double* alpha, const double* a,
             const int* lda, const double* beta, double* c, const int* ldc) {
   syrk_impl<double>("DSYRK ", uplo, trans, n, k, alpha, a, lda, beta, c, ldc);
+}
+
+void sgemmtr_(const char* uplo, const char* transa, const char* transb, const int* n, const int* k,
+              const float* alpha, const float* a, const int* lda, const float* b, const int* ldb, const float* beta,
+              float* c, const int* ldc) {
+  gemmtr_impl<float>("SGEMMTR", uplo, transa, transb, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
+}
+
+void dgemmtr_(const char* uplo, const char* transa, const char* transb, const int* n, const int* k,
+              const double* alpha, const double* a, const int* lda, const double* b, const int* ldb,
+              const double* beta, double* c, const int* ldc) {
+  gemmtr_impl<double>("DGEMMTR", uplo, transa, transb, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
+}
+
+void ssyr2k_(const char* uplo, const char* trans, const int* n, const int* k, const float* alpha, const float* a,
+             const int* lda, const float* b, const int* ldb, const float* beta, float* c, const int* ldc) {
+  syr2k_impl<float>("SSYR2K", uplo, trans, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
+}
+
+void dsyr2k_(const char* uplo, const char* trans, const int* n, const int* k, const double* alpha, const double* a,
+             const int* lda, const double* b, const int* ldb, const double* beta, double* c, const int* ldc) {
+  syr2k_impl<double>("DSYR2K", uplo, trans, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
+}
+
+void ssymm_(const char* side, const char* uplo, const int* m, const int* n, const float* alpha, const float* a,
+            const int* lda, const float* b, const int* ldb, const float* beta, float* c, const int* ldc) {
+  symm_impl<float>("SSYMM ", side, uplo, m, n, alpha, a, lda, b, ldb, beta, c, ldc);
+}
+
+void dsymm_(const char* side, const char* uplo, const int* m, const int* n, const double* alpha, const double* a,
+            const int* lda, const double* b, const int* ldb, const double* beta, double* c, const int* ldc) {
+  symm_impl<double>("DSYMM ", side, uplo, m, n, alpha, a, lda, b, ldb, beta, c, ldc);
+}
+
+void strmm_(const char* side, const char* uplo, const char* transa, const char* diag, const int* m, const int* n,
+            const float* alpha, const float* a, const int* lda, float* b, const int* ldb) {
+  tri_impl<float, false>("STRMM ", side, uplo, transa, diag, m, n, alpha, a, lda, b, ldb);
+}
+
+void dtrmm_(const char* side, const char* uplo, const char* transa, const char* diag, const int* m, const int* n,
+            const double* alpha, const double* a, const int* lda, double* b, const int* ldb) {
+  tri_impl<double, false>("DTRMM ", side, uplo, transa, diag, m, n, alpha, a, lda, b, ldb);
+}
+
+void strsm_(const char* side, const char* uplo, const char* transa, const char* diag, const int* m, const int* n,
+            const float* alpha, const float* a, const int* lda, float* b, const int* ldb) {
+  tri_impl<float, true>("STRSM ", side, uplo, transa, diag, m, n, alpha, a, lda, b, ldb);
+}
+
+void dtrsm_(const char* side, const char* uplo, const char* transa, const char* diag, const int* m, const int* n,
+            const double* alpha, const double* a, const int* lda, double* b, const int* ldb) {
+  tri_impl<double, true>("DTRSM ", side, uplo, transa, diag, m, n, alpha, a, lda, b, ldb);
 }
 
 void eh_blas_set_stream(void* stream) { g_stream = static_cast<hipStream_t>(stream); }

@@ -388,6 +388,27 @@ static eh_status run_gemm(int opa, int opb, int64_t m, int64_t n, int64_t k, T a
   return launch_product<T>(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, split_prec, terms, config, stream);
 }
 
+template <typename T>
+eh_status device_product(int opa, int opb, int64_t m, int64_t n, int64_t k, T alpha, const T* A, int64_t lda,
+                         const T* B, int64_t ldb, T beta, T* C, int64_t ldc, hipStream_t stream, int tri) {
+  if (tri == 0) return run_gemm<T>(opa, opb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, EH_PREC_PLAIN, -1, stream);
+  if (m == 0 || n == 0) return EH_SUCCESS;
+  if (k == 0 || alpha == T(0)) {
+    if (beta == T(1)) return EH_SUCCESS;
+    hipError_t e = launch_scale_c<T>(C, (int)m, (int)n, ldc, beta, stream, tri);
+    return e == hipSuccess ? EH_SUCCESS : hip_fail(e, "scale_c launch");
+  }
+  int config = -1;
+  eh_status st = plan_config(std::is_same<T, float>::value ? EH_F32 : EH_F64, opa, opb, m, n, k, &config);
+  if (st != EH_SUCCESS) return st;
+  return launch_product<T>(opa != EH_OP_N, opb != EH_OP_N, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, false, 0,
+                           config, stream, tri);
+}
+template eh_status device_product<float>(int, int, int64_t, int64_t, int64_t, float, const float*, int64_t,
+                                         const float*, int64_t, float, float*, int64_t, hipStream_t, int);
+template eh_status device_product<double>(int, int, int64_t, int64_t, int64_t, double, const double*, int64_t,
+                                          const double*, int64_t, double, double*, int64_t, hipStream_t, int);
+
 eh_status gemm_dispatch(int dtype, int opa, int opb, int64_t m, int64_t n, int64_t k, const void* alpha,
                         const void* A, int64_t lda, const void* B, int64_t ldb, const void* beta, void* C,
                         int64_t ldc, int prec, int config, void* stream) {
@@ -496,18 +517,10 @@ eh_status eh_syrk(eh_dtype dtype, eh_uplo uplo, eh_op trans, int64_t n, int64_t 
   auto run = [&](auto zero) -> eh_status {
     using T = decltype(zero);
     const T al = *static_cast<const T*>(alpha), be = *static_cast<const T*>(beta);
-    T* Cp = static_cast<T*>(C);
-    if (k == 0 || al == T(0)) {  // A is not read
-      if (be == T(1)) return EH_SUCCESS;
-      hipError_t e = launch_scale_c<T>(Cp, (int)n, (int)n, ldc, be, s, tri);
-      return e == hipSuccess ? EH_SUCCESS : hip_fail(e, "scale_c launch");
-    }
-    const int opa = tr ? EH_OP_T : EH_OP_N, opb = tr ? EH_OP_N : EH_OP_T;
-    int config = -1;
-    eh_status st = plan_config(dtype, opa, opb, n, n, k, &config);
-    if (st != EH_SUCCESS) return st;
     const T* Ap = static_cast<const T*>(A);
-    return launch_product<T>(tr, !tr, n, n, k, al, Ap, lda, Ap, lda, be, Cp, ldc, false, 0, config, s, tri);
+    // k == 0 or alpha == 0 only scales the triangle (A is not read)
+    return device_product<T>(tr ? EH_OP_T : EH_OP_N, tr ? EH_OP_N : EH_OP_T, n, n, k, al, Ap, lda, Ap, lda, be,
+                             static_cast<T*>(C), ldc, s, tri);
   };
   return dtype == EH_F64 ? run(0.0) : run(0.0f);
 }

@@ -29,6 +29,12 @@ eh_status workspace_get(hipStream_t stream, size_t bytes, void** out);
 eh_status check_gemm_args(int dtype, int opa, int opb, int64_t m, int64_t n, int64_t k, const void* alpha,
                           const void* A, int64_t lda, const void* B, int64_t ldb, const void* beta, const void* C,
                           int64_t ldc);
+// Plain-precision device product C = alpha*op(A)*op(B) + beta*C (operands
+// already checked) on the planner's configuration; tri != 0 updates only the
+// lower (1) or upper (2) triangle of C.  k == 0 or alpha == 0 only scales C.
+template <typename T>
+eh_status device_product(int opa, int opb, int64_t m, int64_t n, int64_t k, T alpha, const T* A, int64_t lda,
+                         const T* B, int64_t ldb, T beta, T* C, int64_t ldc, hipStream_t stream, int tri = 0);
 eh_status gemm_dispatch(int dtype, int opa, int opb, int64_t m, int64_t n, int64_t k, const void* alpha,
                         const void* A, int64_t lda, const void* B, int64_t ldb, const void* beta, void* C,
                         int64_t ldc, int prec, int config, void* stream);

@@ -53,6 +53,31 @@ hipError_t launch_gemv(const GemvPlan& g, bool trans, int m, int k, T alpha, con
                        uint32_t bytesA, const T* x, int64_t incx, T beta, T* y, int64_t incy, void* workspace,
                        hipStream_t s);
 
+// Dense form of the n x n matrix stored in one triangle of A (lower: i >= j):
+// sym mirrors the stored triangle, otherwise the other triangle is zero and
+// unit puts ones on the diagonal.  D has leading dimension ldd.
+template <typename T>
+hipError_t launch_tri_dense(const T* A, int64_t lda, int n, T* D, int64_t ldd, bool lower, bool sym, bool unit,
+                            hipStream_t s);
+
+// Triangular solve of one diagonal block for nvec right-hand sides, in
+// place: M x = alpha b with M(i, k) = mt ? A[k + i*lda] : A[i + k*lda] of
+// order nb <= kTrsmLeaf (lower or upper, unit diagonal or not); component i
+// of right-hand side v is B[i*es + v*vs].
+constexpr int kTrsmLeaf = 32;
+template <typename T>
+struct TrsmLeaf {
+  const T* A;
+  int64_t lda;
+  int mt, lower, unit, nb;
+  T* B;
+  int64_t es, vs;
+  int nvec;
+  T alpha;
+};
+template <typename T>
+hipError_t launch_trsm_leaf(const TrsmLeaf<T>& p, hipStream_t s);
+
 // Split-BF16 FP32 product over packed operands.  Each operand is first packed
 // into bf16 planes P[ks][term][row][32] (rows padded to split_rows_padded,
 // ks = k / 32 slices) by launch_pack_split, then the product kernel streams

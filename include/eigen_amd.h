@@ -22,6 +22,11 @@
  *   eh_syrk
  *       ?syrk_ (blas/level3_impl.h:594-689) /
  *       general_matrix_matrix_triangular_product (GeneralMatrixMatrixTriangular.h).
+ *   eh_gemmtr / eh_syr2k / eh_symm / eh_trmm / eh_trsm
+ *       ?gemmtr_ / ?syr2k_ / ?symm_ / ?trmm_ / ?trsm_ (blas/level3_impl.h:98-758)
+ *       and the level-3 kernels they run (GeneralMatrixMatrixTriangular.h,
+ *       SelfadjointMatrixMatrix.h, TriangularMatrixMatrix.h,
+ *       TriangularSolverMatrix.h), on the same MFMA product kernels.
  *   eh_plan / eh_set_tune_mode
  *       computeProductBlockingSizes + evaluateProductBlockingSizesHeuristic
  *       (GeneralBlockPanelKernel.h:201-452): the blocking choice, here a
@@ -130,6 +135,73 @@ typedef enum { EH_UPPER = 0, EH_LOWER = 1 } eh_uplo;
  * or k == 0 scales the triangle without reading A. */
 eh_status eh_syrk(eh_dtype dtype, eh_uplo uplo, eh_op trans, int64_t n, int64_t k, const void* alpha, const void* A,
                   int64_t lda, const void* beta, void* C, int64_t ldc, void* stream);
+
+/* ---- other level-3 routines on the product kernels -----------------------
+ * All take device pointers and host scalars, run on `stream` and return
+ * without synchronising.  Temporaries (a dense copy of a triangle, a copy of
+ * a block of B) come from the stream-ordered allocator.  FP32 runs on FP32
+ * MFMA (the split-BF16 precision applies to eh_gemm only). */
+
+typedef enum { EH_LEFT = 0, EH_RIGHT = 1 } eh_side;
+typedef enum { EH_NON_UNIT = 0, EH_UNIT = 1 } eh_diag;
+
+/* Product with a triangle of C: C = alpha*op(A)*op(B) + beta*C on the `uplo`
+ * triangle of the n x n result only (op(A) n x k, op(B) k x n).  Replaces
+ * ?gemmtr_ (blas/level3_impl.h:98-228) and the
+ * general_matrix_matrix_triangular_product it runs
+ * (GeneralMatrixMatrixTriangular.h:38-330). */
+eh_status eh_gemmtr(eh_dtype dtype, eh_uplo uplo, eh_op opa, eh_op opb, int64_t n, int64_t k, const void* alpha,
+                    const void* A, int64_t lda, const void* B, int64_t ldb, const void* beta, void* C, int64_t ldc,
+                    void* stream);
+
+/* Symmetric rank-2k update of one triangle of C (n x n):
+ *   trans = N:  C = alpha*A*B^T + alpha*B*A^T + beta*C   (A, B n x k)
+ *   trans = T:  C = alpha*A^T*B + alpha*B^T*A + beta*C   (A, B k x n)
+ * Replaces ?syr2k_ (blas/level3_impl.h:690-758), which evaluates the two
+ * products into the triangle (triangularView<>() += ...); here two masked
+ * products, the second accumulating onto the first. */
+eh_status eh_syr2k(eh_dtype dtype, eh_uplo uplo, eh_op trans, int64_t n, int64_t k, const void* alpha, const void* A,
+                   int64_t lda, const void* B, int64_t ldb, const void* beta, void* C, int64_t ldc, void* stream);
+
+/* Symmetric product, A symmetric and stored in its `uplo` triangle:
+ *   side = LEFT:   C = alpha*A*B + beta*C   (A m x m)
+ *   side = RIGHT:  C = alpha*B*A + beta*C   (A n x n)
+ * Replaces ?symm_ (blas/level3_impl.h:505-592) and
+ * product_selfadjoint_matrix (SelfadjointMatrixMatrix.h:323-520), whose
+ * packing mirrors the stored triangle (symm_pack_lhs/rhs, :20-250); here a
+ * tiled kernel writes the mirrored dense matrix once (HBM-bound, order^2
+ * elements) and the product kernels take it as a plain operand. */
+eh_status eh_symm(eh_dtype dtype, eh_side side, eh_uplo uplo, int64_t m, int64_t n, const void* alpha, const void* A,
+                  int64_t lda, const void* B, int64_t ldb, const void* beta, void* C, int64_t ldc, void* stream);
+
+/* Triangular product in place, A triangular in its `uplo` triangle (unit
+ * diagonal not read when diag = UNIT):
+ *   side = LEFT:   B = alpha*op(A)*B   (A m x m)
+ *   side = RIGHT:  B = alpha*B*op(A)   (A n x n)
+ * Replaces ?trmm_ (blas/level3_impl.h:366-503) and
+ * product_triangular_matrix_matrix (TriangularMatrixMatrix.h:84-420).
+ * Recursive 2x2 blocking: the off-diagonal blocks are plain products
+ * accumulated in place; a diagonal block of order <= 256 is made dense (zeros
+ * off the triangle) and multiplied from a copy of its rows (columns) of B.
+ * alpha == 0 sets B to zero without reading A (the netlib rule). */
+eh_status eh_trmm(eh_dtype dtype, eh_side side, eh_uplo uplo, eh_op transa, eh_diag diag, int64_t m, int64_t n,
+                  const void* alpha, const void* A, int64_t lda, void* B, int64_t ldb, void* stream);
+
+/* Triangular solve in place, A triangular in its `uplo` triangle:
+ *   side = LEFT:   op(A)*X = alpha*B   (A m x m)
+ *   side = RIGHT:  X*op(A) = alpha*B   (A n x n)
+ * X overwrites B.  Replaces ?trsm_ (blas/level3_impl.h:228-364) and
+ * triangular_solve_matrix (TriangularSolverMatrix.h:20-330).  Recursive 2x2
+ * blocking: a block of unknowns is solved, its contribution is subtracted
+ * from the remaining right-hand sides by a product on the MFMA kernels, and
+ * diagonal blocks of order <= 32 are solved by one thread per right-hand side
+ * (multiplying by the reciprocal of the diagonal, as the reference does).
+ * alpha is applied to the right-hand sides before the solve (the reference
+ * scales the solution afterwards: equal up to rounding); alpha == 0 sets B
+ * to zero without reading A.  No check for a zero diagonal: as in the
+ * reference, it produces infinities. */
+eh_status eh_trsm(eh_dtype dtype, eh_side side, eh_uplo uplo, eh_op transa, eh_diag diag, int64_t m, int64_t n,
+                  const void* alpha, const void* A, int64_t lda, void* B, int64_t ldb, void* stream);
 
 /* ---- planning ------------------------------------------------------------ */
 

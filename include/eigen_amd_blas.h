@@ -1,5 +1,5 @@
 /*
- * eigen_amd_blas.h -- BLAS level-3 GEMM entry points of the MI355X library.
+ * eigen_amd_blas.h -- BLAS level-3 entry points of the MI355X library.
  *
  * Same symbols, argument lists, validation order, error codes and quick
  * returns as Eigen's BLAS (blas/level3_impl.h:12-96, declared in
@@ -47,6 +47,41 @@ void ssyrk_(const char* uplo, const char* trans, const int* n, const int* k, con
             const int* lda, const float* beta, float* c, const int* ldc);
 void dsyrk_(const char* uplo, const char* trans, const int* n, const int* k, const double* alpha, const double* a,
             const int* lda, const double* beta, double* c, const int* ldc);
+/* The other real level-3 routines, with the symbols, arguments, validation
+ * order, info codes and quick returns of Eigen's BLAS; device or host
+ * pointers as for ?gemm_, the call returns when the output is complete.
+ *   ?gemmtr_  blas/level3_impl.h:98-228   info 1 uplo, 2 transa, 3 transb, 4 n,
+ *             5 k, 8 lda, 10 ldb, 13 ldc (name "SGEMMTR", 7 characters)
+ *   ?syr2k_   blas/level3_impl.h:690-758  info 1 uplo, 2 trans, 3 n, 4 k,
+ *             7 lda, 9 ldb, 12 ldc
+ *   ?symm_    blas/level3_impl.h:505-592  info 1 side, 2 uplo, 3 m, 4 n,
+ *             7 lda, 9 ldb, 12 ldc
+ *   ?trmm_    blas/level3_impl.h:366-503  info 1 side, 2 uplo, 3 transa,
+ *   ?trsm_    blas/level3_impl.h:228-364  4 diag, 5 m, 6 n, 9 lda, 11 ldb
+ * Only the uplo triangle of C is written by ?gemmtr_ / ?syr2k_; host C
+ * travels whole, so the other triangle comes back unchanged. */
+void sgemmtr_(const char* uplo, const char* transa, const char* transb, const int* n, const int* k,
+              const float* alpha, const float* a, const int* lda, const float* b, const int* ldb, const float* beta,
+              float* c, const int* ldc);
+void dgemmtr_(const char* uplo, const char* transa, const char* transb, const int* n, const int* k,
+              const double* alpha, const double* a, const int* lda, const double* b, const int* ldb,
+              const double* beta, double* c, const int* ldc);
+void ssyr2k_(const char* uplo, const char* trans, const int* n, const int* k, const float* alpha, const float* a,
+             const int* lda, const float* b, const int* ldb, const float* beta, float* c, const int* ldc);
+void dsyr2k_(const char* uplo, const char* trans, const int* n, const int* k, const double* alpha, const double* a,
+             const int* lda, const double* b, const int* ldb, const double* beta, double* c, const int* ldc);
+void ssymm_(const char* side, const char* uplo, const int* m, const int* n, const float* alpha, const float* a,
+            const int* lda, const float* b, const int* ldb, const float* beta, float* c, const int* ldc);
+void dsymm_(const char* side, const char* uplo, const int* m, const int* n, const double* alpha, const double* a,
+            const int* lda, const double* b, const int* ldb, const double* beta, double* c, const int* ldc);
+void strmm_(const char* side, const char* uplo, const char* transa, const char* diag, const int* m, const int* n,
+            const float* alpha, const float* a, const int* lda, float* b, const int* ldb);
+void dtrmm_(const char* side, const char* uplo, const char* transa, const char* diag, const int* m, const int* n,
+            const double* alpha, const double* a, const int* lda, double* b, const int* ldb);
+void strsm_(const char* side, const char* uplo, const char* transa, const char* diag, const int* m, const int* n,
+            const float* alpha, const float* a, const int* lda, float* b, const int* ldb);
+void dtrsm_(const char* side, const char* uplo, const char* transa, const char* diag, const int* m, const int* n,
+            const double* alpha, const double* a, const int* lda, double* b, const int* ldb);
 /* Error hook (blas/xerbla.cpp:18): weak default prints the routine and info. */
 void xerbla_(const char* name, int* info, size_t len);
 

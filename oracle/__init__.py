@@ -257,3 +257,100 @@ def normwise_error(Cgpu: np.ndarray, Cref: np.ndarray, k: int, u: float, normA: 
 def higham_mary_bound(absprod: np.ndarray, k: int, u: float, lam: float = 5.0, nprod: int = 2) -> float:
     """lambda * sqrt(k) * u * nprod * || |A||B| ||_F  (test/product_test_helpers.h:56-74 form)."""
     return float(lam * np.sqrt(k) * u * nprod * np.linalg.norm(absprod))
+
+
+# --------------------------------------------------------------------------
+# Other real level-3 routines (restated in float64 numpy; for FP64 inputs the
+# float64 result carries rounding of the same order as the device's, so
+# tests compare against it with normwise bounds, not bit for bit).
+# --------------------------------------------------------------------------
+def _tri_dense(A: np.ndarray, n: int, lower: bool, unit: bool) -> np.ndarray:
+    """The n x n triangle of A (column-major) as a dense float64 matrix; the
+    other triangle is never read, unit puts ones on the diagonal."""
+    T = np.tril(np.asarray(A[:n, :n], np.float64)) if lower else np.triu(np.asarray(A[:n, :n], np.float64))
+    if unit:
+        np.fill_diagonal(T, 1.0)
+    return T
+
+
+def _sym_dense(A: np.ndarray, n: int, lower: bool) -> np.ndarray:
+    """Symmetric matrix from the stored triangle (blas/level3_impl.h:565-571 form)."""
+    T = _tri_dense(A, n, lower, False)
+    return T + np.tril(T, -1).T if lower else T + np.triu(T, 1).T
+
+
+def _op(x: np.ndarray, trans: str) -> np.ndarray:
+    return x.T if trans.upper() in "TC" else x
+
+
+def gemmtr_ref(uplo: str, ta: str, tb: str, alpha, A, B, beta, C0) -> np.ndarray:
+    """?gemmtr_ (blas/level3_impl.h:198-227): beta-scale the uplo triangle
+    (beta == 0 sets it to zero), then add alpha*op(A)*op(B) to it; the other
+    triangle of C is returned unchanged."""
+    C = np.array(C0, np.float64, order="F")
+    n = C.shape[0]
+    P = alpha * (_op(np.asarray(A, np.float64), ta) @ _op(np.asarray(B, np.float64), tb))
+    mask = np.tril(np.ones((n, n), bool)) if uplo.upper() == "L" else np.triu(np.ones((n, n), bool))
+    scaled = np.zeros_like(C) if beta == 0 else beta * C
+    C[mask] = (scaled + P)[mask]
+    return C
+
+
+def syr2k_ref(uplo: str, trans: str, alpha, A, B, beta, C0) -> np.ndarray:
+    """?syr2k_ (blas/level3_impl.h:718-757): beta-scale the uplo triangle, then
+    add alpha*(A*B^T + B*A^T) (trans N) or alpha*(A^T*B + B^T*A) (trans T/C)."""
+    A64, B64 = np.asarray(A, np.float64), np.asarray(B, np.float64)
+    if trans.upper() == "N":
+        P = A64 @ B64.T + B64 @ A64.T
+    else:
+        P = A64.T @ B64 + B64.T @ A64
+    C = np.array(C0, np.float64, order="F")
+    n = C.shape[0]
+    mask = np.tril(np.ones((n, n), bool)) if uplo.upper() == "L" else np.triu(np.ones((n, n), bool))
+    scaled = np.zeros_like(C) if beta == 0 else beta * C
+    C[mask] = (scaled + alpha * P)[mask]
+    return C
+
+
+def symm_ref(side: str, uplo: str, alpha, A, B, beta, C0) -> np.ndarray:
+    """?symm_ (blas/level3_impl.h:534-591): C = beta*C (beta == 0 sets zero),
+    then C += alpha*S*B (side L) or alpha*B*S (side R), S the symmetric matrix
+    of A's uplo triangle."""
+    m, n = C0.shape
+    ka = m if side.upper() == "L" else n
+    S = _sym_dense(A, ka, uplo.upper() == "L")
+    B64 = np.asarray(B, np.float64)[:m, :n]
+    C = np.zeros((m, n)) if beta == 0 else beta * np.asarray(C0, np.float64)
+    return C + alpha * (S @ B64 if side.upper() == "L" else B64 @ S)
+
+
+def trmm_ref(side: str, uplo: str, trans: str, diag: str, alpha, A, B0) -> np.ndarray:
+    """?trmm_ (blas/level3_impl.h:482-502): B = alpha*op(T)*B (side L) or
+    alpha*B*op(T) (side R), T the uplo triangle of A (unit diagonal when diag
+    is U)."""
+    m, n = B0.shape
+    ka = m if side.upper() == "L" else n
+    T = _op(_tri_dense(A, ka, uplo.upper() == "L", diag.upper() == "U"), trans)
+    B64 = np.asarray(B0, np.float64)
+    return alpha * (T @ B64 if side.upper() == "L" else B64 @ T)
+
+
+def trsm_ref(side: str, uplo: str, trans: str, diag: str, alpha, A, B0) -> np.ndarray:
+    """?trsm_ (blas/level3_impl.h:339-364): X with op(T)*X = B (side L) or
+    X*op(T) = B (side R), then scaled by alpha (the reference's order); T the
+    uplo triangle of A.  Substitution in float64, the textbook recurrence
+    over the rows of X (side R: over the rows of X^T, solving op(T)^T X^T = B^T)."""
+    m, n = B0.shape
+    left = side.upper() == "L"
+    ka = m if left else n
+    T = _op(_tri_dense(A, ka, uplo.upper() == "L", diag.upper() == "U"), trans)
+    lower = (uplo.upper() == "L") != (trans.upper() in "TC")  # triangle of op(T)
+    Bt = np.array(B0, np.float64)
+    if not left:
+        T, Bt, lower = T.T, Bt.T.copy(), not lower
+    X = np.zeros_like(Bt)
+    for i in (range(ka) if lower else range(ka - 1, -1, -1)):
+        acc = Bt[i].copy()
+        acc -= T[i, :i] @ X[:i] if lower else T[i, i + 1:] @ X[i + 1:]
+        X[i] = acc / T[i, i]
+    return alpha * (X if left else X.T)

@@ -18,7 +18,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def _declared(header: str):
     text = open(os.path.join(ROOT, "include", header)).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b((?:eh_|sgemm_|dgemm_|xerbla_)\w*)\s*\(", text)))
+    return sorted(set(re.findall(r"\b((?:eh_|[sd](?:gemm|gemmtr|syrk|syr2k|symm|trmm|trsm)_|xerbla_)\w*)\s*\(", text)))
 
 
 def _exported(path: str):
@@ -204,3 +204,35 @@ def test_strided_destination_argument_checks():
     assert L.eh_gemm_strided(*args(2, 6)) == E.ERR_INVALID          # columns would overlap: ldc < (m-1)*incc + 1
     assert L.eh_gemm_strided(*args(1, 3)) == E.ERR_INVALID          # incc == 1 is eh_gemm: ldc >= m
     assert "ldc" in E.lib().eh_last_error().decode()
+
+
+def test_level3_argument_checks():
+    """The level-3 entry points reject bad flags and leading dimensions, and
+    return at once for empty problems, before any device work."""
+    L = E.lib()
+    one = C.c_double(1.0)
+    a = C.byref(one)
+    # eh_trsm / eh_trmm(dtype, side, uplo, transa, diag, m, n, alpha, A, lda, B, ldb, stream)
+    for f in (L.eh_trsm, L.eh_trmm):
+        assert f(E.F64, 2, 0, 0, 0, 4, 3, a, 1, 4, 1, 4, None) == E.ERR_INVALID       # side
+        assert f(E.F64, 0, 2, 0, 0, 4, 3, a, 1, 4, 1, 4, None) == E.ERR_INVALID       # uplo
+        assert f(E.F64, 0, 0, 3, 0, 4, 3, a, 1, 4, 1, 4, None) == E.ERR_INVALID       # transa
+        assert f(E.F64, 0, 0, 0, 2, 4, 3, a, 1, 4, 1, 4, None) == E.ERR_INVALID       # diag
+        assert f(E.F64, 0, 0, 0, 0, 4, 3, a, 1, 3, 1, 4, None) == E.ERR_INVALID       # lda < m (left)
+        assert f(E.F64, 1, 0, 0, 0, 4, 3, a, 1, 2, 1, 4, None) == E.ERR_INVALID       # lda < n (right)
+        assert f(E.F64, 1, 0, 0, 0, 4, 3, a, 1, 3, 1, 3, None) == E.ERR_INVALID       # ldb < m
+        assert f(E.F64, 0, 0, 0, 0, 0, 3, a, 1, 1, 1, 1, None) == E.SUCCESS           # m == 0
+        assert f(E.F64, 0, 0, 0, 0, 4, 0, a, 1, 4, 1, 4, None) == E.SUCCESS           # n == 0
+    # eh_symm(dtype, side, uplo, m, n, alpha, A, lda, B, ldb, beta, C, ldc, stream)
+    assert L.eh_symm(E.F64, 0, 0, 4, 3, a, 1, 3, 1, 4, a, 1, 4, None) == E.ERR_INVALID   # lda < m
+    assert L.eh_symm(E.F64, 1, 0, 4, 3, a, 1, 3, 1, 3, a, 1, 4, None) == E.ERR_INVALID   # ldb < m
+    assert L.eh_symm(E.F64, 1, 0, 4, 3, a, 1, 3, 1, 4, a, 1, 3, None) == E.ERR_INVALID   # ldc < m
+    assert L.eh_symm(E.F64, 0, 0, 0, 3, a, 1, 1, 1, 1, a, 1, 1, None) == E.SUCCESS
+    # eh_syr2k(dtype, uplo, trans, n, k, alpha, A, lda, B, ldb, beta, C, ldc, stream)
+    assert L.eh_syr2k(E.F32, 0, 0, 4, 3, a, 1, 3, 1, 4, a, 1, 4, None) == E.ERR_INVALID  # lda < n
+    assert L.eh_syr2k(E.F32, 0, 1, 4, 3, a, 1, 3, 1, 2, a, 1, 4, None) == E.ERR_INVALID  # ldb < k
+    assert L.eh_syr2k(E.F32, 0, 0, 0, 3, a, 1, 1, 1, 1, a, 1, 1, None) == E.SUCCESS
+    # eh_gemmtr(dtype, uplo, opa, opb, n, k, alpha, A, lda, B, ldb, beta, C, ldc, stream)
+    assert L.eh_gemmtr(E.F32, 5, 0, 0, 4, 3, a, 1, 4, 1, 3, a, 1, 4, None) == E.ERR_INVALID  # uplo
+    assert L.eh_gemmtr(E.F32, 0, 0, 0, 4, 3, a, 1, 4, 1, 2, a, 1, 4, None) == E.ERR_INVALID  # ldb < k
+    assert L.eh_gemmtr(E.F32, 0, 0, 0, 0, 3, a, 1, 1, 1, 3, a, 1, 1, None) == E.SUCCESS

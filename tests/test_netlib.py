@@ -3,7 +3,8 @@ reference, compiled into oracle/_ref and linked against our sgemm_/dgemm_).
 
 The input deck mirrors the reference's sblat3.dat / dblat3.dat (N in
 {0,1,2,3,5,9}, alpha in {0,1,0.7}, beta in {0,1,1.3}, threshold 16) with the
-routines the library provides switched on: GEMM and SYRK.  The tester supplies its own XERBLA, which
+routines the library provides switched on: all six real level-3 routines
+(GEMM, SYMM, TRMM, TRSM, SYRK, SYR2K).  The tester supplies its own XERBLA, which
 overrides our weak xerbla_, to check the error exits; the computational tests
 compare every call with its own column-by-column recomputation (DMMCH).
 The error-exit run uses N = 0 only and needs no GPU."""
@@ -19,8 +20,9 @@ REF = os.path.join(ROOT, "oracle", "_ref")
 
 # routines in the order of the reference decks (sblat3.dat / dblat3.dat) and
 # whether the library provides them
-ROUTINES = [("GEMM ", True), ("SYMM ", False), ("TRMM ", False), ("TRSM ", False), ("SYRK ", True),
-            ("SYR2K", False)]
+ROUTINES = [("GEMM ", True), ("SYMM ", True), ("TRMM ", True), ("TRSM ", True), ("SYRK ", True),
+            ("SYR2K", True)]
+NAMES = ["GEMM", "SYMM", "TRMM", "TRSM", "SYRK", "SYR2K"]
 
 
 def deck(prec: str, ns, error_exits=True) -> str:
@@ -54,10 +56,10 @@ def run_tester(prec: str, ns, error_exits=True) -> str:
 @pytest.mark.parametrize("prec", ["s", "d"])
 def test_error_exits(prec):
     out = run_tester(prec, [0])
-    for name in ("GEMM", "SYRK"):
-        assert f"{prec.upper()}{name}  PASSED THE TESTS OF ERROR-EXITS" in out, out
+    for name in NAMES:
+        assert re.search(rf"{prec.upper()}{name}\s+PASSED THE TESTS OF ERROR-EXITS", out), out
         # N = 0: quick returns only
-        assert re.search(rf"{prec.upper()}{name}  PASSED THE COMPUTATIONAL TESTS \(\s*\d+ CALLS\)", out), out
+        assert re.search(rf"{prec.upper()}{name}\s+PASSED THE COMPUTATIONAL TESTS \(\s*\d+ CALLS\)", out), out
     assert f"{prec.upper()}GEMM  PASSED THE COMPUTATIONAL TESTS (    81 CALLS)" in out, out
     assert "FAILED" not in out and "SUSPECT" not in out, out
 
@@ -66,9 +68,9 @@ def test_error_exits(prec):
 @pytest.mark.parametrize("prec", ["s", "d"])
 def test_computational(gpu, prec):
     out = run_tester(prec, [0, 1, 2, 3, 5, 9])
-    for name in ("GEMM", "SYRK"):
-        assert f"{prec.upper()}{name}  PASSED THE TESTS OF ERROR-EXITS" in out, out
+    for name in NAMES:
+        assert re.search(rf"{prec.upper()}{name}\s+PASSED THE TESTS OF ERROR-EXITS", out), out
+        m = re.search(rf"{prec.upper()}{name}\s+PASSED THE COMPUTATIONAL TESTS \(\s*(\d+) CALLS\)", out)
+        assert m and int(m.group(1)) > 1000, (name, out)
     assert f"{prec.upper()}GEMM  PASSED THE COMPUTATIONAL TESTS ( 17496 CALLS)" in out, out
-    m = re.search(rf"{prec.upper()}SYRK  PASSED THE COMPUTATIONAL TESTS \(\s*(\d+) CALLS\)", out)
-    assert m and int(m.group(1)) > 1000, out
     assert "FAILED" not in out and "SUSPECT" not in out, out
