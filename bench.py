@@ -191,6 +191,8 @@ def main() -> int:
         torch.cuda.empty_cache()
         out["detail"].update(bench_dgemm(E, torch, dev, args))
         out["detail"]["gemv_16384"] = bench_gemv(E, torch, dev, args)
+        out["detail"]["level3_fp64"] = bench_level3(E, torch, dev, args)
+        torch.cuda.empty_cache()
         out["detail"]["batched"] = bench_batched(E, torch, args)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
@@ -222,6 +224,38 @@ def bench_dgemm(E, torch, dev, args):
                      "planner": ["heuristic", "mlp", "table"][E.lib().eh_get_tune_mode()]}
         log(f"{name}: {out[name]}")
         del dA, dB, dC
+    return out
+
+
+def bench_level3(E, torch, dev, args):
+    """The other FP64 level-3 routines at order 8192 (B, C 8192 x 8192): TRSM and
+    TRMM (left, lower, no transpose; m^2 n flops), SYMM (left, lower; 2 m^2 n),
+    SYRK and SYR2K (lower, no transpose, k = 8192; n^2 k and 2 n^2 k).  The
+    triangle has a diagonal of 2 and off-diagonal entries U[-1/n, 1/n] (well
+    conditioned, so repeated solves stay in the normal range)."""
+    n = 8192
+    A = torch.empty(n * n, dtype=torch.float64, device=dev)
+    B = torch.empty(n * n, dtype=torch.float64, device=dev)
+    Cm = torch.empty(n * n, dtype=torch.float64, device=dev)
+    E.fill(E.F64, A, n * n, 31, E.DIST_UNIFORM, -1.0 / n, 1.0 / n)
+    A.view(n, n).diagonal().fill_(2.0)
+    E.fill(E.F64, B, n * n, 32, E.DIST_UNIFORM, -1, 1)
+    E.fill(E.F64, Cm, n * n, 33, E.DIST_UNIFORM, -1, 1)
+    nn = float(n) ** 3
+    cases = {
+        "dtrsm_8192": (nn, lambda: E.trsm(E.F64, E.LEFT, E.LOWER, E.OP_N, E.NON_UNIT, n, n, 1.0, A, n, B, n)),
+        "dtrmm_8192": (nn, lambda: E.trmm(E.F64, E.LEFT, E.LOWER, E.OP_N, E.NON_UNIT, n, n, 1.0, A, n, B, n)),
+        "dsymm_8192": (2 * nn, lambda: E.symm(E.F64, E.LEFT, E.LOWER, n, n, 1.0, A, n, B, n, 0.5, Cm, n)),
+        "dsyrk_8192": (nn, lambda: E.syrk(E.F64, E.LOWER, E.OP_N, n, n, 1.0, B, n, 0.5, Cm, n)),
+        "dsyr2k_8192": (2 * nn, lambda: E.syr2k(E.F64, E.LOWER, E.OP_N, n, n, 1.0, A, n, B, n, 0.5, Cm, n)),
+    }
+    out = {}
+    for name, (flop, fn) in cases.items():
+        wall, kern = timed(fn, 3, 1)
+        tf = flop / (wall * 1e-3) / 1e12
+        out[name] = {"ms": wall, "tflops": tf, "frac_of_fp64_peak": tf / PEAK["fp64"]}
+        log(f"{name}: {out[name]}")
+    del A, B, Cm
     return out
 
 

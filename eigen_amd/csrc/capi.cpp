@@ -286,6 +286,9 @@ static eh_status launch_product(bool ta, bool tb, int64_t m, int64_t n, int64_t 
   {
     if (config < 0 || config >= kNumConfigs) return fail(EH_ERR_INVALID, "config out of range");
     config_decode(config, &tile, &splits);
+    // a triangular update of a square result runs square tiles, which launch
+    // only the tiles that meet the triangle (balanced over the XCDs)
+    if (tri != 0 && m == n && kTileM[tile] != kTileN[tile]) tile = m >= 2048 ? 3 : 0;
   }
 
   // depth per split: a whole number of k slices

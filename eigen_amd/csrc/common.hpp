@@ -109,4 +109,36 @@ __device__ inline void tile_coords(int logical, int ntm, int ntn, int gm, int& t
   tn = in / rows;
 }
 
+// Tile (r, c), c <= r, of the lower triangle of an nt x nt tile grid for
+// logical index L in [0, nt*(nt+1)/2): the triangle is walked in groups of
+// gm tile rows, column by column inside a group (the order tile_coords uses
+// for a full grid), so a contiguous range of indices -- what xcd_remap gives
+// one XCD -- covers a compact block of tiles that share operand panels, and
+// every XCD gets the same number of tiles.  Wave-uniform integer arithmetic.
+__host__ __device__ inline void tri_tile_coords(int L, int nt, int gm, int& r, int& c) {
+  // tiles before group g (all groups before it are full)
+  auto prefix = [gm](int64_t g) { return gm * gm * g * (g - 1) / 2 + g * gm * (gm + 1) / 2; };
+  const int groups = (nt + gm - 1) / gm;
+  int g = (int)(sqrtf(2.0f * (float)L) / (float)gm);
+  if (g > groups - 1) g = groups - 1;
+  while (g + 1 < groups && prefix(g + 1) <= L) ++g;
+  while (g > 0 && prefix(g) > L) --g;
+  int o = L - (int)prefix(g);
+  const int base = g * gm;
+  const int R = nt - base < gm ? nt - base : gm;  // rows in this group
+  if (o < R * base) {  // the rectangle left of the diagonal block
+    c = o / R;
+    r = base + o % R;
+    return;
+  }
+  o -= R * base;  // the R x R lower triangle on the diagonal, column c' has R - c' tiles
+  int cc = 0;
+  while (o >= R - cc) {
+    o -= R - cc;
+    ++cc;
+  }
+  c = base + cc;
+  r = base + cc + o;
+}
+
 }  // namespace eh

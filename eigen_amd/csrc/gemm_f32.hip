@@ -8,8 +8,7 @@ namespace eh {
 // ---------------------------------------------------------------------------
 template <typename T, int BM, int BN, int WGM, int WGN, int RS, bool TA, bool TB>
 static hipError_t launch_one(const GemmParams<T>& p, int splits, hipStream_t s) {
-  const int ntm = (p.M + BM - 1) / BM, ntn = (p.N + BN - 1) / BN;
-  dim3 grid(ntm * ntn, splits);
+  dim3 grid((unsigned)grid_tiles<BM, BN>(p), splits);
   hipLaunchKernelGGL((gemm_mfma_kernel<T, BM, BN, WGM, WGN, TA, TB, RS>), grid, dim3(64 * WGM * WGN), 0, s, p);
   return hipGetLastError();
 }

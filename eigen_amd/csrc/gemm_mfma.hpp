@@ -20,6 +20,19 @@
 
 namespace eh {
 
+// A triangular update (tri != 0) of a square result with square tiles
+// launches only the tiles on or beside the triangle (tri_tile_coords);
+// otherwise the full grid runs and tiles off the triangle return at once.
+template <int BM, int BN, typename T>
+__host__ __device__ inline bool tri_compact(const GemmParams<T>& p) {
+  return p.tri != 0 && BM == BN && p.M == p.N;
+}
+template <int BM, int BN, typename T>
+inline int64_t grid_tiles(const GemmParams<T>& p) {
+  const int64_t ntm = (p.M + BM - 1) / BM, ntn = (p.N + BN - 1) / BN;
+  return tri_compact<BM, BN>(p) ? ntm * (ntm + 1) / 2 : ntm * ntn;
+}
+
 template <typename T, int BM, int BN, int WGM, int WGN, bool TA, bool TB, int RS>
 __global__ void __launch_bounds__(64 * WGM * WGN) gemm_mfma_kernel(GemmParams<T> p) {
   using F = Mfma<T>;
@@ -47,7 +60,15 @@ __global__ void __launch_bounds__(64 * WGM * WGN) gemm_mfma_kernel(GemmParams<T>
 
   const int ntm = (p.M + BM - 1) / BM, ntn = (p.N + BN - 1) / BN;
   int tm, tn;
-  tile_coords(xcd_remap(blockIdx.x, gridDim.x), ntm, ntn, 8, tm, tn);
+  if (tri_compact<BM, BN>(p)) {
+    // the grid holds only the tiles that meet the triangle (lower: tn <= tm)
+    int r, c;
+    tri_tile_coords(xcd_remap(blockIdx.x, gridDim.x), ntm, 8, r, c);
+    tm = p.tri == 1 ? r : c;
+    tn = p.tri == 1 ? c : r;
+  } else {
+    tile_coords(xcd_remap(blockIdx.x, gridDim.x), ntm, ntn, 8, tm, tn);
+  }
   const int m0 = tm * BM, n0 = tn * BN;
   // a tile with no element in the updated triangle has nothing to do
   if ((p.tri == 1 && m0 + BM <= n0) || (p.tri == 2 && n0 + BN <= m0)) return;

@@ -60,23 +60,21 @@ template <typename T>
 hipError_t launch_tri_dense(const T* A, int64_t lda, int n, T* D, int64_t ldd, bool lower, bool sym, bool unit,
                             hipStream_t s);
 
-// Triangular solve of one diagonal block for nvec right-hand sides, in
-// place: M x = alpha b with M(i, k) = mt ? A[k + i*lda] : A[i + k*lda] of
-// order nb <= kTrsmLeaf (lower or upper, unit diagonal or not); component i
-// of right-hand side v is B[i*es + v*vs].
-constexpr int kTrsmLeaf = 32;
+// Inverses of the diagonal blocks of order kTrsmLeaf of the triangular
+// matrix M(i, k) = mt ? A[k + i*lda] : A[i + k*lda] of the given order
+// (lower or upper, unit diagonal or not): block b goes to W + b*kTrsmLeaf^2
+// as a dense kTrsmLeaf x kTrsmLeaf column-major matrix (zeros outside the
+// triangle, and past the order in the last block).
+constexpr int kTrsmLeaf = 64;
 template <typename T>
-struct TrsmLeaf {
+struct TrtriParams {
   const T* A;
   int64_t lda;
-  int mt, lower, unit, nb;
-  T* B;
-  int64_t es, vs;
-  int nvec;
-  T alpha;
+  int mt, lower, unit, order;
+  T* W;
 };
 template <typename T>
-hipError_t launch_trsm_leaf(const TrsmLeaf<T>& p, hipStream_t s);
+hipError_t launch_trtri(const TrtriParams<T>& p, hipStream_t s);
 
 // Split-BF16 FP32 product over packed operands.  Each operand is first packed
 // into bf16 planes P[ks][term][row][32] (rows padded to split_rows_padded,

@@ -8,8 +8,11 @@
 //   GEMMTR, SYR2K  one or two products restricted to a triangle of C;
 //   SYMM           the mirrored dense matrix (one HBM pass), then one product;
 //   TRMM, TRSM     recursive 2x2 blocking of the triangle: at each level the
-//                  off-diagonal block is a plain product on the rest of B,
-//                  so all but O(order * leaf * rhs) of the flops run on MFMA.
+//                  off-diagonal block is a plain product on the rest of B;
+//                  the leaves are products too (TRMM: the dense diagonal
+//                  block; TRSM: the inverse of the diagonal block, all of
+//                  them computed by one batched launch), so every flop but
+//                  the O(order * 64^2) of the inversions runs on MFMA.
 // Arguments are checked with the reference's rules (blas/level3_impl.h); the
 // BLAS wrappers in blas.cpp add the xerbla_ codes.
 #include <hip/hip_runtime.h>
@@ -74,55 +77,67 @@ int64_t half(int64_t sz, int64_t leaf) {
 }
 
 // ---- TRSM --------------------------------------------------------------------
+// The diagonal blocks of order kTrsmLeaf are inverted up front (one launch);
+// a leaf of the recursion multiplies its right-hand sides by the inverse, on
+// the MFMA kernels, from a copy of them.
+template <typename T>
+struct TrsmWork {
+  const T* inv;  // inverse of diagonal block b at inv + b*kTrsmLeaf^2 (ld kTrsmLeaf)
+  T* copy;       // kTrsmLeaf x n (left) or m x kTrsmLeaf (right)
+};
+
 // E[d:d+sz, d:d+sz] X = alpha*B, B = rows d.. of the right-hand sides (sz x n).
 template <typename T>
-eh_status trsm_left(const Tri<T>& E, int64_t d, int64_t sz, T* B, int64_t ldb, int64_t n, T alpha, hipStream_t s) {
-  if (sz <= kTrsmLeaf) {
-    TrsmLeaf<T> p{E.diag(d), E.lda, (int)E.ta, (int)E.lower, (int)E.unit, (int)sz, B, 1, ldb, (int)n, alpha};
-    hipError_t e = launch_trsm_leaf<T>(p, s);
-    return e == hipSuccess ? EH_SUCCESS : hip_fail(e, "trsm leaf launch");
+eh_status trsm_left(const Tri<T>& E, int64_t d, int64_t sz, T* B, int64_t ldb, int64_t n, T alpha,
+                    const TrsmWork<T>& w, hipStream_t s) {
+  if (sz <= kTrsmLeaf) {  // X = alpha * inv(E_dd) * B
+    hipError_t e = hipMemcpy2DAsync(w.copy, sz * sizeof(T), B, ldb * sizeof(T), sz * sizeof(T), n,
+                                    hipMemcpyDeviceToDevice, s);
+    if (e != hipSuccess) return hip_fail(e, "trsm block copy");
+    return device_product<T>(EH_OP_N, EH_OP_N, sz, n, sz, alpha, w.inv + (d / kTrsmLeaf) * kTrsmLeaf * kTrsmLeaf,
+                             kTrsmLeaf, w.copy, sz, T(0), B, ldb, s);
   }
   const int64_t h = half(sz, kTrsmLeaf);
   T* B1 = B;
   T* B2 = B + h;
   if (E.lower) {  // X1 = E11 \ alpha*B1; B2 = alpha*B2 - E21*X1; X2 = E22 \ B2
-    EH_TRY(trsm_left(E, d, h, B1, ldb, n, alpha, s));
+    EH_TRY(trsm_left(E, d, h, B1, ldb, n, alpha, w, s));
     EH_TRY(device_product<T>(E.op(), EH_OP_N, sz - h, n, h, T(-1), E.block(d + h, d), E.lda, B1, ldb, alpha, B2, ldb,
                              s));
-    return trsm_left(E, d + h, sz - h, B2, ldb, n, T(1), s);
+    return trsm_left(E, d + h, sz - h, B2, ldb, n, T(1), w, s);
   }
   // X2 = E22 \ alpha*B2; B1 = alpha*B1 - E12*X2; X1 = E11 \ B1
-  EH_TRY(trsm_left(E, d + h, sz - h, B2, ldb, n, alpha, s));
+  EH_TRY(trsm_left(E, d + h, sz - h, B2, ldb, n, alpha, w, s));
   EH_TRY(device_product<T>(E.op(), EH_OP_N, h, n, sz - h, T(-1), E.block(d, d + h), E.lda, B2, ldb, alpha, B1, ldb,
                            s));
-  return trsm_left(E, d, h, B1, ldb, n, T(1), s);
+  return trsm_left(E, d, h, B1, ldb, n, T(1), w, s);
 }
 
 // X E[d:d+sz, d:d+sz] = alpha*B, B = columns d.. of the right-hand sides (m x sz).
 template <typename T>
 eh_status trsm_right(const Tri<T>& E, int64_t d, int64_t sz, T* B, int64_t ldb, int64_t m, T alpha,
-                     hipStream_t s) {
-  if (sz <= kTrsmLeaf) {
-    // row r of X solves E^T x = alpha*b: the leaf matrix is E^T (the other
-    // triangle), right-hand sides are the rows of B
-    TrsmLeaf<T> p{E.diag(d), E.lda, (int)!E.ta, (int)!E.lower, (int)E.unit, (int)sz, B, ldb, 1, (int)m, alpha};
-    hipError_t e = launch_trsm_leaf<T>(p, s);
-    return e == hipSuccess ? EH_SUCCESS : hip_fail(e, "trsm leaf launch");
+                     const TrsmWork<T>& w, hipStream_t s) {
+  if (sz <= kTrsmLeaf) {  // X = alpha * B * inv(E_dd)
+    hipError_t e = hipMemcpy2DAsync(w.copy, m * sizeof(T), B, ldb * sizeof(T), m * sizeof(T), sz,
+                                    hipMemcpyDeviceToDevice, s);
+    if (e != hipSuccess) return hip_fail(e, "trsm block copy");
+    return device_product<T>(EH_OP_N, EH_OP_N, m, sz, sz, alpha, w.copy, m,
+                             w.inv + (d / kTrsmLeaf) * kTrsmLeaf * kTrsmLeaf, kTrsmLeaf, T(0), B, ldb, s);
   }
   const int64_t h = half(sz, kTrsmLeaf);
   T* B1 = B;
   T* B2 = B + h * ldb;
   if (E.lower) {  // X2 E22 = alpha*B2; X1 E11 = alpha*B1 - X2 E21
-    EH_TRY(trsm_right(E, d + h, sz - h, B2, ldb, m, alpha, s));
+    EH_TRY(trsm_right(E, d + h, sz - h, B2, ldb, m, alpha, w, s));
     EH_TRY(device_product<T>(EH_OP_N, E.op(), m, h, sz - h, T(-1), B2, ldb, E.block(d + h, d), E.lda, alpha, B1, ldb,
                              s));
-    return trsm_right(E, d, h, B1, ldb, m, T(1), s);
+    return trsm_right(E, d, h, B1, ldb, m, T(1), w, s);
   }
   // X1 E11 = alpha*B1; X2 E22 = alpha*B2 - X1 E12
-  EH_TRY(trsm_right(E, d, h, B1, ldb, m, alpha, s));
+  EH_TRY(trsm_right(E, d, h, B1, ldb, m, alpha, w, s));
   EH_TRY(device_product<T>(EH_OP_N, E.op(), m, sz - h, h, T(-1), B1, ldb, E.block(d, d + h), E.lda, alpha, B2, ldb,
                            s));
-  return trsm_right(E, d + h, sz - h, B2, ldb, m, T(1), s);
+  return trsm_right(E, d + h, sz - h, B2, ldb, m, T(1), w, s);
 }
 
 // ---- TRMM --------------------------------------------------------------------
@@ -226,7 +241,18 @@ eh_status trsm_t(int side, int uplo, int transa, int diag, int64_t m, int64_t n,
   if (!A) return fail(EH_ERR_INVALID, "A is NULL");
   const bool ta = transa != EH_OP_N;
   const Tri<T> E{A, lda, ta, (uplo == EH_LOWER) != ta, diag == EH_UNIT};
-  return side == EH_LEFT ? trsm_left<T>(E, 0, m, B, ldb, n, alpha, s) : trsm_right<T>(E, 0, n, B, ldb, m, alpha, s);
+  const int64_t ka = side == EH_LEFT ? m : n;
+  const int64_t blocks = (ka + kTrsmLeaf - 1) / kTrsmLeaf;
+  const int64_t inv = blocks * kTrsmLeaf * kTrsmLeaf;
+  const int64_t copy = side == EH_LEFT ? kTrsmLeaf * n : m * kTrsmLeaf;
+  Temp t;
+  EH_TRY(t.get((size_t)(inv + copy) * sizeof(T), s));
+  const TrsmWork<T> w{static_cast<T*>(t.ptr), static_cast<T*>(t.ptr) + inv};
+  const TrtriParams<T> tp{A, lda, (int)E.ta, (int)E.lower, (int)E.unit, (int)ka, static_cast<T*>(t.ptr)};
+  hipError_t e = launch_trtri<T>(tp, s);
+  if (e != hipSuccess) return hip_fail(e, "diagonal block inversion launch");
+  return side == EH_LEFT ? trsm_left<T>(E, 0, m, B, ldb, n, alpha, w, s)
+                         : trsm_right<T>(E, 0, n, B, ldb, m, alpha, w, s);
 }
 
 template <typename T>

@@ -12,11 +12,13 @@
 //                      (zeros outside it, optional unit diagonal).  64x64
 //                      tiles; the mirrored half goes through LDS so that both
 //                      the reads and the writes are coalesced.  HBM-bound.
-//   trsm_leaf_kernel   a diagonal block of at most 32 rows of a triangular
-//                      solve: one thread per right-hand side, the solution
-//                      vector in registers, the block (and the reciprocals of
-//                      its diagonal, as TriangularSolverMatrix.h:67 uses them)
-//                      in LDS, read as wave-wide broadcasts.
+//   trtri_kernel       the inverses of all 64 x 64 diagonal blocks of a
+//                      triangular matrix in one launch (one workgroup per
+//                      block, one thread per column of the inverse, the
+//                      column in registers, the block and the reciprocals of
+//                      its diagonal in LDS); the solve then multiplies each
+//                      diagonal block of right-hand sides by its inverse on
+//                      the MFMA kernels.
 #include <hip/hip_runtime.h>
 
 #include "common.hpp"
@@ -70,57 +72,72 @@ hipError_t launch_tri_dense(const T* A, int64_t lda, int n, T* D, int64_t ldd, b
   return hipGetLastError();
 }
 
-// Solve M x = alpha b for every right-hand side v < nvec, in place in B:
-// element i of vector v is B[i*es + v*vs]; M(i, k) = mt ? A[k + i*lda]
-// : A[i + k*lda] is lower or upper triangular of order nb <= NB.  Entries of
-// M outside its triangle (and past nb) are zero in LDS and the reciprocal
-// diagonal past nb is one, so the unrolled substitution needs no guards:
-// padded components stay zero.
+// Inverses of the diagonal blocks of a triangular matrix, one workgroup per
+// block: block b is M_b(i, k) = mt ? A[k + i*lda] : A[i + k*lda] at A + b*NB
+// (1 + lda), of order nb = min(NB, order - b*NB), lower or upper, unit
+// diagonal or not.  Thread v solves M_b x = e_v by substitution, multiplying
+// by the reciprocal of the diagonal as TriangularSolverMatrix.h:67 does, and
+// writes column v of inv(M_b) to W + b*NB*NB (NB x NB, column-major, zero
+// outside the triangle and past nb).  Entries of M_b outside its triangle
+// and past nb are zero in LDS and the reciprocal diagonal past nb is one, so
+// the unrolled substitution needs no guards.  Each dot product runs as two
+// interleaved partial sums (half the dependent chain).
 template <typename T, int NB>
-__global__ void __launch_bounds__(256) trsm_leaf_kernel(TrsmLeaf<T> p) {
+__global__ void __launch_bounds__(NB) trtri_kernel(TrtriParams<T> p) {
   __shared__ T M[NB][NB];
   __shared__ T dinv[NB];
-  for (int idx = threadIdx.x; idx < NB * NB; idx += 256) {
+  const int b = blockIdx.x;
+  const int nb = min(NB, p.order - b * NB);
+  const T* A = p.A + (int64_t)b * NB * (1 + p.lda);
+  for (int idx = threadIdx.x; idx < NB * NB; idx += NB) {
     const int i = idx % NB, k = idx / NB;
-    const bool in = i < p.nb && k < p.nb && (p.lower ? k < i : k > i);
-    M[i][k] = in ? (p.mt ? p.A[k + (int64_t)i * p.lda] : p.A[i + (int64_t)k * p.lda]) : T(0);
+    const bool in = i < nb && k < nb && (p.lower ? k < i : k > i);
+    M[i][k] = in ? (p.mt ? A[k + (int64_t)i * p.lda] : A[i + (int64_t)k * p.lda]) : T(0);
   }
-  for (int i = threadIdx.x; i < NB; i += 256)
-    dinv[i] = (i >= p.nb || p.unit) ? T(1) : T(1) / p.A[i + (int64_t)i * p.lda];
+  if (threadIdx.x < NB) {
+    const int i = threadIdx.x;
+    dinv[i] = (i >= nb || p.unit) ? T(1) : T(1) / A[i + (int64_t)i * p.lda];
+  }
   __syncthreads();
-  const int v = blockIdx.x * 256 + threadIdx.x;
-  if (v >= p.nvec) return;
-  T* b = p.B + (int64_t)v * p.vs;
+  const int v = threadIdx.x;
   T x[NB];
 #pragma unroll
-  for (int i = 0; i < NB; ++i) x[i] = i < p.nb ? p.alpha * b[(int64_t)i * p.es] : T(0);
+  for (int i = 0; i < NB; ++i) x[i] = (i == v && i < nb) ? T(1) : T(0);
   if (p.lower) {
 #pragma unroll
     for (int i = 0; i < NB; ++i) {
-      T s = x[i];
+      T s0 = x[i], s1 = T(0);
 #pragma unroll
-      for (int k = 0; k < i; ++k) s = fma_t(-M[i][k], x[k], s);
-      x[i] = s * dinv[i];
+      for (int k = 0; k + 1 < i; k += 2) {
+        s0 = fma_t(-M[i][k], x[k], s0);
+        s1 = fma_t(-M[i][k + 1], x[k + 1], s1);
+      }
+      if (i & 1) s0 = fma_t(-M[i][i - 1], x[i - 1], s0);
+      x[i] = (s0 + s1) * dinv[i];
     }
   } else {
 #pragma unroll
     for (int i = NB - 1; i >= 0; --i) {
-      T s = x[i];
+      T s0 = x[i], s1 = T(0);
 #pragma unroll
-      for (int k = i + 1; k < NB; ++k) s = fma_t(-M[i][k], x[k], s);
-      x[i] = s * dinv[i];
+      for (int k = i + 1; k + 1 < NB; k += 2) {
+        s0 = fma_t(-M[i][k], x[k], s0);
+        s1 = fma_t(-M[i][k + 1], x[k + 1], s1);
+      }
+      if ((NB - 1 - i) & 1) s0 = fma_t(-M[i][NB - 1], x[NB - 1], s0);
+      x[i] = (s0 + s1) * dinv[i];
     }
   }
+  T* w = p.W + (int64_t)b * NB * NB + (int64_t)v * NB;
 #pragma unroll
-  for (int i = 0; i < NB; ++i)
-    if (i < p.nb) b[(int64_t)i * p.es] = x[i];
+  for (int i = 0; i < NB; ++i) w[i] = x[i];
 }
 
 template <typename T>
-hipError_t launch_trsm_leaf(const TrsmLeaf<T>& p, hipStream_t s) {
-  if (p.nb < 1 || p.nb > kTrsmLeaf) return hipErrorInvalidValue;
-  if (p.nvec <= 0) return hipSuccess;
-  hipLaunchKernelGGL((trsm_leaf_kernel<T, kTrsmLeaf>), dim3((unsigned)((p.nvec + 255) / 256)), dim3(256), 0, s, p);
+hipError_t launch_trtri(const TrtriParams<T>& p, hipStream_t s) {
+  if (p.order <= 0) return hipSuccess;
+  const unsigned blocks = (unsigned)((p.order + kTrsmLeaf - 1) / kTrsmLeaf);
+  hipLaunchKernelGGL((trtri_kernel<T, kTrsmLeaf>), dim3(blocks), dim3(kTrsmLeaf), 0, s, p);
   return hipGetLastError();
 }
 
@@ -128,7 +145,7 @@ template hipError_t launch_tri_dense<float>(const float*, int64_t, int, float*, 
                                             hipStream_t);
 template hipError_t launch_tri_dense<double>(const double*, int64_t, int, double*, int64_t, bool, bool, bool,
                                              hipStream_t);
-template hipError_t launch_trsm_leaf<float>(const TrsmLeaf<float>&, hipStream_t);
-template hipError_t launch_trsm_leaf<double>(const TrsmLeaf<double>&, hipStream_t);
+template hipError_t launch_trtri<float>(const TrtriParams<float>&, hipStream_t);
+template hipError_t launch_trtri<double>(const TrtriParams<double>&, hipStream_t);
 
 }  // namespace eh

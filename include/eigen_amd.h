@@ -193,9 +193,11 @@ eh_status eh_trmm(eh_dtype dtype, eh_side side, eh_uplo uplo, eh_op transa, eh_d
  * X overwrites B.  Replaces ?trsm_ (blas/level3_impl.h:228-364) and
  * triangular_solve_matrix (TriangularSolverMatrix.h:20-330).  Recursive 2x2
  * blocking: a block of unknowns is solved, its contribution is subtracted
- * from the remaining right-hand sides by a product on the MFMA kernels, and
- * diagonal blocks of order <= 32 are solved by one thread per right-hand side
- * (multiplying by the reciprocal of the diagonal, as the reference does).
+ * from the remaining right-hand sides by a product on the MFMA kernels.  The
+ * 64 x 64 diagonal blocks are inverted up front by one batched launch
+ * (substitution, multiplying by the reciprocal of the diagonal as the
+ * reference does) and a leaf multiplies its right-hand sides by the inverse
+ * on the MFMA kernels, from a copy of them.
  * alpha is applied to the right-hand sides before the solve (the reference
  * scales the solution afterwards: equal up to rounding); alpha == 0 sets B
  * to zero without reading A.  No check for a zero diagonal: as in the
