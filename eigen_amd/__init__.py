@@ -130,6 +130,8 @@ def lib():
         for f in (L.eh_trmm, L.eh_trsm):
             f.restype = i32
             f.argtypes = [i32, i32, i32, i32, i32, i64, i64, vp, vp, i64, vp, i64, vp]
+        L.eh_gemm_bf16.restype = i32
+        L.eh_gemm_bf16.argtypes = [i32, i32, i64, i64, i64, vp, vp, i64, vp, i64, vp, vp, i64, vp]
         L.eh_gemm_strided.restype = i32
         L.eh_gemm_strided.argtypes = [i32, i32, i32, i64, i64, i64, vp, vp, i64, vp, i64, vp, vp, i64, i64, i32, vp]
         L.eh_num_configs.restype = i32
@@ -252,6 +254,15 @@ def syrk(dtype: int, uplo: int, trans: int, n: int, k: int, alpha: float, A, lda
     a, b = _scalar(dtype, alpha), _scalar(dtype, beta)
     s = current_stream() if stream is None else stream
     check(lib().eh_syrk(dtype, uplo, trans, n, k, C.byref(a), _ptr(A), lda, C.byref(b), _ptr(Cm), ldc, s), "eh_syrk")
+
+
+def gemm_bf16(opa: int, opb: int, m: int, n: int, k: int, alpha: float, A, lda: int, B, ldb: int, beta: float, Cm,
+              ldc: int, stream=None) -> None:
+    """C (float32) = alpha*op(A)*op(B) + beta*C with A, B bfloat16 (eh_gemm_bf16)."""
+    a, b = C.c_float(alpha), C.c_float(beta)
+    s = current_stream() if stream is None else stream
+    check(lib().eh_gemm_bf16(opa, opb, m, n, k, C.byref(a), _ptr(A), lda, _ptr(B), ldb, C.byref(b), _ptr(Cm), ldc, s),
+          "eh_gemm_bf16")
 
 
 def gemmtr(dtype: int, uplo: int, opa: int, opb: int, n: int, k: int, alpha: float, A, lda: int, B, ldb: int,

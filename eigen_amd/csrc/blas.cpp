@@ -431,6 +431,51 @@ void tri_impl(const char* name, const char* side, const char* uplo, const char* 
   finish(s);
 }
 
+// sbgemm_: bfloat16 A and B, float alpha, beta and C, with ?gemm_'s
+// validation order and info codes (the reference declares this OpenBLAS
+// entry point, GeneralMatrixMatrix_BLAS.h:127-131, and has no
+// implementation of its own).
+void sbgemm_impl(const char* opa, const char* opb, const int* m, const int* n, const int* k, const float* palpha,
+                 const unsigned short* a, const int* lda, const unsigned short* b, const int* ldb, const float* pbeta,
+                 float* c, const int* ldc) {
+  static const char name[] = "SBGEMM";
+  int info = 0;
+  const int oa = op_code(*opa), ob = op_code(*opb);
+  if (oa == OP_BAD)
+    info = 1;
+  else if (ob == OP_BAD)
+    info = 2;
+  else if (*m < 0)
+    info = 3;
+  else if (*n < 0)
+    info = 4;
+  else if (*k < 0)
+    info = 5;
+  else if (*lda < std::max(1, oa == OP_N ? *m : *k))
+    info = 8;
+  else if (*ldb < std::max(1, ob == OP_N ? *k : *n))
+    info = 10;
+  else if (*ldc < std::max(1, *m))
+    info = 13;
+  if (info) return xerbla_(name, &info, 6);
+  if (*m == 0 || *n == 0) return;
+  const float alpha = *palpha, beta = *pbeta;
+  const bool need_ab = *k > 0 && alpha != 0.0f;
+  if (!need_ab && beta == 1.0f) return;
+  hipStream_t s = g_stream;
+  std::lock_guard<std::mutex> lk(g_stage_mutex);
+  DevMat<unsigned short> A(a, *lda), B(b, *ldb);
+  DevMat<float> C(c, *ldc);
+  if ((need_ab && (!dev_view(g_sa, a, oa == OP_N ? *m : *k, oa == OP_N ? *k : *m, *lda, true, s, &A) ||
+                   !dev_view(g_sb, b, ob == OP_N ? *k : *n, ob == OP_N ? *n : *k, *ldb, true, s, &B))) ||
+      !dev_view(g_sc, c, *m, *n, *ldc, beta != 0.0f, s, &C))
+    return (void)std::fprintf(stderr, "eigen_amd BLAS: staging failed\n");
+  if (eh_gemm_bf16((eh_op)oa, (eh_op)ob, *m, *n, *k, &alpha, A.p, A.ld, B.p, B.ld, &beta, C.p, C.ld, s) != EH_SUCCESS)
+    return report(name);
+  if (!copy_back(C, c, s)) std::fprintf(stderr, "eigen_amd BLAS: copying C back failed\n");
+  finish(s);
+}
+
 }  // namespace
 
 extern "C" {
@@ -511,6 +556,13 @@ void strsm_(const char* side, const char* uplo, const char* transa, const char* 
 void dtrsm_(const char* side, const char* uplo, const char* transa, const char* diag, const int* m, const int* n,
             const double* alpha, const double* a, const int* lda, double* b, const int* ldb) {
   tri_impl<double, true>("DTRSM ", side, uplo, transa, diag, m, n, alpha, a, lda, b, ldb);
+}
+
+void sbgemm_(const char* transa, const char* transb, const int* m, const int* n, const int* k, const float* alpha,
+             const void* a, const int* lda, const void* b, const int* ldb, const float* beta, float* c,
+             const int* ldc) {
+  sbgemm_impl(transa, transb, m, n, k, alpha, static_cast<const unsigned short*>(a), lda,
+              static_cast<const unsigned short*>(b), ldb, beta, c, ldc);
 }
 
 void eh_blas_set_stream(void* stream) { g_stream = static_cast<hipStream_t>(stream); }

@@ -223,6 +223,65 @@ static eh_status launch_split_product(const GemmParams<float>& g, bool ta, bool 
   return EH_SUCCESS;
 }
 
+// bf16 inputs, FP32 accumulation and result: both operands are repacked
+// into the one-term plane layout, then the packed BF16 MFMA product runs
+// with a single product per k slice (and the split-K reduction when K is
+// cut).  Operand extents must stay below 4 GB (32-bit buffer offsets).
+static eh_status launch_bf16_product(bool ta, bool tb, int m, int n, int k, float alpha, const unsigned short* A,
+                                     int64_t lda, const unsigned short* B, int64_t ldb, float beta, float* C,
+                                     int64_t ldc, hipStream_t stream) {
+  const uint64_t extA = ta ? (uint64_t)(m - 1) * lda + k : (uint64_t)(k - 1) * lda + m;
+  const uint64_t extB = !tb ? (uint64_t)(n - 1) * ldb + k : (uint64_t)(k - 1) * ldb + n;
+  if ((extA + 64) * 2 >= kMaxExtent || (extB + 64) * 2 >= kMaxExtent)
+    return fail(EH_ERR_UNSUPPORTED, "bf16 operands must span less than 4 GB");
+  const int KS = split_kslices(k);
+  const int64_t t256 = ((int64_t)(m + 255) / 256) * ((n + 127) / 128);
+  const int tile = t256 >= 256 ? 2 : 0;
+  const int64_t tiles = tile ? t256 : ((int64_t)(m + 127) / 128) * ((n + 127) / 128);
+  int splits = 1;
+  while (splits < 8 && tiles * splits < 256 && k / (2 * splits) >= 512) splits *= 2;
+  const int per = (KS + splits - 1) / splits;
+  splits = (KS + per - 1) / per;
+  const size_t bytesA = split_packed_bytes(m, k, 1), bytesB = split_packed_bytes(n, k, 1);
+  const size_t bytesW = splits > 1 ? (size_t)splits * m * n * sizeof(float) : 0;
+  void* ws = nullptr;
+  eh_status st = workspace_get(stream, bytesA + bytesB + bytesW, &ws);
+  if (st != EH_SUCCESS) return st;
+  unsigned char* base = static_cast<unsigned char*>(ws);
+  hipError_t e = launch_pack_bf16(A, lda, ta, m, k, base, stream);
+  if (e == hipSuccess) e = launch_pack_bf16(B, ldb, !tb, n, k, base + bytesA, stream);
+  if (e != hipSuccess) return hip_fail(e, "bf16 pack launch");
+  SplitParams sp{};
+  sp.Ap = base;
+  sp.Bp = base + bytesA;
+  sp.C = C;
+  sp.W = splits > 1 ? reinterpret_cast<float*>(base + bytesA + bytesB) : nullptr;
+  sp.ldc = ldc;
+  sp.M = m;
+  sp.N = n;
+  sp.KS = KS;
+  sp.Mp = split_rows_padded(m);
+  sp.Np = split_rows_padded(n);
+  sp.ks_per_split = per;
+  sp.alpha = alpha;
+  sp.beta = beta;
+  e = launch_gemm_split_packed(sp, 1, tile, splits, stream);
+  if (e != hipSuccess) return hip_fail(e, "bf16 product launch");
+  if (splits > 1) {
+    GemmParams<float> r{};
+    r.C = C;
+    r.W = sp.W;
+    r.ldc = ldc;
+    r.M = m;
+    r.N = n;
+    r.alpha = alpha;
+    r.beta = beta;
+    e = launch_splitk_reduce<float>(r, splits, stream);
+    if (e != hipSuccess) return hip_fail(e, "split-K reduce launch");
+  }
+  return EH_SUCCESS;
+}
+
 template <typename T>
 static eh_status launch_product(bool ta, bool tb, int64_t m, int64_t n, int64_t k, T alpha, const T* A,
                                 int64_t lda, const T* B, int64_t ldb, T beta, T* C, int64_t ldc, bool split_prec,
@@ -500,6 +559,23 @@ eh_status eh_gemm_strided(eh_dtype dtype, eh_op opa, eh_op opb, int64_t m, int64
                                                      *static_cast<const float*>(beta), s);
   if (W) (void)hipFreeAsync(W, s);
   return e == hipSuccess ? EH_SUCCESS : hip_fail(e, "strided combine launch");
+}
+
+eh_status eh_gemm_bf16(eh_op opa, eh_op opb, int64_t m, int64_t n, int64_t k, const float* alpha, const void* A,
+                       int64_t lda, const void* B, int64_t ldb, const float* beta, float* C, int64_t ldc,
+                       void* stream) {
+  eh_status st = check_gemm_args(EH_F32, opa, opb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+  if (st != EH_SUCCESS) return st;
+  if (m == 0 || n == 0) return EH_SUCCESS;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (k == 0 || *alpha == 0.0f) {
+    if (*beta == 1.0f) return EH_SUCCESS;
+    hipError_t e = launch_scale_c<float>(C, (int)m, (int)n, ldc, *beta, s);
+    return e == hipSuccess ? EH_SUCCESS : hip_fail(e, "scale_c launch");
+  }
+  return launch_bf16_product(opa != EH_OP_N, opb != EH_OP_N, (int)m, (int)n, (int)k, *alpha,
+                             static_cast<const unsigned short*>(A), lda, static_cast<const unsigned short*>(B), ldb,
+                             *beta, C, ldc, s);
 }
 
 eh_status eh_syrk(eh_dtype dtype, eh_uplo uplo, eh_op trans, int64_t n, int64_t k, const void* alpha, const void* A,

@@ -256,9 +256,10 @@ __global__ void __launch_bounds__(64 * WGM * WGN) gemm_split_packed_kernel(Split
     }
     // cross products with p + q < TERMS, smallest first; consecutive MFMAs go
     // to different accumulators
-    constexpr int NPROD = TERMS == 3 ? 6 : 3;
-    constexpr int PA[6] = {TERMS == 3 ? 2 : 1, 0, TERMS == 3 ? 1 : 0, 1, 0, 0};
-    constexpr int PB[6] = {0, TERMS == 3 ? 2 : 1, TERMS == 3 ? 1 : 0, 0, 1, 0};
+    // (one term: the single product of bf16 inputs)
+    constexpr int NPROD = TERMS == 3 ? 6 : (TERMS == 2 ? 3 : 1);
+    constexpr int PA[6] = {TERMS == 3 ? 2 : (TERMS == 2 ? 1 : 0), 0, TERMS == 3 ? 1 : 0, 1, 0, 0};
+    constexpr int PB[6] = {0, TERMS == 3 ? 2 : (TERMS == 2 ? 1 : 0), TERMS == 3 ? 1 : 0, 0, 1, 0};
 #pragma unroll
     for (int s = 0; s < 2; ++s)
 #pragma unroll
@@ -309,6 +310,58 @@ __global__ void __launch_bounds__(256) split_bf16_kernel(const float* x, int64_t
   }
 }
 
+// Pack of a bf16 operand (one plane, no split): op(X) (rows x K, bf16 bits)
+// -> P[ks][r][0..31] with zeros past `rows` and past K.  Same layout as the
+// one-term split planes, so the product kernel is shared.  Loads are 2-byte
+// buffer loads by consecutive lanes along the contiguous index (any element
+// alignment is valid for bf16 operands).
+template <bool KCONT>
+__global__ void __launch_bounds__(256) pack_bf16_kernel(const unsigned short* __restrict__ X, int64_t ld, int rows,
+                                                        int K, int Rp, int KS, u16x8* __restrict__ P) {
+  constexpr int TR = 64, TK = 64;
+  __shared__ unsigned short tile[TR][TK + 2];
+  const int r0 = blockIdx.x * TR, k0 = blockIdx.y * TK;
+  const int tid = threadIdx.x;
+  const uint64_t extent = KCONT ? ((uint64_t)(rows - 1) * ld + K) * 2 : ((uint64_t)(K - 1) * ld + rows) * 2;
+  const int64_t base_el = KCONT ? (int64_t)r0 * ld + k0 : r0 + (int64_t)k0 * ld;
+  const uint64_t left = extent > (uint64_t)base_el * 2 ? extent - (uint64_t)base_el * 2 : 0;
+  const __amdgpu_buffer_rsrc_t rs =
+      make_rsrc(X + base_el, (uint32_t)(left < 0xFFFFFFF0ull ? left : 0xFFFFFFF0ull));
+#pragma unroll
+  for (int i = 0; i < TR * TK / 256; ++i) {
+    const int q = tid + 256 * i;
+    const int r = KCONT ? q / TK : q % TR;  // lanes walk the contiguous index
+    const int k = KCONT ? q % TK : q / TR;
+    const uint32_t off = (uint32_t)((KCONT ? (int64_t)r * ld + k : r + (int64_t)k * ld) * 2);
+    const unsigned short v = __builtin_amdgcn_raw_buffer_load_b16(rs, (int)off, 0, 0);
+    tile[r][k] = (r0 + r < rows && k0 + k < K) ? v : (unsigned short)0;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const int q = tid + 256 * i;
+    const int c = q & 3, r = (q >> 2) & (TR - 1), sl = q >> 8;
+    const int ks = k0 / 32 + sl;
+    if (ks >= KS) continue;
+    u16x8 t;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) t[e] = tile[r][sl * 32 + c * 8 + e];
+    P[((int64_t)ks * Rp + r0 + r) * 4 + c] = t;
+  }
+}
+
+hipError_t launch_pack_bf16(const unsigned short* X, int64_t ld, bool kcont, int rows, int K, void* P,
+                            hipStream_t s) {
+  const int Rp = split_rows_padded(rows), KS = split_kslices(K);
+  dim3 grid(Rp / 64, (K + 63) / 64);
+  if (kcont)
+    hipLaunchKernelGGL(pack_bf16_kernel<true>, grid, dim3(256), 0, s, X, ld, rows, K, Rp, KS, static_cast<u16x8*>(P));
+  else
+    hipLaunchKernelGGL(pack_bf16_kernel<false>, grid, dim3(256), 0, s, X, ld, rows, K, Rp, KS,
+                       static_cast<u16x8*>(P));
+  return hipGetLastError();
+}
+
 template <int TERMS>
 static hipError_t launch_pack_t(const float* X, int64_t ld, bool kcont, int rows, int K, int Rp, int KS, void* P,
                                 hipStream_t s) {
@@ -350,6 +403,10 @@ hipError_t launch_gemm_split_packed(const SplitParams& p, int terms, int tile, i
     if (tile == 0) return launch_packed_t<128, 128, 2, 2, 2, 3>(p, splits, s);
     if (tile == 2) return launch_packed_t<256, 128, 4, 2, 2, 2>(p, splits, s);
     return launch_packed_t<256, 128, 2, 2, 2, 2>(p, splits, s);
+  }
+  if (terms == 1) {  // bf16 inputs: one product per slice, deeper pipelines
+    if (tile == 0) return launch_packed_t<128, 128, 2, 2, 1, 4>(p, splits, s);
+    return launch_packed_t<256, 128, 4, 2, 1, 4>(p, splits, s);
   }
   return hipErrorInvalidValue;
 }

@@ -98,6 +98,9 @@ struct SplitParams {
 hipError_t launch_pack_split(const float* X, int64_t ld, bool kcont, int rows, int K, int terms, void* P,
                              hipStream_t s);
 hipError_t launch_gemm_split_packed(const SplitParams& p, int terms, int tile, int splits, hipStream_t s);
+// bf16 operand (bits) -> one plane in the same packed layout (terms = 1).
+hipError_t launch_pack_bf16(const unsigned short* X, int64_t ld, bool kcont, int rows, int K, void* P,
+                            hipStream_t s);
 hipError_t launch_split_bf16(const float* x, int64_t n, int terms, unsigned short* t0, unsigned short* t1,
                              unsigned short* t2, hipStream_t s);
 

@@ -16,6 +16,9 @@
  *       the whole GEBP product C = alpha*op(A)*op(B) + beta*C, where the
  *       reference's beta handling lives one level up in ?gemm_
  *       (blas/level3_impl.h:80-95) and is fused here.
+ *   eh_gemm_bf16
+ *       the sbgemm redirect of bfloat16 products
+ *       (Eigen/src/Core/products/GeneralMatrixMatrix_BLAS.h:120-202).
  *   eh_gemm_strided
  *       the same driver with a result increment (resIncr,
  *       GeneralMatrixMatrix.h:98-137) for destinations with an inner stride.
@@ -117,6 +120,19 @@ eh_status eh_gemm_config(eh_dtype dtype, eh_op opa, eh_op opb, int64_t m, int64_
 eh_status eh_gemm_strided(eh_dtype dtype, eh_op opa, eh_op opb, int64_t m, int64_t n, int64_t k, const void* alpha,
                           const void* A, int64_t lda, const void* B, int64_t ldb, const void* beta, void* C,
                           int64_t incc, int64_t ldc, eh_precision prec, void* stream);
+
+/* bfloat16 inputs, FP32 result: C = alpha*op(A)*op(B) + beta*C with A and B
+ * bfloat16 (raw 16-bit patterns, column-major, any element alignment) and
+ * alpha, beta, C float.  Products of bf16 values are exact in FP32 and are
+ * accumulated in FP32 on the BF16 matrix cores (v_mfma_f32_32x32x16_bf16).
+ * This is the sbgemm contract that Eigen binds for bfloat16 products under
+ * EIGEN_USE_OPENBLAS_BFLOAT16 (GeneralMatrixMatrix_BLAS.h:120-202), which
+ * then rounds the FP32 result to bfloat16 itself; beta == 0 does not read C,
+ * alpha == 0 or k == 0 does not read A or B.  Operands must span less than
+ * 4 GB (EH_ERR_UNSUPPORTED otherwise). */
+eh_status eh_gemm_bf16(eh_op opa, eh_op opb, int64_t m, int64_t n, int64_t k, const float* alpha, const void* A,
+                       int64_t lda, const void* B, int64_t ldb, const float* beta, float* C, int64_t ldc,
+                       void* stream);
 
 /* Triangle of a symmetric result. */
 typedef enum { EH_UPPER = 0, EH_LOWER = 1 } eh_uplo;

@@ -82,6 +82,15 @@ void strsm_(const char* side, const char* uplo, const char* transa, const char* 
             const float* alpha, const float* a, const int* lda, float* b, const int* ldb);
 void dtrsm_(const char* side, const char* uplo, const char* transa, const char* diag, const int* m, const int* n,
             const double* alpha, const double* a, const int* lda, double* b, const int* ldb);
+/* bfloat16 product with an FP32 result (the OpenBLAS sbgemm prototype that
+ * Eigen declares and calls for bfloat16 products under
+ * EIGEN_USE_OPENBLAS_BFLOAT16, GeneralMatrixMatrix_BLAS.h:127-131): a and b
+ * hold bfloat16 values (16-bit patterns), alpha, beta and c are float.
+ * Validation order and info codes as ?gemm_ (name "SBGEMM"); device or host
+ * pointers as for ?gemm_. */
+void sbgemm_(const char* transa, const char* transb, const int* m, const int* n, const int* k, const float* alpha,
+             const void* a, const int* lda, const void* b, const int* ldb, const float* beta, float* c,
+             const int* ldc);
 /* Error hook (blas/xerbla.cpp:18): weak default prints the routine and info. */
 void xerbla_(const char* name, int* info, size_t len);
 
