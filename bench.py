@@ -193,6 +193,8 @@ def main() -> int:
         out["detail"]["gemv_16384"] = bench_gemv(E, torch, dev, args)
         out["detail"]["level3_fp64"] = bench_level3(E, torch, dev, args)
         torch.cuda.empty_cache()
+        out["detail"]["types"] = bench_types(E, torch, dev, args)
+        torch.cuda.empty_cache()
         out["detail"]["batched"] = bench_batched(E, torch, args)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
@@ -256,6 +258,35 @@ def bench_level3(E, torch, dev, args):
         out[name] = {"ms": wall, "tflops": tf, "frac_of_fp64_peak": tf / PEAK["fp64"]}
         log(f"{name}: {out[name]}")
     del A, B, Cm
+    return out
+
+
+def bench_types(E, torch, dev, args):
+    """Other scalar types: bf16 inputs with an FP32 result (8192^3, U[-1,1]
+    rounded to bf16; 2mnk flops) and complex products cgemm / zgemm (4096^3,
+    real and imaginary parts U[-1,1]; 8mnk real flops)."""
+    out = {}
+    n = 8192
+    A = (torch.rand(n * n, device=dev) * 2 - 1).to(torch.bfloat16)
+    B = (torch.rand(n * n, device=dev) * 2 - 1).to(torch.bfloat16)
+    Cm = torch.zeros(n * n, device=dev)
+    fn = lambda: E.gemm_bf16(0, 0, n, n, n, 1.0, A.data_ptr(), n, B.data_ptr(), n, 0.0, Cm, n)
+    wall, kern = timed(fn, max(3, args.steps // 2), args.warmup)
+    tf = 2.0 * n ** 3 / (wall * 1e-3) / 1e12
+    out["bf16_8192"] = {"ms": wall, "tflops": tf, "frac_of_bf16_peak": tf / PEAK["bf16"]}
+    del A, B, Cm
+    n = 4096
+    for name, dtype, tdt, peak in (("cgemm_4096", E.F32, torch.float32, "fp32"),
+                                   ("zgemm_4096", E.F64, torch.float64, "fp64")):
+        A = torch.rand(2 * n * n, dtype=tdt, device=dev) * 2 - 1
+        B = torch.rand(2 * n * n, dtype=tdt, device=dev) * 2 - 1
+        Cm = torch.rand(2 * n * n, dtype=tdt, device=dev) * 2 - 1
+        fn = lambda: E.gemm_complex(dtype, 0, 0, n, n, n, 1.0 + 0.5j, A, n, B, n, 0.5, Cm, n, prec=E.PREC_PLAIN)
+        wall, kern = timed(fn, max(3, args.steps // 2), args.warmup)
+        tf = 8.0 * n ** 3 / (wall * 1e-3) / 1e12
+        out[name] = {"ms": wall, "tflops": tf, f"frac_of_{peak}_peak": tf / PEAK[peak]}
+        del A, B, Cm
+    log(f"types: {out}")
     return out
 
 

@@ -130,6 +130,8 @@ def lib():
         for f in (L.eh_trmm, L.eh_trsm):
             f.restype = i32
             f.argtypes = [i32, i32, i32, i32, i32, i64, i64, vp, vp, i64, vp, i64, vp]
+        L.eh_gemm_complex.restype = i32
+        L.eh_gemm_complex.argtypes = [i32, i32, i32, i64, i64, i64, vp, vp, i64, vp, i64, vp, vp, i64, i32, vp]
         L.eh_gemm_bf16.restype = i32
         L.eh_gemm_bf16.argtypes = [i32, i32, i64, i64, i64, vp, vp, i64, vp, i64, vp, vp, i64, vp]
         L.eh_gemm_strided.restype = i32
@@ -254,6 +256,18 @@ def syrk(dtype: int, uplo: int, trans: int, n: int, k: int, alpha: float, A, lda
     a, b = _scalar(dtype, alpha), _scalar(dtype, beta)
     s = current_stream() if stream is None else stream
     check(lib().eh_syrk(dtype, uplo, trans, n, k, C.byref(a), _ptr(A), lda, C.byref(b), _ptr(Cm), ldc, s), "eh_syrk")
+
+
+def gemm_complex(dtype: int, opa: int, opb: int, m: int, n: int, k: int, alpha: complex, A, lda: int, B, ldb: int,
+                 beta: complex, Cm, ldc: int, prec: int = PREC_AUTO, stream=None) -> None:
+    """Complex C = alpha*op(A)*op(B) + beta*C on interleaved storage (eh_gemm_complex);
+    dtype F32 = complex64, F64 = complex128; op OP_C conjugates."""
+    T = C.c_float if dtype == F32 else C.c_double
+    a = (T * 2)(complex(alpha).real, complex(alpha).imag)
+    b = (T * 2)(complex(beta).real, complex(beta).imag)
+    s = current_stream() if stream is None else stream
+    check(lib().eh_gemm_complex(dtype, opa, opb, m, n, k, a, _ptr(A), lda, _ptr(B), ldb, b, _ptr(Cm), ldc, prec, s),
+          "eh_gemm_complex")
 
 
 def gemm_bf16(opa: int, opb: int, m: int, n: int, k: int, alpha: float, A, lda: int, B, ldb: int, beta: float, Cm,

@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <complex>
 #include <cstdio>
 #include <mutex>
 
@@ -431,6 +432,50 @@ void tri_impl(const char* name, const char* side, const char* uplo, const char* 
   finish(s);
 }
 
+// cgemm_ / zgemm_: ?gemm_'s checks, on complex elements (Z is a pair of reals).
+template <typename T>
+void complex_gemm_impl(const char* name, const char* opa, const char* opb, const int* m, const int* n, const int* k,
+                       const T* palpha, const std::complex<T>* a, const int* lda, const std::complex<T>* b,
+                       const int* ldb, const T* pbeta, std::complex<T>* c, const int* ldc) {
+  using Z = std::complex<T>;
+  int info = 0;
+  const int oa = op_code(*opa), ob = op_code(*opb);
+  if (oa == OP_BAD)
+    info = 1;
+  else if (ob == OP_BAD)
+    info = 2;
+  else if (*m < 0)
+    info = 3;
+  else if (*n < 0)
+    info = 4;
+  else if (*k < 0)
+    info = 5;
+  else if (*lda < std::max(1, oa == OP_N ? *m : *k))
+    info = 8;
+  else if (*ldb < std::max(1, ob == OP_N ? *k : *n))
+    info = 10;
+  else if (*ldc < std::max(1, *m))
+    info = 13;
+  if (info) return xerbla_(name, &info, 6);
+  if (*m == 0 || *n == 0) return;
+  const bool need_ab = *k > 0 && (palpha[0] != T(0) || palpha[1] != T(0));
+  const bool beta_one = pbeta[0] == T(1) && pbeta[1] == T(0);
+  if (!need_ab && beta_one) return;
+  hipStream_t s = g_stream;
+  std::lock_guard<std::mutex> lk(g_stage_mutex);
+  DevMat<Z> A(a, *lda), B(b, *ldb), C(c, *ldc);
+  const bool read_c = pbeta[0] != T(0) || pbeta[1] != T(0);
+  if ((need_ab && (!dev_view(g_sa, a, oa == OP_N ? *m : *k, oa == OP_N ? *k : *m, *lda, true, s, &A) ||
+                   !dev_view(g_sb, b, ob == OP_N ? *k : *n, ob == OP_N ? *n : *k, *ldb, true, s, &B))) ||
+      !dev_view(g_sc, c, *m, *n, *ldc, read_c, s, &C))
+    return (void)std::fprintf(stderr, "eigen_amd BLAS: staging failed\n");
+  if (eh_gemm_complex(dtype_of(sizeof(T)), (eh_op)oa, (eh_op)ob, *m, *n, *k, palpha, A.p, A.ld, B.p, B.ld, pbeta, C.p,
+                      C.ld, EH_PREC_AUTO, s) != EH_SUCCESS)
+    return report(name);
+  if (!copy_back(C, c, s)) std::fprintf(stderr, "eigen_amd BLAS: copying C back failed\n");
+  finish(s);
+}
+
 // sbgemm_: bfloat16 A and B, float alpha, beta and C, with ?gemm_'s
 // validation order and info codes (the reference declares this OpenBLAS
 // entry point, GeneralMatrixMatrix_BLAS.h:127-131, and has no
@@ -556,6 +601,22 @@ void strsm_(const char* side, const char* uplo, const char* transa, const char* 
 void dtrsm_(const char* side, const char* uplo, const char* transa, const char* diag, const int* m, const int* n,
             const double* alpha, const double* a, const int* lda, double* b, const int* ldb) {
   tri_impl<double, true>("DTRSM ", side, uplo, transa, diag, m, n, alpha, a, lda, b, ldb);
+}
+
+void cgemm_(const char* transa, const char* transb, const int* m, const int* n, const int* k, const void* alpha,
+            const void* a, const int* lda, const void* b, const int* ldb, const void* beta, void* c, const int* ldc) {
+  using Z = std::complex<float>;
+  complex_gemm_impl<float>("CGEMM ", transa, transb, m, n, k, static_cast<const float*>(alpha),
+                           static_cast<const Z*>(a), lda, static_cast<const Z*>(b), ldb,
+                           static_cast<const float*>(beta), static_cast<Z*>(c), ldc);
+}
+
+void zgemm_(const char* transa, const char* transb, const int* m, const int* n, const int* k, const void* alpha,
+            const void* a, const int* lda, const void* b, const int* ldb, const void* beta, void* c, const int* ldc) {
+  using Z = std::complex<double>;
+  complex_gemm_impl<double>("ZGEMM ", transa, transb, m, n, k, static_cast<const double*>(alpha),
+                            static_cast<const Z*>(a), lda, static_cast<const Z*>(b), ldb,
+                            static_cast<const double*>(beta), static_cast<Z*>(c), ldc);
 }
 
 void sbgemm_(const char* transa, const char* transb, const int* m, const int* n, const int* k, const float* alpha,

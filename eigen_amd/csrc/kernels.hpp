@@ -76,6 +76,20 @@ struct TrtriParams {
 template <typename T>
 hipError_t launch_trtri(const TrtriParams<T>& p, hipStream_t s);
 
+// Complex product as one real product (complex_kernels.hip): A3 (2m x 2k
+// real, ld3) = the 2x2 real blocks of alpha*op(A); B3 (2k x n real, ld3) =
+// op(B) with real and imaginary parts in consecutive rows; C = beta*C for a
+// complex beta.  T is the real type; A, B, C hold interleaved (re, im) pairs
+// with leading dimensions in complex elements.
+template <typename T>
+hipError_t launch_complex_pack_a(const T* A, int64_t lda, int m, int k, bool ta, bool conj, T ar, T ai, T* A3,
+                                 int64_t ld3, hipStream_t s);
+template <typename T>
+hipError_t launch_complex_pack_b(const T* B, int64_t ldb, int k, int n, bool tb, bool conj, T* B3, int64_t ld3,
+                                 hipStream_t s);
+template <typename T>
+hipError_t launch_complex_scale(T* C, int m, int n, int64_t ldc, T br, T bi, hipStream_t s);
+
 // Split-BF16 FP32 product over packed operands.  Each operand is first packed
 // into bf16 planes P[ks][term][row][32] (rows padded to split_rows_padded,
 // ks = k / 32 slices) by launch_pack_split, then the product kernel streams

@@ -16,6 +16,8 @@
  *       the whole GEBP product C = alpha*op(A)*op(B) + beta*C, where the
  *       reference's beta handling lives one level up in ?gemm_
  *       (blas/level3_impl.h:80-95) and is fused here.
+ *   eh_gemm_complex
+ *       cgemm_/zgemm_ and Eigen's complex GEBP (GeneralBlockPanelKernel.h:631-1050).
  *   eh_gemm_bf16
  *       the sbgemm redirect of bfloat16 products
  *       (Eigen/src/Core/products/GeneralMatrixMatrix_BLAS.h:120-202).
@@ -133,6 +135,22 @@ eh_status eh_gemm_strided(eh_dtype dtype, eh_op opa, eh_op opb, int64_t m, int64
 eh_status eh_gemm_bf16(eh_op opa, eh_op opb, int64_t m, int64_t n, int64_t k, const float* alpha, const void* A,
                        int64_t lda, const void* B, int64_t ldb, const float* beta, float* C, int64_t ldc,
                        void* stream);
+
+/* Complex product C = alpha*op(A)*op(B) + beta*C on interleaved complex
+ * storage (std::complex<float> for EH_F32, std::complex<double> for EH_F64;
+ * leading dimensions in complex elements; alpha and beta point to one host
+ * complex scalar, i.e. two reals).  op = C conjugates.  Replaces
+ * cgemm_/zgemm_ (blas/level3_impl.h:12-96 instantiated for complex scalars)
+ * and Eigen's complex GEBP (GeneralBlockPanelKernel.h:631-1050).  Runs as ONE
+ * real product of order (2m) x n x (2k) on the FP32 / FP64 MFMA kernels: each
+ * complex entry of alpha*op(A) is repacked as the 2x2 real block
+ * [Re -Im; Im Re], B (when op(B) = B) and C are used in place as real
+ * matrices with twice the leading dimension.  FP32 follows `prec` like
+ * eh_gemm.  beta == 0 does not read C; alpha == 0 or k == 0 does not read A
+ * or B.  m and k must be below 2^30. */
+eh_status eh_gemm_complex(eh_dtype dtype, eh_op opa, eh_op opb, int64_t m, int64_t n, int64_t k, const void* alpha,
+                          const void* A, int64_t lda, const void* B, int64_t ldb, const void* beta, void* C,
+                          int64_t ldc, eh_precision prec, void* stream);
 
 /* Triangle of a symmetric result. */
 typedef enum { EH_UPPER = 0, EH_LOWER = 1 } eh_uplo;

@@ -82,6 +82,13 @@ void strsm_(const char* side, const char* uplo, const char* transa, const char* 
             const float* alpha, const float* a, const int* lda, float* b, const int* ldb);
 void dtrsm_(const char* side, const char* uplo, const char* transa, const char* diag, const int* m, const int* n,
             const double* alpha, const double* a, const int* lda, double* b, const int* ldb);
+/* Complex GEMM: alpha, beta, a, b, c are complex (pairs of float / double,
+ * Fortran COMPLEX / COMPLEX*16 layout).  Same validation order and info codes
+ * as ?gemm_ (names "CGEMM " / "ZGEMM "); op 'C' conjugates. */
+void cgemm_(const char* transa, const char* transb, const int* m, const int* n, const int* k, const void* alpha,
+            const void* a, const int* lda, const void* b, const int* ldb, const void* beta, void* c, const int* ldc);
+void zgemm_(const char* transa, const char* transb, const int* m, const int* n, const int* k, const void* alpha,
+            const void* a, const int* lda, const void* b, const int* ldb, const void* beta, void* c, const int* ldc);
 /* bfloat16 product with an FP32 result (the OpenBLAS sbgemm prototype that
  * Eigen declares and calls for bfloat16 products under
  * EIGEN_USE_OPENBLAS_BFLOAT16, GeneralMatrixMatrix_BLAS.h:127-131): a and b

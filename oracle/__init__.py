@@ -354,3 +354,15 @@ def trsm_ref(side: str, uplo: str, trans: str, diag: str, alpha, A, B0) -> np.nd
         acc -= T[i, :i] @ X[:i] if lower else T[i, i + 1:] @ X[i + 1:]
         X[i] = acc / T[i, i]
     return alpha * (X if left else X.T)
+
+
+def cgemm_ref(ta: str, tb: str, alpha, A, B, beta, C0) -> np.ndarray:
+    """Complex ?gemm_ (blas/level3_impl.h:12-96 for complex scalars): C = beta*C
+    (beta == 0 sets zero, C not read) + alpha*op(A)*op(B), op in N / T / C
+    (conjugate transpose).  Evaluated in extended precision (clongdouble)."""
+    def op(x, t):
+        x = np.asarray(x, np.clongdouble)
+        return x if t.upper() == "N" else (x.T if t.upper() == "T" else x.conj().T)
+    P = op(A, ta) @ op(B, tb)
+    base = np.zeros(P.shape, np.clongdouble) if beta == 0 else complex(beta) * np.asarray(C0, np.clongdouble)
+    return base + complex(alpha) * P

@@ -1,5 +1,6 @@
 """Netlib level-3 known-answer testers (blas/testing/sblat3.f, dblat3.f of the
-reference, compiled into oracle/_ref and linked against our sgemm_/dgemm_).
+reference, compiled into oracle/_ref and linked against the library's BLAS),
+and cblat3.f / zblat3.f for the complex GEMM.
 
 The input deck mirrors the reference's sblat3.dat / dblat3.dat (N in
 {0,1,2,3,5,9}, alpha in {0,1,0.7}, beta in {0,1,1.3}, threshold 16) with the
@@ -25,8 +26,14 @@ ROUTINES = [("GEMM ", True), ("SYMM ", True), ("TRMM ", True), ("TRSM ", True), 
 NAMES = ["GEMM", "SYMM", "TRMM", "TRSM", "SYRK", "SYR2K"]
 
 
+# complex decks (cblat3.dat / zblat3.dat): the library provides GEMM only
+ROUTINES_COMPLEX = [("GEMM ", True), ("HEMM ", False), ("SYMM ", False), ("TRMM ", False), ("TRSM ", False),
+                    ("HERK ", False), ("SYRK ", False), ("HER2K", False), ("SYR2K", False)]
+
+
 def deck(prec: str, ns, error_exits=True) -> str:
     up = prec.upper()
+    cplx = prec in "cz"
     lines = [f"'{prec}blat3.summ'     NAME OF SUMMARY OUTPUT FILE", "6                 UNIT NUMBER OF SUMMARY FILE",
              f"'{prec}blat3.snap'     NAME OF SNAPSHOT OUTPUT FILE", "-1                UNIT NUMBER OF SNAPSHOT FILE",
              "F        LOGICAL FLAG, T TO REWIND SNAPSHOT FILE AFTER EACH RECORD.",
@@ -34,9 +41,12 @@ def deck(prec: str, ns, error_exits=True) -> str:
              f"{'T' if error_exits else 'F'}        LOGICAL FLAG, T TO TEST ERROR EXITS.",
              "16.0     THRESHOLD VALUE OF TEST RATIO",
              f"{len(ns)}                 NUMBER OF VALUES OF N", " ".join(str(n) for n in ns) + "       VALUES OF N",
-             "3                 NUMBER OF VALUES OF ALPHA", "0.0 1.0 0.7       VALUES OF ALPHA",
-             "3                 NUMBER OF VALUES OF BETA", "0.0 1.0 1.3       VALUES OF BETA"]
-    lines += [f"{up}{name} {'T' if on else 'F'} PUT F FOR NO TEST. SAME COLUMNS." for name, on in ROUTINES]
+             "3                 NUMBER OF VALUES OF ALPHA",
+             "(0.0,0.0) (1.0,0.0) (0.7,-0.9)       VALUES OF ALPHA" if cplx else "0.0 1.0 0.7       VALUES OF ALPHA",
+             "3                 NUMBER OF VALUES OF BETA",
+             "(0.0,0.0) (1.0,0.0) (1.3,-1.1)       VALUES OF BETA" if cplx else "0.0 1.0 1.3       VALUES OF BETA"]
+    lines += [f"{up}{name} {'T' if on else 'F'} PUT F FOR NO TEST. SAME COLUMNS."
+              for name, on in (ROUTINES_COMPLEX if cplx else ROUTINES)]
     return "\n".join(lines) + "\n"
 
 
@@ -73,4 +83,23 @@ def test_computational(gpu, prec):
         m = re.search(rf"{prec.upper()}{name}\s+PASSED THE COMPUTATIONAL TESTS \(\s*(\d+) CALLS\)", out)
         assert m and int(m.group(1)) > 1000, (name, out)
     assert f"{prec.upper()}GEMM  PASSED THE COMPUTATIONAL TESTS ( 17496 CALLS)" in out, out
+    assert "FAILED" not in out and "SUSPECT" not in out, out
+
+
+@pytest.mark.parametrize("prec", ["c", "z"])
+def test_complex_error_exits(prec):
+    out = run_tester(prec, [0])
+    assert re.search(rf"{prec.upper()}GEMM\s+PASSED THE TESTS OF ERROR-EXITS", out), out
+    assert f"{prec.upper()}GEMM  PASSED THE COMPUTATIONAL TESTS (    81 CALLS)" in out, out
+    assert "FAILED" not in out and "SUSPECT" not in out, out
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", ["c", "z"])
+def test_complex_computational(gpu, prec):
+    """CGEMM / ZGEMM: every op combination (N, T, C), complex alpha and beta."""
+    out = run_tester(prec, [0, 1, 2, 3, 5, 9])
+    assert re.search(rf"{prec.upper()}GEMM\s+PASSED THE TESTS OF ERROR-EXITS", out), out
+    m = re.search(rf"{prec.upper()}GEMM\s+PASSED THE COMPUTATIONAL TESTS \(\s*(\d+) CALLS\)", out)
+    assert m and int(m.group(1)) > 10000, out
     assert "FAILED" not in out and "SUSPECT" not in out, out
