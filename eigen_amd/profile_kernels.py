@@ -1,6 +1,6 @@
 """Short driver for profiler runs, and the summariser of their output.
 
-    python3 -m eigen_amd.profile_kernels run {split3,plain,dgemm,batched,copy} [reps]
+    python3 -m eigen_amd.profile_kernels run {split3,plain,dgemm,batched,dtrsm,zgemm,copy} [reps]
     python3 -m eigen_amd.profile_kernels summarize <rocprofv3 output root>
 
 `run` launches one workload a few times (the program under
@@ -10,6 +10,8 @@
   plain    sgemm 8192^3 on FP32 MFMA
   dgemm    dgemm 4096^3, C = 1.5*A*B + 0.5*C
   batched  20,000 uneven products in one grouped launch (captured graph)
+  dtrsm    dtrsm 8192 x 8192 (left, lower, no transpose), diagonal 2, off-diagonal U[-1/n, 1/n]
+  zgemm    zgemm 4096^3 (complex128, one real product over 2x2 real blocks)
   copy     a 1 GiB device-to-device copy: calibrates the HBM byte counters
 
 `summarize` reads directories named trace_<workload> (kernel-trace csv),
@@ -31,7 +33,8 @@ import sys
 COPY_BYTES = 1 << 30
 # dominant kernel of each workload (substring of its name)
 DOMINANT = {"split3": "gemm_split_packed_kernel", "plain": "gemm_mfma_kernel<float", "dgemm": "gemm_mfma_kernel<double",
-            "batched": "grouped_gemm_kernel", "copy": "copyBuffer"}
+            "batched": "grouped_gemm_kernel", "dtrsm": "gemm_mfma_kernel<double", "zgemm": "gemm_mfma_kernel<double",
+            "copy": "copyBuffer"}
 
 
 def run(what: str, reps: int) -> None:
@@ -63,6 +66,20 @@ def run(what: str, reps: int) -> None:
         g = E.Grouped(b.problems)
         for _ in range(reps):
             g.graph_launch()
+    elif what == "dtrsm":
+        n = 8192
+        A = torch.empty(n * n, dtype=torch.float64, device="cuda")
+        B = torch.empty(n * n, dtype=torch.float64, device="cuda")
+        E.fill(E.F64, A, n * n, 31, E.DIST_UNIFORM, -1.0 / n, 1.0 / n)
+        A.view(n, n).diagonal().fill_(2.0)
+        E.fill(E.F64, B, n * n, 32, E.DIST_UNIFORM, -1, 1)
+        for _ in range(reps):
+            E.trsm(E.F64, E.LEFT, E.LOWER, E.OP_N, E.NON_UNIT, n, n, 1.0, A, n, B, n)
+    elif what == "zgemm":
+        n = 4096
+        A, B, Cm = (torch.rand(2 * n * n, dtype=torch.float64, device="cuda") * 2 - 1 for _ in range(3))
+        for _ in range(reps):
+            E.gemm_complex(E.F64, 0, 0, n, n, n, 1.0 + 0.5j, A, n, B, n, 0.5, Cm, n)
     elif what == "copy":
         x = torch.empty(COPY_BYTES // 4, dtype=torch.float32, device="cuda")
         y = torch.empty_like(x)
