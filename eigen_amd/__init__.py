@@ -130,6 +130,8 @@ def lib():
         for f in (L.eh_trmm, L.eh_trsm):
             f.restype = i32
             f.argtypes = [i32, i32, i32, i32, i32, i64, i64, vp, vp, i64, vp, i64, vp]
+        L.eh_gemv.restype = i32
+        L.eh_gemv.argtypes = [i32, i32, i64, i64, vp, vp, i64, vp, i64, vp, vp, i64, vp]
         L.eh_gemm_complex.restype = i32
         L.eh_gemm_complex.argtypes = [i32, i32, i32, i64, i64, i64, vp, vp, i64, vp, i64, vp, vp, i64, i32, vp]
         L.eh_gemm_bf16.restype = i32
@@ -256,6 +258,16 @@ def syrk(dtype: int, uplo: int, trans: int, n: int, k: int, alpha: float, A, lda
     a, b = _scalar(dtype, alpha), _scalar(dtype, beta)
     s = current_stream() if stream is None else stream
     check(lib().eh_syrk(dtype, uplo, trans, n, k, C.byref(a), _ptr(A), lda, C.byref(b), _ptr(Cm), ldc, s), "eh_syrk")
+
+
+def gemv(dtype: int, trans: int, m: int, n: int, alpha: float, A, lda: int, x, incx: int, beta: float, y,
+         incy: int, stream=None) -> None:
+    """y = alpha*op(A)*x + beta*y, A m x n, strided vectors (eh_gemv). x and y
+    may be device pointers (ints) already offset as BLAS expects."""
+    a, b = _scalar(dtype, alpha), _scalar(dtype, beta)
+    s = current_stream() if stream is None else stream
+    check(lib().eh_gemv(dtype, trans, m, n, C.byref(a), _ptr(A), lda, _ptr(x), incx, C.byref(b), _ptr(y), incy, s),
+          "eh_gemv")
 
 
 def gemm_complex(dtype: int, opa: int, opb: int, m: int, n: int, k: int, alpha: complex, A, lda: int, B, ldb: int,

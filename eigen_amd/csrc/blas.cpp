@@ -15,6 +15,7 @@
 #include <complex>
 #include <cstdio>
 #include <mutex>
+#include <vector>
 
 #include "../../include/eigen_amd.h"
 #include "../../include/eigen_amd_blas.h"
@@ -432,6 +433,76 @@ void tri_impl(const char* name, const char* side, const char* uplo, const char* 
   finish(s);
 }
 
+// ?gemv_ with the checks of blas/level2_impl.h:45-61: info 1 trans, 2 m,
+// 3 n, 6 lda, 8 incx, 11 incy; quick return for m == 0, n == 0, or
+// alpha == 0 with beta == 1.  Host vectors are gathered into contiguous
+// staging (a negative increment runs from the last element) and scattered
+// back.
+template <typename T>
+void gemv_impl(const char* name, const char* op, const int* m, const int* n, const T* palpha, const T* a,
+               const int* lda, const T* x, const int* incx, const T* pbeta, T* y, const int* incy) {
+  int info = 0;
+  const int o = op_code(*op);
+  if (o == OP_BAD)
+    info = 1;
+  else if (*m < 0)
+    info = 2;
+  else if (*n < 0)
+    info = 3;
+  else if (*lda < std::max(1, *m))
+    info = 6;
+  else if (*incx == 0)
+    info = 8;
+  else if (*incy == 0)
+    info = 11;
+  if (info) return xerbla_(name, &info, 6);
+  const T alpha = *palpha, beta = *pbeta;
+  if (*m == 0 || *n == 0 || (alpha == T(0) && beta == T(1))) return;
+  const int lx = o == OP_N ? *n : *m, ly = o == OP_N ? *m : *n;
+  hipStream_t s = g_stream;
+  std::lock_guard<std::mutex> lk(g_stage_mutex);
+  DevMat<T> A(a, *lda);
+  if (alpha != T(0) && !dev_view(g_sa, a, *m, *n, *lda, true, s, &A))
+    return (void)std::fprintf(stderr, "eigen_amd BLAS: staging failed\n");
+  // element i of a strided vector of length len
+  auto at = [](int64_t i, int64_t len, int inc) { return (inc > 0 ? i : i - (len - 1)) * inc; };
+  const T* dx = x;
+  int64_t dincx = *incx;
+  std::vector<T> hx, hy;
+  if (alpha != T(0) && !is_device_ptr(x)) {
+    hx.resize(lx);
+    for (int64_t i = 0; i < lx; ++i) hx[i] = x[at(i, lx, *incx)];
+    if (!g_sb.ensure((size_t)lx * sizeof(T)) ||
+        hipMemcpyAsync(g_sb.ptr, hx.data(), (size_t)lx * sizeof(T), hipMemcpyHostToDevice, s) != hipSuccess)
+      return (void)std::fprintf(stderr, "eigen_amd BLAS: staging x failed\n");
+    dx = static_cast<const T*>(g_sb.ptr);
+    dincx = 1;
+  }
+  T* dy = y;
+  int64_t dincy = *incy;
+  const bool host_y = !is_device_ptr(y);
+  if (host_y) {
+    hy.resize(ly);
+    if (beta != T(0))
+      for (int64_t i = 0; i < ly; ++i) hy[i] = y[at(i, ly, *incy)];
+    if (!g_sc.ensure((size_t)ly * sizeof(T)) ||
+        (beta != T(0) &&
+         hipMemcpyAsync(g_sc.ptr, hy.data(), (size_t)ly * sizeof(T), hipMemcpyHostToDevice, s) != hipSuccess))
+      return (void)std::fprintf(stderr, "eigen_amd BLAS: staging y failed\n");
+    dy = static_cast<T*>(g_sc.ptr);
+    dincy = 1;
+  }
+  if (eh_gemv(dtype_of(sizeof(T)), (eh_op)o, *m, *n, &alpha, A.p, A.ld, dx, dincx, &beta, dy, dincy, s) != EH_SUCCESS)
+    return report(name);
+  if (host_y) {
+    if (hipMemcpyAsync(hy.data(), dy, (size_t)ly * sizeof(T), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+      return (void)std::fprintf(stderr, "eigen_amd BLAS: copying y back failed\n");
+    for (int64_t i = 0; i < ly; ++i) y[at(i, ly, *incy)] = hy[i];
+  }
+  finish(s);
+}
+
 // cgemm_ / zgemm_: ?gemm_'s checks, on complex elements (Z is a pair of reals).
 template <typename T>
 void complex_gemm_impl(const char* name, const char* opa, const char* opb, const int* m, const int* n, const int* k,
@@ -601,6 +672,16 @@ void strsm_(const char* side, const char* uplo, const char* transa, const char* 
 void dtrsm_(const char* side, const char* uplo, const char* transa, const char* diag, const int* m, const int* n,
             const double* alpha, const double* a, const int* lda, double* b, const int* ldb) {
   tri_impl<double, true>("DTRSM ", side, uplo, transa, diag, m, n, alpha, a, lda, b, ldb);
+}
+
+void sgemv_(const char* trans, const int* m, const int* n, const float* alpha, const float* a, const int* lda,
+            const float* x, const int* incx, const float* beta, float* y, const int* incy) {
+  gemv_impl<float>("SGEMV ", trans, m, n, alpha, a, lda, x, incx, beta, y, incy);
+}
+
+void dgemv_(const char* trans, const int* m, const int* n, const double* alpha, const double* a, const int* lda,
+            const double* x, const int* incx, const double* beta, double* y, const int* incy) {
+  gemv_impl<double>("DGEMV ", trans, m, n, alpha, a, lda, x, incx, beta, y, incy);
 }
 
 void cgemm_(const char* transa, const char* transb, const int* m, const int* n, const int* k, const void* alpha,

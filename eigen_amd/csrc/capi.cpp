@@ -471,6 +471,47 @@ template eh_status device_product<float>(int, int, int64_t, int64_t, int64_t, fl
 template eh_status device_product<double>(int, int, int64_t, int64_t, int64_t, double, const double*, int64_t,
                                           const double*, int64_t, double, double*, int64_t, hipStream_t, int);
 
+// y = alpha*op(A)*x + beta*y with strided vectors (BLAS ?gemv_).  A
+// matrix spanning 4 GB or more is cut along the dimension that multiplies
+// lda: the input dimension for op = N (later parts accumulate with beta = 1),
+// the output rows for op = T.
+template <typename T>
+static eh_status gemv_run(bool trans, int64_t rows, int64_t k, T alpha, const T* A, int64_t lda, const T* x,
+                          int64_t incx, T beta, T* y, int64_t incy, hipStream_t s) {
+  if (rows == 0) return EH_SUCCESS;
+  if (k == 0 || alpha == T(0)) {  // y = beta*y (A and x not read)
+    if (beta == T(1)) return EH_SUCCESS;
+    hipError_t e = launch_scale_c<T>(y, 1, (int)rows, incy, beta, s);
+    return e == hipSuccess ? EH_SUCCESS : hip_fail(e, "scale launch");
+  }
+  const uint64_t extent = ((trans ? (uint64_t)(rows - 1) * lda + k : (uint64_t)(k - 1) * lda + rows) + 64) * sizeof(T);
+  if (extent >= kMaxExtent) {
+    if (!trans && k > 1) {
+      const int64_t h = k / 2;
+      eh_status st = gemv_run<T>(false, rows, h, alpha, A, lda, x, incx, beta, y, incy, s);
+      if (st != EH_SUCCESS) return st;
+      return gemv_run<T>(false, rows, k - h, alpha, A + h * lda, lda, x + h * incx, incx, T(1), y, incy, s);
+    }
+    if (trans && rows > 1) {
+      const int64_t h = rows / 2;
+      eh_status st = gemv_run<T>(true, h, k, alpha, A, lda, x, incx, beta, y, incy, s);
+      if (st != EH_SUCCESS) return st;
+      return gemv_run<T>(true, rows - h, k, alpha, A + h * lda, lda, x, incx, beta, y + h * incy, incy, s);
+    }
+    return fail(EH_ERR_UNSUPPORTED, "matrix-vector operand too large for 32-bit addressing");
+  }
+  const GemvPlan g = gemv_plan(trans, (int)rows, (int)k, (int)sizeof(T));
+  void* ws = nullptr;
+  const size_t wsb = gemv_workspace_bytes(g, (int)rows, (int)sizeof(T));
+  if (wsb > 0) {
+    eh_status st = workspace_get(s, wsb, &ws);
+    if (st != EH_SUCCESS) return st;
+  }
+  hipError_t e = launch_gemv<T>(g, trans, (int)rows, (int)k, alpha, A, lda, (uint32_t)(extent - 64 * sizeof(T)), x, incx,
+                                beta, y, incy, ws, s);
+  return e == hipSuccess ? EH_SUCCESS : hip_fail(e, "gemv launch");
+}
+
 eh_status gemm_dispatch(int dtype, int opa, int opb, int64_t m, int64_t n, int64_t k, const void* alpha,
                         const void* A, int64_t lda, const void* B, int64_t ldb, const void* beta, void* C,
                         int64_t ldc, int prec, int config, void* stream) {
@@ -559,6 +600,31 @@ eh_status eh_gemm_strided(eh_dtype dtype, eh_op opa, eh_op opb, int64_t m, int64
                                                      *static_cast<const float*>(beta), s);
   if (W) (void)hipFreeAsync(W, s);
   return e == hipSuccess ? EH_SUCCESS : hip_fail(e, "strided combine launch");
+}
+
+eh_status eh_gemv(eh_dtype dtype, eh_op trans, int64_t m, int64_t n, const void* alpha, const void* A, int64_t lda,
+                  const void* x, int64_t incx, const void* beta, void* y, int64_t incy, void* stream) {
+  if (dtype != EH_F32 && dtype != EH_F64) return fail(EH_ERR_INVALID, "dtype must be EH_F32 or EH_F64");
+  if (!valid_op(trans)) return fail(EH_ERR_INVALID, "trans must be N, T or C");
+  if (m < 0 || n < 0) return fail(EH_ERR_INVALID, "m, n must be >= 0");
+  if (m > INT32_MAX || n > INT32_MAX) return fail(EH_ERR_UNSUPPORTED, "m, n must fit int32");
+  if (lda < std::max<int64_t>(1, m)) return fail(EH_ERR_INVALID, "lda too small");
+  if (incx == 0 || incy == 0) return fail(EH_ERR_INVALID, "incx and incy must be nonzero");
+  if (!alpha || !beta) return fail(EH_ERR_INVALID, "alpha and beta must point to host scalars");
+  if (m == 0 || n == 0) return EH_SUCCESS;
+  const bool tr = trans != EH_OP_N;
+  const int64_t rows = tr ? n : m, k = tr ? m : n;  // y has `rows` elements, x has k
+  if (!y || !A || !x) return fail(EH_ERR_INVALID, "A, x or y is NULL");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  auto run = [&](auto zero) -> eh_status {
+    using T = decltype(zero);
+    // a negative increment walks the vector from its last element (BLAS)
+    const T* xp = static_cast<const T*>(x) + (incx < 0 ? (1 - k) * incx : 0);
+    T* yp = static_cast<T*>(y) + (incy < 0 ? (1 - rows) * incy : 0);
+    return gemv_run<T>(tr, rows, k, *static_cast<const T*>(alpha), static_cast<const T*>(A), lda, xp, incx,
+                       *static_cast<const T*>(beta), yp, incy, s);
+  };
+  return dtype == EH_F64 ? run(0.0) : run(0.0f);
 }
 
 eh_status eh_gemm_bf16(eh_op opa, eh_op opb, int64_t m, int64_t n, int64_t k, const float* alpha, const void* A,

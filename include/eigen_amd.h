@@ -16,6 +16,9 @@
  *       the whole GEBP product C = alpha*op(A)*op(B) + beta*C, where the
  *       reference's beta handling lives one level up in ?gemm_
  *       (blas/level3_impl.h:80-95) and is fused here.
+ *   eh_gemv
+ *       ?gemv_ (blas/level2_impl.h:27-84) / general_matrix_vector_product
+ *       (GeneralMatrixVector.h:71-289).
  *   eh_gemm_complex
  *       cgemm_/zgemm_ and Eigen's complex GEBP (GeneralBlockPanelKernel.h:631-1050).
  *   eh_gemm_bf16
@@ -122,6 +125,18 @@ eh_status eh_gemm_config(eh_dtype dtype, eh_op opa, eh_op opb, int64_t m, int64_
 eh_status eh_gemm_strided(eh_dtype dtype, eh_op opa, eh_op opb, int64_t m, int64_t n, int64_t k, const void* alpha,
                           const void* A, int64_t lda, const void* B, int64_t ldb, const void* beta, void* C,
                           int64_t incc, int64_t ldc, eh_precision prec, void* stream);
+
+/* Matrix-vector product with strided vectors (the BLAS ?gemv_ contract,
+ * blas/level2_impl.h:27-84): A is m x n;
+ *   trans = N:  y = alpha*A*x + beta*y     (x: n elements, y: m)
+ *   trans = T:  y = alpha*A^T*x + beta*y   (x: m elements, y: n)
+ * x(i) = x[i*incx] and y(i) = y[i*incy]; a negative increment starts at the
+ * last element, as in BLAS.  beta == 0 does not read y; alpha == 0 does not
+ * read A or x.  This is also the kernel Eigen's products with one output
+ * column or row reach (GeneralMatrixMatrix.h:480-490 ->
+ * GeneralMatrixVector.h:71-289), which eh_gemm takes for n == 1 or m == 1. */
+eh_status eh_gemv(eh_dtype dtype, eh_op trans, int64_t m, int64_t n, const void* alpha, const void* A, int64_t lda,
+                  const void* x, int64_t incx, const void* beta, void* y, int64_t incy, void* stream);
 
 /* bfloat16 inputs, FP32 result: C = alpha*op(A)*op(B) + beta*C with A and B
  * bfloat16 (raw 16-bit patterns, column-major, any element alignment) and

@@ -82,6 +82,16 @@ void strsm_(const char* side, const char* uplo, const char* transa, const char* 
             const float* alpha, const float* a, const int* lda, float* b, const int* ldb);
 void dtrsm_(const char* side, const char* uplo, const char* transa, const char* diag, const int* m, const int* n,
             const double* alpha, const double* a, const int* lda, double* b, const int* ldb);
+/* Matrix-vector product y = alpha*op(A)*x + beta*y (blas/level2_impl.h:27-84,
+ * the routine GeneralMatrixVector_BLAS.h:125-126 redirects Eigen's
+ * matrix-vector products to under EIGEN_USE_BLAS): info 1 trans, 2 m, 3 n,
+ * 6 lda, 8 incx, 11 incy (names "SGEMV " / "DGEMV "); quick return for
+ * m == 0, n == 0, or alpha == 0 with beta == 1; negative increments walk
+ * the vectors backwards.  Device or host pointers as for ?gemm_. */
+void sgemv_(const char* trans, const int* m, const int* n, const float* alpha, const float* a, const int* lda,
+            const float* x, const int* incx, const float* beta, float* y, const int* incy);
+void dgemv_(const char* trans, const int* m, const int* n, const double* alpha, const double* a, const int* lda,
+            const double* x, const int* incx, const double* beta, double* y, const int* incy);
 /* Complex GEMM: alpha, beta, a, b, c are complex (pairs of float / double,
  * Fortran COMPLEX / COMPLEX*16 layout).  Same validation order and info codes
  * as ?gemm_ (names "CGEMM " / "ZGEMM "); op 'C' conjugates. */

@@ -18,7 +18,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def _declared(header: str):
     text = open(os.path.join(ROOT, "include", header)).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b((?:eh_|[sd](?:gemm|gemmtr|syrk|syr2k|symm|trmm|trsm)_|sbgemm_|[cz]gemm_|xerbla_)\w*)\s*\(", text)))
+    return sorted(set(re.findall(r"\b((?:eh_|[sd](?:gemm|gemmtr|syrk|syr2k|symm|trmm|trsm)_|sbgemm_|[cz]gemm_|[sd]gemv_|xerbla_)\w*)\s*\(", text)))
 
 
 def _exported(path: str):
@@ -236,3 +236,52 @@ def test_level3_argument_checks():
     assert L.eh_gemmtr(E.F32, 5, 0, 0, 4, 3, a, 1, 4, 1, 3, a, 1, 4, None) == E.ERR_INVALID  # uplo
     assert L.eh_gemmtr(E.F32, 0, 0, 0, 4, 3, a, 1, 4, 1, 2, a, 1, 4, None) == E.ERR_INVALID  # ldb < k
     assert L.eh_gemmtr(E.F32, 0, 0, 0, 0, 3, a, 1, 1, 1, 3, a, 1, 1, None) == E.SUCCESS
+
+
+_GEMV_SCRIPT = r"""
+import ctypes as C, sys
+sys.path.insert(0, {root!r})
+import eigen_amd as E
+L = E.blas()
+i = lambda v: C.byref(C.c_int(v))
+one = C.c_double(1.0)
+buf = (C.c_double * 64)()
+t, m, n, lda, incx, incy = {args!r}
+L.dgemv_(t.encode(), i(m), i(n), C.byref(one), buf, i(lda), buf, i(incx), C.byref(one), buf, i(incy))
+sys.stdout.flush()
+"""
+
+GEMV_INVALID = [("X", 2, 2, 2, 1, 1), ("N", -1, 2, 2, 1, 1), ("T", 2, -3, 2, 1, 1), ("N", 4, 2, 3, 1, 1),
+                ("T", 3, 2, 3, 0, 1), ("N", 3, 2, 3, 1, 0), ("C", 0, 2, 0, 1, 1)]
+
+
+@pytest.mark.parametrize("args", GEMV_INVALID)
+def test_dgemv_error_exits_match_eigen_blas(args):
+    """Info codes of dgemv_ against Eigen's own dgemv_ (blas/level2_impl.h:45-59,
+    compiled into oracle/_ref with a recording xerbla_)."""
+    if not O.ref_available():
+        pytest.skip("oracle/_ref absent")
+    R = O.blas_ref()
+    R.ref_blas_reset()
+    t, m, n, lda, incx, incy = args
+    i = lambda v: C.byref(C.c_int(v))
+    one = C.c_double(1.0)
+    buf = (C.c_double * 64)()
+    R.dgemv_(t.encode(), i(m), i(n), C.byref(one), buf, i(lda), buf, i(incx), C.byref(one), buf, i(incy))
+    want = R.ref_blas_last_info()
+    assert want != 0
+    r = subprocess.run([sys.executable, "-c", _GEMV_SCRIPT.format(root=ROOT, args=args)], capture_output=True,
+                       text=True, timeout=120)
+    assert f"ERROR #{want}: DGEMV" in r.stdout, (want, r.stdout, r.stderr)
+
+
+def test_gemv_argument_checks():
+    L = E.lib()
+    one = C.c_double(1.0)
+    a = C.byref(one)
+    # eh_gemv(dtype, trans, m, n, alpha, A, lda, x, incx, beta, y, incy, stream)
+    assert L.eh_gemv(E.F64, 3, 4, 3, a, 1, 4, 1, 1, a, 1, 1, None) == E.ERR_INVALID   # trans
+    assert L.eh_gemv(E.F64, 0, 4, 3, a, 1, 3, 1, 1, a, 1, 1, None) == E.ERR_INVALID   # lda < m
+    assert L.eh_gemv(E.F64, 0, 4, 3, a, 1, 4, 1, 0, a, 1, 1, None) == E.ERR_INVALID   # incx == 0
+    assert L.eh_gemv(E.F64, 0, 4, 3, a, 1, 4, 1, 1, a, 1, 0, None) == E.ERR_INVALID   # incy == 0
+    assert L.eh_gemv(E.F64, 0, 0, 3, a, None, 1, None, 1, a, None, 1, None) == E.SUCCESS  # m == 0
