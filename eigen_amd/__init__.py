@@ -132,6 +132,8 @@ def lib():
             f.argtypes = [i32, i32, i32, i32, i32, i64, i64, vp, vp, i64, vp, i64, vp]
         L.eh_gemv.restype = i32
         L.eh_gemv.argtypes = [i32, i32, i64, i64, vp, vp, i64, vp, i64, vp, vp, i64, vp]
+        L.eh_gemv_complex.restype = i32
+        L.eh_gemv_complex.argtypes = [i32, i32, i64, i64, vp, vp, i64, vp, i64, vp, vp, i64, vp]
         L.eh_gemm_complex.restype = i32
         L.eh_gemm_complex.argtypes = [i32, i32, i32, i64, i64, i64, vp, vp, i64, vp, i64, vp, vp, i64, i32, vp]
         L.eh_gemm_bf16.restype = i32
@@ -280,6 +282,17 @@ def gemm_complex(dtype: int, opa: int, opb: int, m: int, n: int, k: int, alpha: 
     s = current_stream() if stream is None else stream
     check(lib().eh_gemm_complex(dtype, opa, opb, m, n, k, a, _ptr(A), lda, _ptr(B), ldb, b, _ptr(Cm), ldc, prec, s),
           "eh_gemm_complex")
+
+
+def gemv_complex(dtype: int, trans: int, m: int, n: int, alpha: complex, A, lda: int, x, incx: int, beta: complex,
+                 y, incy: int, stream=None) -> None:
+    """Complex y = alpha*op(A)*x + beta*y with strided vectors (eh_gemv_complex)."""
+    T = C.c_float if dtype == F32 else C.c_double
+    a = (T * 2)(complex(alpha).real, complex(alpha).imag)
+    b = (T * 2)(complex(beta).real, complex(beta).imag)
+    s = current_stream() if stream is None else stream
+    check(lib().eh_gemv_complex(dtype, trans, m, n, a, _ptr(A), lda, _ptr(x), incx, b, _ptr(y), incy, s),
+          "eh_gemv_complex")
 
 
 def gemm_bf16(opa: int, opb: int, m: int, n: int, k: int, alpha: float, A, lda: int, B, ldb: int, beta: float, Cm,

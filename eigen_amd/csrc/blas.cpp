@@ -503,6 +503,75 @@ void gemv_impl(const char* name, const char* op, const int* m, const int* n, con
   finish(s);
 }
 
+// cgemv_ / zgemv_: ?gemv_'s checks on complex elements (Z is a pair of reals).
+template <typename T>
+void complex_gemv_impl(const char* name, const char* op, const int* m, const int* n, const T* palpha,
+                       const std::complex<T>* a, const int* lda, const std::complex<T>* x, const int* incx,
+                       const T* pbeta, std::complex<T>* y, const int* incy) {
+  using Z = std::complex<T>;
+  int info = 0;
+  const int o = op_code(*op);
+  if (o == OP_BAD)
+    info = 1;
+  else if (*m < 0)
+    info = 2;
+  else if (*n < 0)
+    info = 3;
+  else if (*lda < std::max(1, *m))
+    info = 6;
+  else if (*incx == 0)
+    info = 8;
+  else if (*incy == 0)
+    info = 11;
+  if (info) return xerbla_(name, &info, 6);
+  const bool alpha_zero = palpha[0] == T(0) && palpha[1] == T(0);
+  const bool beta_one = pbeta[0] == T(1) && pbeta[1] == T(0), beta_zero = pbeta[0] == T(0) && pbeta[1] == T(0);
+  if (*m == 0 || *n == 0 || (alpha_zero && beta_one)) return;
+  const int lx = o == OP_N ? *n : *m, ly = o == OP_N ? *m : *n;
+  hipStream_t s = g_stream;
+  std::lock_guard<std::mutex> lk(g_stage_mutex);
+  DevMat<Z> A(a, *lda);
+  if (!alpha_zero && !dev_view(g_sa, a, *m, *n, *lda, true, s, &A))
+    return (void)std::fprintf(stderr, "eigen_amd BLAS: staging failed\n");
+  auto at = [](int64_t i, int64_t len, int inc) { return (inc > 0 ? i : i - (len - 1)) * inc; };
+  const Z* dx = x;
+  int64_t dincx = *incx;
+  std::vector<Z> hx, hy;
+  if (!alpha_zero && !is_device_ptr(x)) {
+    hx.resize(lx);
+    for (int64_t i = 0; i < lx; ++i) hx[i] = x[at(i, lx, *incx)];
+    if (!g_sb.ensure((size_t)lx * sizeof(Z)) ||
+        hipMemcpyAsync(g_sb.ptr, hx.data(), (size_t)lx * sizeof(Z), hipMemcpyHostToDevice, s) != hipSuccess)
+      return (void)std::fprintf(stderr, "eigen_amd BLAS: staging x failed\n");
+    dx = static_cast<const Z*>(g_sb.ptr);
+    dincx = 1;
+  }
+  Z* dy = y;
+  int64_t dincy = *incy;
+  const bool host_y = !is_device_ptr(y);
+  if (host_y) {
+    hy.resize(ly);
+    if (!beta_zero)
+      for (int64_t i = 0; i < ly; ++i) hy[i] = y[at(i, ly, *incy)];
+    if (!g_sc.ensure((size_t)ly * sizeof(Z)) ||
+        (!beta_zero &&
+         hipMemcpyAsync(g_sc.ptr, hy.data(), (size_t)ly * sizeof(Z), hipMemcpyHostToDevice, s) != hipSuccess))
+      return (void)std::fprintf(stderr, "eigen_amd BLAS: staging y failed\n");
+    dy = static_cast<Z*>(g_sc.ptr);
+    dincy = 1;
+  }
+  if (eh_gemv_complex(dtype_of(sizeof(T)), (eh_op)o, *m, *n, palpha, A.p, A.ld, dx, dincx, pbeta, dy, dincy, s) !=
+      EH_SUCCESS)
+    return report(name);
+  if (host_y) {
+    if (hipMemcpyAsync(hy.data(), dy, (size_t)ly * sizeof(Z), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+      return (void)std::fprintf(stderr, "eigen_amd BLAS: copying y back failed\n");
+    for (int64_t i = 0; i < ly; ++i) y[at(i, ly, *incy)] = hy[i];
+  }
+  finish(s);
+}
+
 // cgemm_ / zgemm_: ?gemm_'s checks, on complex elements (Z is a pair of reals).
 template <typename T>
 void complex_gemm_impl(const char* name, const char* opa, const char* opb, const int* m, const int* n, const int* k,
@@ -682,6 +751,21 @@ void sgemv_(const char* trans, const int* m, const int* n, const float* alpha, c
 void dgemv_(const char* trans, const int* m, const int* n, const double* alpha, const double* a, const int* lda,
             const double* x, const int* incx, const double* beta, double* y, const int* incy) {
   gemv_impl<double>("DGEMV ", trans, m, n, alpha, a, lda, x, incx, beta, y, incy);
+}
+
+void cgemv_(const char* trans, const int* m, const int* n, const void* alpha, const void* a, const int* lda,
+            const void* x, const int* incx, const void* beta, void* y, const int* incy) {
+  using Z = std::complex<float>;
+  complex_gemv_impl<float>("CGEMV ", trans, m, n, static_cast<const float*>(alpha), static_cast<const Z*>(a), lda,
+                           static_cast<const Z*>(x), incx, static_cast<const float*>(beta), static_cast<Z*>(y), incy);
+}
+
+void zgemv_(const char* trans, const int* m, const int* n, const void* alpha, const void* a, const int* lda,
+            const void* x, const int* incx, const void* beta, void* y, const int* incy) {
+  using Z = std::complex<double>;
+  complex_gemv_impl<double>("ZGEMV ", trans, m, n, static_cast<const double*>(alpha), static_cast<const Z*>(a), lda,
+                            static_cast<const Z*>(x), incx, static_cast<const double*>(beta), static_cast<Z*>(y),
+                            incy);
 }
 
 void cgemm_(const char* transa, const char* transb, const int* m, const int* n, const int* k, const void* alpha,

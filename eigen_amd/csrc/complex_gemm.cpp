@@ -73,10 +73,68 @@ eh_status run(int dtype, int opa, int opb, int64_t m, int64_t n, int64_t k, cons
                        s);
 }
 
+// y = alpha*op(A)*x + beta*y, complex, on the real view of A: one real
+// product with two right-hand sides (A read once) between a gather and a
+// combine pass (complex_kernels.hip).
+template <typename T>
+eh_status gemv_run(int dtype, int op, int64_t m, int64_t n, const T* alpha, const T* A, int64_t lda, const T* x,
+                   int64_t incx, const T* beta, T* y, int64_t incy, hipStream_t s) {
+  const int64_t lenx = op == EH_OP_N ? n : m, leny = op == EH_OP_N ? m : n;
+  if (alpha[0] == T(0) && alpha[1] == T(0)) {  // y = beta*y
+    if (beta[0] == T(1) && beta[1] == T(0)) return EH_SUCCESS;
+    hipError_t e = launch_complex_scale<T>(y, 1, (int)leny, incy, beta[0], beta[1], s);
+    return e == hipSuccess ? EH_SUCCESS : hip_fail(e, "complex scale launch");
+  }
+  const int opc = op == EH_OP_N ? 0 : (op == EH_OP_T ? 1 : 2);
+  const int64_t ldx = op == EH_OP_N ? n : 2 * m, ldp = op == EH_OP_N ? 2 * m : n;
+  Temp t;
+  eh_status st = t.get((size_t)(2 * ldx + 2 * ldp) * sizeof(T), s);
+  if (st != EH_SUCCESS) return st;
+  T* X = static_cast<T*>(t.ptr);
+  T* P = X + 2 * ldx;
+  hipError_t e = launch_complex_gemv_gather<T>(x, incx, (int)lenx, opc, X, ldx, s);
+  if (e != hipSuccess) return hip_fail(e, "complex gather launch");
+  const T one = T(1), zero = T(0);
+  if (op == EH_OP_N)
+    st = gemm_dispatch(dtype, EH_OP_N, EH_OP_N, 2 * m, 2, n, &one, A, 2 * lda, X, ldx, &zero, P, ldp, EH_PREC_PLAIN, -1,
+                       s);
+  else
+    st = gemm_dispatch(dtype, EH_OP_T, EH_OP_N, n, 2, 2 * m, &one, A, 2 * lda, X, ldx, &zero, P, ldp, EH_PREC_PLAIN, -1,
+                       s);
+  if (st != EH_SUCCESS) return st;
+  e = launch_complex_gemv_combine<T>(P, ldp, (int)leny, opc, alpha[0], alpha[1], beta[0], beta[1], y, incy, s);
+  return e == hipSuccess ? EH_SUCCESS : hip_fail(e, "complex combine launch");
+}
+
 }  // namespace
 }  // namespace eh
 
 using namespace eh;
+
+extern "C" eh_status eh_gemv_complex(eh_dtype dtype, eh_op trans, int64_t m, int64_t n, const void* alpha,
+                                     const void* A, int64_t lda, const void* x, int64_t incx, const void* beta,
+                                     void* y, int64_t incy, void* stream) {
+  if (dtype != EH_F32 && dtype != EH_F64) return fail(EH_ERR_INVALID, "dtype must be EH_F32 or EH_F64");
+  if (trans != EH_OP_N && trans != EH_OP_T && trans != EH_OP_C) return fail(EH_ERR_INVALID, "trans must be N, T or C");
+  if (m < 0 || n < 0) return fail(EH_ERR_INVALID, "m, n must be >= 0");
+  if (m > INT32_MAX / 2 || n > INT32_MAX / 2) return fail(EH_ERR_UNSUPPORTED, "m, n must be below 2^30");
+  if (lda < std::max<int64_t>(1, m)) return fail(EH_ERR_INVALID, "lda too small");
+  if (incx == 0 || incy == 0) return fail(EH_ERR_INVALID, "incx and incy must be nonzero");
+  if (!alpha || !beta) return fail(EH_ERR_INVALID, "alpha and beta must point to host scalars");
+  if (m == 0 || n == 0) return EH_SUCCESS;
+  if (!A || !x || !y) return fail(EH_ERR_INVALID, "A, x or y is NULL");
+  const int64_t lenx = trans == EH_OP_N ? n : m, leny = trans == EH_OP_N ? m : n;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  auto run = [&](auto zero) -> eh_status {
+    using T = decltype(zero);
+    // a negative increment walks the vector from its last element (BLAS); 2 reals per element
+    const T* xp = static_cast<const T*>(x) + 2 * (incx < 0 ? (1 - lenx) * incx : 0);
+    T* yp = static_cast<T*>(y) + 2 * (incy < 0 ? (1 - leny) * incy : 0);
+    return gemv_run<T>(dtype, trans, m, n, static_cast<const T*>(alpha), static_cast<const T*>(A), lda, xp, incx,
+                       static_cast<const T*>(beta), yp, incy, s);
+  };
+  return dtype == EH_F64 ? run(0.0) : run(0.0f);
+}
 
 extern "C" eh_status eh_gemm_complex(eh_dtype dtype, eh_op opa, eh_op opb, int64_t m, int64_t n, int64_t k,
                                      const void* alpha, const void* A, int64_t lda, const void* B, int64_t ldb,

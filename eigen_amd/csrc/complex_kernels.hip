@@ -77,6 +77,58 @@ __global__ void __launch_bounds__(256) complex_scale_kernel(T* C, int m, int n, 
   }
 }
 
+// Complex matrix-vector product y = alpha*op(A)*x + beta*y on the real view R
+// of A (2m x n real, leading dimension 2*lda: rows 2i, 2i+1 are Re, Im of
+// row i).  gemv_gather builds the two real right-hand sides X (len x 2,
+// column-major, ld = len2) from x (element i at x[i*incx], already based at
+// the element BLAS calls first):
+//   op = N (len = n, len2 = n):  X = [Re x, Im x]               P = R X   (2m x 2)
+//   op = T (len = m, len2 = 2m): X(2i) = (Re, Im), X(2i+1) = (-Im, Re)
+//   op = C:                      X(2i) = (Re, Im), X(2i+1) = (Im, -Re)  P = R^T X (n x 2)
+// and gemv_combine forms y from P:
+//   op = N:  y_i = (P(2i,0) - P(2i+1,1)) + i (P(2i,1) + P(2i+1,0))
+//   op = T/C: y_j = P(j,0) + i P(j,1)
+// scaled by alpha, plus beta*y (beta == 0: y not read).
+template <typename T>
+__global__ void __launch_bounds__(256) gemv_gather_kernel(const T* x, int64_t incx, int len, int op, T* X,
+                                                          int64_t ld) {
+  for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < len; i += (int64_t)gridDim.x * 256) {
+    const Cplx<T> v = reinterpret_cast<const Cplx<T>*>(x)[i * incx];
+    if (op == 0) {
+      X[i] = v.re;
+      X[i + ld] = v.im;
+    } else {
+      X[2 * i] = v.re;
+      X[2 * i + ld] = v.im;
+      X[2 * i + 1] = op == 2 ? v.im : -v.im;
+      X[2 * i + 1 + ld] = op == 2 ? -v.re : v.re;
+    }
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) gemv_combine_kernel(const T* P, int64_t ldp, int len, int op, T ar, T ai,
+                                                           T br, T bi, T* y, int64_t incy) {
+  for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < len; i += (int64_t)gridDim.x * 256) {
+    T pr, pi;
+    if (op == 0) {
+      pr = P[2 * i] - P[2 * i + 1 + ldp];
+      pi = P[2 * i + ldp] + P[2 * i + 1];
+    } else {
+      pr = P[i];
+      pi = P[i + ldp];
+    }
+    Cplx<T>* dst = reinterpret_cast<Cplx<T>*>(y) + i * incy;
+    T re = ar * pr - ai * pi, im = ar * pi + ai * pr;
+    if (br != T(0) || bi != T(0)) {
+      const Cplx<T> c = *dst;
+      re += br * c.re - bi * c.im;
+      im += br * c.im + bi * c.re;
+    }
+    *dst = Cplx<T>{re, im};
+  }
+}
+
 static unsigned grid_for(int64_t total) {
   int64_t b = (total + 255) / 256;
   if (b > 65536) b = 65536;
@@ -105,6 +157,27 @@ hipError_t launch_complex_scale(T* C, int m, int n, int64_t ldc, T br, T bi, hip
   return hipGetLastError();
 }
 
+template <typename T>
+hipError_t launch_complex_gemv_gather(const T* x, int64_t incx, int len, int op, T* X, int64_t ld, hipStream_t s) {
+  hipLaunchKernelGGL(gemv_gather_kernel<T>, dim3(grid_for(len)), dim3(256), 0, s, x, incx, len, op, X, ld);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_complex_gemv_combine(const T* P, int64_t ldp, int len, int op, T ar, T ai, T br, T bi, T* y,
+                                       int64_t incy, hipStream_t s) {
+  hipLaunchKernelGGL(gemv_combine_kernel<T>, dim3(grid_for(len)), dim3(256), 0, s, P, ldp, len, op, ar, ai, br, bi, y,
+                     incy);
+  return hipGetLastError();
+}
+
+template hipError_t launch_complex_gemv_gather<float>(const float*, int64_t, int, int, float*, int64_t, hipStream_t);
+template hipError_t launch_complex_gemv_gather<double>(const double*, int64_t, int, int, double*, int64_t,
+                                                       hipStream_t);
+template hipError_t launch_complex_gemv_combine<float>(const float*, int64_t, int, int, float, float, float, float,
+                                                       float*, int64_t, hipStream_t);
+template hipError_t launch_complex_gemv_combine<double>(const double*, int64_t, int, int, double, double, double,
+                                                        double, double*, int64_t, hipStream_t);
 template hipError_t launch_complex_pack_a<float>(const float*, int64_t, int, int, bool, bool, float, float, float*,
                                                  int64_t, hipStream_t);
 template hipError_t launch_complex_pack_a<double>(const double*, int64_t, int, int, bool, bool, double, double,

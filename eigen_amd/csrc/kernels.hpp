@@ -89,6 +89,14 @@ hipError_t launch_complex_pack_b(const T* B, int64_t ldb, int k, int n, bool tb,
                                  hipStream_t s);
 template <typename T>
 hipError_t launch_complex_scale(T* C, int m, int n, int64_t ldc, T br, T bi, hipStream_t s);
+// Complex matrix-vector product on the real view of A: the real right-hand
+// sides from x (op 0 = N, 1 = T, 2 = C) and the combination of the real
+// product P into y (complex_kernels.hip documents the layouts).
+template <typename T>
+hipError_t launch_complex_gemv_gather(const T* x, int64_t incx, int len, int op, T* X, int64_t ld, hipStream_t s);
+template <typename T>
+hipError_t launch_complex_gemv_combine(const T* P, int64_t ldp, int len, int op, T ar, T ai, T br, T bi, T* y,
+                                       int64_t incy, hipStream_t s);
 
 // Split-BF16 FP32 product over packed operands.  Each operand is first packed
 // into bf16 planes P[ks][term][row][32] (rows padded to split_rows_padded,

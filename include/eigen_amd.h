@@ -167,6 +167,17 @@ eh_status eh_gemm_complex(eh_dtype dtype, eh_op opa, eh_op opb, int64_t m, int64
                           const void* A, int64_t lda, const void* B, int64_t ldb, const void* beta, void* C,
                           int64_t ldc, eh_precision prec, void* stream);
 
+/* Complex matrix-vector product with strided vectors (the cgemv_/zgemv_
+ * contract, blas/level2_impl.h:27-84 for complex scalars): A m x n
+ * interleaved complex, y = alpha*op(A)*x + beta*y, op = C conjugates, alpha
+ * and beta point to one host complex scalar.  A is read once: its real view
+ * (2m x n, leading dimension 2*lda) multiplies two real right-hand sides
+ * built from x on the FP32 / FP64 MFMA kernels, and a combine pass forms y.
+ * Increments as eh_gemv; m and n must be below 2^30. */
+eh_status eh_gemv_complex(eh_dtype dtype, eh_op trans, int64_t m, int64_t n, const void* alpha, const void* A,
+                          int64_t lda, const void* x, int64_t incx, const void* beta, void* y, int64_t incy,
+                          void* stream);
+
 /* Triangle of a symmetric result. */
 typedef enum { EH_UPPER = 0, EH_LOWER = 1 } eh_uplo;
 

@@ -99,6 +99,12 @@ void cgemm_(const char* transa, const char* transb, const int* m, const int* n, 
             const void* a, const int* lda, const void* b, const int* ldb, const void* beta, void* c, const int* ldc);
 void zgemm_(const char* transa, const char* transb, const int* m, const int* n, const int* k, const void* alpha,
             const void* a, const int* lda, const void* b, const int* ldb, const void* beta, void* c, const int* ldc);
+/* Complex matrix-vector product (the ?gemv_ contract on complex elements,
+ * names "CGEMV " / "ZGEMV "); op 'C' conjugates. */
+void cgemv_(const char* trans, const int* m, const int* n, const void* alpha, const void* a, const int* lda,
+            const void* x, const int* incx, const void* beta, void* y, const int* incy);
+void zgemv_(const char* trans, const int* m, const int* n, const void* alpha, const void* a, const int* lda,
+            const void* x, const int* incx, const void* beta, void* y, const int* incy);
 /* bfloat16 product with an FP32 result (the OpenBLAS sbgemm prototype that
  * Eigen declares and calls for bfloat16 products under
  * EIGEN_USE_OPENBLAS_BFLOAT16, GeneralMatrixMatrix_BLAS.h:127-131): a and b

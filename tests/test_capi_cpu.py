@@ -18,7 +18,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def _declared(header: str):
     text = open(os.path.join(ROOT, "include", header)).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b((?:eh_|[sd](?:gemm|gemmtr|syrk|syr2k|symm|trmm|trsm)_|sbgemm_|[cz]gemm_|[sd]gemv_|xerbla_)\w*)\s*\(", text)))
+    return sorted(set(re.findall(r"\b((?:eh_|[sd](?:gemm|gemmtr|syrk|syr2k|symm|trmm|trsm)_|sbgemm_|[cz]gemm_|[sdcz]gemv_|xerbla_)\w*)\s*\(", text)))
 
 
 def _exported(path: str):

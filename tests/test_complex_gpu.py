@@ -116,3 +116,71 @@ def test_zgemm_host_pointers(gpu):
     L.zgemm_(b"C", b"N", i(m), i(n), i(k), za, A.ctypes.data_as(C.c_void_p), i(k), B.ctypes.data_as(C.c_void_p),
              i(k + 1), zb, Cm.ctypes.data_as(C.c_void_p), i(m))
     assert np.array_equal(Cm, want)
+
+
+def _strided_z(vals, inc, dt):
+    n = len(vals)
+    buf = np.full(max(1, 1 + (n - 1) * abs(inc)), np.nan + 1j * np.nan, dtype=dt)
+    for i, v in enumerate(vals):
+        buf[(i if inc > 0 else i - (n - 1)) * inc] = v
+    return buf
+
+
+def _gather_z(buf, n, inc):
+    return np.array([buf[(i if inc > 0 else i - (n - 1)) * inc] for i in range(n)])
+
+
+@pytest.mark.parametrize("dt", [np.complex64, np.complex128])
+@pytest.mark.parametrize("trans,incs", list(itertools.product("NTC", [(1, 1), (-2, 3), (2, -1)])))
+def test_complex_gemv_bound(gpu, dt, trans, incs):
+    """eh_gemv_complex: bound 4 k u || |alpha||op(A)||x| + |beta||y0| || against the
+    extended-precision product; gaps between strided elements stay NaN."""
+    rng = np.random.default_rng(25)
+    incx, incy = incs
+    for m, n, alpha, beta in ((1, 1, 1.0, 0.0), (300, 129, 0.7 - 0.9j, 1.3 - 1.1j), (64, 1000, -0.5j, 0.0)):
+        A = _z(rng, (m, n), dt, m + 2)
+        lx, ly = (n, m) if trans == "N" else (m, n)
+        x = rng.uniform(-1, 1, lx) + 1j * rng.uniform(-1, 1, lx)
+        y0 = rng.uniform(-1, 1, ly) + 1j * rng.uniform(-1, 1, ly)
+        xb, yb = _strided_z(x, incx, dt), _strided_z(y0, incy, dt)
+        if beta == 0:
+            yb[~np.isnan(yb)] = np.nan  # beta == 0: y is not read
+        dy = _dev(yb)
+        E.gemv_complex(ED[dt], OPS[trans], m, n, alpha, _dev(A), A.shape[0], _dev(xb), incx, beta, dy, incy)
+        got_buf = _back(dy, (len(yb), 1))[:, 0]
+        got = _gather_z(got_buf, ly, incy).astype(np.complex128)
+        gaps = np.ones(len(yb), bool)
+        for i in range(ly):
+            gaps[(i if incy > 0 else i - (ly - 1)) * incy] = False
+        assert np.isnan(got_buf[gaps]).all()
+        Am = A[:m, :n]
+        want = O.cgemm_ref(trans, "N", alpha, Am, x.astype(dt).reshape(-1, 1), beta,
+                           None if beta == 0 else y0.astype(dt).reshape(-1, 1))[:, 0].astype(np.complex128)
+        Eb = abs(alpha) * (_op_abs(Am, trans) @ np.abs(x.astype(dt).astype(np.complex128)))
+        if beta != 0:
+            Eb = Eb + abs(beta) * np.abs(y0.astype(dt).astype(np.complex128))
+        assert np.linalg.norm(got - want) <= 4 * lx * U[dt] * np.linalg.norm(Eb), (m, n)
+
+
+def test_zgemv_host_vs_eigen_blas(gpu):
+    """zgemv_ on host arrays with negative increments against Eigen's own zgemv_."""
+    if not O.ref_available():
+        pytest.skip("oracle/_ref absent")
+    R = O.blas_ref()
+    L = E.blas()
+    rng = np.random.default_rng(26)
+    m, n, incx, incy = 150, 90, -2, 3
+    A = _z(rng, (m, n), np.complex128, m + 1)
+    for trans in "NTC":
+        lx, ly = (n, m) if trans == "N" else (m, n)
+        xb = _strided_z(rng.uniform(-1, 1, lx) + 1j * rng.uniform(-1, 1, lx), incx, np.complex128)
+        yb = _strided_z(rng.uniform(-1, 1, ly) + 1j * rng.uniform(-1, 1, ly), incy, np.complex128)
+        ours, ref = yb.copy(), yb.copy()
+        i = lambda v: C.byref(C.c_int(v))
+        za, zb = (C.c_double * 2)(0.7, -0.9), (C.c_double * 2)(1.3, -1.1)
+        p = lambda a: a.ctypes.data_as(C.c_void_p)
+        for lib, out in ((L, ours), (R, ref)):
+            lib.zgemv_(trans.encode(), i(m), i(n), za, p(A), i(m + 1), p(xb), i(incx), zb, p(out), i(incy))
+        ok = ~np.isnan(ref)
+        assert np.array_equal(ok, ~np.isnan(ours))
+        assert np.abs(ours[ok] - ref[ok]).max() < 1e-12 * lx, trans

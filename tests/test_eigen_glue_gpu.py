@@ -3,7 +3,10 @@
 * test_device_glue    -- C.device(dev).noalias() {=,+=,-=} expr on device Maps
   through Eigen's AssignmentWithDevice seam (include/eigen_amd_eigen.h);
 * test_blas_redirect  -- unchanged C.noalias() += alpha*A*B with
-  EIGEN_USE_BLAS routed to libeigen_amd_blas (host and device Maps).
+  EIGEN_USE_BLAS routed to libeigen_amd_blas (host and device Maps);
+* test_blas_level3_redirect -- unchanged matrix-vector, triangular solve,
+  triangular and self-adjoint products, rank update and complex products
+  with EIGEN_USE_BLAS, each BLAS symbol counted on its way to the library.
 
 Each program compares with Eigen's own CPU evaluation (or an exact loop) on
 small-integer operands, bit for bit, and prints PASS/FAIL per case."""
@@ -18,7 +21,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 BUILD = os.path.join(HERE, "eigen_glue", "_build")
 
 
-@pytest.mark.parametrize("prog", ["test_device_glue", "test_blas_redirect"])
+@pytest.mark.parametrize("prog", ["test_device_glue", "test_blas_redirect", "test_blas_level3_redirect"])
 def test_eigen_program(gpu, prog):
     path = os.path.join(BUILD, prog)
     if not os.path.exists(path):
