@@ -235,8 +235,9 @@ static eh_status launch_bf16_product(bool ta, bool tb, int m, int n, int k, floa
   if ((extA + 64) * 2 >= kMaxExtent || (extB + 64) * 2 >= kMaxExtent)
     return fail(EH_ERR_UNSUPPORTED, "bf16 operands must span less than 4 GB");
   const int KS = split_kslices(k);
-  const int64_t t256 = ((int64_t)(m + 255) / 256) * ((n + 127) / 128);
-  const int tile = t256 >= 256 ? 2 : 0;
+  // 256x256 tiles (tile 1) once they fill the chip, 128x128 (tile 0) below
+  const int64_t t256 = ((int64_t)(m + 255) / 256) * ((n + 255) / 256);
+  const int tile = t256 >= 256 ? 1 : 0;
   const int64_t tiles = tile ? t256 : ((int64_t)(m + 127) / 128) * ((n + 127) / 128);
   int splits = 1;
   while (splits < 8 && tiles * splits < 256 && k / (2 * splits) >= 512) splits *= 2;

@@ -404,9 +404,14 @@ hipError_t launch_gemm_split_packed(const SplitParams& p, int terms, int tile, i
     if (tile == 2) return launch_packed_t<256, 128, 4, 2, 2, 2>(p, splits, s);
     return launch_packed_t<256, 128, 2, 2, 2, 2>(p, splits, s);
   }
-  if (terms == 1) {  // bf16 inputs: one product per slice, deeper pipelines
+  if (terms == 1) {
+    // bf16 inputs: one product per slice, so the LDS reads per MFMA decide
+    // the rate.  tile 0: 128x128, 4 waves of 64x64 (4 stages); tile 1:
+    // 256x256, 4 waves of 128x128 (accumulators in AGPRs, 2 stages): half the
+    // LDS reads per MFMA, 814 vs 637 TFLOP/s at 8192^3 (the 256x128 8-wave
+    // tile), 659 vs 606 at 4096^3; below that the 128x128 tile wins.
     if (tile == 0) return launch_packed_t<128, 128, 2, 2, 1, 4>(p, splits, s);
-    return launch_packed_t<256, 128, 4, 2, 1, 4>(p, splits, s);
+    return launch_packed_t<256, 256, 2, 2, 1, 2>(p, splits, s);
   }
   return hipErrorInvalidValue;
 }

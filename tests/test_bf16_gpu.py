@@ -129,3 +129,26 @@ def test_sbgemm_host_pointers(gpu):
     L.sbgemm_(b"N", b"T", i(m), i(n), i(k), f(1.5), A.ctypes.data_as(C.c_void_p), i(m + 2),
               B.ctypes.data_as(C.c_void_p), i(n), f(0.25), Cm.ctypes.data_as(C.c_void_p), i(m + 1))
     assert np.array_equal(Cm[:m], want[:m])
+
+
+@pytest.mark.parametrize("ta,tb", [(0, 0), (1, 1)])
+def test_bf16_large_tile_sampled_rows(gpu, ta, tb):
+    """A product large enough for the 256x256 tile (ragged n), checked on 96
+    sampled rows against the long-double oracle with the same bound."""
+    rng = np.random.default_rng(8)
+    m, n, k = 4096, 4160, 96
+    sa = (k, m) if ta else (m, k)
+    sb = (n, k) if tb else (k, n)
+    A = _bf16(rng, *sa, sa[0])
+    B = _bf16(rng, *sb, sb[0] + 1)
+    tA, pA = _dev16(A)
+    tB, pB = _dev16(B)
+    dC = _dev32(np.full((m, n), np.nan, np.float32))
+    E.gemm_bf16(ta, tb, m, n, k, 1.0, pA, A.shape[0], pB, B.shape[0], 0.0, dC, m)
+    got = from_dev(dC, (m, n)).astype(np.float64)
+    rows = np.sort(rng.choice(m, 96, replace=False))
+    Af = np.asfortranarray(_widen(A[: sa[0], : sa[1]]).reshape(sa))
+    Bf = np.asfortranarray(_widen(B[: sb[0], : sb[1]]).reshape(sb))
+    want, absprod = O.gemm_ref(bool(ta), bool(tb), 1.0, Af, Bf, rows=rows, with_abs=True)
+    assert np.isfinite(got).all()
+    assert np.linalg.norm(got[rows] - want) <= k * U32 * np.linalg.norm(absprod)
