@@ -285,3 +285,16 @@ def test_gemv_argument_checks():
     assert L.eh_gemv(E.F64, 0, 4, 3, a, 1, 4, 1, 0, a, 1, 1, None) == E.ERR_INVALID   # incx == 0
     assert L.eh_gemv(E.F64, 0, 4, 3, a, 1, 4, 1, 1, a, 1, 0, None) == E.ERR_INVALID   # incy == 0
     assert L.eh_gemv(E.F64, 0, 0, 3, a, None, 1, None, 1, a, None, 1, None) == E.SUCCESS  # m == 0
+
+
+def test_triangular_tile_enumeration(tmp_path):
+    """The compact grid of triangular updates (tri_tile_coords) covers every
+    tile on or below the diagonal exactly once, for all grids up to 300 x 300
+    tiles (host build of the device function)."""
+    exe = tmp_path / "tri_tiles_check"
+    src = os.path.join(ROOT, "tests", "tri_tiles_check.cpp")
+    r = subprocess.run(["/opt/rocm/bin/hipcc", "-O2", "-std=c++17", "-o", str(exe), src], capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0 and "ALL PASSED" in out.stdout, out.stdout
