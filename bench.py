@@ -194,6 +194,7 @@ def main() -> int:
         out["detail"]["level3_fp64"] = bench_level3(E, torch, dev, args)
         torch.cuda.empty_cache()
         out["detail"]["types"] = bench_types(E, torch, dev, args)
+        out["detail"]["batched_strided"] = bench_batched_strided(E, torch, dev, args)
         torch.cuda.empty_cache()
         out["detail"]["batched"] = bench_batched(E, torch, args)
     if not args.no_cpu:
@@ -258,6 +259,28 @@ def bench_level3(E, torch, dev, args):
         out[name] = {"ms": wall, "tflops": tf, "frac_of_fp64_peak": tf / PEAK["fp64"]}
         log(f"{name}: {out[name]}")
     del A, B, Cm
+    return out
+
+
+def bench_batched_strided(E, torch, dev, args):
+    """Arrays of fixed-size matrices through eh_gemm_batched_strided: 100,000
+    products of 4x4 FP32 and 20,000 of 16x16 FP64 (C = A*B, U[-1,1]);
+    algorithmic bytes = A, B read and C written once."""
+    out = {}
+    for name, dtype, tdt, es, size, count in (("f32_4x4x4", E.F32, torch.float32, 4, 4, 100000),
+                                              ("f64_16x16x16", E.F64, torch.float64, 8, 16, 20000)):
+        e = size * size
+        A = torch.rand(e * count, dtype=tdt, device=dev) * 2 - 1
+        B = torch.rand(e * count, dtype=tdt, device=dev) * 2 - 1
+        Cm = torch.zeros(e * count, dtype=tdt, device=dev)
+        fn = lambda: E.gemm_batched_strided(dtype, 0, 0, size, size, size, 1.0, A, size, e, B, size, e, 0.0, Cm, size,
+                                            e, count)
+        wall, kern = timed(fn, max(10, args.steps), args.warmup)
+        flop, nbytes = 2.0 * size ** 3 * count, 3.0 * e * es * count
+        out[name] = {"products": count, "ms": wall, "gflops": flop / (wall * 1e-3) / 1e9,
+                     "gbs": nbytes / (wall * 1e-3) / 1e9, "frac_of_hbm": nbytes / (wall * 1e-3) / 1e9 / HBM_PEAK_GBS}
+        del A, B, Cm
+    log(f"batched_strided: {out}")
     return out
 
 

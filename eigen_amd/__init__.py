@@ -130,6 +130,9 @@ def lib():
         for f in (L.eh_trmm, L.eh_trsm):
             f.restype = i32
             f.argtypes = [i32, i32, i32, i32, i32, i64, i64, vp, vp, i64, vp, i64, vp]
+        L.eh_gemm_batched_strided.restype = i32
+        L.eh_gemm_batched_strided.argtypes = [i32, i32, i32, i64, i64, i64, vp, vp, i64, i64, vp, i64, i64, vp, vp,
+                                              i64, i64, i64, vp]
         L.eh_gemv.restype = i32
         L.eh_gemv.argtypes = [i32, i32, i64, i64, vp, vp, i64, vp, i64, vp, vp, i64, vp]
         L.eh_gemv_complex.restype = i32
@@ -260,6 +263,17 @@ def syrk(dtype: int, uplo: int, trans: int, n: int, k: int, alpha: float, A, lda
     a, b = _scalar(dtype, alpha), _scalar(dtype, beta)
     s = current_stream() if stream is None else stream
     check(lib().eh_syrk(dtype, uplo, trans, n, k, C.byref(a), _ptr(A), lda, C.byref(b), _ptr(Cm), ldc, s), "eh_syrk")
+
+
+def gemm_batched_strided(dtype: int, opa: int, opb: int, m: int, n: int, k: int, alpha: float, A, lda: int,
+                         stride_a: int, B, ldb: int, stride_b: int, beta: float, Cm, ldc: int, stride_c: int,
+                         count: int, stream=None) -> None:
+    """`count` products of one shape at fixed element strides (eh_gemm_batched_strided)."""
+    a, b = _scalar(dtype, alpha), _scalar(dtype, beta)
+    s = current_stream() if stream is None else stream
+    check(lib().eh_gemm_batched_strided(dtype, opa, opb, m, n, k, C.byref(a), _ptr(A), lda, stride_a, _ptr(B), ldb,
+                                        stride_b, C.byref(b), _ptr(Cm), ldc, stride_c, count, s),
+          "eh_gemm_batched_strided")
 
 
 def gemv(dtype: int, trans: int, m: int, n: int, alpha: float, A, lda: int, x, incx: int, beta: float, y,

@@ -356,9 +356,189 @@ static int64_t tile_cost(const GroupedDesc& d) {
   return 2 + (d.k + step - 1) / step;
 }
 
+// Work list of `count` products of one shape at fixed byte strides, built on
+// the device: descriptor p is `base` with its operands advanced by p strides,
+// product p owns tiles [p*T, (p+1)*T), and with equal tile costs the W waves'
+// equal-cost runs start at tile floor(v * count * T / W).
+__global__ void __launch_bounds__(256) strided_build_kernel(GroupedDesc base, int64_t sa, int64_t sb, int64_t sc,
+                                                            int64_t count, int64_t T, int64_t W, GroupedDesc* d,
+                                                            int64_t* begin, int64_t* wave_tile, int32_t* wave_prod) {
+  const int64_t total = count * T;
+  const int64_t n = max(count + 1, W + 1);
+  for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    if (i < count) {
+      GroupedDesc g = base;
+      g.A = static_cast<const char*>(base.A) + i * sa;
+      g.B = static_cast<const char*>(base.B) + i * sb;
+      g.C = static_cast<char*>(base.C) + i * sc;
+      d[i] = g;
+    }
+    if (i <= count) begin[i] = i * T;
+    if (i <= W) wave_tile[i] = i == W ? total : (int64_t)((__int128)total * i / W);
+    if (i < W) {
+      const int64_t t = (int64_t)((__int128)total * i / W);
+      wave_prod[i] = (int32_t)min(t / T, count - 1);
+    }
+  }
+}
+
+// Products of at most 4 x 4 x 4 (Matrix2/3/4-sized), one thread each: the
+// operands of product p are loaded whole into registers (unconditional loads
+// clamped into the product, zeroed by selects, so no load waits on another),
+// C(i, j) = alpha * sum_k op(A)(i, k) op(B)(k, j) + beta * C(i, j) with the
+// k sum in order.  A 32 x 32 tile of the grouped kernel would spend 1/64 of
+// its work on such a product.
+template <typename T>
+__global__ void __launch_bounds__(256) tiny_batched_kernel(const T* __restrict__ A, int64_t lda, int64_t sa,
+                                                           const T* __restrict__ B, int64_t ldb, int64_t sb,
+                                                           T* __restrict__ C, int64_t ldc, int64_t sc, int m, int n,
+                                                           int k, int ta, int tb, T alpha, T beta, int64_t count) {
+  const int64_t p = blockIdx.x * 256ll + threadIdx.x;
+  if (p >= count) return;
+  T a[4][4], b[4][4];
+  const bool reads = k > 0 && alpha != T(0);
+  const T* Ap = A + p * sa;
+  const T* Bp = B + p * sb;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) {
+      const int ic = min(i, m - 1), kc = min(kk, max(k - 1, 0));
+      const T va = reads ? Ap[ta ? kc + (int64_t)ic * lda : ic + (int64_t)kc * lda] : T(0);
+      a[i][kk] = (i < m && kk < k) ? va : T(0);
+    }
+#pragma unroll
+  for (int kk = 0; kk < 4; ++kk)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int jc = min(j, n - 1), kc = min(kk, max(k - 1, 0));
+      const T vb = reads ? Bp[tb ? jc + (int64_t)kc * ldb : kc + (int64_t)jc * ldb] : T(0);
+      b[kk][j] = (j < n && kk < k) ? vb : T(0);
+    }
+  T* Cp = C + p * sc;
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      if (i >= m || j >= n) continue;
+      T acc = T(0);
+#pragma unroll
+      for (int kk = 0; kk < 4; ++kk) acc = fma_t(a[i][kk], b[kk][j], acc);
+      T* dst = Cp + i + (int64_t)j * ldc;
+      *dst = beta == T(0) ? alpha * acc : fma_t(alpha, acc, beta * *dst);
+    }
+}
+
 }  // namespace eh
 
 using namespace eh;
+
+extern "C" eh_status eh_gemm_batched_strided(eh_dtype dtype, eh_op opa, eh_op opb, int64_t m, int64_t n, int64_t k,
+                                             const void* alpha, const void* A, int64_t lda, int64_t stride_a,
+                                             const void* B, int64_t ldb, int64_t stride_b, const void* beta, void* C,
+                                             int64_t ldc, int64_t stride_c, int64_t count, void* stream) {
+  eh_status st = check_gemm_args(dtype, opa, opb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+  if (st != EH_SUCCESS) return st;
+  if (count < 0 || stride_a < 0 || stride_b < 0 || stride_c < 0)
+    return fail(EH_ERR_INVALID, "count and strides must be >= 0");
+  if (count > 1 && m > 0 && n > 0 && stride_c < (n - 1) * ldc + m)
+    return fail(EH_ERR_INVALID, "stride_c too small: results of different products would overlap");
+  if (m == 0 || n == 0 || count == 0) return EH_SUCCESS;
+  if (count > INT32_MAX) return fail(EH_ERR_UNSUPPORTED, "count must fit int32");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (m <= 4 && n <= 4 && k <= 4) {  // fixed-size products up to 4 x 4: one thread each
+    const unsigned blocks = (unsigned)((count + 255) / 256);
+    if (dtype == EH_F64)
+      hipLaunchKernelGGL(tiny_batched_kernel<double>, dim3(blocks), dim3(256), 0, s, static_cast<const double*>(A), lda,
+                         stride_a, static_cast<const double*>(B), ldb, stride_b, static_cast<double*>(C), ldc, stride_c,
+                         (int)m, (int)n, (int)k, (int)(opa != EH_OP_N), (int)(opb != EH_OP_N),
+                         *static_cast<const double*>(alpha), *static_cast<const double*>(beta), count);
+    else
+      hipLaunchKernelGGL(tiny_batched_kernel<float>, dim3(blocks), dim3(256), 0, s, static_cast<const float*>(A), lda,
+                         stride_a, static_cast<const float*>(B), ldb, stride_b, static_cast<float*>(C), ldc, stride_c,
+                         (int)m, (int)n, (int)k, (int)(opa != EH_OP_N), (int)(opb != EH_OP_N),
+                         *static_cast<const float*>(alpha), *static_cast<const float*>(beta), count);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? EH_SUCCESS : hip_fail(e, "tiny batched launch");
+  }
+  const int es = dtype == EH_F64 ? 8 : 4;
+  const double al = dtype == EH_F64 ? *static_cast<const double*>(alpha) : *static_cast<const float*>(alpha);
+  const double be = dtype == EH_F64 ? *static_cast<const double*>(beta) : *static_cast<const float*>(beta);
+  const bool ta = opa != EH_OP_N, tb = opb != EH_OP_N;
+  const bool reads = k > 0 && al != 0.0;
+  constexpr uint64_t kMaxBytes = 0x7FFFFFFFull;
+  const uint64_t ea = reads ? (ta ? extent_bytes(k, m, lda, es) : extent_bytes(m, k, lda, es)) : 0;
+  const uint64_t eb = reads ? (tb ? extent_bytes(n, k, ldb, es) : extent_bytes(k, n, ldb, es)) : 0;
+  const int64_t tm = (m + 31) / 32, tn = (n + 31) / 32;
+  if (ea > kMaxBytes || eb > kMaxBytes || m > (1 << 30) || n > (1 << 30) || tm * tn > (1ll << 30) ||
+      count * tm * tn > (1ll << 40)) {
+    // past the grouped kernel's 32-bit offsets: one product at a time
+    for (int64_t p = 0; p < count; ++p) {
+      st = gemm_dispatch(dtype, opa, opb, m, n, k, alpha, static_cast<const char*>(A) + p * stride_a * es, lda,
+                         static_cast<const char*>(B) + p * stride_b * es, ldb, beta,
+                         static_cast<char*>(C) + p * stride_c * es, ldc, EH_PREC_PLAIN, -1, stream);
+      if (st != EH_SUCCESS) return st;
+    }
+    return EH_SUCCESS;
+  }
+  GroupedDesc base{};
+  base.A = A;
+  base.B = B;
+  base.C = C;
+  base.ldc = ldc;
+  base.alpha = reads ? al : 0.0;
+  base.beta = be;
+  base.bytesA = (uint32_t)ea;
+  base.bytesB = (uint32_t)eb;
+  base.lda = (uint32_t)std::min<int64_t>(lda, kMaxBytes);
+  base.ldb = (uint32_t)std::min<int64_t>(ldb, kMaxBytes);
+  base.dtype = dtype;
+  base.ta = ta;
+  base.tb = tb;
+  base.m = (int32_t)m;
+  base.n = (int32_t)n;
+  base.k = reads ? (int32_t)k : 0;
+  base.tiles_m = (int32_t)tm;
+  const int64_t T = tm * tn, tiles = count * T;
+  // resident waves, as for eh_grouped_create (queried once per process)
+  static int64_t resident = 0;
+  if (resident == 0) {
+    int dev = 0, cus = 256, per_cu = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess &&
+        prop.multiProcessorCount > 0)
+      cus = prop.multiProcessorCount;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, grouped_gemm_kernel, 64 * kWavesPerGroup, 0) !=
+            hipSuccess ||
+        per_cu < 1)
+      per_cu = 1;
+    resident = (int64_t)cus * per_cu;
+  }
+  int64_t groups = std::min<int64_t>(resident, (tiles + kWavesPerGroup - 1) / kWavesPerGroup);
+  if (groups > 8) groups = (groups + 7) / 8 * 8;
+  groups = std::max<int64_t>(groups, 1);
+  const int64_t W = groups * kWavesPerGroup;
+  // work list in the per-stream workspace: descriptors, tile starts, wave runs
+  const size_t bd = (size_t)count * sizeof(GroupedDesc), bb = (size_t)(count + 1) * sizeof(int64_t),
+               bw = (size_t)(W + 1) * sizeof(int64_t), bp = (size_t)W * sizeof(int32_t);
+  void* ws = nullptr;
+  st = workspace_get(s, bd + bb + bw + bp, &ws);
+  if (st != EH_SUCCESS) return st;
+  char* p = static_cast<char*>(ws);
+  GroupedDesc* d = reinterpret_cast<GroupedDesc*>(p);
+  int64_t* begin = reinterpret_cast<int64_t*>(p + bd);
+  int64_t* wave_tile = reinterpret_cast<int64_t*>(p + bd + bb);
+  int32_t* wave_prod = reinterpret_cast<int32_t*>(p + bd + bb + bw);
+  const int64_t nb = std::min<int64_t>((std::max(count, W) + 256) / 256, 65536);
+  hipLaunchKernelGGL(strided_build_kernel, dim3((unsigned)nb), dim3(256), 0, s, base, stride_a * es, stride_b * es,
+                     stride_c * es, count, T, W, d, begin, wave_tile, wave_prod);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(e, "batched build launch");
+  hipLaunchKernelGGL(grouped_gemm_kernel, dim3((unsigned)groups), dim3(64 * kWavesPerGroup), 0, s, d, begin, wave_tile,
+                     wave_prod);
+  e = hipGetLastError();
+  return e == hipSuccess ? EH_SUCCESS : hip_fail(e, "grouped kernel launch");
+}
 
 struct eh_grouped {
   std::vector<eh_gemm_problem> host;   // as given (for the per-product path)

@@ -43,7 +43,7 @@
  *   eh_split_bf16
  *       float -> bfloat16 RTNE (Eigen/src/Core/arch/Default/BFloat16.h:434)
  *       applied term by term: the split used by the FP32-accurate BF16 path.
- *   eh_grouped_*
+ *   eh_grouped_*, eh_gemm_batched_strided
  *       no single reference function: Eigen runs many small products as a
  *       host loop of general_matrix_matrix_product / lazy products
  *       (Eigen/src/Core/ProductEvaluators.h:606-692); here one persistent
@@ -308,6 +308,22 @@ eh_status eh_grouped_run_per_product(eh_grouped* g, void* stream);
 /* Number of 32x32 output tiles and of non-empty products. */
 eh_status eh_grouped_info(eh_grouped* g, int64_t* tiles, int64_t* products);
 eh_status eh_grouped_destroy(eh_grouped* g);
+
+/* `count` products of one shape with operands at fixed element strides
+ * (product p: A + p*stride_a, B + p*stride_b, C + p*stride_c), e.g. arrays of
+ * fixed-size matrices: the batched form of Eigen's small and fixed-size
+ * products, which Eigen evaluates one at a time (coefficient-based below
+ * ProductEvaluators.h:606-692's threshold, GEBP above).  The work list is
+ * written on the device (one small kernel: descriptors and the equal-cost
+ * split of the count*tiles 32x32 tiles over the resident waves) and the
+ * grouped kernel runs it: two launches, no host loop, no allocation beyond
+ * the per-stream workspace.  Results must not overlap
+ * (stride_c >= (n-1)*ldc + m when count > 1).  Products whose operands span
+ * 2 GB or more run one product at a time. */
+eh_status eh_gemm_batched_strided(eh_dtype dtype, eh_op opa, eh_op opb, int64_t m, int64_t n, int64_t k,
+                                  const void* alpha, const void* A, int64_t lda, int64_t stride_a, const void* B,
+                                  int64_t ldb, int64_t stride_b, const void* beta, void* C, int64_t ldc,
+                                  int64_t stride_c, int64_t count, void* stream);
 
 /* ---- autotuner ------------------------------------------------------------- */
 

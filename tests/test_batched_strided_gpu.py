@@ -1,0 +1,110 @@
+"""Strided batched products on the grouped kernel (eh_gemm_batched_strided).
+
+Bars: bit-exact on small-integer inputs (every partial sum is exact) for every
+op combination, fixed-size 4x4 products by the tens of thousands (the
+one-thread-per-product path for shapes up to 4x4x4), ragged
+shapes with padded leading dimensions and gaps between products (which must
+stay untouched); alpha == 0 does not read A or B, beta == 0 does not read C;
+random inputs within the FP32 product bound k u || |A||B| ||_F per product."""
+import itertools
+
+import numpy as np
+import pytest
+
+import eigen_amd as E
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+ED = {np.float32: E.F32, np.float64: E.F64}
+
+
+def _dev(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a)).to("cuda:0")
+
+
+def _host(t):
+    import torch
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def _batch(rng, count, rows, cols, ld, stride, dt, ints=True):
+    """Flat buffer holding `count` column-major rows x cols matrices (leading
+    dimension ld) at `stride` elements apart; NaN everywhere else."""
+    buf = np.full(stride * (count - 1) + ld * (cols - 1) + rows if count else 1, np.nan, dtype=dt)
+    mats = []
+    for p in range(count):
+        m = rng.integers(-8, 9, (rows, cols)).astype(dt) if ints else rng.uniform(-1, 1, (rows, cols)).astype(dt)
+        for j in range(cols):
+            buf[p * stride + j * ld: p * stride + j * ld + rows] = m[:, j]
+        mats.append(m)
+    return buf, mats
+
+
+def _unpack(buf, p, rows, cols, ld, stride):
+    return np.stack([buf[p * stride + j * ld: p * stride + j * ld + rows] for j in range(cols)], axis=1)
+
+
+# every op combination on ragged shapes; the 20,000-product 4x4 batch (fixed-size
+# matrices) with op N, N
+CASES = [(ta, tb, m, n, k, count) for ta, tb in itertools.product((0, 1), (0, 1))
+         for (m, n, k, count) in ((33, 17, 50, 100), (1, 1, 1, 5), (70, 40, 3, 7), (3, 2, 4, 1000))] + \
+    [(0, 0, 4, 4, 4, 20000)]
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+@pytest.mark.parametrize("ta,tb,m,n,k,count", CASES)
+def test_batched_strided_exact(gpu, dt, ta, tb, m, n, k, count):
+    rng = np.random.default_rng(m + n + k + count)
+    ra, ca = (k, m) if ta else (m, k)
+    rb, cb = (n, k) if tb else (k, n)
+    lda, ldb, ldc = ra + 1, rb, m + 2
+    sa, sb, sc = lda * ca + 3, ldb * cb, ldc * n + 5
+    A, am = _batch(rng, count, ra, ca, lda, sa, dt)
+    B, bm = _batch(rng, count, rb, cb, ldb, sb, dt)
+    C0, cm = _batch(rng, count, m, n, ldc, sc, dt)
+    dC = _dev(C0)
+    E.gemm_batched_strided(ED[dt], ta, tb, m, n, k, 2.0, _dev(A), lda, sa, _dev(B), ldb, sb, -1.0, dC, ldc, sc, count)
+    got = _host(dC)
+    assert np.array_equal(np.isnan(got), np.isnan(C0)), "elements outside the products written"
+    for p in range(0, count, max(1, count // 200)):
+        a = am[p].astype(np.int64)
+        b = bm[p].astype(np.int64)
+        want = 2 * ((a.T if ta else a) @ (b.T if tb else b)) - cm[p].astype(np.int64)
+        assert np.array_equal(_unpack(got, p, m, n, ldc, sc), want.astype(dt)), p
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+@pytest.mark.parametrize("m,n,k,count", [(20, 12, 9, 30), (4, 3, 2, 300)])  # grouped kernel; one thread per product
+def test_batched_strided_alpha_beta_zero(gpu, dt, m, n, k, count):
+    rng = np.random.default_rng(3)
+    C0, cm = _batch(rng, count, m, n, m, m * n, dt)
+    nan = np.full(m * k * count + 8, np.nan, dt)
+    dC = _dev(C0)
+    E.gemm_batched_strided(ED[dt], 0, 0, m, n, k, 0.0, _dev(nan), m, m * k, _dev(nan), k, k * n, 0.5, dC, m, m * n,
+                           count)
+    assert np.array_equal(_host(dC), 0.5 * C0)
+    A, am = _batch(rng, count, m, k, m, m * k, dt)
+    B, bm = _batch(rng, count, k, n, k, k * n, dt)
+    dC = _dev(np.full(m * n * count, np.nan, dt))
+    E.gemm_batched_strided(ED[dt], 0, 0, m, n, k, 1.0, _dev(A), m, m * k, _dev(B), k, k * n, 0.0, dC, m, m * n, count)
+    got = _host(dC)
+    for p in range(count):
+        assert np.array_equal(_unpack(got, p, m, n, m, m * n), (am[p].astype(np.int64) @ bm[p].astype(np.int64)).astype(dt))
+
+
+def test_batched_strided_random_bound(gpu):
+    rng = np.random.default_rng(4)
+    m, n, k, count = 100, 90, 500, 40
+    A, am = _batch(rng, count, m, k, m, m * k, np.float32, ints=False)
+    B, bm = _batch(rng, count, k, n, k, k * n, np.float32, ints=False)
+    dC = _dev(np.zeros(m * n * count, np.float32))
+    E.gemm_batched_strided(E.F32, 0, 0, m, n, k, 1.0, _dev(A), m, m * k, _dev(B), k, k * n, 0.0, dC, m, m * n, count)
+    got = _host(dC)
+    for p in range(0, count, 7):
+        want, absprod = O.gemm_ref(False, False, 1.0, np.asfortranarray(am[p]), np.asfortranarray(bm[p]),
+                                   with_abs=True)
+        err = np.linalg.norm(_unpack(got, p, m, n, m, m * n).astype(np.float64) - want)
+        assert err <= k * 2.0 ** -24 * np.linalg.norm(absprod)

@@ -298,3 +298,14 @@ def test_triangular_tile_enumeration(tmp_path):
     assert r.returncode == 0, r.stderr
     out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=300)
     assert out.returncode == 0 and "ALL PASSED" in out.stdout, out.stdout
+
+
+def test_batched_strided_argument_checks():
+    L = E.lib()
+    one = C.c_double(1.0)
+    a = C.byref(one)
+    # (dtype, opa, opb, m, n, k, alpha, A, lda, sa, B, ldb, sb, beta, C, ldc, sc, count, stream)
+    assert L.eh_gemm_batched_strided(E.F64, 0, 0, 4, 3, 2, a, 1, 4, 8, 1, 2, 6, a, 1, 4, 11, 2, None) == E.ERR_INVALID
+    assert L.eh_gemm_batched_strided(E.F64, 0, 0, 4, 3, 2, a, 1, 4, -1, 1, 2, 6, a, 1, 4, 12, 2, None) == E.ERR_INVALID
+    assert L.eh_gemm_batched_strided(E.F64, 0, 0, 4, 3, 2, a, 1, 3, 8, 1, 2, 6, a, 1, 4, 12, 2, None) == E.ERR_INVALID
+    assert L.eh_gemm_batched_strided(E.F64, 0, 0, 4, 3, 2, a, 1, 4, 8, 1, 2, 6, a, 1, 4, 12, 0, None) == E.SUCCESS
