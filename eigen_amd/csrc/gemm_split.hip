@@ -315,7 +315,11 @@ __global__ void __launch_bounds__(256) split_bf16_kernel(const float* x, int64_t
 // one-term split planes, so the product kernel is shared.  Loads are 2-byte
 // buffer loads by consecutive lanes along the contiguous index (any element
 // alignment is valid for bf16 operands).
-template <bool KCONT>
+// WIDE: 16-byte loads of 8 elements along the contiguous index, valid when
+// every chunk starts on a 4-byte boundary (even element offset: an even
+// leading dimension and a 4-byte aligned base) and the operand ends on one
+// (the range check is per dword); otherwise 2-byte loads.
+template <bool KCONT, bool WIDE>
 __global__ void __launch_bounds__(256) pack_bf16_kernel(const unsigned short* __restrict__ X, int64_t ld, int rows,
                                                         int K, int Rp, int KS, u16x8* __restrict__ P) {
   constexpr int TR = 64, TK = 64;
@@ -327,14 +331,31 @@ __global__ void __launch_bounds__(256) pack_bf16_kernel(const unsigned short* __
   const uint64_t left = extent > (uint64_t)base_el * 2 ? extent - (uint64_t)base_el * 2 : 0;
   const __amdgpu_buffer_rsrc_t rs =
       make_rsrc(X + base_el, (uint32_t)(left < 0xFFFFFFF0ull ? left : 0xFFFFFFF0ull));
+  if constexpr (WIDE) {
 #pragma unroll
-  for (int i = 0; i < TR * TK / 256; ++i) {
-    const int q = tid + 256 * i;
-    const int r = KCONT ? q / TK : q % TR;  // lanes walk the contiguous index
-    const int k = KCONT ? q % TK : q / TR;
-    const uint32_t off = (uint32_t)((KCONT ? (int64_t)r * ld + k : r + (int64_t)k * ld) * 2);
-    const unsigned short v = __builtin_amdgcn_raw_buffer_load_b16(rs, (int)off, 0, 0);
-    tile[r][k] = (r0 + r < rows && k0 + k < K) ? v : (unsigned short)0;
+    for (int i = 0; i < TR * TK / (256 * 8); ++i) {
+      const int q = tid + 256 * i;  // one 8-element chunk
+      const int r = KCONT ? q / (TK / 8) : (q % (TR / 8)) * 8;
+      const int k = KCONT ? (q % (TK / 8)) * 8 : q / (TR / 8);
+      const uint32_t off = (uint32_t)((KCONT ? (int64_t)r * ld + k : r + (int64_t)k * ld) * 2);
+      const u32x4 v = buf_load16(rs, off, 0);
+      const unsigned short* e8 = reinterpret_cast<const unsigned short*>(&v);
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int rr = KCONT ? r : r + e, kk = KCONT ? k + e : k;
+        tile[rr][kk] = (r0 + rr < rows && k0 + kk < K) ? e8[e] : (unsigned short)0;
+      }
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < TR * TK / 256; ++i) {
+      const int q = tid + 256 * i;
+      const int r = KCONT ? q / TK : q % TR;  // lanes walk the contiguous index
+      const int k = KCONT ? q % TK : q / TR;
+      const uint32_t off = (uint32_t)((KCONT ? (int64_t)r * ld + k : r + (int64_t)k * ld) * 2);
+      const unsigned short v = __builtin_amdgcn_raw_buffer_load_b16(rs, (int)off, 0, 0);
+      tile[r][k] = (r0 + r < rows && k0 + k < K) ? v : (unsigned short)0;
+    }
   }
   __syncthreads();
 #pragma unroll
@@ -354,11 +375,19 @@ hipError_t launch_pack_bf16(const unsigned short* X, int64_t ld, bool kcont, int
                             hipStream_t s) {
   const int Rp = split_rows_padded(rows), KS = split_kslices(K);
   dim3 grid(Rp / 64, (K + 63) / 64);
-  if (kcont)
-    hipLaunchKernelGGL(pack_bf16_kernel<true>, grid, dim3(256), 0, s, X, ld, rows, K, Rp, KS, static_cast<u16x8*>(P));
+  // the buffer range check works per dword: the operand must also end on a
+  // dword boundary (an even count of contiguous elements per row or column)
+  // so that the dword holding its last element is inside the range
+  const bool wide = (reinterpret_cast<uintptr_t>(X) % 4) == 0 && ld % 2 == 0 && (kcont ? K % 2 == 0 : rows % 2 == 0);
+  auto* out = static_cast<u16x8*>(P);
+  if (kcont && wide)
+    hipLaunchKernelGGL((pack_bf16_kernel<true, true>), grid, dim3(256), 0, s, X, ld, rows, K, Rp, KS, out);
+  else if (kcont)
+    hipLaunchKernelGGL((pack_bf16_kernel<true, false>), grid, dim3(256), 0, s, X, ld, rows, K, Rp, KS, out);
+  else if (wide)
+    hipLaunchKernelGGL((pack_bf16_kernel<false, true>), grid, dim3(256), 0, s, X, ld, rows, K, Rp, KS, out);
   else
-    hipLaunchKernelGGL(pack_bf16_kernel<false>, grid, dim3(256), 0, s, X, ld, rows, K, Rp, KS,
-                       static_cast<u16x8*>(P));
+    hipLaunchKernelGGL((pack_bf16_kernel<false, false>), grid, dim3(256), 0, s, X, ld, rows, K, Rp, KS, out);
   return hipGetLastError();
 }
 

@@ -500,9 +500,9 @@ extern "C" eh_status eh_gemm_batched_strided(eh_dtype dtype, eh_op opa, eh_op op
   base.k = reads ? (int32_t)k : 0;
   base.tiles_m = (int32_t)tm;
   const int64_t T = tm * tn, tiles = count * T;
-  // resident waves, as for eh_grouped_create (queried once per process)
-  static int64_t resident = 0;
-  if (resident == 0) {
+  // resident workgroups, as for eh_grouped_create (queried once per process;
+  // the initialisation of a function-local static is thread-safe)
+  static const int64_t resident = [] {
     int dev = 0, cus = 256, per_cu = 0;
     hipDeviceProp_t prop;
     if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess &&
@@ -512,8 +512,8 @@ extern "C" eh_status eh_gemm_batched_strided(eh_dtype dtype, eh_op opa, eh_op op
             hipSuccess ||
         per_cu < 1)
       per_cu = 1;
-    resident = (int64_t)cus * per_cu;
-  }
+    return (int64_t)cus * per_cu;
+  }();
   int64_t groups = std::min<int64_t>(resident, (tiles + kWavesPerGroup - 1) / kWavesPerGroup);
   if (groups > 8) groups = (groups + 7) / 8 * 8;
   groups = std::max<int64_t>(groups, 1);

@@ -1,6 +1,6 @@
 """Short driver for profiler runs, and the summariser of their output.
 
-    python3 -m eigen_amd.profile_kernels run {split3,plain,dgemm,batched,dtrsm,zgemm,copy} [reps]
+    python3 -m eigen_amd.profile_kernels run {split3,plain,dgemm,batched,dtrsm,zgemm,bf16,copy} [reps]
     python3 -m eigen_amd.profile_kernels summarize <rocprofv3 output root>
 
 `run` launches one workload a few times (the program under
@@ -12,6 +12,7 @@
   batched  20,000 uneven products in one grouped launch (captured graph)
   dtrsm    dtrsm 8192 x 8192 (left, lower, no transpose), diagonal 2, off-diagonal U[-1/n, 1/n]
   zgemm    zgemm 4096^3 (complex128, one real product over 2x2 real blocks)
+  bf16     bf16 inputs, FP32 result, 8192^3 (U[-1,1] rounded to bf16)
   copy     a 1 GiB device-to-device copy: calibrates the HBM byte counters
 
 `summarize` reads directories named trace_<workload> (kernel-trace csv),
@@ -34,7 +35,7 @@ COPY_BYTES = 1 << 30
 # dominant kernel of each workload (substring of its name)
 DOMINANT = {"split3": "gemm_split_packed_kernel", "plain": "gemm_mfma_kernel<float", "dgemm": "gemm_mfma_kernel<double",
             "batched": "grouped_gemm_kernel", "dtrsm": "gemm_mfma_kernel<double", "zgemm": "gemm_mfma_kernel<double",
-            "copy": "copyBuffer"}
+            "bf16": "gemm_split_packed_kernel", "copy": "copyBuffer"}
 
 
 def run(what: str, reps: int) -> None:
@@ -80,6 +81,13 @@ def run(what: str, reps: int) -> None:
         A, B, Cm = (torch.rand(2 * n * n, dtype=torch.float64, device="cuda") * 2 - 1 for _ in range(3))
         for _ in range(reps):
             E.gemm_complex(E.F64, 0, 0, n, n, n, 1.0 + 0.5j, A, n, B, n, 0.5, Cm, n)
+    elif what == "bf16":
+        n = 8192
+        A = (torch.rand(n * n, device="cuda") * 2 - 1).to(torch.bfloat16)
+        B = (torch.rand(n * n, device="cuda") * 2 - 1).to(torch.bfloat16)
+        Cm = torch.zeros(n * n, device="cuda")
+        for _ in range(reps):
+            E.gemm_bf16(0, 0, n, n, n, 1.0, A.data_ptr(), n, B.data_ptr(), n, 0.0, Cm, n)
     elif what == "copy":
         x = torch.empty(COPY_BYTES // 4, dtype=torch.float32, device="cuda")
         y = torch.empty_like(x)
