@@ -20,22 +20,6 @@
 namespace eh {
 namespace {
 
-struct Temp {
-  void* ptr = nullptr;
-  hipStream_t s = nullptr;
-  eh_status get(size_t bytes, hipStream_t stream) {
-    s = stream;
-    hipError_t e = hipMallocAsync(&ptr, std::max<size_t>(bytes, 256), stream);
-    if (e != hipSuccess) {
-      ptr = nullptr;
-      return fail(EH_ERR_ALLOC, std::string("complex product temporary: ") + hipGetErrorString(e));
-    }
-    return EH_SUCCESS;
-  }
-  ~Temp() {
-    if (ptr) (void)hipFreeAsync(ptr, s);
-  }
-};
 
 template <typename T>
 eh_status run(int dtype, int opa, int opb, int64_t m, int64_t n, int64_t k, const T* alpha, const T* A, int64_t lda,
@@ -52,9 +36,9 @@ eh_status run(int dtype, int opa, int opb, int64_t m, int64_t n, int64_t k, cons
     beff = T(1);
   }
   const bool pack_b = opb != EH_OP_N;
-  Temp t;
+  StreamTemp t;
   const size_t a3 = (size_t)4 * m * k, b3 = pack_b ? (size_t)2 * k * n : 0;
-  eh_status st = t.get((a3 + b3) * sizeof(T), s);
+  eh_status st = t.get((a3 + b3) * sizeof(T), s, "complex product temporary");
   if (st != EH_SUCCESS) return st;
   T* A3 = static_cast<T*>(t.ptr);
   hipError_t e = launch_complex_pack_a<T>(A, lda, (int)m, (int)k, opa != EH_OP_N, opa == EH_OP_C, ar, ai, A3, 2 * m, s);
@@ -87,8 +71,8 @@ eh_status gemv_run(int dtype, int op, int64_t m, int64_t n, const T* alpha, cons
   }
   const int opc = op == EH_OP_N ? 0 : (op == EH_OP_T ? 1 : 2);
   const int64_t ldx = op == EH_OP_N ? n : 2 * m, ldp = op == EH_OP_N ? 2 * m : n;
-  Temp t;
-  eh_status st = t.get((size_t)(2 * ldx + 2 * ldp) * sizeof(T), s);
+  StreamTemp t;
+  eh_status st = t.get((size_t)(2 * ldx + 2 * ldp) * sizeof(T), s, "complex product temporary");
   if (st != EH_SUCCESS) return st;
   T* X = static_cast<T*>(t.ptr);
   T* P = X + 2 * ldx;

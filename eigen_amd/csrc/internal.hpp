@@ -26,6 +26,29 @@ eh_tune_mode tune_mode();
 eh_status plan_config(int dtype, int opa, int opb, int64_t m, int64_t n, int64_t k, int* config);
 
 eh_status workspace_get(hipStream_t stream, size_t bytes, void** out);
+
+// A stream-ordered temporary (hipMallocAsync) released by hipFreeAsync on the
+// same stream when the owner goes out of scope, i.e. after the work queued
+// on that stream that uses it.
+struct StreamTemp {
+  void* ptr = nullptr;
+  hipStream_t s = nullptr;
+  eh_status get(size_t bytes, hipStream_t stream, const char* what) {
+    s = stream;
+    hipError_t e = hipMallocAsync(&ptr, bytes < 256 ? 256 : bytes, stream);
+    if (e != hipSuccess) {
+      ptr = nullptr;
+      return fail(EH_ERR_ALLOC, std::string(what) + ": " + hipGetErrorString(e));
+    }
+    return EH_SUCCESS;
+  }
+  StreamTemp() = default;
+  StreamTemp(const StreamTemp&) = delete;
+  StreamTemp& operator=(const StreamTemp&) = delete;
+  ~StreamTemp() {
+    if (ptr) (void)hipFreeAsync(ptr, s);
+  }
+};
 eh_status check_gemm_args(int dtype, int opa, int opb, int64_t m, int64_t n, int64_t k, const void* alpha,
                           const void* A, int64_t lda, const void* B, int64_t ldb, const void* beta, const void* C,
                           int64_t ldc);

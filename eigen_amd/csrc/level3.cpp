@@ -30,24 +30,6 @@ namespace {
 bool valid_op(int op) { return op == EH_OP_N || op == EH_OP_T || op == EH_OP_C; }
 bool valid_dtype(int d) { return d == EH_F32 || d == EH_F64; }
 
-// Stream-ordered temporary, released on scope exit (after the work queued
-// on the stream that uses it).
-struct Temp {
-  void* ptr = nullptr;
-  hipStream_t s = nullptr;
-  eh_status get(size_t bytes, hipStream_t stream) {
-    s = stream;
-    hipError_t e = hipMallocAsync(&ptr, std::max<size_t>(bytes, 256), stream);
-    if (e != hipSuccess) {
-      ptr = nullptr;
-      return fail(EH_ERR_ALLOC, std::string("level-3 temporary: ") + hipGetErrorString(e));
-    }
-    return EH_SUCCESS;
-  }
-  ~Temp() {
-    if (ptr) (void)hipFreeAsync(ptr, s);
-  }
-};
 
 #define EH_TRY(x)                         \
   do {                                    \
@@ -245,8 +227,8 @@ eh_status trsm_t(int side, int uplo, int transa, int diag, int64_t m, int64_t n,
   const int64_t blocks = (ka + kTrsmLeaf - 1) / kTrsmLeaf;
   const int64_t inv = blocks * kTrsmLeaf * kTrsmLeaf;
   const int64_t copy = side == EH_LEFT ? kTrsmLeaf * n : m * kTrsmLeaf;
-  Temp t;
-  EH_TRY(t.get((size_t)(inv + copy) * sizeof(T), s));
+  StreamTemp t;
+  EH_TRY(t.get((size_t)(inv + copy) * sizeof(T), s, "level-3 temporary"));
   const TrsmWork<T> w{static_cast<T*>(t.ptr), static_cast<T*>(t.ptr) + inv};
   const TrtriParams<T> tp{A, lda, (int)E.ta, (int)E.lower, (int)E.unit, (int)ka, static_cast<T*>(t.ptr)};
   hipError_t e = launch_trtri<T>(tp, s);
@@ -265,8 +247,8 @@ eh_status trmm_t(int side, int uplo, int transa, int diag, int64_t m, int64_t n,
   const int64_t ka = side == EH_LEFT ? m : n;
   const int64_t leaf = std::min(ka, kTrmmLeaf);
   const int64_t copy = side == EH_LEFT ? leaf * n : m * leaf;
-  Temp t;
-  EH_TRY(t.get((size_t)(leaf * leaf + copy) * sizeof(T), s));
+  StreamTemp t;
+  EH_TRY(t.get((size_t)(leaf * leaf + copy) * sizeof(T), s, "level-3 temporary"));
   const TrmmWork<T> w{static_cast<T*>(t.ptr), static_cast<T*>(t.ptr) + leaf * leaf};
   return side == EH_LEFT ? trmm_left<T>(E, 0, m, B, ldb, n, alpha, w, s)
                          : trmm_right<T>(E, 0, n, B, ldb, m, alpha, w, s);
@@ -279,8 +261,8 @@ eh_status symm_t(int side, int uplo, int64_t m, int64_t n, T alpha, const T* A, 
   if (alpha == T(0))  // only beta*C (A and B are not read)
     return device_product<T>(EH_OP_N, EH_OP_N, m, n, 0, alpha, A, lda, B, ldb, beta, C, ldc, s);
   if (!A || !B) return fail(EH_ERR_INVALID, "A or B is NULL");
-  Temp t;
-  EH_TRY(t.get((size_t)ka * ka * sizeof(T), s));
+  StreamTemp t;
+  EH_TRY(t.get((size_t)ka * ka * sizeof(T), s, "level-3 temporary"));
   T* D = static_cast<T*>(t.ptr);
   hipError_t e = launch_tri_dense<T>(A, lda, (int)ka, D, ka, uplo == EH_LOWER, true, false, s);
   if (e != hipSuccess) return hip_fail(e, "symmetric expansion launch");
