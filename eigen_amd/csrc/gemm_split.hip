@@ -79,7 +79,12 @@ __global__ void __launch_bounds__(256) pack_split_kernel(const float* __restrict
   // row, 64 rows contiguous per plane and k slice).
   constexpr int TR = 64, TK = 64;
   __shared__ float tile[TR][TK + 1];
-  const int r0 = blockIdx.x * TR, k0 = blockIdx.y * TK;
+  // consecutive workgroups walk the contiguous index of X, so the ones running
+  // together read neighbouring segments: a 2-D grid (row tiles in x) when rows
+  // are contiguous, a 1-D grid with the k tiles fastest when k is
+  const int ktiles = (K + TK - 1) / TK;
+  const int r0 = (KCONT ? blockIdx.x / ktiles : blockIdx.x) * TR;
+  const int k0 = (KCONT ? blockIdx.x % ktiles : blockIdx.y) * TK;
   const int tid = threadIdx.x;
   // 16-byte loads of 4 consecutive elements along the contiguous index (a
   // buffer load needs 4-byte alignment only); reads past the operand return
@@ -324,7 +329,9 @@ __global__ void __launch_bounds__(256) pack_bf16_kernel(const unsigned short* __
                                                         int K, int Rp, int KS, u16x8* __restrict__ P) {
   constexpr int TR = 64, TK = 64;
   __shared__ unsigned short tile[TR][TK + 2];
-  const int r0 = blockIdx.x * TR, k0 = blockIdx.y * TK;
+  const int ktiles = (K + TK - 1) / TK;  // grid shapes as in pack_split_kernel
+  const int r0 = (KCONT ? blockIdx.x / ktiles : blockIdx.x) * TR;
+  const int k0 = (KCONT ? blockIdx.x % ktiles : blockIdx.y) * TK;
   const int tid = threadIdx.x;
   const uint64_t extent = KCONT ? ((uint64_t)(rows - 1) * ld + K) * 2 : ((uint64_t)(K - 1) * ld + rows) * 2;
   const int64_t base_el = KCONT ? (int64_t)r0 * ld + k0 : r0 + (int64_t)k0 * ld;
@@ -380,10 +387,11 @@ hipError_t launch_pack_bf16(const unsigned short* X, int64_t ld, bool kcont, int
   // so that the dword holding its last element is inside the range
   const bool wide = (reinterpret_cast<uintptr_t>(X) % 4) == 0 && ld % 2 == 0 && (kcont ? K % 2 == 0 : rows % 2 == 0);
   auto* out = static_cast<u16x8*>(P);
+  const dim3 kgrid(grid.x * grid.y);  // k-contiguous operands: 1-D, k tiles fastest
   if (kcont && wide)
-    hipLaunchKernelGGL((pack_bf16_kernel<true, true>), grid, dim3(256), 0, s, X, ld, rows, K, Rp, KS, out);
+    hipLaunchKernelGGL((pack_bf16_kernel<true, true>), kgrid, dim3(256), 0, s, X, ld, rows, K, Rp, KS, out);
   else if (kcont)
-    hipLaunchKernelGGL((pack_bf16_kernel<true, false>), grid, dim3(256), 0, s, X, ld, rows, K, Rp, KS, out);
+    hipLaunchKernelGGL((pack_bf16_kernel<true, false>), kgrid, dim3(256), 0, s, X, ld, rows, K, Rp, KS, out);
   else if (wide)
     hipLaunchKernelGGL((pack_bf16_kernel<false, true>), grid, dim3(256), 0, s, X, ld, rows, K, Rp, KS, out);
   else
@@ -396,7 +404,7 @@ static hipError_t launch_pack_t(const float* X, int64_t ld, bool kcont, int rows
                                 hipStream_t s) {
   dim3 grid(Rp / 64, (K + 63) / 64);
   if (kcont)
-    hipLaunchKernelGGL((pack_split_kernel<TERMS, true>), grid, dim3(256), 0, s, X, ld, rows, K, Rp, KS,
+    hipLaunchKernelGGL((pack_split_kernel<TERMS, true>), dim3(grid.x * grid.y), dim3(256), 0, s, X, ld, rows, K, Rp, KS,
                        static_cast<u16x8*>(P));
   else
     hipLaunchKernelGGL((pack_split_kernel<TERMS, false>), grid, dim3(256), 0, s, X, ld, rows, K, Rp, KS,
