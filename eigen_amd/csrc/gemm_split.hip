@@ -444,11 +444,13 @@ hipError_t launch_gemm_split_packed(const SplitParams& p, int terms, int tile, i
   if (terms == 1) {
     // bf16 inputs: one product per slice, so the LDS reads per MFMA decide
     // the rate.  tile 0: 128x128, 4 waves of 64x64 (4 stages); tile 1:
-    // 256x256, 4 waves of 128x128 (accumulators in AGPRs, 2 stages): half the
-    // LDS reads per MFMA, 814 vs 637 TFLOP/s at 8192^3 (the 256x128 8-wave
-    // tile), 659 vs 606 at 4096^3; below that the 128x128 tile wins.
+    // 256x256, 8 waves of 128x64 (2 stages, two waves per SIMD): 3/4 of the
+    // LDS reads per MFMA of 64x64 wave tiles, 903 TFLOP/s at 8192^3 against
+    // 637 for the 256x128 8-wave tile and 866 for 4 waves of 128x128 (one
+    // wave per SIMD, accumulators in AGPRs); 814 vs 776 at 4096^3.  Below
+    // 256 tiles of 256x256 the 128x128 tile wins.
     if (tile == 0) return launch_packed_t<128, 128, 2, 2, 1, 4>(p, splits, s);
-    return launch_packed_t<256, 256, 2, 2, 1, 2>(p, splits, s);
+    return launch_packed_t<256, 256, 2, 4, 1, 2>(p, splits, s);
   }
   return hipErrorInvalidValue;
 }
