@@ -141,8 +141,18 @@ __global__ void __launch_bounds__(256) gemv_t_kernel(GemvParams<T> p) {
 template <typename T>
 __global__ void __launch_bounds__(256) gemv_reduce_kernel(GemvParams<T> p, int splits) {
   for (int i = blockIdx.x * 256 + threadIdx.x; i < p.m; i += gridDim.x * 256) {
+    // partials summed in split order; loads issued eight at a time so that a
+    // long split list costs splits/8 memory latencies, not splits
     T s = T(0);
-    for (int z = 0; z < splits; ++z) s += p.W[(int64_t)z * p.m + i];
+    int z = 0;
+    for (; z + 8 <= splits; z += 8) {
+      T v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = p.W[(int64_t)(z + u) * p.m + i];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) s += v[u];
+    }
+    for (; z < splits; ++z) s += p.W[(int64_t)z * p.m + i];
     T* dst = p.y + (int64_t)i * p.incy;
     *dst = (p.beta == T(0)) ? p.alpha * s : fma_t(p.alpha, s, p.beta * *dst);
   }
