@@ -54,20 +54,66 @@ def timed(fn, steps: int, warmup: int):
     return wall, ev0.elapsed_time(ev1) / steps
 
 
-def fp64_rows_error(dA, dB, dC, m, n, k, rows, alpha=1.0, beta=0.0, C0=None, dtype=np.float32):
-    """Normwise error ||C - C_ref||_F / (K u ||A||_F ||B||_F) on sampled rows,
-    C_ref computed on the host in FP64 (numpy) from the same device inputs."""
-    A = dA.cpu().numpy().astype(np.float64).reshape((k, m)).T  # column-major m x k
-    B = dB.cpu().numpy().astype(np.float64).reshape((n, k)).T
-    Cg = dC.cpu().numpy().astype(np.float64).reshape((n, m)).T
-    ref = alpha * (A[rows] @ B)
-    if C0 is not None and beta != 0.0:
-        ref += beta * C0[rows]
+def exact_product(A, B):
+    """A @ B for float64 (or narrower) operands, accurate to ~2^-60 relative.
+
+    Each operand is cut into four slices of 16 significant bits relative to
+    the largest magnitude of its row (A) or column (B), so every slice product
+    (at most 34 bits) and every partial sum of a k-term dot product of slices
+    (k <= 2^17) is an exact float64; each of the 16 slice products is
+    therefore computed exactly by float64 BLAS in any summation order, and the
+    exact results are summed in extended precision (Ozaki, Ogita, Oishi &
+    Rump, "Error-free transformations of matrix multiplication", Numer.
+    Algorithms 59, 2012).  What the four slices leave of an operand is below
+    2^-64 of its row's (column's) largest entry."""
+    A = np.asarray(A, np.float64)
+    B = np.asarray(B, np.float64)
+    k = A.shape[1]
+    assert k <= 1 << 17
+
+    def slices(X, axis):
+        mx = np.max(np.abs(X), axis=axis, keepdims=True)
+        e = np.where(mx > 0, np.ceil(np.log2(np.where(mx > 0, mx, 1.0))), 0.0)
+        out, R = [], X.copy()
+        for s_ in range(4):
+            # adding 1.5 * 2^q rounds R to a multiple of 2^(q - 52) = 2^(e - 16(s+1))
+            sigma = np.ldexp(1.5, (e + 52 - 16 * (s_ + 1)).astype(np.int64))
+            hi = (R + sigma) - sigma
+            out.append(hi)
+            R = R - hi
+        return out
+    sa, sb = slices(A, 1), slices(B, 0)
+    acc = np.zeros((A.shape[0], B.shape[1]), np.longdouble)
+    for a in sa:
+        for b in sb:
+            acc += (a @ b).astype(np.longdouble)
+    return acc
+
+
+def normwise(got, ref, k, u, normA, normB, scale=1.0):
+    """||C - C_ref||_F / (K u ||A||_F ||B||_F) -- the sampled part of C scaled
+    by `scale` (sqrt of the sampling ratio) to the whole matrix."""
+    den = k * u * normA * normB
+    d = float(np.linalg.norm(np.asarray(got, np.longdouble) - np.asarray(ref, np.longdouble))) * scale
+    return d / den if den > 0 else d
+
+
+def sampled_rows_error(dA, dB, dC, m, n, k, rows, dtype, alpha=1.0, beta=0.0, C0_rows=None):
+    """Normwise error of C (m x n, column-major device buffers) on sampled
+    rows against the exact product of the same inputs (FP64 host arithmetic,
+    error-free for FP64 inputs), rows scaled to the whole matrix."""
+    import torch
+    idx = torch.from_numpy(np.asarray(rows, np.int64)).to(dA.device)
+    A_rows = dA.view(k, m)[:, idx].T.double().cpu().numpy()
+    B = dB.view(n, k).T.double().cpu().numpy()
+    got = dC.view(n, m)[:, idx].T.double().cpu().numpy()
+    ref = alpha * exact_product(A_rows, B)
+    if C0_rows is not None and beta != 0.0:
+        ref = ref + beta * np.asarray(C0_rows, np.longdouble)
     u = float(np.finfo(dtype).eps) / 2
-    num = np.linalg.norm(Cg[rows] - ref)
-    # norms of the sampled rows of A scaled to the whole matrix
-    nA = np.linalg.norm(A[rows]) * np.sqrt(m / len(rows))
-    return float(num / (k * u * nA * np.linalg.norm(B)))
+    nA = float(torch.linalg.norm(dA.double()))
+    nB = float(torch.linalg.norm(dB.double()))
+    return normwise(got, ref, k, u, abs(alpha) * nA, nB, np.sqrt(m / len(rows)))
 
 
 def measured_traffic(workload: str):
@@ -83,7 +129,22 @@ def measured_traffic(workload: str):
         return None
 
 
-def cpu_baseline(steps: int = 3):
+def host_cpu() -> str:
+    """CPU model, logical CPUs visible and OMP_NUM_THREADS of this host."""
+    model = "unknown CPU"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return (f"{model}, {os.cpu_count()} logical CPUs visible, "
+            f"OMP_NUM_THREADS={os.environ.get('OMP_NUM_THREADS', 'unset')}")
+
+
+def cpu_baseline(reps: int = 7):
     """Eigen's CPU GEBP (built from the reference tree into oracle/_ref) on this host."""
     try:
         import oracle as O
@@ -98,7 +159,7 @@ def cpu_baseline(steps: int = 3):
         t0 = time.perf_counter()
         O.eigen_gemm(False, False, 1.0, A, B, Cm)  # warm-up
         first = time.perf_counter() - t0
-        reps = max(1, min(steps, int(20.0 // max(first, 1e-3))))
+        reps = max(5, min(reps, int(20.0 // max(first, 1e-3))))  # median of >= 5, ~20 s at most
         ts = []
         for _ in range(reps):
             Cm[:] = 0
@@ -109,8 +170,8 @@ def cpu_baseline(steps: int = 3):
         return {"value": round(2.0 * m * n * k / t / 1e12, 4), "unit": "TFLOP/s", "cores": int(threads),
                 "kind": "reference",
                 "sample": f"Eigen C.noalias() += A*B, FP32 8192^3 on host Maps, same input distribution, "
-                          f"OpenMP {threads} threads, {L.ref_simd().decode()}, median of {reps} after 1 warm-up "
-                          f"({os.cpu_count()} logical CPUs visible)"}
+                          f"OpenMP {threads} threads, {L.ref_simd().decode()}, median of {reps} after 1 warm-up; "
+                          f"{host_cpu()}"}
     except Exception as e:  # the baseline is reported, never required
         log("cpu baseline failed:", e)
         return None
@@ -153,9 +214,17 @@ def main() -> int:
         if args.no_extra and name != "split3":
             continue
         steps = args.steps if name == "split3" else max(3, args.steps // 2)
+        E.kernel_timing(True)  # HIP events around each kernel launch, on its stream
         wall, kern = timed(sgemm(prec), steps, args.warmup)
-        err = fp64_rows_error(dA, dB, dC, m, n, k, rows)
-        res[name] = {"ms": wall, "kernel_ms": kern, "tflops": flop / (wall * 1e-3) / 1e12, "normwise_err": err}
+        prod_cls = E.KC_PRODUCT if name == "plain" else E.KC_PACKED_PRODUCT
+        prod_ms, prod_n = E.kernel_timing_read(prod_cls)
+        pack_ms, pack_n = E.kernel_timing_read(E.KC_PACK)
+        E.kernel_timing(False)
+        launches = steps + args.warmup
+        err = sampled_rows_error(dA, dB, dC, m, n, k, rows, np.float32)
+        res[name] = {"ms": wall, "event_ms": kern, "tflops": flop / (wall * 1e-3) / 1e12, "normwise_err": err,
+                     "product_kernel_ms": prod_ms / max(prod_n, 1), "product_launches": prod_n,
+                     "pack_kernel_ms": pack_ms / max(launches, 1) if pack_n else 0.0}
         log(f"{name}: {res[name]}")
 
     head = res["split3"]
@@ -171,15 +240,24 @@ def main() -> int:
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
+        "arithmetic": "bf16 x bf16 products, f32 accumulation (3-term split of each f32 operand)",
         "data": "synthetic (device splitmix64, sign*U(1,2)*2^U(-8,8))",
         "config": {"workload": "sgemm C=A*B M=N=K=8192 column-major, split-BF16 3 terms / 6 products on BF16 "
                                "MFMA, FP32 accumulate", "M": m, "N": n, "K": k,
                    "flop_per_step": flop, "flop_counting": "2MNK once"},
-        # BF16 MFMA work actually issued: 6 products of 2MNK each
-        "roofline": {"bound": "mfma", "achieved": round(6 * flop / (head["kernel_ms"] * 1e-3) / 1e12, 2),
+        # roofline of the dominant kernel (the packed BF16 product): the
+        # algorithmic 2MNK counted once over its own event-timed launch time,
+        # against the BF16 MFMA peak
+        "roofline": {"bound": "mfma",
+                     "achieved": round(flop / (head["product_kernel_ms"] * 1e-3) / 1e12, 2),
                      "peak": PEAK["bf16"], "unit": "TFLOP/s",
-                     "frac": round(6 * flop / (head["kernel_ms"] * 1e-3) / 1e12 / PEAK["bf16"], 4),
+                     "frac": round(flop / (head["product_kernel_ms"] * 1e-3) / 1e12 / PEAK["bf16"], 4),
                      "traffic": measured_traffic("split3")},
+        # BF16 MFMA work actually issued (6 cross products of 2MNK each) over the
+        # same kernel time: the matrix cores' utilisation, not the roofline
+        "mfma_utilisation": round(6 * flop / (head["product_kernel_ms"] * 1e-3) / 1e12 / PEAK["bf16"], 4),
+        "traffic_source": "profiles/traffic.json: FETCH_SIZE + WRITE_SIZE per launch of this kernel and workload "
+                          "from rocprofv3 --pmc runs, each scaled by its reading for a copy of known size",
         "cpu_baseline": None,
         "detail": {"split3": res["split3"]},
     }
@@ -210,7 +288,8 @@ def main() -> int:
 
 
 def bench_dgemm(E, torch, dev, args):
-    """FP64 dgemm drop-in: C = 1.5*A*B + 0.5*C, entries N(0,1)."""
+    """FP64 dgemm drop-in: C = 1.5*A*B + 0.5*C, entries N(0,1); normwise error on
+    16 sampled rows against the exact product of the same inputs."""
     out = {}
     for name, (m, n, k) in (("dgemm_4096", (4096, 4096, 4096)), ("dgemm_tall", (16384, 256, 16384))):
         dA = torch.empty(m * k, dtype=torch.float64, device=dev)
@@ -220,14 +299,31 @@ def bench_dgemm(E, torch, dev, args):
         E.fill(E.F64, dB, k * n, 12, E.DIST_NORMAL, 0, 1)
         E.fill(E.F64, dC, m * n, 13, E.DIST_NORMAL, 0, 1)
         fn = lambda: E.gemm(E.F64, E.OP_N, E.OP_N, m, n, k, 1.5, dA, m, dB, k, 0.5, dC, m)
+        E.kernel_timing(True)
         wall, kern = timed(fn, max(3, args.steps // 2), args.warmup)
+        kms, kn = E.kernel_timing_read(E.KC_PRODUCT)
+        E.kernel_timing(False)
         tf = 2.0 * m * n * k / (wall * 1e-3) / 1e12
-        out[name] = {"ms": wall, "kernel_ms": kern, "tflops": tf, "frac_of_fp64_peak": tf / PEAK["fp64"],
-                     "config": E.plan(E.F64, 0, 0, m, n, k), "config_tile_split": E.config_info(E.plan(E.F64, 0, 0, m, n, k)),
+        cfg = E.plan(E.F64, 0, 0, m, n, k)
+        out[name] = {"ms": wall, "event_ms": kern, "product_kernel_ms": kms / max(kn, 1), "tflops": tf,
+                     "frac_of_fp64_peak": tf / PEAK["fp64"], "config": cfg, "config_tile_split": E.config_info(cfg),
                      "planner": ["heuristic", "mlp", "table"][E.lib().eh_get_tune_mode()]}
+        # one more product from a fresh C for the error
+        rows = np.arange(0, m, m // 16)
+        E.fill(E.F64, dC, m * n, 13, E.DIST_NORMAL, 0, 1)
+        idx = torch.from_numpy(rows).to(dev)
+        C0_rows = dC.view(n, m)[:, idx].T.cpu().numpy()
+        fn()
+        torch.cuda.synchronize()
+        out[name]["normwise_err"] = sampled_rows_error(dA, dB, dC, m, n, k, rows, np.float64, 1.5, 0.5, C0_rows)
         log(f"{name}: {out[name]}")
         del dA, dB, dC
     return out
+
+
+def _exact_cols(T, X, cols):
+    """T @ X[:, cols] exactly (host), T and X float64 arrays."""
+    return exact_product(T, X[:, cols])
 
 
 def bench_level3(E, torch, dev, args):
@@ -235,15 +331,21 @@ def bench_level3(E, torch, dev, args):
     TRMM (left, lower, no transpose; m^2 n flops), SYMM (left, lower; 2 m^2 n),
     SYRK and SYR2K (lower, no transpose, k = 8192; n^2 k and 2 n^2 k).  The
     triangle has a diagonal of 2 and off-diagonal entries U[-1/n, 1/n] (well
-    conditioned, so repeated solves stay in the normal range)."""
+    conditioned, so repeated solves stay in the normal range).  After the
+    timing, each routine runs once more on fresh inputs and its normwise error
+    on 8 sampled columns is taken against the exact product (TRSM: the
+    backward error ||T X - B|| / (n u ||T|| ||X||))."""
     n = 8192
     A = torch.empty(n * n, dtype=torch.float64, device=dev)
     B = torch.empty(n * n, dtype=torch.float64, device=dev)
     Cm = torch.empty(n * n, dtype=torch.float64, device=dev)
-    E.fill(E.F64, A, n * n, 31, E.DIST_UNIFORM, -1.0 / n, 1.0 / n)
-    A.view(n, n).diagonal().fill_(2.0)
-    E.fill(E.F64, B, n * n, 32, E.DIST_UNIFORM, -1, 1)
-    E.fill(E.F64, Cm, n * n, 33, E.DIST_UNIFORM, -1, 1)
+
+    def fresh():
+        E.fill(E.F64, A, n * n, 31, E.DIST_UNIFORM, -1.0 / n, 1.0 / n)
+        A.view(n, n).diagonal().fill_(2.0)
+        E.fill(E.F64, B, n * n, 32, E.DIST_UNIFORM, -1, 1)
+        E.fill(E.F64, Cm, n * n, 33, E.DIST_UNIFORM, -1, 1)
+    fresh()
     nn = float(n) ** 3
     cases = {
         "dtrsm_8192": (nn, lambda: E.trsm(E.F64, E.LEFT, E.LOWER, E.OP_N, E.NON_UNIT, n, n, 1.0, A, n, B, n)),
@@ -252,11 +354,44 @@ def bench_level3(E, torch, dev, args):
         "dsyrk_8192": (nn, lambda: E.syrk(E.F64, E.LOWER, E.OP_N, n, n, 1.0, B, n, 0.5, Cm, n)),
         "dsyr2k_8192": (2 * nn, lambda: E.syr2k(E.F64, E.LOWER, E.OP_N, n, n, 1.0, A, n, B, n, 0.5, Cm, n)),
     }
+    u = float(np.finfo(np.float64).eps) / 2
+    cols = np.arange(5, n, n // 8)
+    host = lambda t: t.view(n, n).T.cpu().numpy()  # column-major n x n
     out = {}
     for name, (flop, fn) in cases.items():
         wall, kern = timed(fn, 3, 1)
         tf = flop / (wall * 1e-3) / 1e12
         out[name] = {"ms": wall, "tflops": tf, "frac_of_fp64_peak": tf / PEAK["fp64"]}
+        fresh()
+        Ah, Bh, Ch = host(A), host(B), host(Cm)
+        fn()
+        torch.cuda.synchronize()
+        lo = np.tril(Ah)
+        sym = lo + np.tril(lo, -1).T
+        if name == "dtrsm_8192":
+            X = host(B)
+            res = _exact_cols(lo, X, cols) - Bh[:, cols]
+            err = float(np.linalg.norm(np.asarray(res, np.float64))) * np.sqrt(n / len(cols)) / (
+                n * u * np.linalg.norm(lo) * np.linalg.norm(X))
+        elif name == "dtrmm_8192":
+            err = normwise(host(B)[:, cols], _exact_cols(lo, Bh, cols), n, u, np.linalg.norm(lo), np.linalg.norm(Bh),
+                           np.sqrt(n / len(cols)))
+        elif name == "dsymm_8192":
+            ref = _exact_cols(sym, Bh, cols) + 0.5 * np.asarray(Ch[:, cols], np.longdouble)
+            err = normwise(host(Cm)[:, cols], ref, n, u, np.linalg.norm(sym), np.linalg.norm(Bh), np.sqrt(n / len(cols)))
+        else:  # lower triangle of the rank-k / rank-2k update, on the sampled columns
+            if name == "dsyrk_8192":
+                ref = exact_product(Bh, Bh[cols, :].T) + 0.5 * np.asarray(Ch[:, cols], np.longdouble)
+                nab = np.linalg.norm(Bh) ** 2
+            else:
+                ref = (exact_product(Ah, Bh[cols, :].T) + exact_product(Bh, Ah[cols, :].T)
+                       + 0.5 * np.asarray(Ch[:, cols], np.longdouble))
+                nab = 2 * np.linalg.norm(Ah) * np.linalg.norm(Bh)
+            got = host(Cm)[:, cols]
+            mask = np.arange(n)[:, None] >= cols[None, :]
+            d = np.where(mask, np.asarray(got, np.longdouble) - ref, 0)
+            err = float(np.linalg.norm(np.asarray(d, np.float64))) * np.sqrt(n / len(cols)) / (n * u * nab)
+        out[name]["normwise_err"] = err
         log(f"{name}: {out[name]}")
     del A, B, Cm
     return out
@@ -265,7 +400,8 @@ def bench_level3(E, torch, dev, args):
 def bench_batched_strided(E, torch, dev, args):
     """Arrays of fixed-size matrices through eh_gemm_batched_strided: 100,000
     products of 4x4 FP32 and 20,000 of 16x16 FP64 (C = A*B, U[-1,1]);
-    algorithmic bytes = A, B read and C written once."""
+    algorithmic bytes = A, B read and C written once.  Error: the largest
+    normwise error over every 100th product against the exact product."""
     out = {}
     for name, dtype, tdt, es, size, count in (("f32_4x4x4", E.F32, torch.float32, 4, 4, 100000),
                                               ("f64_16x16x16", E.F64, torch.float64, 8, 16, 20000)):
@@ -277,8 +413,15 @@ def bench_batched_strided(E, torch, dev, args):
                                             e, count)
         wall, kern = timed(fn, max(10, args.steps), args.warmup)
         flop, nbytes = 2.0 * size ** 3 * count, 3.0 * e * es * count
+        u = float(np.finfo(np.float32 if dtype == E.F32 else np.float64).eps) / 2
+        Ah = A.view(count, size, size).transpose(1, 2).cpu().numpy()
+        Bh = B.view(count, size, size).transpose(1, 2).cpu().numpy()
+        Ch = Cm.view(count, size, size).transpose(1, 2).cpu().numpy()
+        errs = [normwise(Ch[p], exact_product(Ah[p], Bh[p]), size, u, np.linalg.norm(Ah[p]), np.linalg.norm(Bh[p]))
+                for p in range(0, count, 100)]
         out[name] = {"products": count, "ms": wall, "gflops": flop / (wall * 1e-3) / 1e9,
-                     "gbs": nbytes / (wall * 1e-3) / 1e9, "frac_of_hbm": nbytes / (wall * 1e-3) / 1e9 / HBM_PEAK_GBS}
+                     "gbs": nbytes / (wall * 1e-3) / 1e9, "frac_of_hbm": nbytes / (wall * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "normwise_err_max": float(np.max(errs))}
         del A, B, Cm
     log(f"batched_strided: {out}")
     return out
@@ -287,18 +430,25 @@ def bench_batched_strided(E, torch, dev, args):
 def bench_types(E, torch, dev, args):
     """Other scalar types: bf16 inputs with an FP32 result (8192^3, U[-1,1]
     rounded to bf16; 2mnk flops) and complex products cgemm / zgemm (4096^3,
-    real and imaginary parts U[-1,1]; 8mnk real flops)."""
+    real and imaginary parts U[-1,1]; 8mnk real flops).  Errors on sampled
+    rows against the exact product of the same (widened) inputs."""
     out = {}
     n = 8192
     A = (torch.rand(n * n, device=dev) * 2 - 1).to(torch.bfloat16)
     B = (torch.rand(n * n, device=dev) * 2 - 1).to(torch.bfloat16)
     Cm = torch.zeros(n * n, device=dev)
     fn = lambda: E.gemm_bf16(0, 0, n, n, n, 1.0, A.data_ptr(), n, B.data_ptr(), n, 0.0, Cm, n)
+    E.kernel_timing(True)
     wall, kern = timed(fn, max(3, args.steps // 2), args.warmup)
+    kms, kn = E.kernel_timing_read(E.KC_PACKED_PRODUCT)
+    E.kernel_timing(False)
     tf = 2.0 * n ** 3 / (wall * 1e-3) / 1e12
-    out["bf16_8192"] = {"ms": wall, "tflops": tf, "frac_of_bf16_peak": tf / PEAK["bf16"]}
+    out["bf16_8192"] = {"ms": wall, "tflops": tf, "frac_of_bf16_peak": tf / PEAK["bf16"],
+                        "product_kernel_ms": kms / max(kn, 1),
+                        "normwise_err": sampled_rows_error(A, B, Cm, n, n, n, np.arange(0, n, n // 32), np.float32)}
     del A, B, Cm
     n = 4096
+    rows = np.arange(0, n, n // 16)
     for name, dtype, tdt, peak in (("cgemm_4096", E.F32, torch.float32, "fp32"),
                                    ("zgemm_4096", E.F64, torch.float64, "fp64")):
         A = torch.rand(2 * n * n, dtype=tdt, device=dev) * 2 - 1
@@ -308,6 +458,22 @@ def bench_types(E, torch, dev, args):
         wall, kern = timed(fn, max(3, args.steps // 2), args.warmup)
         tf = 8.0 * n ** 3 / (wall * 1e-3) / 1e12
         out[name] = {"ms": wall, "tflops": tf, f"frac_of_{peak}_peak": tf / PEAK[peak]}
+        # error of one more product from a fresh C
+        Cm.uniform_(-1, 1)
+        cz = lambda t: t.view(-1, 2)[:, 0].double() + 1j * t.view(-1, 2)[:, 1].double()
+        Ah = cz(A).view(n, n).T[rows].cpu().numpy()
+        Bh = cz(B).view(n, n).T.cpu().numpy()
+        C0 = cz(Cm).view(n, n).T[rows].cpu().numpy()
+        fn()
+        torch.cuda.synchronize()
+        got = cz(Cm).view(n, n).T[rows].cpu().numpy()
+        pr = (exact_product(Ah.real, Bh.real) - exact_product(Ah.imag, Bh.imag)
+              + 1j * (exact_product(Ah.real, Bh.imag) + exact_product(Ah.imag, Bh.real)))
+        ref = (1.0 + 0.5j) * pr + 0.5 * C0
+        u = float(np.finfo(np.float32 if dtype == E.F32 else np.float64).eps) / 2
+        nA = float(torch.linalg.norm(A.double())) * abs(1.0 + 0.5j)
+        out[name]["normwise_err"] = float(np.linalg.norm(np.asarray(got - ref, np.complex128))) * np.sqrt(
+            n / len(rows)) / (n * u * nA * float(torch.linalg.norm(B.double())))
         del A, B, Cm
     log(f"types: {out}")
     return out
@@ -316,7 +482,8 @@ def bench_types(E, torch, dev, args):
 def bench_gemv(E, torch, dev, args):
     """Matrix-vector products (the n == 1 path): y = A*x and y = A^T*x with A
     16384 x 16384, FP32 and FP64, U[-1,1]; algorithmic bytes = the matrix read
-    once (the vectors are 0.01% of it)."""
+    once (the vectors are 0.01% of it).  Error: normwise against the exact
+    product of the same inputs."""
     out = {}
     n = 16384
     for dtype, name, tdt, es in ((E.F32, "f32", torch.float32, 4), (E.F64, "f64", torch.float64, 8)):
@@ -325,12 +492,21 @@ def bench_gemv(E, torch, dev, args):
         y = torch.zeros(n, dtype=tdt, device=dev)
         E.fill(dtype, A, n * n, 21, E.DIST_UNIFORM, -1, 1)
         E.fill(dtype, x, n, 22, E.DIST_UNIFORM, -1, 1)
+        Ah = A.view(n, n).T.double().cpu().numpy()
+        xh = x.double().cpu().numpy()[:, None]
+        u = float(np.finfo(np.float32 if dtype == E.F32 else np.float64).eps) / 2
         for ta in (0, 1):
             fn = lambda: E.gemm(dtype, ta, 0, n, 1, n, 1.0, A, n, x, n, 0.0, y, n)
+            E.kernel_timing(True)
             wall, kern = timed(fn, max(10, args.steps), args.warmup)
-            gbs = n * n * es / (kern * 1e-3) / 1e9
-            out[f"{name}_{'T' if ta else 'N'}"] = {"ms": wall, "kernel_ms": kern, "gbs": gbs,
-                                                   "frac_of_hbm": gbs / HBM_PEAK_GBS}
+            kms, kn = E.kernel_timing_read(E.KC_GEMV)
+            E.kernel_timing(False)
+            gbs = n * n * es / (kms / max(kn, 1) * 1e-3) / 1e9
+            ref = exact_product(Ah.T if ta else Ah, xh)[:, 0]
+            err = normwise(y.double().cpu().numpy(), ref, n, u, np.linalg.norm(Ah), np.linalg.norm(xh))
+            out[f"{name}_{'T' if ta else 'N'}"] = {"ms": wall, "event_ms": kern, "kernel_ms": kms / max(kn, 1),
+                                                   "gbs": gbs, "frac_of_hbm": gbs / HBM_PEAK_GBS,
+                                                   "normwise_err": err}
         del A, x, y
     log(f"gemv: {out}")
     return out
@@ -338,18 +514,32 @@ def bench_gemv(E, torch, dev, args):
 
 def bench_batched(E, torch, args):
     """20,000 uneven products (M, N, K log-uniform in [4, 256], half FP32, half FP64, U[-1,1]):
-    the persistent grouped kernel replayed as a hipGraph vs one launch per product."""
-    from eigen_amd.batched import uneven_batch
+    the persistent grouped kernel replayed as a hipGraph vs one launch per product.
+    Error: the largest normwise error over every 100th product against the exact product."""
+    from eigen_amd.batched import operand, uneven_batch
     b = uneven_batch(20000, seed=0)
     g = E.Grouped(b.problems)
     wall_g, kern_g = timed(g.graph_launch, max(5, args.steps), args.warmup)
+    E.kernel_timing(True)
     wall_r, _ = timed(g.run, max(5, args.steps // 2), 1)
+    kms, kn = E.kernel_timing_read(E.KC_GROUPED)
+    E.kernel_timing(False)
     wall_p, _ = timed(g.run_per_product, 2, 1)
+    g.graph_launch()
+    torch.cuda.synchronize()
+    errs = []
+    for p in range(0, len(b.problems), 100):
+        A_, B_, C_ = (operand(b, p, w) for w in range(3))
+        u = float(np.finfo(A_.dtype).eps) / 2
+        errs.append(normwise(C_, exact_product(A_, B_), A_.shape[1], u, np.linalg.norm(A_), np.linalg.norm(B_)))
+    kernel_ms = kms / max(kn, 1)
     res = {"products": len(b.problems), "flop": b.flops, "bytes": b.bytes,
-           "graph_ms": wall_g, "graph_kernel_ms": kern_g, "run_ms": wall_r, "per_product_ms": wall_p,
+           "graph_ms": wall_g, "graph_event_ms": kern_g, "run_ms": wall_r, "kernel_ms": kernel_ms,
+           "per_product_ms": wall_p,
            "gflops": b.flops / (wall_g * 1e-3) / 1e9, "gbs": b.bytes / (wall_g * 1e-3) / 1e9,
            "frac_of_hbm": b.bytes / (wall_g * 1e-3) / 1e9 / HBM_PEAK_GBS,
-           "speedup_vs_per_product": wall_p / wall_g}
+           "kernel_frac_of_hbm": b.bytes / (kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+           "speedup_vs_per_product": wall_p / wall_g, "normwise_err_max": float(np.max(errs))}
     log(f"batched: {res}")
     g.close()
     res["_batch"] = b
@@ -369,7 +559,7 @@ def cpu_baseline_dgemm(m: int, n: int, k: int):
         B = O.fill((k, n), np.float64, 12, O.NORMAL, 0, 1)
         C0 = O.fill((m, n), np.float64, 13, O.NORMAL, 0, 1)
         ts = []
-        for r in range(4):  # one warm-up, median of three
+        for r in range(6):  # one warm-up, median of five
             Cm = np.asfortranarray(C0.copy())
             t0 = time.perf_counter()
             L.ref_gemm_beta_f64(0, 0, m, n, k, 1.5, O._ptr(A), m, O._ptr(B), k, 0.5, O._ptr(Cm), m)
@@ -378,7 +568,8 @@ def cpu_baseline_dgemm(m: int, n: int, k: int):
         t = float(np.median(ts))
         return {"tflops": round(2.0 * m * n * k / t / 1e12, 4), "ms": round(t * 1e3, 2),
                 "threads": int(L.ref_get_threads()), "kind": "reference",
-                "sample": "Eigen C = 1.5*A*B + 0.5*C (dgemm semantics), FP64 N(0,1), median of 3 after 1 warm-up"}
+                "sample": "Eigen C = 1.5*A*B + 0.5*C (dgemm semantics), FP64 N(0,1), median of 5 after 1 warm-up; "
+                          + host_cpu()}
     except Exception as e:
         log("cpu dgemm baseline failed:", e)
         return None
@@ -405,14 +596,14 @@ def cpu_baseline_batched(b):
         args_ = [cnt, O._ptr(dt), O._ptr(ms), O._ptr(ns), O._ptr(ks), ptrs[0], ptrs[1], ptrs[2]]
         L.ref_grouped(*args_, 1)  # warm-up
         ts = []
-        for _ in range(3):
+        for _ in range(5):
             t0 = time.perf_counter()
             L.ref_grouped(*args_, 1)
             ts.append(time.perf_counter() - t0)
         t = float(np.median(ts))
         return {"ms": t * 1e3, "gflops": b.flops / t / 1e9, "threads": int(L.ref_get_threads()),
                 "kind": "reference", "sample": "Eigen c.noalias() = a*b per product, OpenMP schedule(dynamic) "
-                                                "over the 20,000 products, median of 3"}
+                                                "over the 20,000 products, median of 5; " + host_cpu()}
     except Exception as e:
         log("cpu batched baseline failed:", e)
         return None

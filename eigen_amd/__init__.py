@@ -198,6 +198,8 @@ def _declare_extra(L) -> None:
         "eh_mlp_predict": (i32, [vp, vp, i64, vp, vp]),
         "eh_planner_set_mlp": (i32, [vp, vp]),
         "eh_planner_set_mlp_params": (i32, [vp]),
+        "eh_kernel_timing": (i32, [i32]),
+        "eh_kernel_timing_read": (i32, [i32, C.POINTER(f64), C.POINTER(i64)]),
     }
     for name, (res, args) in table.items():
         if hasattr(L, name):
@@ -379,6 +381,22 @@ def fill(dtype: int, X, n: int, seed: int, dist: int = DIST_UNIFORM, p0: float =
 def split_bf16(x, n: int, terms: int, t0, t1=None, t2=None, stream=None) -> None:
     s = current_stream() if stream is None else stream
     check(lib().eh_split_bf16(_ptr(x), n, terms, _ptr(t0), _ptr(t1), _ptr(t2), s), "eh_split_bf16")
+
+
+# kernel classes of eh_kernel_timing_read
+KC_PRODUCT, KC_PACKED_PRODUCT, KC_PACK, KC_GROUPED, KC_GEMV = range(5)
+
+
+def kernel_timing(enable: bool) -> None:
+    """Start (clearing earlier records) or stop recording HIP events around kernel launches."""
+    check(lib().eh_kernel_timing(1 if enable else 0), "eh_kernel_timing")
+
+
+def kernel_timing_read(cls: int):
+    """(total ms, launches) of the recorded launches of one kernel class; clears them."""
+    ms, n = C.c_double(0.0), C.c_int64(0)
+    check(lib().eh_kernel_timing_read(cls, C.byref(ms), C.byref(n)), "eh_kernel_timing_read")
+    return ms.value, n.value
 
 
 def num_configs() -> int:

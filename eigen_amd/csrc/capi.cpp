@@ -9,13 +9,18 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <memory>
 #include <string>
+#include <thread>
+#include <tuple>
 #include <utility>
+#include <vector>
 
 #include "../../include/eigen_amd.h"
 #include "internal.hpp"
@@ -97,38 +102,104 @@ static eh_precision default_sgemm_precision() {
 }
 
 // ---- workspace --------------------------------------------------------------
-// One growable device buffer per (device, stream) for split-K partials.
-// Growing frees the old buffer with hipFree, which waits for the device, so
-// work still queued on it finishes first.
-struct Workspace {
-  void* ptr = nullptr;
-  size_t bytes = 0;
-};
-static std::map<std::pair<int, hipStream_t>, Workspace> g_ws;
+// One growable device buffer per (device, stream) for split-K partials, packed
+// operands and work lists; hipStreamPerThread names a different stream in
+// every thread, so it is keyed per thread as well.  A caller holds a lease
+// (a shared owner) on the buffer until its launches are queued: growing the
+// workspace only drops the table's owner, and the old buffer is freed when
+// its last lease goes.  The free is hipFree, which waits for the device, so
+// the work queued with the buffer completes before the memory is released.
+static std::map<std::tuple<int, hipStream_t, std::thread::id>, std::shared_ptr<Workspace>> g_ws;
 
-eh_status workspace_get(hipStream_t stream, size_t bytes, void** out) {
+Workspace::~Workspace() {
+  if (ptr) (void)hipFree(ptr);
+}
+
+static thread_local void* t_fixed_ptr = nullptr;
+static thread_local size_t t_fixed_bytes = 0;
+
+FixedScratch::FixedScratch(void* ptr, size_t bytes) : prev_ptr(t_fixed_ptr), prev_bytes(t_fixed_bytes) {
+  t_fixed_ptr = ptr;
+  t_fixed_bytes = bytes;
+}
+FixedScratch::~FixedScratch() {
+  t_fixed_ptr = prev_ptr;
+  t_fixed_bytes = prev_bytes;
+}
+bool fixed_scratch_active() { return t_fixed_ptr != nullptr; }
+
+eh_status workspace_get(hipStream_t stream, size_t bytes, WorkspaceLease* out, void** ptr) {
+  if (t_fixed_ptr) {
+    if (bytes > t_fixed_bytes)
+      return fail(EH_ERR_UNSUPPORTED, "product needs " + std::to_string(bytes) + " bytes of scratch; the fixed scratch holds " +
+                                          std::to_string(t_fixed_bytes));
+    *ptr = t_fixed_ptr;
+    return EH_SUCCESS;
+  }
   int dev = 0;
   hipError_t e = hipGetDevice(&dev);
   if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+  const std::thread::id tid = stream == hipStreamPerThread ? std::this_thread::get_id() : std::thread::id();
+  std::shared_ptr<Workspace> old;  // released after the lock, outside the table
   std::lock_guard<std::mutex> lk(g_mutex);
-  Workspace& w = g_ws[{dev, stream}];
-  if (w.bytes < bytes) {
-    if (w.ptr) {
-      e = hipFree(w.ptr);
-      if (e != hipSuccess) return hip_fail(e, "hipFree(workspace)");
-      w.ptr = nullptr;
-      w.bytes = 0;
-    }
-    size_t want = std::max(bytes, (size_t)1 << 20);
-    e = hipMalloc(&w.ptr, want);
+  std::shared_ptr<Workspace>& w = g_ws[std::make_tuple(dev, stream, tid)];
+  if (!w || w->bytes < bytes) {
+    old = std::move(w);
+    auto fresh = std::make_shared<Workspace>();
+    const size_t want = std::max(bytes, old ? std::max(old->bytes, (size_t)1 << 20) : (size_t)1 << 20);
+    e = hipMalloc(&fresh->ptr, want);
     if (e != hipSuccess) {
-      w.ptr = nullptr;
+      fresh->ptr = nullptr;
+      w = std::move(old);
       return fail(EH_ERR_ALLOC, std::string("hipMalloc(workspace): ") + hipGetErrorString(e));
     }
-    w.bytes = want;
+    fresh->bytes = want;
+    w = std::move(fresh);
   }
-  *out = w.ptr;
+  *out = w;
+  *ptr = w->ptr;
   return EH_SUCCESS;
+}
+
+// ---- kernel event timing ------------------------------------------------------
+static std::atomic<int> g_timing_on{0};
+static std::mutex g_timing_mutex;
+static std::vector<std::pair<hipEvent_t, hipEvent_t>> g_timed[EH_KC_COUNT];
+
+KernelTimer::KernelTimer(int c, hipStream_t st) : cls(c), s(st) {
+  if (!g_timing_on.load(std::memory_order_relaxed)) return;
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(s, &cap) != hipSuccess || cap != hipStreamCaptureStatusNone) return;
+  if (hipEventCreate(&start) != hipSuccess) {
+    start = nullptr;
+    return;
+  }
+  if (hipEventRecord(start, s) != hipSuccess) {
+    (void)hipEventDestroy(start);
+    start = nullptr;
+  }
+}
+
+KernelTimer::~KernelTimer() {
+  if (!start) return;
+  hipEvent_t stop = nullptr;
+  if (hipEventCreate(&stop) != hipSuccess || hipEventRecord(stop, s) != hipSuccess) {
+    if (stop) (void)hipEventDestroy(stop);
+    (void)hipEventDestroy(start);
+    return;
+  }
+  std::lock_guard<std::mutex> lk(g_timing_mutex);
+  g_timed[cls].emplace_back(start, stop);
+}
+
+static void timing_clear_locked() {
+  for (auto& v : g_timed) {
+    for (auto& ev : v) {
+      (void)hipEventDestroy(ev.first);
+      (void)hipEventDestroy(ev.second);
+    }
+    v.clear();
+  }
 }
 
 // ---- argument checks ------------------------------------------------------
@@ -192,11 +263,19 @@ static eh_status launch_split_product(const GemmParams<float>& g, bool ta, bool 
   const size_t bytesA = split_packed_bytes(m, k, terms), bytesB = split_packed_bytes(n, k, terms);
   const size_t bytesW = splits > 1 ? (size_t)splits * m * n * sizeof(float) : 0;
   void* ws = nullptr;
-  eh_status st = workspace_get(stream, bytesA + bytesB + bytesW, &ws);
+  WorkspaceLease lease;
+  eh_status st = workspace_get(stream, bytesA + bytesB + bytesW, &lease, &ws);
   if (st != EH_SUCCESS) return st;
   unsigned char* base = static_cast<unsigned char*>(ws);
-  hipError_t e = launch_pack_split(g.A, g.lda, ta, m, k, terms, base, stream);
-  if (e == hipSuccess) e = launch_pack_split(g.B, g.ldb, !tb, n, k, terms, base + bytesA, stream);
+  hipError_t e;
+  {
+    KernelTimer t(EH_KC_PACK, stream);
+    e = launch_pack_split(g.A, g.lda, ta, m, k, terms, base, stream);
+  }
+  if (e == hipSuccess) {
+    KernelTimer t(EH_KC_PACK, stream);
+    e = launch_pack_split(g.B, g.ldb, !tb, n, k, terms, base + bytesA, stream);
+  }
   if (e != hipSuccess) return hip_fail(e, "pack launch");
   SplitParams sp{};
   sp.Ap = base;
@@ -212,7 +291,10 @@ static eh_status launch_split_product(const GemmParams<float>& g, bool ta, bool 
   sp.ks_per_split = per;
   sp.alpha = g.alpha;
   sp.beta = g.beta;
-  e = launch_gemm_split_packed(sp, terms, tile, splits, stream);
+  {
+    KernelTimer t(EH_KC_PACKED_PRODUCT, stream);
+    e = launch_gemm_split_packed(sp, terms, tile, splits, stream);
+  }
   if (e != hipSuccess) return hip_fail(e, "split product launch");
   if (splits > 1) {
     GemmParams<float> r = g;
@@ -246,11 +328,19 @@ static eh_status launch_bf16_product(bool ta, bool tb, int m, int n, int k, floa
   const size_t bytesA = split_packed_bytes(m, k, 1), bytesB = split_packed_bytes(n, k, 1);
   const size_t bytesW = splits > 1 ? (size_t)splits * m * n * sizeof(float) : 0;
   void* ws = nullptr;
-  eh_status st = workspace_get(stream, bytesA + bytesB + bytesW, &ws);
+  WorkspaceLease lease;
+  eh_status st = workspace_get(stream, bytesA + bytesB + bytesW, &lease, &ws);
   if (st != EH_SUCCESS) return st;
   unsigned char* base = static_cast<unsigned char*>(ws);
-  hipError_t e = launch_pack_bf16(A, lda, ta, m, k, base, stream);
-  if (e == hipSuccess) e = launch_pack_bf16(B, ldb, !tb, n, k, base + bytesA, stream);
+  hipError_t e;
+  {
+    KernelTimer t(EH_KC_PACK, stream);
+    e = launch_pack_bf16(A, lda, ta, m, k, base, stream);
+  }
+  if (e == hipSuccess) {
+    KernelTimer t(EH_KC_PACK, stream);
+    e = launch_pack_bf16(B, ldb, !tb, n, k, base + bytesA, stream);
+  }
   if (e != hipSuccess) return hip_fail(e, "bf16 pack launch");
   SplitParams sp{};
   sp.Ap = base;
@@ -266,7 +356,10 @@ static eh_status launch_bf16_product(bool ta, bool tb, int m, int n, int k, floa
   sp.ks_per_split = per;
   sp.alpha = alpha;
   sp.beta = beta;
-  e = launch_gemm_split_packed(sp, 1, tile, splits, stream);
+  {
+    KernelTimer t(EH_KC_PACKED_PRODUCT, stream);
+    e = launch_gemm_split_packed(sp, 1, tile, splits, stream);
+  }
   if (e != hipSuccess) return hip_fail(e, "bf16 product launch");
   if (splits > 1) {
     GemmParams<float> r{};
@@ -356,13 +449,17 @@ static eh_status launch_product(bool ta, bool tb, int64_t m, int64_t n, int64_t 
   per = (per + kslice - 1) / kslice * kslice;
   p.k_per_split = (int)per;
   splits = (int)((k + per - 1) / per);  // drop empty trailing splits
+  WorkspaceLease lease;
   if (splits > 1) {
     void* ws = nullptr;
-    eh_status st = workspace_get(stream, (size_t)splits * m * n * sizeof(T), &ws);
+    eh_status st = workspace_get(stream, (size_t)splits * m * n * sizeof(T), &lease, &ws);
     if (st != EH_SUCCESS) return st;
     p.W = static_cast<T*>(ws);
   }
-  e = launch_gemm_mfma<T>(p, tile, ta, tb, splits, stream);
+  {
+    KernelTimer t(EH_KC_PRODUCT, stream);
+    e = launch_gemm_mfma<T>(p, tile, ta, tb, splits, stream);
+  }
   if (e != hipSuccess) return hip_fail(e, "gemm launch");
   if (splits > 1) {
     e = launch_splitk_reduce<T>(p, splits, stream);
@@ -404,11 +501,13 @@ static bool vector_product(int opa, int opb, int64_t m, int64_t n, int64_t k, T 
   if (extent >= kMaxExtent) return false;
   const GemvPlan g = gemv_plan(trans, rows, (int)k, (int)sizeof(T));
   void* ws = nullptr;
+  WorkspaceLease lease;
   const size_t wsb = gemv_workspace_bytes(g, rows, (int)sizeof(T));
   if (wsb > 0) {
-    *st = workspace_get(stream, wsb, &ws);
+    *st = workspace_get(stream, wsb, &lease, &ws);
     if (*st != EH_SUCCESS) return true;
   }
+  KernelTimer t(EH_KC_GEMV, stream);
   hipError_t e = launch_gemv<T>(g, trans, rows, (int)k, alpha, mat, ldm, (uint32_t)(extent - 64 * sizeof(T)), x, incx,
                                 beta, C, incy, ws, stream);
   *st = e == hipSuccess ? EH_SUCCESS : hip_fail(e, "gemv launch");
@@ -503,11 +602,13 @@ static eh_status gemv_run(bool trans, int64_t rows, int64_t k, T alpha, const T*
   }
   const GemvPlan g = gemv_plan(trans, (int)rows, (int)k, (int)sizeof(T));
   void* ws = nullptr;
+  WorkspaceLease lease;
   const size_t wsb = gemv_workspace_bytes(g, (int)rows, (int)sizeof(T));
   if (wsb > 0) {
-    eh_status st = workspace_get(s, wsb, &ws);
+    eh_status st = workspace_get(s, wsb, &lease, &ws);
     if (st != EH_SUCCESS) return st;
   }
+  KernelTimer t(EH_KC_GEMV, s);
   hipError_t e = launch_gemv<T>(g, trans, (int)rows, (int)k, alpha, A, lda, (uint32_t)(extent - 64 * sizeof(T)), x, incx,
                                 beta, y, incy, ws, s);
   return e == hipSuccess ? EH_SUCCESS : hip_fail(e, "gemv launch");
@@ -745,6 +846,36 @@ const char* eh_status_string(eh_status s) {
 }
 
 const char* eh_last_error(void) { return g_last_error.c_str(); }
+
+eh_status eh_kernel_timing(int enable) {
+  std::lock_guard<std::mutex> lk(g_timing_mutex);
+  timing_clear_locked();
+  g_timing_on.store(enable ? 1 : 0);
+  return EH_SUCCESS;
+}
+
+eh_status eh_kernel_timing_read(eh_kernel_class cls, double* total_ms, int64_t* launches) {
+  if ((int)cls < 0 || (int)cls >= EH_KC_COUNT) return fail(EH_ERR_INVALID, "bad kernel class");
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+  {
+    std::lock_guard<std::mutex> lk(g_timing_mutex);
+    ev.swap(g_timed[cls]);
+  }
+  double sum = 0.0;
+  eh_status st = EH_SUCCESS;
+  for (auto& p : ev) {
+    float ms = 0.0f;
+    hipError_t e = hipEventSynchronize(p.second);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, p.first, p.second);
+    if (e != hipSuccess && st == EH_SUCCESS) st = hip_fail(e, "kernel timing event");
+    sum += ms;
+    (void)hipEventDestroy(p.first);
+    (void)hipEventDestroy(p.second);
+  }
+  if (total_ms) *total_ms = sum;
+  if (launches) *launches = (int64_t)ev.size();
+  return st;
+}
 
 const char* eh_version(void) { return "eigen_amd 0.1 gfx950"; }
 

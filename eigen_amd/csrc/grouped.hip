@@ -522,7 +522,8 @@ extern "C" eh_status eh_gemm_batched_strided(eh_dtype dtype, eh_op opa, eh_op op
   const size_t bd = (size_t)count * sizeof(GroupedDesc), bb = (size_t)(count + 1) * sizeof(int64_t),
                bw = (size_t)(W + 1) * sizeof(int64_t), bp = (size_t)W * sizeof(int32_t);
   void* ws = nullptr;
-  st = workspace_get(s, bd + bb + bw + bp, &ws);
+  WorkspaceLease lease;
+  st = workspace_get(s, bd + bb + bw + bp, &lease, &ws);
   if (st != EH_SUCCESS) return st;
   char* p = static_cast<char*>(ws);
   GroupedDesc* d = reinterpret_cast<GroupedDesc*>(p);
@@ -534,9 +535,12 @@ extern "C" eh_status eh_gemm_batched_strided(eh_dtype dtype, eh_op opa, eh_op op
                      stride_c * es, count, T, W, d, begin, wave_tile, wave_prod);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return hip_fail(e, "batched build launch");
-  hipLaunchKernelGGL(grouped_gemm_kernel, dim3((unsigned)groups), dim3(64 * kWavesPerGroup), 0, s, d, begin, wave_tile,
-                     wave_prod);
-  e = hipGetLastError();
+  {
+    KernelTimer t(EH_KC_GROUPED, s);
+    hipLaunchKernelGGL(grouped_gemm_kernel, dim3((unsigned)groups), dim3(64 * kWavesPerGroup), 0, s, d, begin,
+                       wave_tile, wave_prod);
+    e = hipGetLastError();
+  }
   return e == hipSuccess ? EH_SUCCESS : hip_fail(e, "grouped kernel launch");
 }
 
@@ -557,6 +561,7 @@ struct eh_grouped {
 
 static eh_status grouped_enqueue(eh_grouped* g, hipStream_t s) {
   if (g->count > 0) {
+    KernelTimer t(EH_KC_GROUPED, s);
     hipLaunchKernelGGL(grouped_gemm_kernel, dim3((unsigned)g->groups), dim3(64 * kWavesPerGroup), 0, s, g->d_desc,
                        g->d_begin, g->d_wave_tile, g->d_wave_prod);
     hipError_t e = hipGetLastError();

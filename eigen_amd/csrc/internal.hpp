@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <memory>
 #include <string>
 
 #include "../../include/eigen_amd.h"
@@ -25,7 +26,50 @@ eh_tune_mode tune_mode();
 // Planner (planner.cpp): configuration for a shape under the current mode.
 eh_status plan_config(int dtype, int opa, int opb, int64_t m, int64_t n, int64_t k, int* config);
 
-eh_status workspace_get(hipStream_t stream, size_t bytes, void** out);
+// Per-stream device workspace (capi.cpp).  The lease keeps the memory
+// allocated while it lives; hold it until the launches that use it are queued.
+struct Workspace {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  Workspace() = default;
+  Workspace(const Workspace&) = delete;
+  Workspace& operator=(const Workspace&) = delete;
+  ~Workspace();
+};
+using WorkspaceLease = std::shared_ptr<Workspace>;
+// *ptr = `bytes` of device workspace for work queued on `stream`.  Normally
+// the per-stream workspace (grown with hipMalloc on demand; *lease keeps it
+// alive).  While a FixedScratch is installed on the calling thread, *ptr is
+// that caller-provided buffer instead (no allocation, *lease stays empty),
+// and a request larger than it fails with EH_ERR_UNSUPPORTED.
+eh_status workspace_get(hipStream_t stream, size_t bytes, WorkspaceLease* lease, void** ptr);
+
+// Installs a fixed device buffer as this thread's workspace for its lifetime
+// (the allocation-free entry points); while installed, the planner does not
+// tune unseen shapes either (tuning allocates).
+struct FixedScratch {
+  FixedScratch(void* ptr, size_t bytes);
+  ~FixedScratch();
+  FixedScratch(const FixedScratch&) = delete;
+  FixedScratch& operator=(const FixedScratch&) = delete;
+  void* prev_ptr;
+  size_t prev_bytes;
+};
+bool fixed_scratch_active();
+
+// Event timing of kernel launches for measurement (eh_kernel_timing): while
+// enabled, a KernelTimer records one HIP event on `s` when constructed and
+// one when destroyed, around the launch it scopes (not while `s` is being
+// captured into a graph).
+struct KernelTimer {
+  KernelTimer(int cls, hipStream_t s);
+  ~KernelTimer();
+  KernelTimer(const KernelTimer&) = delete;
+  KernelTimer& operator=(const KernelTimer&) = delete;
+  int cls;
+  hipStream_t s;
+  hipEvent_t start = nullptr;
+};
 
 // A stream-ordered temporary (hipMallocAsync) released by hipFreeAsync on the
 // same stream when the owner goes out of scope, i.e. after the work queued

@@ -122,6 +122,7 @@ int table_plan(int dtype, int opa, int opb, int64_t m, int64_t n, int64_t k) {
     if (it != g_table.end()) return it->second;
   }
   int best = -1;
+  if (fixed_scratch_active()) return -1;  // tuning allocates: the caller's planner fallback decides
   if (tune_shape(dtype, opa, opb, m, n, k, &best) != EH_SUCCESS || best < 0) return -1;
   std::lock_guard<std::mutex> lk(g_tune_mutex);
   g_table[key_of(dtype, opa, opb, m, n, k)] = best;

@@ -296,12 +296,18 @@ typedef struct eh_gemm_problem {
 
 typedef struct eh_grouped eh_grouped; /* opaque batch handle */
 
-/* Validates and uploads `count` descriptors (host array); the device work
- * list is built on every run.  Products must not write overlapping C. */
+/* Validates `count` descriptors (host array), builds the work list on the
+ * host once -- one descriptor per product, the first output tile of each
+ * product, and one [begin, end) run of tiles of equal estimated cost per
+ * resident wave -- and uploads it to the device.  Products whose operands
+ * span 2^31 bytes or more are kept aside for the single-product path.
+ * Products must not write overlapping C. */
 eh_status eh_grouped_create(const eh_gemm_problem* problems, int64_t count, eh_grouped** out);
-/* All products in one persistent kernel (plus a work-list build kernel). */
+/* All products of the work list in one launch of the persistent grouped
+ * kernel, then the set-aside large products one eh_gemm each. */
 eh_status eh_grouped_run(eh_grouped* g, void* stream);
-/* The same two launches captured once in a hipGraph and replayed. */
+/* The same launches captured once in a hipGraph (kernel nodes only) and
+ * replayed on every call. */
 eh_status eh_grouped_graph_launch(eh_grouped* g, void* stream);
 /* Reference path: one eh_gemm launch per product. */
 eh_status eh_grouped_run_per_product(eh_grouped* g, void* stream);
@@ -386,6 +392,25 @@ eh_status eh_fill(eh_dtype dtype, void* x, int64_t n, uint64_t seed, int dist, d
  * (t1/t2 may be NULL when terms < 2/3).  Device pointers. */
 eh_status eh_split_bf16(const float* x, int64_t n, int terms, uint16_t* t0, uint16_t* t1, uint16_t* t2,
                         void* stream);
+
+/* Event timing of kernel launches, for measurement (no reference
+ * counterpart; the role of a profiler's kernel trace, inside the process).
+ * While enabled, every launch of a kernel of the listed classes that is not
+ * being captured into a graph is bracketed by two HIP events recorded on the
+ * stream it is launched on.  eh_kernel_timing(1) clears earlier records and
+ * enables, eh_kernel_timing(0) clears and disables; eh_kernel_timing_read
+ * waits for the recorded events of one class, returns the sum of their
+ * elapsed times and the number of launches, and clears them. */
+typedef enum eh_kernel_class {
+  EH_KC_PRODUCT = 0,        /* plain FP32 / FP64 MFMA product kernels */
+  EH_KC_PACKED_PRODUCT = 1, /* split-BF16 and bf16 products over packed planes */
+  EH_KC_PACK = 2,           /* split / bf16 operand packing */
+  EH_KC_GROUPED = 3,        /* persistent grouped kernel */
+  EH_KC_GEMV = 4,           /* matrix-vector kernels */
+  EH_KC_COUNT = 5
+} eh_kernel_class;
+eh_status eh_kernel_timing(int enable);
+eh_status eh_kernel_timing_read(eh_kernel_class cls, double* total_ms, int64_t* launches);
 
 eh_status eh_synchronize(void* stream);
 const char* eh_status_string(eh_status s);

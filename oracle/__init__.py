@@ -86,6 +86,10 @@ def lib():
         L.orc_bf16_rtne_n.argtypes = [vp, i64, vp]
         L.orc_gemm_split_ref.argtypes = [i32, i32, i32, i64, i64, i64, f64, vp, i64, vp, i64, f64, vp, i64,
                                          vp, i64, vp]
+        L.orc_gemm_split_pairs_ref.argtypes = [i32, i32, i32, i32, i64, i64, i64, f64, vp, i64, vp, i64, f64, vp,
+                                               i64, vp, i64, vp]
+        L.orc_split_kept_pairs.restype = i32
+        L.orc_split_kept_pairs.argtypes = [i32]
         _lib = L
     return _lib
 
@@ -220,7 +224,16 @@ def bf16_to_f32(h: np.ndarray) -> np.ndarray:
     return (np.asarray(h, dtype=np.uint32) << 16).view(np.float32)
 
 
-def gemm_split_ref(terms: int, ta: bool, tb: bool, alpha, A, B, beta=0.0, C0=None, rows=None):
+def split_kept_pairs(terms: int) -> int:
+    """Bit 3p+q set: the split product keeps t_p(A) * t_q(B) (p + q < terms)."""
+    return int(lib().orc_split_kept_pairs(terms))
+
+
+def gemm_split_ref(terms: int, ta: bool, tb: bool, alpha, A, B, beta=0.0, C0=None, rows=None, pairs=None):
+    """Long-double value of the split product alpha*sum_{(p,q) kept} t_p(op(A)) t_q(op(B)) + beta*C0.
+
+    `pairs` (9-bit mask, bit 3p+q) overrides the kept set of the split with
+    `terms` terms (default: p + q < terms)."""
     m = A.shape[1] if ta else A.shape[0]
     k = A.shape[0] if ta else A.shape[1]
     n = B.shape[0] if tb else B.shape[1]
@@ -230,9 +243,10 @@ def gemm_split_ref(terms: int, ta: bool, tb: bool, alpha, A, B, beta=0.0, C0=Non
     rows_arr = None if rows is None else np.ascontiguousarray(rows, dtype=np.int64)
     nrows = m if rows is None else rows_arr.size
     out = np.empty((nrows, n), np.float64)
-    lib().orc_gemm_split_ref(terms, int(ta), int(tb), m, n, k, float(alpha), _ptr(A), A.shape[0], _ptr(B),
-                             B.shape[0], float(beta), _ptr(C0), C0.shape[0],
-                             None if rows_arr is None else _ptr(rows_arr), nrows, _ptr(out))
+    mask = split_kept_pairs(terms) if pairs is None else int(pairs)
+    lib().orc_gemm_split_pairs_ref(terms, mask, int(ta), int(tb), m, n, k, float(alpha), _ptr(A), A.shape[0],
+                                   _ptr(B), B.shape[0], float(beta), _ptr(C0), C0.shape[0],
+                                   None if rows_arr is None else _ptr(rows_arr), nrows, _ptr(out))
     return out
 
 

@@ -283,14 +283,18 @@ ORC_API void orc_bf16_split(const float* x, int64_t n, int terms, uint16_t* t0, 
   }
 }
 
-/* Long-double value of the split product restricted to the kept cross
- * products.  Term pairs are (p,q) with p+q <= terms-1 (hi*hi, hi*mid,
- * mid*hi for 2 terms; plus hi*lo, lo*hi, mid*mid for 3 terms).  This is what
- * the split GEMM computes before its own FP32 accumulation rounding.
- *   out[r*n+j] = alpha * sum_k sum_{p+q<terms} ta_p(i,k) tb_q(k,j) + beta*C0 */
-ORC_API void orc_gemm_split_ref(int terms, int ta, int tb, int64_t m, int64_t n, int64_t k, double alpha,
-                                const float* A, int64_t lda, const float* B, int64_t ldb, double beta,
-                                const float* C0, int64_t ldc, const int64_t* rows, int64_t nrows, double* out) {
+/* Long-double value of a split product restricted to a set of cross
+ * products.  Bit p*3+q of `pairs` keeps the product t_p(op(A)) * t_q(op(B))
+ * (p indexes the terms of A, q those of B).  The split product of
+ * gemm_split.hip keeps the pairs with p + q < terms (hi*hi, hi*mid, mid*hi
+ * for 2 terms; plus hi*lo, lo*hi, mid*mid for 3 terms): this is the value it
+ * computes before its own FP32 accumulation rounding.  Other masks give the
+ * values a kernel with a wrong kept set would approach.
+ *   out[r*n+j] = alpha * sum_k sum_{(p,q) kept} ta_p(i,k) tb_q(k,j) + beta*C0 */
+ORC_API void orc_gemm_split_pairs_ref(int terms, int pairs, int ta, int tb, int64_t m, int64_t n, int64_t k,
+                                      double alpha, const float* A, int64_t lda, const float* B, int64_t ldb,
+                                      double beta, const float* C0, int64_t ldc, const int64_t* rows,
+                                      int64_t nrows, double* out) {
   (void)m;
   long double* bt = (long double*)malloc(sizeof(long double) * 3 * (size_t)(k > 0 ? k : 1));
   for (int64_t j = 0; j < n; ++j) {
@@ -311,8 +315,9 @@ ORC_API void orc_gemm_split_ref(int terms, int ta, int tb, int64_t m, int64_t n,
         orc_bf16_split(&x, 1, terms, &s0, &s1, &s2);
         long double a[3] = {orc_bf16_to_f32(s0), terms > 1 ? orc_bf16_to_f32(s1) : 0.0L,
                             terms > 2 ? orc_bf16_to_f32(s2) : 0.0L};
-        for (int p = 0; p < terms; ++p)
-          for (int q = 0; p + q < terms; ++q) s += a[p] * bt[3 * kk + q];
+        for (int p = 0; p < 3; ++p)
+          for (int q = 0; q < 3; ++q)
+            if (pairs >> (3 * p + q) & 1) s += a[p] * bt[3 * kk + q];
       }
       long double v = (long double)alpha * s;
       if (beta != 0.0) v += (long double)beta * (long double)C0[i + j * ldc];
@@ -320,6 +325,22 @@ ORC_API void orc_gemm_split_ref(int terms, int ta, int tb, int64_t m, int64_t n,
     }
   }
   free(bt);
+}
+
+/* The kept set of the split product with `terms` terms: pairs p + q < terms. */
+ORC_API int orc_split_kept_pairs(int terms) {
+  int mask = 0;
+  for (int p = 0; p < 3; ++p)
+    for (int q = 0; q < 3; ++q)
+      if (p + q < terms) mask |= 1 << (3 * p + q);
+  return mask;
+}
+
+ORC_API void orc_gemm_split_ref(int terms, int ta, int tb, int64_t m, int64_t n, int64_t k, double alpha,
+                                const float* A, int64_t lda, const float* B, int64_t ldb, double beta,
+                                const float* C0, int64_t ldc, const int64_t* rows, int64_t nrows, double* out) {
+  orc_gemm_split_pairs_ref(terms, orc_split_kept_pairs(terms), ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C0,
+                           ldc, rows, nrows, out);
 }
 
 ORC_API void orc_bf16_rtne_n(const float* x, int64_t n, uint16_t* out) {

@@ -83,6 +83,90 @@ def test_bf16_split_is_exact():
     assert [int(v) for v in inf3[0]] == [0x7F80, 0xFF80] and not inf3[1].any() and not inf3[2].any()
 
 
+# ---------------------------------------------------------------- split product value
+def test_split_kept_pairs_are_the_published_sets():
+    # bf16x3 keeps hi*hi, hi*mid, mid*hi; bf16x6 adds hi*lo, lo*hi, mid*mid
+    # (Henry, Tang & Heinecke 2019): the pairs p + q < terms, bit 3p + q
+    bits = lambda pairs: {(p, q) for p in range(3) for q in range(3) if O.split_kept_pairs(pairs) >> (3 * p + q) & 1}
+    assert bits(1) == {(0, 0)}
+    assert bits(2) == {(0, 0), (0, 1), (1, 0)}
+    assert bits(3) == {(0, 0), (0, 1), (1, 0), (0, 2), (2, 0), (1, 1)}
+
+
+@pytest.mark.parametrize("terms", [1, 2, 3])
+def test_split_ref_bf16_exact_inputs_equal_the_product(terms):
+    # integers |x| <= 256 are bfloat16 values: every lower term is zero and the
+    # split product is the exact product, bit for bit
+    A = O.fill((37, 53), np.float32, 71, O.INT, -256, 256)
+    B = O.fill((53, 29), np.float32, 72, O.INT, -256, 256)
+    C0 = O.fill((37, 29), np.float32, 73, O.INT, -8, 8)
+    want = O.gemm_ref(False, False, 1.25, A, B, 0.5, C0)
+    assert np.array_equal(O.gemm_split_ref(terms, False, False, 1.25, A, B, 0.5, C0), want)
+
+
+def test_split_ref_all_nine_pairs_is_the_exact_product():
+    # t0 + t1 + t2 == x exactly, so keeping all nine cross products gives the
+    # product itself (to long-double summation rounding)
+    A = O.fill((40, 300), np.float32, 74, O.WIDE, -8, 8)
+    B = O.fill((300, 30), np.float32, 75, O.WIDE, -8, 8)
+    ex, ab = O.gemm_ref(False, False, 1.0, A, B, with_abs=True)
+    s9 = O.gemm_split_ref(3, False, False, 1.0, A, B, pairs=0x1FF)
+    assert np.linalg.norm(s9 - ex) <= 2.0 ** -50 * np.linalg.norm(ab)
+
+
+@pytest.mark.parametrize("shape", [(64, 64, 1), (40, 30, 200), (20, 20, 2000)])
+def test_split_ref_error_bands(shape):
+    # What each kept set leaves out, relative to || |A||B| ||_F, on wide-range
+    # inputs.  Upper ends are deterministic bounds from the split: a term is at
+    # most 2^-8 of the one before, and two terms leave at most 2^-16 of x, so
+    # 1 term drops <= 2^-7, 2 terms <= 3*2^-16 < 2^-14, 3 terms (t1 t2 + t2 t1
+    # + t2 t2) <= 2^-23 < 2^-22.  Lower ends hold for random inputs (a split
+    # that rounded terms wrongly or kept fewer bits would leave the band); a
+    # kept set with a pair missing rises out of its band (checked below).  The
+    # sets themselves are pinned by test_split_kept_pairs_are_the_published_sets.
+    m, n, k = shape
+    A = O.fill((m, k), np.float32, 61, O.WIDE, -8, 8)
+    B = O.fill((k, n), np.float32, 62, O.WIDE, -8, 8)
+    ex, ab = O.gemm_ref(False, False, 1.0, A, B, with_abs=True)
+    nab = np.linalg.norm(ab)
+    bands = {1: (2.0 ** -13, 2.0 ** -7), 2: (2.0 ** -22, 2.0 ** -14), 3: (2.0 ** -32, 2.0 ** -22)}
+    for terms, (lo, hi) in bands.items():
+        rel = np.linalg.norm(O.gemm_split_ref(terms, False, False, 1.0, A, B) - ex) / nab
+        assert lo <= rel <= hi, (terms, np.log2(rel))
+    # each missing pair of the 3-term set lifts the error out of its band
+    kept = O.split_kept_pairs(3)
+    for b in range(9):
+        if kept >> b & 1:
+            rel = np.linalg.norm(O.gemm_split_ref(3, False, False, 1.0, A, B, pairs=kept ^ (1 << b)) - ex) / nab
+            assert rel > 2.0 ** -22, (b, np.log2(rel))
+
+
+def test_split_ref_single_pair_is_the_term_product():
+    # k = 1: the value of pair (p, q) alone is alpha * t_p(a_i) * t_q(b_j),
+    # with p indexing the terms of A and q those of B
+    a = O.fill((6, 1), np.float32, 81, O.WIDE, -8, 8)
+    b = O.fill((1, 5), np.float32, 82, O.WIDE, -8, 8)
+    ta = [O.bf16_to_f32(t).astype(np.float64) for t in O.bf16_split(a.ravel(), 3)]
+    tb = [O.bf16_to_f32(t).astype(np.float64) for t in O.bf16_split(b.ravel(), 3)]
+    for p in range(3):
+        for q in range(3):
+            got = O.gemm_split_ref(3, False, False, 0.75, a, b, pairs=1 << (3 * p + q))
+            assert np.array_equal(got, 0.75 * np.outer(ta[p], tb[q])), (p, q)
+
+
+def test_split_ref_transposes_rows_and_beta():
+    m, n, k = 23, 17, 41
+    A = O.fill((m, k), np.float32, 91, O.WIDE, -8, 8)
+    B = O.fill((k, n), np.float32, 92, O.WIDE, -8, 8)
+    C0 = O.fill((m, n), np.float32, 93)
+    want = O.gemm_split_ref(3, False, False, 1.5, A, B, 0.5, C0)
+    At, Bt = np.asfortranarray(A.T), np.asfortranarray(B.T)
+    assert np.array_equal(O.gemm_split_ref(3, True, True, 1.5, At, Bt, 0.5, C0), want)
+    assert np.array_equal(O.gemm_split_ref(3, False, False, 1.5, A, B, 0.5, C0, rows=[7, 2, 22]), want[[7, 2, 22]])
+    plain = O.gemm_split_ref(3, False, False, 1.5, A, B)
+    assert np.allclose(want - plain, 0.5 * C0.astype(np.float64), rtol=0, atol=1e-9 * np.abs(plain).max())
+
+
 # ---------------------------------------------------------------- reference product
 def test_gemm_ref_brute_force_tiny():
     rng = np.random.default_rng(0)
