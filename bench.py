@@ -36,12 +36,19 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def timed(fn, steps: int, warmup: int):
-    """Wall-clock ms per step and HIP-event ms per step on the launch stream."""
+def timed(fn, steps: int, warmup: int, kernels=()):
+    """Wall-clock ms per step and HIP-event ms per step on the launch stream.
+
+    `kernels`: kernel classes (eigen_amd.KC_*) whose launches inside the timed
+    steps are bracketed by HIP events on their own stream; their mean launch
+    time in ms is returned as a dict (warm-up launches are not recorded)."""
     import torch
+    import eigen_amd as E
     for _ in range(warmup):
         fn()
     torch.cuda.synchronize()
+    if kernels:
+        E.kernel_timing(True)
     ev0 = torch.cuda.Event(enable_timing=True)
     ev1 = torch.cuda.Event(enable_timing=True)
     t0 = time.perf_counter()
@@ -51,7 +58,15 @@ def timed(fn, steps: int, warmup: int):
     ev1.record()
     torch.cuda.synchronize()
     wall = (time.perf_counter() - t0) * 1e3 / steps
-    return wall, ev0.elapsed_time(ev1) / steps
+    kt = {}
+    for c in kernels:
+        ms, n = E.kernel_timing_read(c)
+        kt[c] = (ms / n if n else 0.0, n)
+    if kernels:
+        E.kernel_timing(False)
+    if not kernels:
+        return wall, ev0.elapsed_time(ev1) / steps
+    return wall, ev0.elapsed_time(ev1) / steps, kt
 
 
 def exact_product(A, B):
@@ -214,17 +229,13 @@ def main() -> int:
         if args.no_extra and name != "split3":
             continue
         steps = args.steps if name == "split3" else max(3, args.steps // 2)
-        E.kernel_timing(True)  # HIP events around each kernel launch, on its stream
-        wall, kern = timed(sgemm(prec), steps, args.warmup)
         prod_cls = E.KC_PRODUCT if name == "plain" else E.KC_PACKED_PRODUCT
-        prod_ms, prod_n = E.kernel_timing_read(prod_cls)
-        pack_ms, pack_n = E.kernel_timing_read(E.KC_PACK)
-        E.kernel_timing(False)
-        launches = steps + args.warmup
+        # HIP events around each kernel launch of the timed steps, on its stream
+        wall, kern, kt = timed(sgemm(prec), steps, args.warmup, (prod_cls, E.KC_PACK))
         err = sampled_rows_error(dA, dB, dC, m, n, k, rows, np.float32)
         res[name] = {"ms": wall, "event_ms": kern, "tflops": flop / (wall * 1e-3) / 1e12, "normwise_err": err,
-                     "product_kernel_ms": prod_ms / max(prod_n, 1), "product_launches": prod_n,
-                     "pack_kernel_ms": pack_ms / max(launches, 1) if pack_n else 0.0}
+                     "product_kernel_ms": kt[prod_cls][0], "product_launches": kt[prod_cls][1],
+                     "pack_kernel_ms": kt[E.KC_PACK][0] * kt[E.KC_PACK][1] / steps}
         log(f"{name}: {res[name]}")
 
     head = res["split3"]
@@ -299,13 +310,10 @@ def bench_dgemm(E, torch, dev, args):
         E.fill(E.F64, dB, k * n, 12, E.DIST_NORMAL, 0, 1)
         E.fill(E.F64, dC, m * n, 13, E.DIST_NORMAL, 0, 1)
         fn = lambda: E.gemm(E.F64, E.OP_N, E.OP_N, m, n, k, 1.5, dA, m, dB, k, 0.5, dC, m)
-        E.kernel_timing(True)
-        wall, kern = timed(fn, max(3, args.steps // 2), args.warmup)
-        kms, kn = E.kernel_timing_read(E.KC_PRODUCT)
-        E.kernel_timing(False)
+        wall, kern, kt = timed(fn, max(3, args.steps // 2), args.warmup, (E.KC_PRODUCT,))
         tf = 2.0 * m * n * k / (wall * 1e-3) / 1e12
         cfg = E.plan(E.F64, 0, 0, m, n, k)
-        out[name] = {"ms": wall, "event_ms": kern, "product_kernel_ms": kms / max(kn, 1), "tflops": tf,
+        out[name] = {"ms": wall, "event_ms": kern, "product_kernel_ms": kt[E.KC_PRODUCT][0], "tflops": tf,
                      "frac_of_fp64_peak": tf / PEAK["fp64"], "config": cfg, "config_tile_split": E.config_info(cfg),
                      "planner": ["heuristic", "mlp", "table"][E.lib().eh_get_tune_mode()]}
         # one more product from a fresh C for the error
@@ -438,13 +446,10 @@ def bench_types(E, torch, dev, args):
     B = (torch.rand(n * n, device=dev) * 2 - 1).to(torch.bfloat16)
     Cm = torch.zeros(n * n, device=dev)
     fn = lambda: E.gemm_bf16(0, 0, n, n, n, 1.0, A.data_ptr(), n, B.data_ptr(), n, 0.0, Cm, n)
-    E.kernel_timing(True)
-    wall, kern = timed(fn, max(3, args.steps // 2), args.warmup)
-    kms, kn = E.kernel_timing_read(E.KC_PACKED_PRODUCT)
-    E.kernel_timing(False)
+    wall, kern, kt = timed(fn, max(3, args.steps // 2), args.warmup, (E.KC_PACKED_PRODUCT,))
     tf = 2.0 * n ** 3 / (wall * 1e-3) / 1e12
     out["bf16_8192"] = {"ms": wall, "tflops": tf, "frac_of_bf16_peak": tf / PEAK["bf16"],
-                        "product_kernel_ms": kms / max(kn, 1),
+                        "product_kernel_ms": kt[E.KC_PACKED_PRODUCT][0],
                         "normwise_err": sampled_rows_error(A, B, Cm, n, n, n, np.arange(0, n, n // 32), np.float32)}
     del A, B, Cm
     n = 4096
@@ -497,14 +502,12 @@ def bench_gemv(E, torch, dev, args):
         u = float(np.finfo(np.float32 if dtype == E.F32 else np.float64).eps) / 2
         for ta in (0, 1):
             fn = lambda: E.gemm(dtype, ta, 0, n, 1, n, 1.0, A, n, x, n, 0.0, y, n)
-            E.kernel_timing(True)
-            wall, kern = timed(fn, max(10, args.steps), args.warmup)
-            kms, kn = E.kernel_timing_read(E.KC_GEMV)
-            E.kernel_timing(False)
-            gbs = n * n * es / (kms / max(kn, 1) * 1e-3) / 1e9
+            wall, kern, kt = timed(fn, max(10, args.steps), args.warmup, (E.KC_GEMV,))
+            kms = kt[E.KC_GEMV][0]
+            gbs = n * n * es / (kms * 1e-3) / 1e9
             ref = exact_product(Ah.T if ta else Ah, xh)[:, 0]
             err = normwise(y.double().cpu().numpy(), ref, n, u, np.linalg.norm(Ah), np.linalg.norm(xh))
-            out[f"{name}_{'T' if ta else 'N'}"] = {"ms": wall, "event_ms": kern, "kernel_ms": kms / max(kn, 1),
+            out[f"{name}_{'T' if ta else 'N'}"] = {"ms": wall, "event_ms": kern, "kernel_ms": kms,
                                                    "gbs": gbs, "frac_of_hbm": gbs / HBM_PEAK_GBS,
                                                    "normwise_err": err}
         del A, x, y
@@ -520,10 +523,7 @@ def bench_batched(E, torch, args):
     b = uneven_batch(20000, seed=0)
     g = E.Grouped(b.problems)
     wall_g, kern_g = timed(g.graph_launch, max(5, args.steps), args.warmup)
-    E.kernel_timing(True)
-    wall_r, _ = timed(g.run, max(5, args.steps // 2), 1)
-    kms, kn = E.kernel_timing_read(E.KC_GROUPED)
-    E.kernel_timing(False)
+    wall_r, _, kt = timed(g.run, max(5, args.steps // 2), 1, (E.KC_GROUPED,))
     wall_p, _ = timed(g.run_per_product, 2, 1)
     g.graph_launch()
     torch.cuda.synchronize()
@@ -532,7 +532,7 @@ def bench_batched(E, torch, args):
         A_, B_, C_ = (operand(b, p, w) for w in range(3))
         u = float(np.finfo(A_.dtype).eps) / 2
         errs.append(normwise(C_, exact_product(A_, B_), A_.shape[1], u, np.linalg.norm(A_), np.linalg.norm(B_)))
-    kernel_ms = kms / max(kn, 1)
+    kernel_ms = kt[E.KC_GROUPED][0]
     res = {"products": len(b.problems), "flop": b.flops, "bytes": b.bytes,
            "graph_ms": wall_g, "graph_event_ms": kern_g, "run_ms": wall_r, "kernel_ms": kernel_ms,
            "per_product_ms": wall_p,

@@ -104,12 +104,18 @@ static eh_precision default_sgemm_precision() {
 // ---- workspace --------------------------------------------------------------
 // One growable device buffer per (device, stream) for split-K partials, packed
 // operands and work lists; hipStreamPerThread names a different stream in
-// every thread, so it is keyed per thread as well.  A caller holds a lease
-// (a shared owner) on the buffer until its launches are queued: growing the
-// workspace only drops the table's owner, and the old buffer is freed when
-// its last lease goes.  The free is hipFree, which waits for the device, so
-// the work queued with the buffer completes before the memory is released.
-static std::map<std::tuple<int, hipStream_t, std::thread::id>, std::shared_ptr<Workspace>> g_ws;
+// every thread, so it is keyed per thread as well.  A call holds the stream's
+// lock (recursive: a call may re-enter through a sub-product) from taking the
+// buffer until its launches are queued, so calls from several threads on one
+// stream queue their launch sequences one after another.  Growing replaces
+// the buffer; the old one is freed when its last reference goes, with
+// hipFree, which waits for the device, so work queued with it completes
+// before the memory is released.
+struct WorkspaceSlot {
+  std::recursive_mutex mu;
+  std::shared_ptr<Workspace> buf;
+};
+static std::map<std::tuple<int, hipStream_t, std::thread::id>, std::unique_ptr<WorkspaceSlot>> g_ws;
 
 Workspace::~Workspace() {
   if (ptr) (void)hipFree(ptr);
@@ -131,8 +137,8 @@ bool fixed_scratch_active() { return t_fixed_ptr != nullptr; }
 eh_status workspace_get(hipStream_t stream, size_t bytes, WorkspaceLease* out, void** ptr) {
   if (t_fixed_ptr) {
     if (bytes > t_fixed_bytes)
-      return fail(EH_ERR_UNSUPPORTED, "product needs " + std::to_string(bytes) + " bytes of scratch; the fixed scratch holds " +
-                                          std::to_string(t_fixed_bytes));
+      return fail(EH_ERR_UNSUPPORTED, "product needs " + std::to_string(bytes) +
+                                          " bytes of scratch; the fixed scratch holds " + std::to_string(t_fixed_bytes));
     *ptr = t_fixed_ptr;
     return EH_SUCCESS;
   }
@@ -140,24 +146,31 @@ eh_status workspace_get(hipStream_t stream, size_t bytes, WorkspaceLease* out, v
   hipError_t e = hipGetDevice(&dev);
   if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
   const std::thread::id tid = stream == hipStreamPerThread ? std::this_thread::get_id() : std::thread::id();
-  std::shared_ptr<Workspace> old;  // released after the lock, outside the table
-  std::lock_guard<std::mutex> lk(g_mutex);
-  std::shared_ptr<Workspace>& w = g_ws[std::make_tuple(dev, stream, tid)];
-  if (!w || w->bytes < bytes) {
-    old = std::move(w);
+  WorkspaceSlot* slot;
+  {
+    std::lock_guard<std::mutex> lk(g_mutex);
+    std::unique_ptr<WorkspaceSlot>& sp = g_ws[std::make_tuple(dev, stream, tid)];
+    if (!sp) sp.reset(new WorkspaceSlot);
+    slot = sp.get();  // slots are never removed
+  }
+  WorkspaceLease lease;
+  lease.lock = std::unique_lock<std::recursive_mutex>(slot->mu);
+  if (!slot->buf || slot->buf->bytes < bytes) {
+    const size_t have = slot->buf ? slot->buf->bytes : 0;
     auto fresh = std::make_shared<Workspace>();
-    const size_t want = std::max(bytes, old ? std::max(old->bytes, (size_t)1 << 20) : (size_t)1 << 20);
+    const size_t want = std::max(bytes, std::max(have, (size_t)1 << 20));
+    slot->buf.reset();  // the old buffer goes once no lease refers to it
     e = hipMalloc(&fresh->ptr, want);
     if (e != hipSuccess) {
       fresh->ptr = nullptr;
-      w = std::move(old);
       return fail(EH_ERR_ALLOC, std::string("hipMalloc(workspace): ") + hipGetErrorString(e));
     }
     fresh->bytes = want;
-    w = std::move(fresh);
+    slot->buf = std::move(fresh);
   }
-  *out = w;
-  *ptr = w->ptr;
+  lease.buf = slot->buf;
+  *ptr = lease.buf->ptr;
+  *out = std::move(lease);
   return EH_SUCCESS;
 }
 
@@ -167,7 +180,7 @@ static std::mutex g_timing_mutex;
 static std::vector<std::pair<hipEvent_t, hipEvent_t>> g_timed[EH_KC_COUNT];
 
 KernelTimer::KernelTimer(int c, hipStream_t st) : cls(c), s(st) {
-  if (!g_timing_on.load(std::memory_order_relaxed)) return;
+  if (!g_timing_on.load(std::memory_order_relaxed) || fixed_scratch_active()) return;
   hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
   if (hipStreamIsCapturing(s, &cap) != hipSuccess || cap != hipStreamCaptureStatusNone) return;
   if (hipEventCreate(&start) != hipSuccess) {

@@ -4,6 +4,7 @@
 #include <stdint.h>
 
 #include <memory>
+#include <mutex>
 #include <string>
 
 #include "../../include/eigen_amd.h"
@@ -26,8 +27,11 @@ eh_tune_mode tune_mode();
 // Planner (planner.cpp): configuration for a shape under the current mode.
 eh_status plan_config(int dtype, int opa, int opb, int64_t m, int64_t n, int64_t k, int* config);
 
-// Per-stream device workspace (capi.cpp).  The lease keeps the memory
-// allocated while it lives; hold it until the launches that use it are queued.
+// Per-stream device workspace (capi.cpp).  A lease holds the stream's
+// workspace lock and a reference to its buffer: hold it until every launch
+// of the call that uses the buffer is queued, so that another thread's call
+// on the same stream can neither interleave its launches with ours (both
+// would use the one buffer) nor free the buffer under us.
 struct Workspace {
   void* ptr = nullptr;
   size_t bytes = 0;
@@ -36,12 +40,15 @@ struct Workspace {
   Workspace& operator=(const Workspace&) = delete;
   ~Workspace();
 };
-using WorkspaceLease = std::shared_ptr<Workspace>;
+struct WorkspaceLease {
+  std::unique_lock<std::recursive_mutex> lock;
+  std::shared_ptr<Workspace> buf;
+};
 // *ptr = `bytes` of device workspace for work queued on `stream`.  Normally
-// the per-stream workspace (grown with hipMalloc on demand; *lease keeps it
-// alive).  While a FixedScratch is installed on the calling thread, *ptr is
-// that caller-provided buffer instead (no allocation, *lease stays empty),
-// and a request larger than it fails with EH_ERR_UNSUPPORTED.
+// the per-stream workspace (grown with hipMalloc on demand).  While a
+// FixedScratch is installed on the calling thread, *ptr is that
+// caller-provided buffer instead (no allocation, no lock), and a request
+// larger than it fails with EH_ERR_UNSUPPORTED.
 eh_status workspace_get(hipStream_t stream, size_t bytes, WorkspaceLease* lease, void** ptr);
 
 // Installs a fixed device buffer as this thread's workspace for its lifetime

@@ -1,10 +1,14 @@
 """Short driver for profiler runs, and the summariser of their output.
 
-    python3 -m eigen_amd.profile_kernels run {split3,plain,dgemm,batched,dtrsm,zgemm,bf16,copy} [reps]
-    python3 -m eigen_amd.profile_kernels summarize <rocprofv3 output root>
+    python3 -m eigen_amd.profile_kernels run {split3,plain,dgemm,batched,dtrsm,zgemm,bf16,copy} [reps] [warmup]
+    python3 -m eigen_amd.profile_kernels summarize <rocprofv3 output root> [traffic.json] [--skip N]
 
-`run` launches one workload a few times (the program under
-`rocprofv3 --kernel-trace --stats` or `rocprofv3 --pmc`):
+`run` launches one workload `warmup` times (default 2), then `reps` times
+with the library's kernel event timing on (HIP events around each launch of
+the dominant kernel class, on its stream), and prints the mean event-timed
+launch of the dominant class ("event_ms ..."): the in-process figure the
+profiler's kernel trace of the same command is compared with.  It is the
+program under `rocprofv3 --kernel-trace --stats` or `rocprofv3 --pmc`:
 
   split3   sgemm 8192^3, split-BF16 three terms (pack A, pack B, packed product)
   plain    sgemm 8192^3 on FP32 MFMA
@@ -15,7 +19,9 @@
   bf16     bf16 inputs, FP32 result, 8192^3 (U[-1,1] rounded to bf16)
   copy     a 1 GiB device-to-device copy: calibrates the HBM byte counters
 
-`summarize` reads directories named trace_<workload> (kernel-trace csv),
+`summarize` reads directories named trace_<workload> (kernel-trace csv; with
+--skip N the first N dispatches of every kernel -- the warm-up launches --
+are left out of the per-dispatch averages),
 pmc_<workload>_<FETCH_SIZE|WRITE_SIZE> and util_<workload> (counter csv of
 one pass over UTIL_COUNTERS) under the given root and
 prints per-kernel duration tables and counter readings as markdown, plus the
@@ -38,12 +44,35 @@ DOMINANT = {"split3": "gemm_split_packed_kernel", "plain": "gemm_mfma_kernel<flo
             "bf16": "gemm_split_packed_kernel", "copy": "copyBuffer"}
 
 
-def run(what: str, reps: int) -> None:
+KCLASS = {"split3": "KC_PACKED_PRODUCT", "bf16": "KC_PACKED_PRODUCT", "plain": "KC_PRODUCT", "dgemm": "KC_PRODUCT",
+          "zgemm": "KC_PRODUCT", "dtrsm": "KC_PRODUCT", "batched": "KC_GROUPED"}
+
+
+def run(what: str, reps: int, warmup: int = 2) -> None:
     import torch
 
     import eigen_amd as E
 
     E.lib()
+    step = _workload(E, torch, what)
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    cls = getattr(E, KCLASS[what]) if what in KCLASS else None
+    if cls is not None:
+        E.kernel_timing(True)
+    for _ in range(reps):
+        step()
+    torch.cuda.synchronize()
+    if cls is not None:
+        ms, n = E.kernel_timing_read(cls)
+        E.kernel_timing(False)
+        print(f"event_ms {what} {KCLASS[what]} {ms / max(n, 1):.4f} over {n} launches", flush=True)
+    print("done", what, flush=True)
+
+
+def _workload(E, torch, what: str):
+    """The workload's inputs on the device and a function running one step."""
     if what in ("split3", "plain"):
         n = 8192
         A = torch.empty(n * n, dtype=torch.float32, device="cuda")
@@ -52,21 +81,18 @@ def run(what: str, reps: int) -> None:
         E.fill(E.F32, A, n * n, 1, E.DIST_WIDE, -8, 8)
         E.fill(E.F32, B, n * n, 2, E.DIST_WIDE, -8, 8)
         prec = E.PREC_SPLIT3 if what == "split3" else E.PREC_PLAIN
-        for _ in range(reps):
-            E.gemm(E.F32, 0, 0, n, n, n, 1.0, A, n, B, n, 0.0, C, n, prec=prec)
+        return lambda: E.gemm(E.F32, 0, 0, n, n, n, 1.0, A, n, B, n, 0.0, C, n, prec=prec)
     elif what == "dgemm":
         n = 4096
         A, B, C = (torch.empty(n * n, dtype=torch.float64, device="cuda") for _ in range(3))
         for i, t in enumerate((A, B, C)):
             E.fill(E.F64, t, n * n, 11 + i, E.DIST_NORMAL, 0, 1)
-        for _ in range(reps):
-            E.gemm(E.F64, 0, 0, n, n, n, 1.5, A, n, B, n, 0.5, C, n)
+        return lambda: E.gemm(E.F64, 0, 0, n, n, n, 1.5, A, n, B, n, 0.5, C, n)
     elif what == "batched":
         from eigen_amd.batched import uneven_batch
         b = uneven_batch(20000, seed=0)
         g = E.Grouped(b.problems)
-        for _ in range(reps):
-            g.graph_launch()
+        return lambda: (g.run(), b)  # b owns the operand buffers
     elif what == "dtrsm":
         n = 8192
         A = torch.empty(n * n, dtype=torch.float64, device="cuda")
@@ -74,29 +100,22 @@ def run(what: str, reps: int) -> None:
         E.fill(E.F64, A, n * n, 31, E.DIST_UNIFORM, -1.0 / n, 1.0 / n)
         A.view(n, n).diagonal().fill_(2.0)
         E.fill(E.F64, B, n * n, 32, E.DIST_UNIFORM, -1, 1)
-        for _ in range(reps):
-            E.trsm(E.F64, E.LEFT, E.LOWER, E.OP_N, E.NON_UNIT, n, n, 1.0, A, n, B, n)
+        return lambda: E.trsm(E.F64, E.LEFT, E.LOWER, E.OP_N, E.NON_UNIT, n, n, 1.0, A, n, B, n)
     elif what == "zgemm":
         n = 4096
         A, B, Cm = (torch.rand(2 * n * n, dtype=torch.float64, device="cuda") * 2 - 1 for _ in range(3))
-        for _ in range(reps):
-            E.gemm_complex(E.F64, 0, 0, n, n, n, 1.0 + 0.5j, A, n, B, n, 0.5, Cm, n)
+        return lambda: E.gemm_complex(E.F64, 0, 0, n, n, n, 1.0 + 0.5j, A, n, B, n, 0.5, Cm, n)
     elif what == "bf16":
         n = 8192
         A = (torch.rand(n * n, device="cuda") * 2 - 1).to(torch.bfloat16)
         B = (torch.rand(n * n, device="cuda") * 2 - 1).to(torch.bfloat16)
         Cm = torch.zeros(n * n, device="cuda")
-        for _ in range(reps):
-            E.gemm_bf16(0, 0, n, n, n, 1.0, A.data_ptr(), n, B.data_ptr(), n, 0.0, Cm, n)
+        return lambda: E.gemm_bf16(0, 0, n, n, n, 1.0, A.data_ptr(), n, B.data_ptr(), n, 0.0, Cm, n)
     elif what == "copy":
         x = torch.empty(COPY_BYTES // 4, dtype=torch.float32, device="cuda")
         y = torch.empty_like(x)
-        for _ in range(reps):
-            y.copy_(x)
-    else:
-        raise SystemExit(f"unknown workload {what}")
-    torch.cuda.synchronize()
-    print("done", what, flush=True)
+        return lambda: y.copy_(x)
+    raise SystemExit(f"unknown workload {what}")
 
 
 def _one(d: str, pattern: str):
@@ -104,18 +123,28 @@ def _one(d: str, pattern: str):
     return list(csv.DictReader(open(f[0]))) if f else None
 
 
-def kernel_table(d: str):
+def kernel_table(d: str, skip: int = 0):
+    """Per-kernel table of a --kernel-trace --stats run.  The stats columns
+    cover every dispatch; 'timed avg' leaves out the first `skip` dispatches
+    of each kernel (the warm-up launches) using the per-dispatch trace."""
     rows = _one(d, "*kernel_stats.csv")
     if rows is None:
         return None, {}
-    out = ["| kernel | calls | avg (us) | min (us) | max (us) | % of GPU time |", "|---|---|---|---|---|---|"]
+    per = {}
+    for r in _one(d, "*kernel_trace.csv") or []:
+        per.setdefault(r["Kernel_Name"], []).append(
+            (int(r["Dispatch_Id"]), (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3))
+    out = ["| kernel | calls | avg (us) | min (us) | max (us) | % of GPU time | timed avg (us) |",
+           "|---|---|---|---|---|---|---|"]
     avg = {}
     for r in rows:
         name = r["Name"]
-        avg[name] = float(r["AverageNs"]) / 1e3
+        ds = [t for _, t in sorted(per.get(name, []))][skip:]
+        avg[name] = sum(ds) / len(ds) if ds else float(r["AverageNs"]) / 1e3
         short = name if len(name) <= 100 else name[:100] + "..."
-        out.append(f"| `{short}` | {r['Calls']} | {avg[name]:.1f} | {float(r['MinNs']) / 1e3:.1f} | "
-                   f"{float(r['MaxNs']) / 1e3:.1f} | {float(r['Percentage']):.1f} |")
+        out.append(f"| `{short}` | {r['Calls']} | {float(r['AverageNs']) / 1e3:.1f} | {float(r['MinNs']) / 1e3:.1f} | "
+                   f"{float(r['MaxNs']) / 1e3:.1f} | {float(r['Percentage']):.1f} | "
+                   f"{avg[name]:.1f} ({len(ds)}) |")
     return "\n".join(out), avg
 
 
@@ -170,7 +199,7 @@ def _dominant(values: dict, workload: str):
     return None, None
 
 
-def summarize(root: str) -> dict:
+def summarize(root: str, skip: int = 0) -> dict:
     traffic = {}
     scale = {}
     for c in ("FETCH_SIZE", "WRITE_SIZE"):
@@ -182,7 +211,7 @@ def summarize(root: str) -> dict:
         print(f"- {c}: {COPY_BYTES / s:.1f} KiB read by the counter for {COPY_BYTES} bytes -> x{s / 1024:.4f} KiB")
     print()
     for d in sorted(glob.glob(os.path.join(root, "trace_*"))):
-        table, _ = kernel_table(d)
+        table, _ = kernel_table(d, skip)
         if table:
             print(f"## {os.path.basename(d)}\n\n{table}\n")
     workloads = sorted({os.path.basename(p).split("_")[1] for p in glob.glob(os.path.join(root, "pmc_*"))})
@@ -219,10 +248,15 @@ def summarize(root: str) -> dict:
 def main(argv=None) -> int:
     argv = sys.argv[1:] if argv is None else argv
     if len(argv) >= 2 and argv[0] == "run":
-        run(argv[1], int(argv[2]) if len(argv) > 2 else 5)
+        run(argv[1], int(argv[2]) if len(argv) > 2 else 5, int(argv[3]) if len(argv) > 3 else 2)
         return 0
     if len(argv) >= 2 and argv[0] == "summarize":
-        traffic = summarize(argv[1])
+        skip = 0
+        if "--skip" in argv:
+            i = argv.index("--skip")
+            skip = int(argv[i + 1])
+            argv = argv[:i] + argv[i + 2:]
+        traffic = summarize(argv[1], skip)
         if len(argv) > 2:
             with open(argv[2], "w") as f:
                 json.dump(traffic, f, indent=1)
