@@ -117,6 +117,8 @@ def lib():
         vp, i64, i32, u64, f64 = C.c_void_p, C.c_int64, C.c_int, C.c_uint64, C.c_double
         L.eh_gemm.restype = i32
         L.eh_gemm.argtypes = [i32, i32, i32, i64, i64, i64, vp, vp, i64, vp, i64, vp, vp, i64, i32, vp]
+        L.eh_gemm_noalloc.restype = i32
+        L.eh_gemm_noalloc.argtypes = [i32, i32, i32, i64, i64, i64, vp, vp, i64, vp, i64, vp, vp, i64, i32, vp]
         L.eh_gemm_config.restype = i32
         L.eh_gemm_config.argtypes = [i32, i32, i32, i64, i64, i64, vp, vp, i64, vp, i64, vp, vp, i64, i32, i32, vp]
         L.eh_syrk.restype = i32

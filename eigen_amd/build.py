@@ -26,7 +26,7 @@ HIPFLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-Wall", "-W
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 LIB_SOURCES = ["gemm_f32.hip", "gemm_f64.hip", "gemm_split.hip", "gemv.hip", "level3_kernels.hip", "fill.hip", "grouped.hip",
-               "mlp.hip", "complex_kernels.hip", "level3.cpp", "complex_gemm.cpp",
+               "mlp.hip", "complex_kernels.hip", "noalloc.hip", "level3.cpp", "complex_gemm.cpp",
                "capi.cpp", "planner.cpp", "tuner.cpp"]
 BLAS_SOURCES = ["blas.cpp"]
 # Per-file code generation choices.  The FP64 kernels keep their

@@ -106,6 +106,26 @@ eh_status eh_gemm(eh_dtype dtype, eh_op opa, eh_op opb, int64_t m, int64_t n, in
                   const void* A, int64_t lda, const void* B, int64_t ldb, const void* beta, void* C, int64_t ldc,
                   eh_precision prec, void* stream);
 
+/* eh_gemm with no device allocation.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under hipGraph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments,
+ * dispatch, kernels and results (bit for bit) as eh_gemm; the workspace
+ * eh_gemm would take from its per-stream pool (split-K partial sums, the
+ * packed bf16 planes of a split-precision product, matrix-vector partial
+ * sums) is instead one fixed device array of EH_NOALLOC_SCRATCH_BYTES per
+ * GPU, part of the library and resident until the process ends.  A product
+ * that needs more scratch than that returns EH_ERR_UNSUPPORTED without
+ * launching anything (e.g. a split-BF16 product needs 6 bytes per element of
+ * op(A) and op(B) with 3 terms, 4 with 2).  The array is shared by all calls
+ * on one GPU: calls that overlap in time on the same GPU race on it, so the
+ * caller serialises them (one stream, or an event between streams).  The
+ * planner never tunes inside this call (EH_TUNE_TABLE falls back to the
+ * network or the heuristic for shapes it has not timed). */
+#define EH_NOALLOC_SCRATCH_BYTES (64u << 20)
+eh_status eh_gemm_noalloc(eh_dtype dtype, eh_op opa, eh_op opb, int64_t m, int64_t n, int64_t k, const void* alpha,
+                          const void* A, int64_t lda, const void* B, int64_t ldb, const void* beta, void* C,
+                          int64_t ldc, eh_precision prec, void* stream);
+
 /* Same, with an explicit configuration index in [0, eh_num_configs()) for
  * the plain kernels (config < 0 = let the planner choose). */
 eh_status eh_gemm_config(eh_dtype dtype, eh_op opa, eh_op opb, int64_t m, int64_t n, int64_t k, const void* alpha,
