@@ -28,7 +28,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAK = {"bf16": 2500.0, "fp32": 157.3, "fp64": 78.6}  # dense MFMA TFLOP/s, MI355X
+PEAK = {"bf16": 2500.0, "fp16": 2500.0, "fp32": 157.3, "fp64": 78.6}  # dense MFMA TFLOP/s, MI355X
 HBM_PEAK_GBS = 8000.0
 
 
@@ -436,8 +436,8 @@ def bench_batched_strided(E, torch, dev, args):
 
 
 def bench_types(E, torch, dev, args):
-    """Other scalar types: bf16 inputs with an FP32 result (8192^3, U[-1,1]
-    rounded to bf16; 2mnk flops) and complex products cgemm / zgemm (4096^3,
+    """Other scalar types: bf16 and IEEE half inputs with an FP32 result (8192^3,
+    U[-1,1] rounded to bf16 / half; 2mnk flops) and complex products cgemm / zgemm (4096^3,
     real and imaginary parts U[-1,1]; 8mnk real flops).  Errors on sampled
     rows against the exact product of the same (widened) inputs."""
     out = {}
@@ -451,6 +451,17 @@ def bench_types(E, torch, dev, args):
     out["bf16_8192"] = {"ms": wall, "tflops": tf, "frac_of_bf16_peak": tf / PEAK["bf16"],
                         "product_kernel_ms": kt[E.KC_PACKED_PRODUCT][0],
                         "normwise_err": sampled_rows_error(A, B, Cm, n, n, n, np.arange(0, n, n // 32), np.float32)}
+    del A, B, Cm
+    # IEEE half inputs, FP32 result, on the FP16 matrix cores (same tiles)
+    A = (torch.rand(n * n, device=dev) * 2 - 1).to(torch.float16)
+    B = (torch.rand(n * n, device=dev) * 2 - 1).to(torch.float16)
+    Cm = torch.zeros(n * n, device=dev)
+    fn = lambda: E.gemm_f16(0, 0, n, n, n, 1.0, A.data_ptr(), n, B.data_ptr(), n, 0.0, Cm, n)
+    wall, kern, kt = timed(fn, max(3, args.steps // 2), args.warmup, (E.KC_PACKED_PRODUCT,))
+    tf = 2.0 * n ** 3 / (wall * 1e-3) / 1e12
+    out["f16_8192"] = {"ms": wall, "tflops": tf, "frac_of_fp16_peak": tf / PEAK["fp16"],
+                       "product_kernel_ms": kt[E.KC_PACKED_PRODUCT][0],
+                       "normwise_err": sampled_rows_error(A, B, Cm, n, n, n, np.arange(0, n, n // 32), np.float32)}
     del A, B, Cm
     n = 4096
     rows = np.arange(0, n, n // 16)

@@ -143,6 +143,8 @@ def lib():
         L.eh_gemm_complex.argtypes = [i32, i32, i32, i64, i64, i64, vp, vp, i64, vp, i64, vp, vp, i64, i32, vp]
         L.eh_gemm_bf16.restype = i32
         L.eh_gemm_bf16.argtypes = [i32, i32, i64, i64, i64, vp, vp, i64, vp, i64, vp, vp, i64, vp]
+        L.eh_gemm_f16.restype = i32
+        L.eh_gemm_f16.argtypes = [i32, i32, i64, i64, i64, vp, vp, i64, vp, i64, vp, vp, i64, vp]
         L.eh_gemm_strided.restype = i32
         L.eh_gemm_strided.argtypes = [i32, i32, i32, i64, i64, i64, vp, vp, i64, vp, i64, vp, vp, i64, i64, i32, vp]
         L.eh_num_configs.restype = i32
@@ -200,6 +202,15 @@ def _declare_extra(L) -> None:
         "eh_mlp_predict": (i32, [vp, vp, i64, vp, vp]),
         "eh_planner_set_mlp": (i32, [vp, vp]),
         "eh_planner_set_mlp_params": (i32, [vp]),
+        "eh_trsm_complex": (i32, [i32] * 5 + [i64, i64, vp, vp, i64, vp, i64, vp]),
+        "eh_trmm_complex": (i32, [i32] * 5 + [i64, i64, vp, vp, i64, vp, i64, vp]),
+        "eh_symm_complex": (i32, [i32, i32, i32, i64, i64, vp, vp, i64, vp, i64, vp, vp, i64, vp]),
+        "eh_hemm": (i32, [i32, i32, i32, i64, i64, vp, vp, i64, vp, i64, vp, vp, i64, vp]),
+        "eh_syrk_complex": (i32, [i32, i32, i32, i64, i64, vp, vp, i64, vp, vp, i64, vp]),
+        "eh_herk": (i32, [i32, i32, i32, i64, i64, vp, vp, i64, vp, vp, i64, vp]),
+        "eh_syr2k_complex": (i32, [i32, i32, i32, i64, i64, vp, vp, i64, vp, i64, vp, vp, i64, vp]),
+        "eh_her2k": (i32, [i32, i32, i32, i64, i64, vp, vp, i64, vp, i64, vp, vp, i64, vp]),
+        "eh_gemmtr_complex": (i32, [i32, i32, i32, i32, i64, i64, vp, vp, i64, vp, i64, vp, vp, i64, vp]),
         "eh_kernel_timing": (i32, [i32]),
         "eh_kernel_timing_read": (i32, [i32, C.POINTER(f64), C.POINTER(i64)]),
     }
@@ -320,6 +331,15 @@ def gemm_bf16(opa: int, opb: int, m: int, n: int, k: int, alpha: float, A, lda: 
     s = current_stream() if stream is None else stream
     check(lib().eh_gemm_bf16(opa, opb, m, n, k, C.byref(a), _ptr(A), lda, _ptr(B), ldb, C.byref(b), _ptr(Cm), ldc, s),
           "eh_gemm_bf16")
+
+
+def gemm_f16(opa: int, opb: int, m: int, n: int, k: int, alpha: float, A, lda: int, B, ldb: int, beta: float, Cm,
+             ldc: int, stream=None) -> None:
+    """C (float32) = alpha*op(A)*op(B) + beta*C with A, B IEEE half (eh_gemm_f16)."""
+    a, b = C.c_float(alpha), C.c_float(beta)
+    s = current_stream() if stream is None else stream
+    check(lib().eh_gemm_f16(opa, opb, m, n, k, C.byref(a), _ptr(A), lda, _ptr(B), ldb, C.byref(b), _ptr(Cm), ldc, s),
+          "eh_gemm_f16")
 
 
 def gemmtr(dtype: int, uplo: int, opa: int, opb: int, n: int, k: int, alpha: float, A, lda: int, B, ldb: int,

@@ -65,6 +65,33 @@ bool is_device_ptr(const void* p) {
 
 void report(const char* what) { std::fprintf(stderr, "eigen_amd BLAS: %s: %s\n", what, eh_last_error()); }
 
+// dtype of an element type: float and complex<float> run as EH_F32,
+// double and complex<double> as EH_F64
+template <typename T>
+struct DtOf {
+  static constexpr eh_dtype value = sizeof(T) == 4 ? EH_F32 : EH_F64;
+};
+template <typename R>
+struct DtOf<std::complex<R>> {
+  static constexpr eh_dtype value = sizeof(R) == 4 ? EH_F32 : EH_F64;
+};
+template <typename T>
+constexpr eh_dtype dtype_of() {
+  return DtOf<T>::value;
+}
+
+// C-ABI signatures the level-3 wrappers dispatch to (real or complex entry)
+using SymmFn = eh_status (*)(eh_dtype, eh_side, eh_uplo, int64_t, int64_t, const void*, const void*, int64_t,
+                             const void*, int64_t, const void*, void*, int64_t, void*);
+using TriFn = eh_status (*)(eh_dtype, eh_side, eh_uplo, eh_op, eh_diag, int64_t, int64_t, const void*, const void*,
+                            int64_t, void*, int64_t, void*);
+using Rank2Fn = eh_status (*)(eh_dtype, eh_uplo, eh_op, int64_t, int64_t, const void*, const void*, int64_t,
+                              const void*, int64_t, const void*, void*, int64_t, void*);
+using RankFn = eh_status (*)(eh_dtype, eh_uplo, eh_op, int64_t, int64_t, const void*, const void*, int64_t,
+                             const void*, void*, int64_t, void*);
+using GemmtrFn = eh_status (*)(eh_dtype, eh_uplo, eh_op, eh_op, int64_t, int64_t, const void*, const void*, int64_t,
+                               const void*, int64_t, const void*, void*, int64_t, void*);
+
 template <typename T>
 void gemm_impl(const char* name, const char* opa, const char* opb, const int* m, const int* n, const int* k,
                const T* palpha, const T* a, const int* lda, const T* b, const int* ldb, const T* pbeta, T* c,
@@ -157,15 +184,18 @@ void gemm_impl(const char* name, const char* opa, const char* opb, const int* m,
 // ?syrk_ with the semantics of blas/level3_impl.h:594-689: validation order
 // and info codes 1 (uplo), 2 (trans), 3 (n), 4 (k), 7 (lda), 10 (ldc); only
 // the uplo triangle of C is read or written.
-template <typename T>
-void syrk_impl(const char* name, const char* uplo, const char* op, const int* n, const int* k, const T* palpha,
-               const T* a, const int* lda, const T* pbeta, T* c, const int* ldc) {
+// Also complex ?syrk_ (trans C invalid) and ?herk_ (blas/level3_impl.h:835-912:
+// alpha and beta real, trans T invalid): element type T, scalar type S.
+template <typename T, typename S = T>
+void syrk_impl(const char* name, const char* uplo, const char* op, const int* n, const int* k, const S* palpha,
+               const T* a, const int* lda, const S* pbeta, T* c, const int* ldc, RankFn fn = eh_syrk,
+               int bad_op = OP_BAD) {
   int info = 0;
   const int ul = (*uplo == 'U' || *uplo == 'u') ? EH_UPPER : (*uplo == 'L' || *uplo == 'l') ? EH_LOWER : -1;
   const int o = op_code(*op);
   if (ul < 0)
     info = 1;
-  else if (o == OP_BAD)
+  else if (o == OP_BAD || o == bad_op)
     info = 2;
   else if (*n < 0)
     info = 3;
@@ -179,11 +209,11 @@ void syrk_impl(const char* name, const char* uplo, const char* op, const int* n,
     xerbla_(name, &info, 6);
     return;
   }
-  const T alpha = *palpha, beta = *pbeta;
-  const bool need_a = *k > 0 && alpha != T(0);
-  if (*n == 0 || (!need_a && beta == T(1))) return;
+  const S alpha = *palpha, beta = *pbeta;
+  const bool need_a = *k > 0 && alpha != S(0);
+  if (*n == 0 || (!need_a && beta == S(1))) return;
 
-  const eh_dtype dt = sizeof(T) == 4 ? EH_F32 : EH_F64;
+  const eh_dtype dt = dtype_of<T>();
   const int arows = o == OP_N ? *n : *k, acols = o == OP_N ? *k : *n;
   const bool dev_a = !need_a || is_device_ptr(a);
   const bool dev_c = is_device_ptr(c);
@@ -209,7 +239,7 @@ void syrk_impl(const char* name, const char* uplo, const char* op, const int* n,
     dc = static_cast<T*>(g_sc.ptr);
     dldc = *n;
   }
-  eh_status st = eh_syrk(dt, (eh_uplo)ul, (eh_op)o, *n, *k, &alpha, da, dlda, &beta, dc, dldc, s);
+  eh_status st = fn(dt, (eh_uplo)ul, (eh_op)o, *n, *k, &alpha, da, dlda, &beta, dc, dldc, s);
   if (st != EH_SUCCESS) return report(name);
   if (!dev_c && hipMemcpy2DAsync(c, (size_t)(*ldc) * sizeof(T), dc, (size_t)(*n) * sizeof(T),
                                  (size_t)(*n) * sizeof(T), *n, hipMemcpyDeviceToHost, s) != hipSuccess)
@@ -265,14 +295,12 @@ void finish(hipStream_t s) {
   if (hipStreamSynchronize(s) != hipSuccess) std::fprintf(stderr, "eigen_amd BLAS: stream sync failed\n");
 }
 
-constexpr eh_dtype dtype_of(size_t es) { return es == 4 ? EH_F32 : EH_F64; }
-
 // ?gemmtr_ with the checks of blas/level3_impl.h:181-197: info 1 uplo,
 // 2 transa, 3 transb, 4 n, 5 k, 8 lda, 10 ldb, 13 ldc.
 template <typename T>
 void gemmtr_impl(const char* name, const char* uplo, const char* opa, const char* opb, const int* n, const int* k,
                  const T* palpha, const T* a, const int* lda, const T* b, const int* ldb, const T* pbeta, T* c,
-                 const int* ldc) {
+                 const int* ldc, GemmtrFn fn = eh_gemmtr) {
   int info = 0;
   const int ul = uplo_code(*uplo), oa = op_code(*opa), ob = op_code(*opb);
   if (ul < 0)
@@ -303,24 +331,26 @@ void gemmtr_impl(const char* name, const char* uplo, const char* opa, const char
                    !dev_view(g_sb, b, ob == OP_N ? *k : *n, ob == OP_N ? *n : *k, *ldb, true, s, &B))) ||
       !dev_view(g_sc, c, *n, *n, *ldc, true, s, &C))  // all of C travels: the other triangle returns unchanged
     return (void)std::fprintf(stderr, "eigen_amd BLAS: staging failed\n");
-  if (eh_gemmtr(dtype_of(sizeof(T)), (eh_uplo)ul, (eh_op)oa, (eh_op)ob, *n, *k, &alpha, A.p, A.ld, B.p, B.ld, &beta,
-                C.p, C.ld, s) != EH_SUCCESS)
+  if (fn(dtype_of<T>(), (eh_uplo)ul, (eh_op)oa, (eh_op)ob, *n, *k, &alpha, A.p, A.ld, B.p, B.ld, &beta, C.p, C.ld,
+         s) != EH_SUCCESS)
     return report(name);
   if (!copy_back(C, c, s)) std::fprintf(stderr, "eigen_amd BLAS: copying C back failed\n");
   finish(s);
 }
 
 // ?syr2k_ with the checks of blas/level3_impl.h:702-716: info 1 uplo,
-// 2 trans, 3 n, 4 k, 7 lda, 9 ldb, 12 ldc.
-template <typename T>
+// 2 trans, 3 n, 4 k, 7 lda, 9 ldb, 12 ldc; also ?her2k_ (:915-942, beta real,
+// trans T invalid) and complex ?syr2k_ (trans C invalid): `bad_op`.
+template <typename T, typename SB = T>
 void syr2k_impl(const char* name, const char* uplo, const char* op, const int* n, const int* k, const T* palpha,
-                const T* a, const int* lda, const T* b, const int* ldb, const T* pbeta, T* c, const int* ldc) {
+                const T* a, const int* lda, const T* b, const int* ldb, const SB* pbeta, T* c, const int* ldc,
+                Rank2Fn fn = eh_syr2k, int bad_op = OP_BAD) {
   int info = 0;
   const int ul = uplo_code(*uplo), o = op_code(*op);
   const int rows = o == OP_N ? *n : *k, cols = o == OP_N ? *k : *n;
   if (ul < 0)
     info = 1;
-  else if (o == OP_BAD)
+  else if (o == OP_BAD || o == bad_op)
     info = 2;
   else if (*n < 0)
     info = 3;
@@ -334,9 +364,10 @@ void syr2k_impl(const char* name, const char* uplo, const char* op, const int* n
     info = 12;
   if (info) return xerbla_(name, &info, 6);
   if (*n == 0) return;
-  const T alpha = *palpha, beta = *pbeta;
+  const T alpha = *palpha;
+  const SB beta = *pbeta;
   const bool need_ab = *k > 0 && alpha != T(0);
-  if (!need_ab && beta == T(1)) return;
+  if (!need_ab && beta == SB(1)) return;
   hipStream_t s = g_stream;
   std::lock_guard<std::mutex> lk(g_stage_mutex);
   DevMat<T> A(a, *lda), B(b, *ldb), C(c, *ldc);
@@ -344,7 +375,7 @@ void syr2k_impl(const char* name, const char* uplo, const char* op, const int* n
                    !dev_view(g_sb, b, rows, cols, *ldb, true, s, &B))) ||
       !dev_view(g_sc, c, *n, *n, *ldc, true, s, &C))
     return (void)std::fprintf(stderr, "eigen_amd BLAS: staging failed\n");
-  if (eh_syr2k(dtype_of(sizeof(T)), (eh_uplo)ul, (eh_op)o, *n, *k, &alpha, A.p, A.ld, B.p, B.ld, &beta, C.p, C.ld, s) !=
+  if (fn(dtype_of<T>(), (eh_uplo)ul, (eh_op)o, *n, *k, &alpha, A.p, A.ld, B.p, B.ld, &beta, C.p, C.ld, s) !=
       EH_SUCCESS)
     return report(name);
   if (!copy_back(C, c, s)) std::fprintf(stderr, "eigen_amd BLAS: copying C back failed\n");
@@ -355,7 +386,8 @@ void syr2k_impl(const char* name, const char* uplo, const char* op, const int* n
 // 2 uplo, 3 m, 4 n, 7 lda, 9 ldb, 12 ldc.
 template <typename T>
 void symm_impl(const char* name, const char* side, const char* uplo, const int* m, const int* n, const T* palpha,
-               const T* a, const int* lda, const T* b, const int* ldb, const T* pbeta, T* c, const int* ldc) {
+               const T* a, const int* lda, const T* b, const int* ldb, const T* pbeta, T* c, const int* ldc,
+               SymmFn fn = eh_symm) {
   int info = 0;
   const int sd = side_code(*side), ul = uplo_code(*uplo);
   const int ka = sd == EH_LEFT ? *m : *n;
@@ -385,8 +417,8 @@ void symm_impl(const char* name, const char* side, const char* uplo, const int* 
                    !dev_view(g_sb, b, *m, *n, *ldb, true, s, &B))) ||
       !dev_view(g_sc, c, *m, *n, *ldc, beta != T(0), s, &C))
     return (void)std::fprintf(stderr, "eigen_amd BLAS: staging failed\n");
-  if (eh_symm(dtype_of(sizeof(T)), (eh_side)sd, (eh_uplo)ul, *m, *n, &alpha, A.p, A.ld, B.p, B.ld, &beta, C.p, C.ld,
-              s) != EH_SUCCESS)
+  if (fn(dtype_of<T>(), (eh_side)sd, (eh_uplo)ul, *m, *n, &alpha, A.p, A.ld, B.p, B.ld, &beta, C.p, C.ld, s) !=
+      EH_SUCCESS)
     return report(name);
   if (!copy_back(C, c, s)) std::fprintf(stderr, "eigen_amd BLAS: copying C back failed\n");
   finish(s);
@@ -396,7 +428,8 @@ void symm_impl(const char* name, const char* side, const char* uplo, const int* 
 // :478-496: info 1 side, 2 uplo, 3 transa, 4 diag, 5 m, 6 n, 9 lda, 11 ldb.
 template <typename T, bool SOLVE>
 void tri_impl(const char* name, const char* side, const char* uplo, const char* opa, const char* diag, const int* m,
-              const int* n, const T* palpha, const T* a, const int* lda, T* b, const int* ldb) {
+              const int* n, const T* palpha, const T* a, const int* lda, T* b, const int* ldb,
+              TriFn fn = SOLVE ? eh_trsm : eh_trmm) {
   int info = 0;
   const int sd = side_code(*side), ul = uplo_code(*uplo), o = op_code(*opa), dg = diag_code(*diag);
   const int ka = sd == EH_LEFT ? *m : *n;
@@ -425,8 +458,7 @@ void tri_impl(const char* name, const char* side, const char* uplo, const char* 
   if ((alpha != T(0) && !dev_view(g_sa, a, ka, ka, *lda, true, s, &A)) ||
       !dev_view(g_sb, b, *m, *n, *ldb, alpha != T(0), s, &B))
     return (void)std::fprintf(stderr, "eigen_amd BLAS: staging failed\n");
-  auto fn = SOLVE ? eh_trsm : eh_trmm;
-  if (fn(dtype_of(sizeof(T)), (eh_side)sd, (eh_uplo)ul, (eh_op)o, (eh_diag)dg, *m, *n, &alpha, A.p, A.ld, B.p, B.ld,
+  if (fn(dtype_of<T>(), (eh_side)sd, (eh_uplo)ul, (eh_op)o, (eh_diag)dg, *m, *n, &alpha, A.p, A.ld, B.p, B.ld,
          s) != EH_SUCCESS)
     return report(name);
   if (!copy_back(B, b, s)) std::fprintf(stderr, "eigen_amd BLAS: copying B back failed\n");
@@ -492,7 +524,7 @@ void gemv_impl(const char* name, const char* op, const int* m, const int* n, con
     dy = static_cast<T*>(g_sc.ptr);
     dincy = 1;
   }
-  if (eh_gemv(dtype_of(sizeof(T)), (eh_op)o, *m, *n, &alpha, A.p, A.ld, dx, dincx, &beta, dy, dincy, s) != EH_SUCCESS)
+  if (eh_gemv(dtype_of<T>(), (eh_op)o, *m, *n, &alpha, A.p, A.ld, dx, dincx, &beta, dy, dincy, s) != EH_SUCCESS)
     return report(name);
   if (host_y) {
     if (hipMemcpyAsync(hy.data(), dy, (size_t)ly * sizeof(T), hipMemcpyDeviceToHost, s) != hipSuccess ||
@@ -560,7 +592,7 @@ void complex_gemv_impl(const char* name, const char* op, const int* m, const int
     dy = static_cast<Z*>(g_sc.ptr);
     dincy = 1;
   }
-  if (eh_gemv_complex(dtype_of(sizeof(T)), (eh_op)o, *m, *n, palpha, A.p, A.ld, dx, dincx, pbeta, dy, dincy, s) !=
+  if (eh_gemv_complex(dtype_of<T>(), (eh_op)o, *m, *n, palpha, A.p, A.ld, dx, dincx, pbeta, dy, dincy, s) !=
       EH_SUCCESS)
     return report(name);
   if (host_y) {
@@ -609,7 +641,7 @@ void complex_gemm_impl(const char* name, const char* opa, const char* opb, const
                    !dev_view(g_sb, b, ob == OP_N ? *k : *n, ob == OP_N ? *n : *k, *ldb, true, s, &B))) ||
       !dev_view(g_sc, c, *m, *n, *ldc, read_c, s, &C))
     return (void)std::fprintf(stderr, "eigen_amd BLAS: staging failed\n");
-  if (eh_gemm_complex(dtype_of(sizeof(T)), (eh_op)oa, (eh_op)ob, *m, *n, *k, palpha, A.p, A.ld, B.p, B.ld, pbeta, C.p,
+  if (eh_gemm_complex(dtype_of<T>(), (eh_op)oa, (eh_op)ob, *m, *n, *k, palpha, A.p, A.ld, B.p, B.ld, pbeta, C.p,
                       C.ld, EH_PREC_AUTO, s) != EH_SUCCESS)
     return report(name);
   if (!copy_back(C, c, s)) std::fprintf(stderr, "eigen_amd BLAS: copying C back failed\n");
@@ -790,6 +822,107 @@ void sbgemm_(const char* transa, const char* transb, const int* m, const int* n,
   sbgemm_impl(transa, transb, m, n, k, alpha, static_cast<const unsigned short*>(a), lda,
               static_cast<const unsigned short*>(b), ldb, beta, c, ldc);
 }
+
+// ---- complex level-3 (blas/level3_impl.h for complex scalars) --------------
+#define EH_CX(R) std::complex<R>
+#define EH_CP(R, p) static_cast<const EH_CX(R)*>(p)
+#define EH_MP(R, p) static_cast<EH_CX(R)*>(p)
+
+void ctrsm_(const char* side, const char* uplo, const char* transa, const char* diag, const int* m, const int* n,
+            const void* alpha, const void* a, const int* lda, void* b, const int* ldb) {
+  tri_impl<EH_CX(float), true>("CTRSM ", side, uplo, transa, diag, m, n, EH_CP(float, alpha), EH_CP(float, a), lda,
+                               EH_MP(float, b), ldb, eh_trsm_complex);
+}
+void ztrsm_(const char* side, const char* uplo, const char* transa, const char* diag, const int* m, const int* n,
+            const void* alpha, const void* a, const int* lda, void* b, const int* ldb) {
+  tri_impl<EH_CX(double), true>("ZTRSM ", side, uplo, transa, diag, m, n, EH_CP(double, alpha), EH_CP(double, a), lda,
+                                EH_MP(double, b), ldb, eh_trsm_complex);
+}
+void ctrmm_(const char* side, const char* uplo, const char* transa, const char* diag, const int* m, const int* n,
+            const void* alpha, const void* a, const int* lda, void* b, const int* ldb) {
+  tri_impl<EH_CX(float), false>("CTRMM ", side, uplo, transa, diag, m, n, EH_CP(float, alpha), EH_CP(float, a), lda,
+                                EH_MP(float, b), ldb, eh_trmm_complex);
+}
+void ztrmm_(const char* side, const char* uplo, const char* transa, const char* diag, const int* m, const int* n,
+            const void* alpha, const void* a, const int* lda, void* b, const int* ldb) {
+  tri_impl<EH_CX(double), false>("ZTRMM ", side, uplo, transa, diag, m, n, EH_CP(double, alpha), EH_CP(double, a),
+                                 lda, EH_MP(double, b), ldb, eh_trmm_complex);
+}
+void csymm_(const char* side, const char* uplo, const int* m, const int* n, const void* alpha, const void* a,
+            const int* lda, const void* b, const int* ldb, const void* beta, void* c, const int* ldc) {
+  symm_impl<EH_CX(float)>("CSYMM ", side, uplo, m, n, EH_CP(float, alpha), EH_CP(float, a), lda, EH_CP(float, b), ldb,
+                          EH_CP(float, beta), EH_MP(float, c), ldc, eh_symm_complex);
+}
+void zsymm_(const char* side, const char* uplo, const int* m, const int* n, const void* alpha, const void* a,
+            const int* lda, const void* b, const int* ldb, const void* beta, void* c, const int* ldc) {
+  symm_impl<EH_CX(double)>("ZSYMM ", side, uplo, m, n, EH_CP(double, alpha), EH_CP(double, a), lda,
+                           EH_CP(double, b), ldb, EH_CP(double, beta), EH_MP(double, c), ldc, eh_symm_complex);
+}
+void chemm_(const char* side, const char* uplo, const int* m, const int* n, const void* alpha, const void* a,
+            const int* lda, const void* b, const int* ldb, const void* beta, void* c, const int* ldc) {
+  symm_impl<EH_CX(float)>("CHEMM ", side, uplo, m, n, EH_CP(float, alpha), EH_CP(float, a), lda, EH_CP(float, b), ldb,
+                          EH_CP(float, beta), EH_MP(float, c), ldc, eh_hemm);
+}
+void zhemm_(const char* side, const char* uplo, const int* m, const int* n, const void* alpha, const void* a,
+            const int* lda, const void* b, const int* ldb, const void* beta, void* c, const int* ldc) {
+  symm_impl<EH_CX(double)>("ZHEMM ", side, uplo, m, n, EH_CP(double, alpha), EH_CP(double, a), lda,
+                           EH_CP(double, b), ldb, EH_CP(double, beta), EH_MP(double, c), ldc, eh_hemm);
+}
+void csyrk_(const char* uplo, const char* trans, const int* n, const int* k, const void* alpha, const void* a,
+            const int* lda, const void* beta, void* c, const int* ldc) {
+  syrk_impl<EH_CX(float)>("CSYRK ", uplo, trans, n, k, EH_CP(float, alpha), EH_CP(float, a), lda, EH_CP(float, beta),
+                          EH_MP(float, c), ldc, eh_syrk_complex, OP_C);
+}
+void zsyrk_(const char* uplo, const char* trans, const int* n, const int* k, const void* alpha, const void* a,
+            const int* lda, const void* beta, void* c, const int* ldc) {
+  syrk_impl<EH_CX(double)>("ZSYRK ", uplo, trans, n, k, EH_CP(double, alpha), EH_CP(double, a), lda,
+                           EH_CP(double, beta), EH_MP(double, c), ldc, eh_syrk_complex, OP_C);
+}
+void cherk_(const char* uplo, const char* trans, const int* n, const int* k, const float* alpha, const void* a,
+            const int* lda, const float* beta, void* c, const int* ldc) {
+  syrk_impl<EH_CX(float), float>("CHERK ", uplo, trans, n, k, alpha, EH_CP(float, a), lda, beta, EH_MP(float, c), ldc,
+                                 eh_herk, OP_T);
+}
+void zherk_(const char* uplo, const char* trans, const int* n, const int* k, const double* alpha, const void* a,
+            const int* lda, const double* beta, void* c, const int* ldc) {
+  syrk_impl<EH_CX(double), double>("ZHERK ", uplo, trans, n, k, alpha, EH_CP(double, a), lda, beta, EH_MP(double, c),
+                                   ldc, eh_herk, OP_T);
+}
+void csyr2k_(const char* uplo, const char* trans, const int* n, const int* k, const void* alpha, const void* a,
+             const int* lda, const void* b, const int* ldb, const void* beta, void* c, const int* ldc) {
+  syr2k_impl<EH_CX(float)>("CSYR2K", uplo, trans, n, k, EH_CP(float, alpha), EH_CP(float, a), lda, EH_CP(float, b),
+                           ldb, EH_CP(float, beta), EH_MP(float, c), ldc, eh_syr2k_complex, OP_C);
+}
+void zsyr2k_(const char* uplo, const char* trans, const int* n, const int* k, const void* alpha, const void* a,
+             const int* lda, const void* b, const int* ldb, const void* beta, void* c, const int* ldc) {
+  syr2k_impl<EH_CX(double)>("ZSYR2K", uplo, trans, n, k, EH_CP(double, alpha), EH_CP(double, a), lda,
+                            EH_CP(double, b), ldb, EH_CP(double, beta), EH_MP(double, c), ldc, eh_syr2k_complex, OP_C);
+}
+void cher2k_(const char* uplo, const char* trans, const int* n, const int* k, const void* alpha, const void* a,
+             const int* lda, const void* b, const int* ldb, const float* beta, void* c, const int* ldc) {
+  syr2k_impl<EH_CX(float), float>("CHER2K", uplo, trans, n, k, EH_CP(float, alpha), EH_CP(float, a), lda,
+                                  EH_CP(float, b), ldb, beta, EH_MP(float, c), ldc, eh_her2k, OP_T);
+}
+void zher2k_(const char* uplo, const char* trans, const int* n, const int* k, const void* alpha, const void* a,
+             const int* lda, const void* b, const int* ldb, const double* beta, void* c, const int* ldc) {
+  syr2k_impl<EH_CX(double), double>("ZHER2K", uplo, trans, n, k, EH_CP(double, alpha), EH_CP(double, a), lda,
+                                    EH_CP(double, b), ldb, beta, EH_MP(double, c), ldc, eh_her2k, OP_T);
+}
+void cgemmtr_(const char* uplo, const char* transa, const char* transb, const int* n, const int* k,
+              const void* alpha, const void* a, const int* lda, const void* b, const int* ldb, const void* beta,
+              void* c, const int* ldc) {
+  gemmtr_impl<EH_CX(float)>("CGEMMTR", uplo, transa, transb, n, k, EH_CP(float, alpha), EH_CP(float, a), lda,
+                            EH_CP(float, b), ldb, EH_CP(float, beta), EH_MP(float, c), ldc, eh_gemmtr_complex);
+}
+void zgemmtr_(const char* uplo, const char* transa, const char* transb, const int* n, const int* k,
+              const void* alpha, const void* a, const int* lda, const void* b, const int* ldb, const void* beta,
+              void* c, const int* ldc) {
+  gemmtr_impl<EH_CX(double)>("ZGEMMTR", uplo, transa, transb, n, k, EH_CP(double, alpha), EH_CP(double, a), lda,
+                             EH_CP(double, b), ldb, EH_CP(double, beta), EH_MP(double, c), ldc, eh_gemmtr_complex);
+}
+#undef EH_CX
+#undef EH_CP
+#undef EH_MP
 
 void eh_blas_set_stream(void* stream) { g_stream = static_cast<hipStream_t>(stream); }
 void* eh_blas_get_stream(void) { return g_stream; }

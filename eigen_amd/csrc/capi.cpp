@@ -318,17 +318,18 @@ static eh_status launch_split_product(const GemmParams<float>& g, bool ta, bool 
   return EH_SUCCESS;
 }
 
-// bf16 inputs, FP32 accumulation and result: both operands are repacked
-// into the one-term plane layout, then the packed BF16 MFMA product runs
-// with a single product per k slice (and the split-K reduction when K is
-// cut).  Operand extents must stay below 4 GB (32-bit buffer offsets).
+// bf16 (or, with f16, IEEE half) inputs, FP32 accumulation and result:
+// both operands are repacked into the one-term plane layout, then the packed
+// BF16 (FP16) MFMA product runs with a single product per k slice (and the
+// split-K reduction when K is cut).  Operand extents must stay below 4 GB
+// (32-bit buffer offsets).
 static eh_status launch_bf16_product(bool ta, bool tb, int m, int n, int k, float alpha, const unsigned short* A,
                                      int64_t lda, const unsigned short* B, int64_t ldb, float beta, float* C,
-                                     int64_t ldc, hipStream_t stream) {
+                                     int64_t ldc, hipStream_t stream, bool f16 = false) {
   const uint64_t extA = ta ? (uint64_t)(m - 1) * lda + k : (uint64_t)(k - 1) * lda + m;
   const uint64_t extB = !tb ? (uint64_t)(n - 1) * ldb + k : (uint64_t)(k - 1) * ldb + n;
   if ((extA + 64) * 2 >= kMaxExtent || (extB + 64) * 2 >= kMaxExtent)
-    return fail(EH_ERR_UNSUPPORTED, "bf16 operands must span less than 4 GB");
+    return fail(EH_ERR_UNSUPPORTED, "16-bit operands must span less than 4 GB");
   const int KS = split_kslices(k);
   // 256x256 tiles (tile 1) once they fill the chip, 128x128 (tile 0) below
   const int64_t t256 = ((int64_t)(m + 255) / 256) * ((n + 255) / 256);
@@ -371,9 +372,9 @@ static eh_status launch_bf16_product(bool ta, bool tb, int m, int n, int k, floa
   sp.beta = beta;
   {
     KernelTimer t(EH_KC_PACKED_PRODUCT, stream);
-    e = launch_gemm_split_packed(sp, 1, tile, splits, stream);
+    e = launch_gemm_split_packed(sp, 1, tile, splits, stream, f16);
   }
-  if (e != hipSuccess) return hip_fail(e, "bf16 product launch");
+  if (e != hipSuccess) return hip_fail(e, "16-bit product launch");
   if (splits > 1) {
     GemmParams<float> r{};
     r.C = C;
@@ -757,6 +758,23 @@ eh_status eh_gemm_bf16(eh_op opa, eh_op opb, int64_t m, int64_t n, int64_t k, co
   return launch_bf16_product(opa != EH_OP_N, opb != EH_OP_N, (int)m, (int)n, (int)k, *alpha,
                              static_cast<const unsigned short*>(A), lda, static_cast<const unsigned short*>(B), ldb,
                              *beta, C, ldc, s);
+}
+
+eh_status eh_gemm_f16(eh_op opa, eh_op opb, int64_t m, int64_t n, int64_t k, const float* alpha, const void* A,
+                      int64_t lda, const void* B, int64_t ldb, const float* beta, float* C, int64_t ldc,
+                      void* stream) {
+  eh_status st = check_gemm_args(EH_F32, opa, opb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+  if (st != EH_SUCCESS) return st;
+  if (m == 0 || n == 0) return EH_SUCCESS;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (k == 0 || *alpha == 0.0f) {
+    if (*beta == 1.0f) return EH_SUCCESS;
+    hipError_t e = launch_scale_c<float>(C, (int)m, (int)n, ldc, *beta, s);
+    return e == hipSuccess ? EH_SUCCESS : hip_fail(e, "scale_c launch");
+  }
+  return launch_bf16_product(opa != EH_OP_N, opb != EH_OP_N, (int)m, (int)n, (int)k, *alpha,
+                             static_cast<const unsigned short*>(A), lda, static_cast<const unsigned short*>(B), ldb,
+                             *beta, C, ldc, s, true);
 }
 
 eh_status eh_syrk(eh_dtype dtype, eh_uplo uplo, eh_op trans, int64_t n, int64_t k, const void* alpha, const void* A,

@@ -3,12 +3,15 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "cplx.hpp"
+
 namespace eh {
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 typedef double doublex4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
 
@@ -51,6 +54,11 @@ struct Mfma<double> {
 
 __device__ inline float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 __device__ inline double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+template <typename R>
+__device__ inline cplx<R> fma_t(cplx<R> a, cplx<R> b, cplx<R> c) {
+  return {fma_t(a.re, b.re, fma_t(-a.im, b.im, c.re)), fma_t(a.re, b.im, fma_t(a.im, b.re, c.im))};
+}
 
 // XOR swizzle of 16-byte chunks inside an LDS row.  For 8 chunks per row
 // (128-byte rows) any 8 consecutive rows at the same logical chunk land on

@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <string>
+#include <type_traits>
 
 #include "../../include/eigen_amd.h"
 #include "internal.hpp"
@@ -91,6 +92,24 @@ eh_status gemv_run(int dtype, int op, int64_t m, int64_t n, const T* alpha, cons
 }
 
 }  // namespace
+
+template <typename R>
+eh_status complex_product(int opa, int opb, int64_t m, int64_t n, int64_t k, cplx<R> alpha, const cplx<R>* A,
+                          int64_t lda, const cplx<R>* B, int64_t ldb, cplx<R> beta, cplx<R>* C, int64_t ldc,
+                          hipStream_t s) {
+  if (m == 0 || n == 0) return EH_SUCCESS;
+  if (m > INT32_MAX / 2 || k > INT32_MAX / 2) return fail(EH_ERR_UNSUPPORTED, "complex m, k must be < 2^30");
+  return run<R>(std::is_same<R, double>::value ? EH_F64 : EH_F32, opa, opb, m, n, k, &alpha.re,
+                reinterpret_cast<const R*>(A), lda, reinterpret_cast<const R*>(B), ldb, &beta.re,
+                reinterpret_cast<R*>(C), ldc, EH_PREC_PLAIN, s);
+}
+template eh_status complex_product<float>(int, int, int64_t, int64_t, int64_t, cplx<float>, const cplx<float>*,
+                                          int64_t, const cplx<float>*, int64_t, cplx<float>, cplx<float>*, int64_t,
+                                          hipStream_t);
+template eh_status complex_product<double>(int, int, int64_t, int64_t, int64_t, cplx<double>, const cplx<double>*,
+                                           int64_t, const cplx<double>*, int64_t, cplx<double>, cplx<double>*, int64_t,
+                                           hipStream_t);
+
 }  // namespace eh
 
 using namespace eh;

@@ -142,7 +142,9 @@ __device__ inline void wait_vmcnt() {
 // each lane fetching the global chunk that belongs at its LDS position.
 // NSTAGE - 1 slices are in flight while one is consumed.
 // ---------------------------------------------------------------------------
-template <int BM, int BN, int WGM, int WGN, int TERMS, int NSTAGE>
+// F16: the planes hold IEEE binary16 operands (one term, no split) and the
+// product runs on v_mfma_f32_32x32x16_f16 instead of the BF16 instruction.
+template <int BM, int BN, int WGM, int WGN, int TERMS, int NSTAGE, bool F16 = false>
 __global__ void __launch_bounds__(64 * WGM * WGN) gemm_split_packed_kernel(SplitParams p) {
   constexpr int NW = WGM * WGN;
   constexpr int ROWB = 64;
@@ -272,10 +274,16 @@ __global__ void __launch_bounds__(64 * WGM * WGN) gemm_split_packed_kernel(Split
 #pragma unroll
         for (int i = 0; i < RM; ++i)
 #pragma unroll
-          for (int j = 0; j < RN; ++j)
-            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a[s][PA[pq]][i]),
-                                                                __builtin_bit_cast(bf16x8, b[s][PB[pq]][j]),
-                                                                acc[i][j], 0, 0, 0);
+          for (int j = 0; j < RN; ++j) {
+            if constexpr (F16)
+              acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(f16x8, a[s][PA[pq]][i]),
+                                                                 __builtin_bit_cast(f16x8, b[s][PB[pq]][j]),
+                                                                 acc[i][j], 0, 0, 0);
+            else
+              acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a[s][PA[pq]][i]),
+                                                                  __builtin_bit_cast(bf16x8, b[s][PB[pq]][j]),
+                                                                  acc[i][j], 0, 0, 0);
+          }
   }
 
   const bool partial = gridDim.y > 1;
@@ -420,15 +428,20 @@ hipError_t launch_pack_split(const float* X, int64_t ld, bool kcont, int rows, i
   return hipErrorInvalidValue;
 }
 
-template <int BM, int BN, int WGM, int WGN, int TERMS, int NSTAGE>
+template <int BM, int BN, int WGM, int WGN, int TERMS, int NSTAGE, bool F16 = false>
 static hipError_t launch_packed_t(const SplitParams& p, int splits, hipStream_t s) {
   const int ntm = (p.M + BM - 1) / BM, ntn = (p.N + BN - 1) / BN;
-  hipLaunchKernelGGL((gemm_split_packed_kernel<BM, BN, WGM, WGN, TERMS, NSTAGE>), dim3(ntm * ntn, splits),
+  hipLaunchKernelGGL((gemm_split_packed_kernel<BM, BN, WGM, WGN, TERMS, NSTAGE, F16>), dim3(ntm * ntn, splits),
                      dim3(64 * WGM * WGN), 0, s, p);
   return hipGetLastError();
 }
 
-hipError_t launch_gemm_split_packed(const SplitParams& p, int terms, int tile, int splits, hipStream_t s) {
+hipError_t launch_gemm_split_packed(const SplitParams& p, int terms, int tile, int splits, hipStream_t s, bool f16) {
+  if (f16) {  // IEEE half operands: the one-term tiles on the FP16 instruction
+    if (terms != 1) return hipErrorInvalidValue;
+    if (tile == 0) return launch_packed_t<128, 128, 2, 2, 1, 4, true>(p, splits, s);
+    return launch_packed_t<256, 256, 2, 4, 1, 2, true>(p, splits, s);
+  }
   // tile 0: 128x128, 4 waves (3 stages); 1: 256x128, 4 waves of 128x64 (2 stages);
   // 2: 256x128, 8 waves of 64x64 -- two waves per SIMD (2 stages)
   if (terms == 3) {

@@ -8,6 +8,7 @@
 #include <string>
 
 #include "../../include/eigen_amd.h"
+#include "cplx.hpp"
 
 namespace eh {
 
@@ -109,6 +110,13 @@ eh_status check_gemm_args(int dtype, int opa, int opb, int64_t m, int64_t n, int
 template <typename T>
 eh_status device_product(int opa, int opb, int64_t m, int64_t n, int64_t k, T alpha, const T* A, int64_t lda,
                          const T* B, int64_t ldb, T beta, T* C, int64_t ldc, hipStream_t stream, int tri = 0);
+// Complex product C = alpha*op(A)*op(B) + beta*C on interleaved storage (op
+// C conjugates), as one real product over 2x2 real blocks (complex_gemm.cpp),
+// plain FP32 / FP64 MFMA; temporaries from the stream-ordered allocator.
+template <typename R>
+eh_status complex_product(int opa, int opb, int64_t m, int64_t n, int64_t k, cplx<R> alpha, const cplx<R>* A,
+                          int64_t lda, const cplx<R>* B, int64_t ldb, cplx<R> beta, cplx<R>* C, int64_t ldc,
+                          hipStream_t s);
 eh_status gemm_dispatch(int dtype, int opa, int opb, int64_t m, int64_t n, int64_t k, const void* alpha,
                         const void* A, int64_t lda, const void* B, int64_t ldb, const void* beta, void* C,
                         int64_t ldc, int prec, int config, void* stream);

@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "cplx.hpp"
+
 namespace eh {
 
 // Parameters of one product C = alpha*op(A)*op(B) + beta*C (column-major).
@@ -55,26 +57,40 @@ hipError_t launch_gemv(const GemvPlan& g, bool trans, int m, int k, T alpha, con
 
 // Dense form of the n x n matrix stored in one triangle of A (lower: i >= j):
 // sym mirrors the stored triangle, otherwise the other triangle is zero and
-// unit puts ones on the diagonal.  D has leading dimension ldd.
+// unit puts ones on the diagonal.  D has leading dimension ldd.  herm (complex
+// types): the mirror is conjugated and the diagonal's imaginary part taken as
+// zero (a Hermitian matrix); conj: every element is conjugated.
 template <typename T>
 hipError_t launch_tri_dense(const T* A, int64_t lda, int n, T* D, int64_t ldd, bool lower, bool sym, bool unit,
-                            hipStream_t s);
+                            hipStream_t s, bool herm = false, bool conj = false);
 
-// Inverses of the diagonal blocks of order kTrsmLeaf of the triangular
-// matrix M(i, k) = mt ? A[k + i*lda] : A[i + k*lda] of the given order
-// (lower or upper, unit diagonal or not): block b goes to W + b*kTrsmLeaf^2
-// as a dense kTrsmLeaf x kTrsmLeaf column-major matrix (zeros outside the
-// triangle, and past the order in the last block).
+// Inverses of the diagonal blocks of order `leaf` of the triangular matrix
+// M(i, k) = mt ? A[k + i*lda] : A[i + k*lda] (conjugated when conj) of the
+// given order (lower or upper, unit diagonal or not): block b goes to
+// W + b*leaf^2 as a dense leaf x leaf column-major matrix (zeros outside the
+// triangle, and past the order in the last block).  leaf = kTrsmLeaf for
+// real types, kTrsmLeafComplex for complex ones (the column of the inverse a
+// thread keeps in registers is twice as wide).
 constexpr int kTrsmLeaf = 64;
+constexpr int kTrsmLeafComplex = 32;
 template <typename T>
 struct TrtriParams {
   const T* A;
   int64_t lda;
   int mt, lower, unit, order;
   T* W;
+  int conj = 0;
 };
 template <typename T>
 hipError_t launch_trtri(const TrtriParams<T>& p, hipStream_t s);
+
+// Triangle of a complex rank update (level3_kernels.hip): on the lower
+// (upper) triangle, C = alpha*W + alpha2*f(W^T) + beta*C with f = conj when
+// herm (W^H), the alpha2 term only when use2, C not read when beta == 0, the
+// diagonal's imaginary part zeroed when herm; W == nullptr: C = beta*C.
+template <typename R>
+hipError_t launch_tri_combine(const cplx<R>* W, int64_t ldw, int n, cplx<R>* C, int64_t ldc, bool lower,
+                              cplx<R> alpha, cplx<R> alpha2, bool use2, cplx<R> beta, bool herm, hipStream_t s);
 
 // Complex product as one real product (complex_kernels.hip): A3 (2m x 2k
 // real, ld3) = the 2x2 real blocks of alpha*op(A); B3 (2k x n real, ld3) =
@@ -119,8 +135,11 @@ struct SplitParams {
 };
 hipError_t launch_pack_split(const float* X, int64_t ld, bool kcont, int rows, int K, int terms, void* P,
                              hipStream_t s);
-hipError_t launch_gemm_split_packed(const SplitParams& p, int terms, int tile, int splits, hipStream_t s);
-// bf16 operand (bits) -> one plane in the same packed layout (terms = 1).
+// f16: IEEE half planes (terms = 1) on the FP16 MFMA instruction.
+hipError_t launch_gemm_split_packed(const SplitParams& p, int terms, int tile, int splits, hipStream_t s,
+                                   bool f16 = false);
+// 16-bit operand (bf16 or half bits) -> one plane in the same packed layout
+// (terms = 1); a bit copy, so it serves both formats.
 hipError_t launch_pack_bf16(const unsigned short* X, int64_t ld, bool kcont, int rows, int K, void* P,
                             hipStream_t s);
 hipError_t launch_split_bf16(const float* x, int64_t n, int terms, unsigned short* t0, unsigned short* t1,

@@ -15,6 +15,12 @@
 //                  the O(order * 64^2) of the inversions runs on MFMA.
 // Arguments are checked with the reference's rules (blas/level3_impl.h); the
 // BLAS wrappers in blas.cpp add the xerbla_ codes.
+//
+// The recursions are templated on the scalar: float, double, or cplx<R> for
+// the complex routines (complex products run as one real product over 2x2
+// real blocks, complex_gemm.cpp; op C conjugates).  The complex rank
+// updates (SYRK, HERK, SYR2K, HER2K, GEMMTR) form the full product W in a
+// temporary and combine its triangle into C (tri_combine, one HBM pass).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -37,6 +43,44 @@ bool valid_dtype(int d) { return d == EH_F32 || d == EH_F64; }
     if (st_ != EH_SUCCESS) return st_;    \
   } while (0)
 
+// Product on the device, real or complex.
+template <typename T>
+eh_status prod(int opa, int opb, int64_t m, int64_t n, int64_t k, T alpha, const T* A, int64_t lda, const T* B,
+               int64_t ldb, T beta, T* C, int64_t ldc, hipStream_t s) {
+  return device_product<T>(opa, opb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s);
+}
+template <typename R>
+eh_status prod(int opa, int opb, int64_t m, int64_t n, int64_t k, cplx<R> alpha, const cplx<R>* A, int64_t lda,
+               const cplx<R>* B, int64_t ldb, cplx<R> beta, cplx<R>* C, int64_t ldc, hipStream_t s) {
+  return complex_product<R>(opa, opb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s);
+}
+
+// B = 0 (m x n)
+template <typename T>
+eh_status zero_b(T* B, int64_t m, int64_t n, int64_t ldb, hipStream_t s) {
+  hipError_t e = launch_scale_c<T>(B, (int)m, (int)n, ldb, T(0), s);
+  return e == hipSuccess ? EH_SUCCESS : hip_fail(e, "zero fill launch");
+}
+template <typename R>
+eh_status zero_b(cplx<R>* B, int64_t m, int64_t n, int64_t ldb, hipStream_t s) {
+  hipError_t e = launch_complex_scale<R>(reinterpret_cast<R*>(B), (int)m, (int)n, ldb, R(0), R(0), s);
+  return e == hipSuccess ? EH_SUCCESS : hip_fail(e, "zero fill launch");
+}
+
+// order of the inverted diagonal blocks of TRSM
+template <typename T>
+constexpr int64_t trsm_leaf() {
+  return kTrsmLeaf;
+}
+template <>
+constexpr int64_t trsm_leaf<cplx<float>>() {
+  return kTrsmLeafComplex;
+}
+template <>
+constexpr int64_t trsm_leaf<cplx<double>>() {
+  return kTrsmLeafComplex;
+}
+
 // The triangular operator E = op(A) of order ka: E(i, k) = ta ? A[k + i*lda]
 // : A[i + k*lda]; `lower` is the triangle of E (the stored triangle of A
 // when not transposed, the other one when transposed).
@@ -45,9 +89,10 @@ struct Tri {
   const T* A;
   int64_t lda;
   bool ta, lower, unit;
+  bool conj;  // E = A^H (complex, op C)
   // E[r0.., c0..] as a product operand: pointer and op code
   const T* block(int64_t r0, int64_t c0) const { return ta ? A + c0 + r0 * lda : A + r0 + c0 * lda; }
-  int op() const { return ta ? EH_OP_T : EH_OP_N; }
+  int op() const { return ta ? (conj ? EH_OP_C : EH_OP_T) : EH_OP_N; }
   const T* diag(int64_t d) const { return A + d + d * lda; }
   bool stored_lower() const { return ta ? !lower : lower; }
 };
@@ -59,38 +104,39 @@ int64_t half(int64_t sz, int64_t leaf) {
 }
 
 // ---- TRSM --------------------------------------------------------------------
-// The diagonal blocks of order kTrsmLeaf are inverted up front (one launch);
+// The diagonal blocks of order trsm_leaf<T>() are inverted up front (one launch);
 // a leaf of the recursion multiplies its right-hand sides by the inverse, on
 // the MFMA kernels, from a copy of them.
 template <typename T>
 struct TrsmWork {
-  const T* inv;  // inverse of diagonal block b at inv + b*kTrsmLeaf^2 (ld kTrsmLeaf)
-  T* copy;       // kTrsmLeaf x n (left) or m x kTrsmLeaf (right)
+  const T* inv;  // inverse of diagonal block b at inv + b*leaf^2 (ld leaf)
+  T* copy;       // leaf x n (left) or m x leaf (right)
 };
 
 // E[d:d+sz, d:d+sz] X = alpha*B, B = rows d.. of the right-hand sides (sz x n).
 template <typename T>
 eh_status trsm_left(const Tri<T>& E, int64_t d, int64_t sz, T* B, int64_t ldb, int64_t n, T alpha,
                     const TrsmWork<T>& w, hipStream_t s) {
-  if (sz <= kTrsmLeaf) {  // X = alpha * inv(E_dd) * B
+  constexpr int64_t leaf = trsm_leaf<T>();
+  if (sz <= leaf) {  // X = alpha * inv(E_dd) * B
     hipError_t e = hipMemcpy2DAsync(w.copy, sz * sizeof(T), B, ldb * sizeof(T), sz * sizeof(T), n,
                                     hipMemcpyDeviceToDevice, s);
     if (e != hipSuccess) return hip_fail(e, "trsm block copy");
-    return device_product<T>(EH_OP_N, EH_OP_N, sz, n, sz, alpha, w.inv + (d / kTrsmLeaf) * kTrsmLeaf * kTrsmLeaf,
-                             kTrsmLeaf, w.copy, sz, T(0), B, ldb, s);
+    return prod(EH_OP_N, EH_OP_N, sz, n, sz, alpha, w.inv + (d / leaf) * leaf * leaf, leaf, w.copy, sz, T(0), B, ldb,
+                s);
   }
-  const int64_t h = half(sz, kTrsmLeaf);
+  const int64_t h = half(sz, leaf);
   T* B1 = B;
   T* B2 = B + h;
   if (E.lower) {  // X1 = E11 \ alpha*B1; B2 = alpha*B2 - E21*X1; X2 = E22 \ B2
     EH_TRY(trsm_left(E, d, h, B1, ldb, n, alpha, w, s));
-    EH_TRY(device_product<T>(E.op(), EH_OP_N, sz - h, n, h, T(-1), E.block(d + h, d), E.lda, B1, ldb, alpha, B2, ldb,
+    EH_TRY(prod(E.op(), EH_OP_N, sz - h, n, h, T(-1), E.block(d + h, d), E.lda, B1, ldb, alpha, B2, ldb,
                              s));
     return trsm_left(E, d + h, sz - h, B2, ldb, n, T(1), w, s);
   }
   // X2 = E22 \ alpha*B2; B1 = alpha*B1 - E12*X2; X1 = E11 \ B1
   EH_TRY(trsm_left(E, d + h, sz - h, B2, ldb, n, alpha, w, s));
-  EH_TRY(device_product<T>(E.op(), EH_OP_N, h, n, sz - h, T(-1), E.block(d, d + h), E.lda, B2, ldb, alpha, B1, ldb,
+  EH_TRY(prod(E.op(), EH_OP_N, h, n, sz - h, T(-1), E.block(d, d + h), E.lda, B2, ldb, alpha, B1, ldb,
                            s));
   return trsm_left(E, d, h, B1, ldb, n, T(1), w, s);
 }
@@ -99,25 +145,25 @@ eh_status trsm_left(const Tri<T>& E, int64_t d, int64_t sz, T* B, int64_t ldb, i
 template <typename T>
 eh_status trsm_right(const Tri<T>& E, int64_t d, int64_t sz, T* B, int64_t ldb, int64_t m, T alpha,
                      const TrsmWork<T>& w, hipStream_t s) {
-  if (sz <= kTrsmLeaf) {  // X = alpha * B * inv(E_dd)
+  constexpr int64_t leaf = trsm_leaf<T>();
+  if (sz <= leaf) {  // X = alpha * B * inv(E_dd)
     hipError_t e = hipMemcpy2DAsync(w.copy, m * sizeof(T), B, ldb * sizeof(T), m * sizeof(T), sz,
                                     hipMemcpyDeviceToDevice, s);
     if (e != hipSuccess) return hip_fail(e, "trsm block copy");
-    return device_product<T>(EH_OP_N, EH_OP_N, m, sz, sz, alpha, w.copy, m,
-                             w.inv + (d / kTrsmLeaf) * kTrsmLeaf * kTrsmLeaf, kTrsmLeaf, T(0), B, ldb, s);
+    return prod(EH_OP_N, EH_OP_N, m, sz, sz, alpha, w.copy, m, w.inv + (d / leaf) * leaf * leaf, leaf, T(0), B, ldb, s);
   }
-  const int64_t h = half(sz, kTrsmLeaf);
+  const int64_t h = half(sz, leaf);
   T* B1 = B;
   T* B2 = B + h * ldb;
   if (E.lower) {  // X2 E22 = alpha*B2; X1 E11 = alpha*B1 - X2 E21
     EH_TRY(trsm_right(E, d + h, sz - h, B2, ldb, m, alpha, w, s));
-    EH_TRY(device_product<T>(EH_OP_N, E.op(), m, h, sz - h, T(-1), B2, ldb, E.block(d + h, d), E.lda, alpha, B1, ldb,
+    EH_TRY(prod(EH_OP_N, E.op(), m, h, sz - h, T(-1), B2, ldb, E.block(d + h, d), E.lda, alpha, B1, ldb,
                              s));
     return trsm_right(E, d, h, B1, ldb, m, T(1), w, s);
   }
   // X1 E11 = alpha*B1; X2 E22 = alpha*B2 - X1 E12
   EH_TRY(trsm_right(E, d, h, B1, ldb, m, alpha, w, s));
-  EH_TRY(device_product<T>(EH_OP_N, E.op(), m, sz - h, h, T(-1), B1, ldb, E.block(d, d + h), E.lda, alpha, B2, ldb,
+  EH_TRY(prod(EH_OP_N, E.op(), m, sz - h, h, T(-1), B1, ldb, E.block(d, d + h), E.lda, alpha, B2, ldb,
                            s));
   return trsm_right(E, d + h, sz - h, B2, ldb, m, T(1), w, s);
 }
@@ -148,20 +194,20 @@ eh_status trmm_left(const Tri<T>& E, int64_t d, int64_t sz, T* B, int64_t ldb, i
     hipError_t e = hipMemcpy2DAsync(w.copy, sz * sizeof(T), B, ldb * sizeof(T), sz * sizeof(T), n,
                                     hipMemcpyDeviceToDevice, s);
     if (e != hipSuccess) return hip_fail(e, "trmm block copy");
-    return device_product<T>(E.op(), EH_OP_N, sz, n, sz, alpha, w.dense, sz, w.copy, sz, T(0), B, ldb, s);
+    return prod(E.op(), EH_OP_N, sz, n, sz, alpha, w.dense, sz, w.copy, sz, T(0), B, ldb, s);
   }
   const int64_t h = half(sz, kTrmmLeaf);
   T* B1 = B;
   T* B2 = B + h;
   if (E.lower) {  // B2 = alpha*(E22*B2 + E21*B1) before B1 changes
     EH_TRY(trmm_left(E, d + h, sz - h, B2, ldb, n, alpha, w, s));
-    EH_TRY(device_product<T>(E.op(), EH_OP_N, sz - h, n, h, alpha, E.block(d + h, d), E.lda, B1, ldb, T(1), B2, ldb,
+    EH_TRY(prod(E.op(), EH_OP_N, sz - h, n, h, alpha, E.block(d + h, d), E.lda, B1, ldb, T(1), B2, ldb,
                              s));
     return trmm_left(E, d, h, B1, ldb, n, alpha, w, s);
   }
   // B1 = alpha*(E11*B1 + E12*B2) before B2 changes
   EH_TRY(trmm_left(E, d, h, B1, ldb, n, alpha, w, s));
-  EH_TRY(device_product<T>(E.op(), EH_OP_N, h, n, sz - h, alpha, E.block(d, d + h), E.lda, B2, ldb, T(1), B1, ldb,
+  EH_TRY(prod(E.op(), EH_OP_N, h, n, sz - h, alpha, E.block(d, d + h), E.lda, B2, ldb, T(1), B1, ldb,
                            s));
   return trmm_left(E, d + h, sz - h, B2, ldb, n, alpha, w, s);
 }
@@ -175,20 +221,20 @@ eh_status trmm_right(const Tri<T>& E, int64_t d, int64_t sz, T* B, int64_t ldb, 
     hipError_t e = hipMemcpy2DAsync(w.copy, m * sizeof(T), B, ldb * sizeof(T), m * sizeof(T), sz,
                                     hipMemcpyDeviceToDevice, s);
     if (e != hipSuccess) return hip_fail(e, "trmm block copy");
-    return device_product<T>(EH_OP_N, E.op(), m, sz, sz, alpha, w.copy, m, w.dense, sz, T(0), B, ldb, s);
+    return prod(EH_OP_N, E.op(), m, sz, sz, alpha, w.copy, m, w.dense, sz, T(0), B, ldb, s);
   }
   const int64_t h = half(sz, kTrmmLeaf);
   T* B1 = B;
   T* B2 = B + h * ldb;
   if (E.lower) {  // B1 = alpha*(B1*E11 + B2*E21) before B2 changes
     EH_TRY(trmm_right(E, d, h, B1, ldb, m, alpha, w, s));
-    EH_TRY(device_product<T>(EH_OP_N, E.op(), m, h, sz - h, alpha, B2, ldb, E.block(d + h, d), E.lda, T(1), B1, ldb,
+    EH_TRY(prod(EH_OP_N, E.op(), m, h, sz - h, alpha, B2, ldb, E.block(d + h, d), E.lda, T(1), B1, ldb,
                              s));
     return trmm_right(E, d + h, sz - h, B2, ldb, m, alpha, w, s);
   }
   // B2 = alpha*(B1*E12 + B2*E22) before B1 changes
   EH_TRY(trmm_right(E, d + h, sz - h, B2, ldb, m, alpha, w, s));
-  EH_TRY(device_product<T>(EH_OP_N, E.op(), m, sz - h, h, alpha, B1, ldb, E.block(d, d + h), E.lda, T(1), B2, ldb,
+  EH_TRY(prod(EH_OP_N, E.op(), m, sz - h, h, alpha, B1, ldb, E.block(d, d + h), E.lda, T(1), B2, ldb,
                            s));
   return trmm_right(E, d, h, B1, ldb, m, alpha, w, s);
 }
@@ -211,26 +257,22 @@ eh_status check_tri_args(int dtype, int side, int uplo, int transa, int diag, in
 }
 
 template <typename T>
-eh_status zero_b(T* B, int64_t m, int64_t n, int64_t ldb, hipStream_t s) {
-  hipError_t e = launch_scale_c<T>(B, (int)m, (int)n, ldb, T(0), s);
-  return e == hipSuccess ? EH_SUCCESS : hip_fail(e, "zero fill launch");
-}
-
-template <typename T>
 eh_status trsm_t(int side, int uplo, int transa, int diag, int64_t m, int64_t n, T alpha, const T* A, int64_t lda,
                  T* B, int64_t ldb, hipStream_t s) {
   if (alpha == T(0)) return zero_b(B, m, n, ldb, s);
   if (!A) return fail(EH_ERR_INVALID, "A is NULL");
   const bool ta = transa != EH_OP_N;
-  const Tri<T> E{A, lda, ta, (uplo == EH_LOWER) != ta, diag == EH_UNIT};
+  const Tri<T> E{A, lda, ta, (uplo == EH_LOWER) != ta, diag == EH_UNIT, transa == EH_OP_C};
   const int64_t ka = side == EH_LEFT ? m : n;
-  const int64_t blocks = (ka + kTrsmLeaf - 1) / kTrsmLeaf;
-  const int64_t inv = blocks * kTrsmLeaf * kTrsmLeaf;
-  const int64_t copy = side == EH_LEFT ? kTrsmLeaf * n : m * kTrsmLeaf;
+  constexpr int64_t leaf = trsm_leaf<T>();
+  const int64_t blocks = (ka + leaf - 1) / leaf;
+  const int64_t inv = blocks * leaf * leaf;
+  const int64_t copy = side == EH_LEFT ? leaf * n : m * leaf;
   StreamTemp t;
   EH_TRY(t.get((size_t)(inv + copy) * sizeof(T), s, "level-3 temporary"));
   const TrsmWork<T> w{static_cast<T*>(t.ptr), static_cast<T*>(t.ptr) + inv};
-  const TrtriParams<T> tp{A, lda, (int)E.ta, (int)E.lower, (int)E.unit, (int)ka, static_cast<T*>(t.ptr)};
+  TrtriParams<T> tp{A, lda, (int)E.ta, (int)E.lower, (int)E.unit, (int)ka, static_cast<T*>(t.ptr)};
+  tp.conj = E.conj;
   hipError_t e = launch_trtri<T>(tp, s);
   if (e != hipSuccess) return hip_fail(e, "diagonal block inversion launch");
   return side == EH_LEFT ? trsm_left<T>(E, 0, m, B, ldb, n, alpha, w, s)
@@ -243,7 +285,7 @@ eh_status trmm_t(int side, int uplo, int transa, int diag, int64_t m, int64_t n,
   if (alpha == T(0)) return zero_b(B, m, n, ldb, s);
   if (!A) return fail(EH_ERR_INVALID, "A is NULL");
   const bool ta = transa != EH_OP_N;
-  const Tri<T> E{A, lda, ta, (uplo == EH_LOWER) != ta, diag == EH_UNIT};
+  const Tri<T> E{A, lda, ta, (uplo == EH_LOWER) != ta, diag == EH_UNIT, transa == EH_OP_C};
   const int64_t ka = side == EH_LEFT ? m : n;
   const int64_t leaf = std::min(ka, kTrmmLeaf);
   const int64_t copy = side == EH_LEFT ? leaf * n : m * leaf;
@@ -254,25 +296,75 @@ eh_status trmm_t(int side, int uplo, int transa, int diag, int64_t m, int64_t n,
                          : trmm_right<T>(E, 0, n, B, ldb, m, alpha, w, s);
 }
 
+// herm (complex): A is Hermitian (the mirror is conjugated, the diagonal's
+// imaginary part is not read), else symmetric.
 template <typename T>
 eh_status symm_t(int side, int uplo, int64_t m, int64_t n, T alpha, const T* A, int64_t lda, const T* B, int64_t ldb,
-                 T beta, T* C, int64_t ldc, hipStream_t s) {
+                 T beta, T* C, int64_t ldc, hipStream_t s, bool herm = false) {
   const int64_t ka = side == EH_LEFT ? m : n;
   if (alpha == T(0))  // only beta*C (A and B are not read)
-    return device_product<T>(EH_OP_N, EH_OP_N, m, n, 0, alpha, A, lda, B, ldb, beta, C, ldc, s);
+    return prod(EH_OP_N, EH_OP_N, m, n, 0, alpha, A, lda, B, ldb, beta, C, ldc, s);
   if (!A || !B) return fail(EH_ERR_INVALID, "A or B is NULL");
   StreamTemp t;
   EH_TRY(t.get((size_t)ka * ka * sizeof(T), s, "level-3 temporary"));
   T* D = static_cast<T*>(t.ptr);
-  hipError_t e = launch_tri_dense<T>(A, lda, (int)ka, D, ka, uplo == EH_LOWER, true, false, s);
+  hipError_t e = launch_tri_dense<T>(A, lda, (int)ka, D, ka, uplo == EH_LOWER, true, false, s, herm);
   if (e != hipSuccess) return hip_fail(e, "symmetric expansion launch");
-  if (side == EH_LEFT) return device_product<T>(EH_OP_N, EH_OP_N, m, n, m, alpha, D, ka, B, ldb, beta, C, ldc, s);
-  return device_product<T>(EH_OP_N, EH_OP_N, m, n, n, alpha, B, ldb, D, ka, beta, C, ldc, s);
+  if (side == EH_LEFT) return prod(EH_OP_N, EH_OP_N, m, n, m, alpha, D, ka, B, ldb, beta, C, ldc, s);
+  return prod(EH_OP_N, EH_OP_N, m, n, n, alpha, B, ldb, D, ka, beta, C, ldc, s);
 }
 
 template <typename T>
 T scalar(const void* p) {
   return *static_cast<const T*>(p);
+}
+
+template <typename R>
+cplx<R> cscalar(const void* p) {
+  const R* q = static_cast<const R*>(p);
+  return {q[0], q[1]};
+}
+
+// Complex update of one triangle of C (n x n): W = op1(X)*op2(Y) over the
+// whole square in a temporary, then C = alpha*W + alpha2*f(W^T) + beta*C on
+// the triangle (tri_combine).  k == 0 or alpha == 0 reads neither X nor Y.
+template <typename R>
+eh_status rank_update(int uplo, int o1, int o2, int64_t n, int64_t k, cplx<R> alpha, const cplx<R>* X, int64_t ldx,
+                      const cplx<R>* Y, int64_t ldy, cplx<R> alpha2, bool use2, bool herm, cplx<R> beta, cplx<R>* C,
+                      int64_t ldc, hipStream_t s) {
+  const bool lower = uplo == EH_LOWER;
+  if (k == 0 || alpha == cplx<R>(0)) {
+    if (beta == cplx<R>(1)) return EH_SUCCESS;  // C untouched (as ?herk / ?her2k leave it)
+    hipError_t e = launch_tri_combine<R>(nullptr, 0, (int)n, C, ldc, lower, cplx<R>(0), cplx<R>(0), false, beta, herm,
+                                         s);
+    return e == hipSuccess ? EH_SUCCESS : hip_fail(e, "triangle scale launch");
+  }
+  if (!X || !Y) return fail(EH_ERR_INVALID, "A or B is NULL");
+  StreamTemp t;
+  EH_TRY(t.get((size_t)n * n * sizeof(cplx<R>), s, "rank update temporary"));
+  cplx<R>* W = static_cast<cplx<R>*>(t.ptr);
+  EH_TRY(prod(o1, o2, n, n, k, cplx<R>(1), X, ldx, Y, ldy, cplx<R>(0), W, n, s));
+  hipError_t e = launch_tri_combine<R>(W, n, (int)n, C, ldc, lower, alpha, alpha2, use2, beta, herm, s);
+  return e == hipSuccess ? EH_SUCCESS : hip_fail(e, "triangle combine launch");
+}
+
+eh_status check_rank_args(int dtype, int uplo, int trans, int64_t n, int64_t k, const void* alpha, const void* A,
+                          int64_t lda, const void* B, int64_t ldb, bool needs_b, const void* beta, const void* C,
+                          int64_t ldc, int bad_trans) {
+  if (!valid_dtype(dtype)) return fail(EH_ERR_INVALID, "dtype must be EH_F32 or EH_F64");
+  if (uplo != EH_UPPER && uplo != EH_LOWER) return fail(EH_ERR_INVALID, "uplo must be EH_UPPER or EH_LOWER");
+  if (!valid_op(trans) || trans == bad_trans) return fail(EH_ERR_INVALID, "trans not allowed for this routine");
+  if (n < 0 || k < 0) return fail(EH_ERR_INVALID, "n, k must be >= 0");
+  if (n > INT32_MAX / 2 || k > INT32_MAX / 2) return fail(EH_ERR_UNSUPPORTED, "complex n, k must be below 2^30");
+  const int64_t rows = trans == EH_OP_N ? n : k;
+  if (lda < std::max<int64_t>(1, rows)) return fail(EH_ERR_INVALID, "lda too small");
+  if (needs_b && ldb < std::max<int64_t>(1, rows)) return fail(EH_ERR_INVALID, "ldb too small");
+  if (ldc < std::max<int64_t>(1, n)) return fail(EH_ERR_INVALID, "ldc too small");
+  if (!alpha || !beta) return fail(EH_ERR_INVALID, "alpha and beta must point to host scalars");
+  if (n > 0 && !C) return fail(EH_ERR_INVALID, "C is NULL");
+  (void)A;
+  (void)B;
+  return EH_SUCCESS;
 }
 
 }  // namespace
@@ -373,5 +465,137 @@ eh_status eh_trsm(eh_dtype dtype, eh_side side, eh_uplo uplo, eh_op transa, eh_d
   return trsm_t<float>(side, uplo, transa, diag, m, n, scalar<float>(alpha), static_cast<const float*>(A), lda,
                        static_cast<float*>(B), ldb, s);
 }
+
+// ---- complex routines -----------------------------------------------------
+
+eh_status eh_trsm_complex(eh_dtype dtype, eh_side side, eh_uplo uplo, eh_op transa, eh_diag diag, int64_t m,
+                          int64_t n, const void* alpha, const void* A, int64_t lda, void* B, int64_t ldb,
+                          void* stream) {
+  EH_TRY(check_tri_args(dtype, side, uplo, transa, diag, m, n, alpha, A, lda, B, ldb));
+  if (m == 0 || n == 0) return EH_SUCCESS;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (dtype == EH_F64)
+    return trsm_t<cplx<double>>(side, uplo, transa, diag, m, n, cscalar<double>(alpha),
+                                static_cast<const cplx<double>*>(A), lda, static_cast<cplx<double>*>(B), ldb, s);
+  return trsm_t<cplx<float>>(side, uplo, transa, diag, m, n, cscalar<float>(alpha), static_cast<const cplx<float>*>(A),
+                             lda, static_cast<cplx<float>*>(B), ldb, s);
+}
+
+eh_status eh_trmm_complex(eh_dtype dtype, eh_side side, eh_uplo uplo, eh_op transa, eh_diag diag, int64_t m,
+                          int64_t n, const void* alpha, const void* A, int64_t lda, void* B, int64_t ldb,
+                          void* stream) {
+  EH_TRY(check_tri_args(dtype, side, uplo, transa, diag, m, n, alpha, A, lda, B, ldb));
+  if (m == 0 || n == 0) return EH_SUCCESS;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (dtype == EH_F64)
+    return trmm_t<cplx<double>>(side, uplo, transa, diag, m, n, cscalar<double>(alpha),
+                                static_cast<const cplx<double>*>(A), lda, static_cast<cplx<double>*>(B), ldb, s);
+  return trmm_t<cplx<float>>(side, uplo, transa, diag, m, n, cscalar<float>(alpha), static_cast<const cplx<float>*>(A),
+                             lda, static_cast<cplx<float>*>(B), ldb, s);
+}
+
+static eh_status symm_complex(eh_dtype dtype, eh_side side, eh_uplo uplo, int64_t m, int64_t n, const void* alpha,
+                              const void* A, int64_t lda, const void* B, int64_t ldb, const void* beta, void* C,
+                              int64_t ldc, void* stream, bool herm) {
+  if (!valid_dtype(dtype)) return fail(EH_ERR_INVALID, "dtype must be EH_F32 or EH_F64");
+  if (side != EH_LEFT && side != EH_RIGHT) return fail(EH_ERR_INVALID, "side must be EH_LEFT or EH_RIGHT");
+  if (uplo != EH_UPPER && uplo != EH_LOWER) return fail(EH_ERR_INVALID, "uplo must be EH_UPPER or EH_LOWER");
+  if (m < 0 || n < 0) return fail(EH_ERR_INVALID, "m, n must be >= 0");
+  if (m > INT32_MAX / 2 || n > INT32_MAX / 2) return fail(EH_ERR_UNSUPPORTED, "complex m, n must be below 2^30");
+  if (lda < std::max<int64_t>(1, side == EH_LEFT ? m : n)) return fail(EH_ERR_INVALID, "lda too small");
+  if (ldb < std::max<int64_t>(1, m)) return fail(EH_ERR_INVALID, "ldb too small");
+  if (ldc < std::max<int64_t>(1, m)) return fail(EH_ERR_INVALID, "ldc too small");
+  if (!alpha || !beta) return fail(EH_ERR_INVALID, "alpha and beta must point to host scalars");
+  if (m == 0 || n == 0) return EH_SUCCESS;
+  if (!C) return fail(EH_ERR_INVALID, "C is NULL");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (dtype == EH_F64)
+    return symm_t<cplx<double>>(side, uplo, m, n, cscalar<double>(alpha), static_cast<const cplx<double>*>(A), lda,
+                                static_cast<const cplx<double>*>(B), ldb, cscalar<double>(beta),
+                                static_cast<cplx<double>*>(C), ldc, s, herm);
+  return symm_t<cplx<float>>(side, uplo, m, n, cscalar<float>(alpha), static_cast<const cplx<float>*>(A), lda,
+                             static_cast<const cplx<float>*>(B), ldb, cscalar<float>(beta),
+                             static_cast<cplx<float>*>(C), ldc, s, herm);
+}
+
+eh_status eh_symm_complex(eh_dtype dtype, eh_side side, eh_uplo uplo, int64_t m, int64_t n, const void* alpha,
+                          const void* A, int64_t lda, const void* B, int64_t ldb, const void* beta, void* C,
+                          int64_t ldc, void* stream) {
+  return symm_complex(dtype, side, uplo, m, n, alpha, A, lda, B, ldb, beta, C, ldc, stream, false);
+}
+
+eh_status eh_hemm(eh_dtype dtype, eh_side side, eh_uplo uplo, int64_t m, int64_t n, const void* alpha, const void* A,
+                  int64_t lda, const void* B, int64_t ldb, const void* beta, void* C, int64_t ldc, void* stream) {
+  return symm_complex(dtype, side, uplo, m, n, alpha, A, lda, B, ldb, beta, C, ldc, stream, true);
+}
+
+// rank updates: run the complex update with the scalar type of `dtype`
+#define EH_RANK_DISPATCH(call)                                       \
+  do {                                                               \
+    hipStream_t s = static_cast<hipStream_t>(stream);                \
+    if (dtype == EH_F64) {                                           \
+      using R = double;                                              \
+      return call;                                                   \
+    }                                                                \
+    using R = float;                                                 \
+    return call;                                                     \
+  } while (0)
+
+eh_status eh_syrk_complex(eh_dtype dtype, eh_uplo uplo, eh_op trans, int64_t n, int64_t k, const void* alpha,
+                          const void* A, int64_t lda, const void* beta, void* C, int64_t ldc, void* stream) {
+  EH_TRY(check_rank_args(dtype, uplo, trans, n, k, alpha, A, lda, nullptr, 0, false, beta, C, ldc, EH_OP_C));
+  if (n == 0) return EH_SUCCESS;
+  const int o1 = trans == EH_OP_N ? EH_OP_N : EH_OP_T, o2 = trans == EH_OP_N ? EH_OP_T : EH_OP_N;
+  EH_RANK_DISPATCH((rank_update<R>(uplo, o1, o2, n, k, cscalar<R>(alpha), static_cast<const cplx<R>*>(A), lda,
+                                   static_cast<const cplx<R>*>(A), lda, cplx<R>(0), false, false, cscalar<R>(beta),
+                                   static_cast<cplx<R>*>(C), ldc, s)));
+}
+
+eh_status eh_herk(eh_dtype dtype, eh_uplo uplo, eh_op trans, int64_t n, int64_t k, const void* alpha, const void* A,
+                  int64_t lda, const void* beta, void* C, int64_t ldc, void* stream) {
+  EH_TRY(check_rank_args(dtype, uplo, trans, n, k, alpha, A, lda, nullptr, 0, false, beta, C, ldc, EH_OP_T));
+  if (n == 0) return EH_SUCCESS;
+  const int o1 = trans == EH_OP_N ? EH_OP_N : EH_OP_C, o2 = trans == EH_OP_N ? EH_OP_C : EH_OP_N;
+  EH_RANK_DISPATCH((rank_update<R>(uplo, o1, o2, n, k, cplx<R>(scalar<R>(alpha)), static_cast<const cplx<R>*>(A), lda,
+                                   static_cast<const cplx<R>*>(A), lda, cplx<R>(0), false, true,
+                                   cplx<R>(scalar<R>(beta)), static_cast<cplx<R>*>(C), ldc, s)));
+}
+
+eh_status eh_syr2k_complex(eh_dtype dtype, eh_uplo uplo, eh_op trans, int64_t n, int64_t k, const void* alpha,
+                           const void* A, int64_t lda, const void* B, int64_t ldb, const void* beta, void* C,
+                           int64_t ldc, void* stream) {
+  EH_TRY(check_rank_args(dtype, uplo, trans, n, k, alpha, A, lda, B, ldb, true, beta, C, ldc, EH_OP_C));
+  if (n == 0) return EH_SUCCESS;
+  const int o1 = trans == EH_OP_N ? EH_OP_N : EH_OP_T, o2 = trans == EH_OP_N ? EH_OP_T : EH_OP_N;
+  // C = alpha*W + alpha*W^T + beta*C with W = op(A)*op(B)^T
+  EH_RANK_DISPATCH((rank_update<R>(uplo, o1, o2, n, k, cscalar<R>(alpha), static_cast<const cplx<R>*>(A), lda,
+                                   static_cast<const cplx<R>*>(B), ldb, cscalar<R>(alpha), true, false,
+                                   cscalar<R>(beta), static_cast<cplx<R>*>(C), ldc, s)));
+}
+
+eh_status eh_her2k(eh_dtype dtype, eh_uplo uplo, eh_op trans, int64_t n, int64_t k, const void* alpha, const void* A,
+                   int64_t lda, const void* B, int64_t ldb, const void* beta, void* C, int64_t ldc, void* stream) {
+  EH_TRY(check_rank_args(dtype, uplo, trans, n, k, alpha, A, lda, B, ldb, true, beta, C, ldc, EH_OP_T));
+  if (n == 0) return EH_SUCCESS;
+  const int o1 = trans == EH_OP_N ? EH_OP_N : EH_OP_C, o2 = trans == EH_OP_N ? EH_OP_C : EH_OP_N;
+  // C = alpha*W + conj(alpha)*W^H + beta*C with W = op(A)*op(B)^H, beta real
+  EH_RANK_DISPATCH((rank_update<R>(uplo, o1, o2, n, k, cscalar<R>(alpha), static_cast<const cplx<R>*>(A), lda,
+                                   static_cast<const cplx<R>*>(B), ldb, conj_if(cscalar<R>(alpha), true), true, true,
+                                   cplx<R>(scalar<R>(beta)), static_cast<cplx<R>*>(C), ldc, s)));
+}
+
+eh_status eh_gemmtr_complex(eh_dtype dtype, eh_uplo uplo, eh_op opa, eh_op opb, int64_t n, int64_t k,
+                            const void* alpha, const void* A, int64_t lda, const void* B, int64_t ldb,
+                            const void* beta, void* C, int64_t ldc, void* stream) {
+  if (uplo != EH_UPPER && uplo != EH_LOWER) return fail(EH_ERR_INVALID, "uplo must be EH_UPPER or EH_LOWER");
+  EH_TRY(check_gemm_args(dtype, opa, opb, n, n, k, alpha, A, lda, B, ldb, beta, C, ldc));
+  if (n > INT32_MAX / 2 || k > INT32_MAX / 2) return fail(EH_ERR_UNSUPPORTED, "complex n, k must be below 2^30");
+  if (n == 0) return EH_SUCCESS;
+  EH_RANK_DISPATCH((rank_update<R>(uplo, opa, opb, n, k, cscalar<R>(alpha), static_cast<const cplx<R>*>(A), lda,
+                                   static_cast<const cplx<R>*>(B), ldb, cplx<R>(0), false, false, cscalar<R>(beta),
+                                   static_cast<cplx<R>*>(C), ldc, s)));
+}
+
+#undef EH_RANK_DISPATCH
 
 }  // extern "C"

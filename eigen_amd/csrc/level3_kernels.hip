@@ -26,9 +26,12 @@
 
 namespace eh {
 
+// T is a real type or cplx<R>; herm (complex only): the mirror is conjugated
+// and the diagonal's imaginary part is dropped (BLAS ?hemm reads only the
+// real part of a Hermitian diagonal); conj: every element is conjugated.
 template <typename T>
 __global__ void __launch_bounds__(256) tri_dense_kernel(const T* A, int64_t lda, int n, T* D, int64_t ldd, int lower,
-                                                        int sym, int unit) {
+                                                        int sym, int unit, int herm, int conj) {
   __shared__ T mir[64][65];
   const int bi = blockIdx.x * 64, bj = blockIdx.y * 64;  // D rows bi.., columns bj..
   const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
@@ -42,7 +45,7 @@ __global__ void __launch_bounds__(256) tri_dense_kernel(const T* A, int64_t lda,
     const int ar = bj + tx;
     for (int r = ty; r < 64; r += 4) {
       const int ac = bi + r;
-      mir[tx][r] = (ar < n && ac < n && stored(ar, ac)) ? A[ar + (int64_t)ac * lda] : T(0);
+      mir[tx][r] = (ar < n && ac < n && stored(ar, ac)) ? conj_if(A[ar + (int64_t)ac * lda], herm) : T(0);
     }
     __syncthreads();
   }
@@ -54,21 +57,23 @@ __global__ void __launch_bounds__(256) tri_dense_kernel(const T* A, int64_t lda,
     T v;
     if (unit && i == j)
       v = T(1);
+    else if (herm && i == j)
+      v = real_part(A[i + (int64_t)j * lda]);
     else if (stored(i, j))
       v = A[i + (int64_t)j * lda];
     else
       v = sym ? mir[r][tx] : T(0);
-    D[i + (int64_t)j * ldd] = v;
+    D[i + (int64_t)j * ldd] = conj_if(v, conj);
   }
 }
 
 template <typename T>
 hipError_t launch_tri_dense(const T* A, int64_t lda, int n, T* D, int64_t ldd, bool lower, bool sym, bool unit,
-                            hipStream_t s) {
+                            hipStream_t s, bool herm, bool conj) {
   if (n <= 0) return hipSuccess;
   const unsigned nb = (unsigned)((n + 63) / 64);
   hipLaunchKernelGGL(tri_dense_kernel<T>, dim3(nb, nb), dim3(256), 0, s, A, lda, n, D, ldd, (int)lower, (int)sym,
-                     (int)unit);
+                     (int)unit, (int)herm, (int)conj);
   return hipGetLastError();
 }
 
@@ -81,7 +86,8 @@ hipError_t launch_tri_dense(const T* A, int64_t lda, int n, T* D, int64_t ldd, b
 // outside the triangle and past nb).  Entries of M_b outside its triangle
 // and past nb are zero in LDS and the reciprocal diagonal past nb is one, so
 // the unrolled substitution needs no guards.  Each dot product runs as two
-// interleaved partial sums (half the dependent chain).
+// interleaved partial sums (half the dependent chain).  T is a real type or
+// cplx<R> (then p.conj conjugates M, for an inverse of A^H's blocks).
 template <typename T, int NB>
 __global__ void __launch_bounds__(NB) trtri_kernel(TrtriParams<T> p) {
   __shared__ T M[NB][NB];
@@ -92,11 +98,11 @@ __global__ void __launch_bounds__(NB) trtri_kernel(TrtriParams<T> p) {
   for (int idx = threadIdx.x; idx < NB * NB; idx += NB) {
     const int i = idx % NB, k = idx / NB;
     const bool in = i < nb && k < nb && (p.lower ? k < i : k > i);
-    M[i][k] = in ? (p.mt ? A[k + (int64_t)i * p.lda] : A[i + (int64_t)k * p.lda]) : T(0);
+    M[i][k] = in ? conj_if(p.mt ? A[k + (int64_t)i * p.lda] : A[i + (int64_t)k * p.lda], p.conj != 0) : T(0);
   }
   if (threadIdx.x < NB) {
     const int i = threadIdx.x;
-    dinv[i] = (i >= nb || p.unit) ? T(1) : T(1) / A[i + (int64_t)i * p.lda];
+    dinv[i] = (i >= nb || p.unit) ? T(1) : recip(conj_if(A[i + (int64_t)i * p.lda], p.conj != 0));
   }
   __syncthreads();
   const int v = threadIdx.x;
@@ -133,19 +139,89 @@ __global__ void __launch_bounds__(NB) trtri_kernel(TrtriParams<T> p) {
   for (int i = 0; i < NB; ++i) w[i] = x[i];
 }
 
+// Complex rank updates: on the lower (upper) triangle of the n x n result,
+//   C(i, j) = alpha*W(i, j) + alpha2*f(W(j, i)) + beta*C(i, j)
+// with f = conj when herm (the W^H term of HER2K) else identity (W^T of
+// SYR2K), the alpha2 term only when use2, C not read when beta == 0, and
+// (herm) the diagonal's imaginary part set to zero as ?herk / ?her2k do.
+// W == nullptr: the W terms are zero (C = beta*C on the triangle).  64 x 64
+// tiles, tiles off the triangle return at once; the transposed term goes
+// through LDS so both its reads and the writes of C are coalesced.
+template <typename R>
+__global__ void __launch_bounds__(256) tri_combine_kernel(const cplx<R>* W, int64_t ldw, int n, cplx<R>* C,
+                                                          int64_t ldc, int lower, cplx<R> alpha, cplx<R> alpha2,
+                                                          int use2, cplx<R> beta, int herm) {
+  __shared__ cplx<R> tr[64][65];
+  const int bi = blockIdx.x * 64, bj = blockIdx.y * 64;  // C rows bi.., columns bj..
+  if (lower ? bi + 63 < bj : bj + 63 < bi) return;      // the whole tile is off the triangle
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  if (W && use2) {
+    // tr[c][r] = W(bj + c, bi + r): lanes walk rows of W (coalesced)
+    const int wr = bj + tx;
+    for (int r = ty; r < 64; r += 4) {
+      const int wc = bi + r;
+      tr[tx][r] = (wr < n && wc < n) ? W[wr + (int64_t)wc * ldw] : cplx<R>(0);
+    }
+    __syncthreads();
+  }
+  const int i = bi + tx;
+  if (i >= n) return;
+  const bool read_c = !(beta == cplx<R>(0));
+  for (int r = ty; r < 64; r += 4) {
+    const int j = bj + r;
+    if (j >= n) break;
+    if (lower ? i < j : i > j) continue;
+    cplx<R> v(0);
+    if (W) {
+      v = alpha * W[i + (int64_t)j * ldw];
+      if (use2) v = v + alpha2 * conj_if(tr[r][tx], herm != 0);
+    }
+    if (read_c) v = v + beta * C[i + (int64_t)j * ldc];
+    if (herm && i == j) v.im = R(0);
+    C[i + (int64_t)j * ldc] = v;
+  }
+}
+
+template <typename R>
+hipError_t launch_tri_combine(const cplx<R>* W, int64_t ldw, int n, cplx<R>* C, int64_t ldc, bool lower,
+                              cplx<R> alpha, cplx<R> alpha2, bool use2, cplx<R> beta, bool herm, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const unsigned nb = (unsigned)((n + 63) / 64);
+  hipLaunchKernelGGL(tri_combine_kernel<R>, dim3(nb, nb), dim3(256), 0, s, W, ldw, n, C, ldc, (int)lower, alpha, alpha2,
+                     (int)use2, beta, (int)herm);
+  return hipGetLastError();
+}
+template hipError_t launch_tri_combine<float>(const cplx<float>*, int64_t, int, cplx<float>*, int64_t, bool,
+                                              cplx<float>, cplx<float>, bool, cplx<float>, bool, hipStream_t);
+template hipError_t launch_tri_combine<double>(const cplx<double>*, int64_t, int, cplx<double>*, int64_t, bool,
+                                               cplx<double>, cplx<double>, bool, cplx<double>, bool, hipStream_t);
+
+template <typename T>
+struct TrsmLeafOf {
+  static constexpr int value = kTrsmLeaf;
+};
+template <typename R>
+struct TrsmLeafOf<cplx<R>> {
+  static constexpr int value = kTrsmLeafComplex;
+};
+
 template <typename T>
 hipError_t launch_trtri(const TrtriParams<T>& p, hipStream_t s) {
   if (p.order <= 0) return hipSuccess;
-  const unsigned blocks = (unsigned)((p.order + kTrsmLeaf - 1) / kTrsmLeaf);
-  hipLaunchKernelGGL((trtri_kernel<T, kTrsmLeaf>), dim3(blocks), dim3(kTrsmLeaf), 0, s, p);
+  constexpr int NB = TrsmLeafOf<T>::value;
+  const unsigned blocks = (unsigned)((p.order + NB - 1) / NB);
+  hipLaunchKernelGGL((trtri_kernel<T, NB>), dim3(blocks), dim3(NB), 0, s, p);
   return hipGetLastError();
 }
 
-template hipError_t launch_tri_dense<float>(const float*, int64_t, int, float*, int64_t, bool, bool, bool,
-                                            hipStream_t);
-template hipError_t launch_tri_dense<double>(const double*, int64_t, int, double*, int64_t, bool, bool, bool,
-                                             hipStream_t);
-template hipError_t launch_trtri<float>(const TrtriParams<float>&, hipStream_t);
-template hipError_t launch_trtri<double>(const TrtriParams<double>&, hipStream_t);
+#define EH_L3_INST(T)                                                                                       \
+  template hipError_t launch_tri_dense<T>(const T*, int64_t, int, T*, int64_t, bool, bool, bool, hipStream_t, \
+                                          bool, bool);                                                      \
+  template hipError_t launch_trtri<T>(const TrtriParams<T>&, hipStream_t);
+EH_L3_INST(float)
+EH_L3_INST(double)
+EH_L3_INST(cplx<float>)
+EH_L3_INST(cplx<double>)
+#undef EH_L3_INST
 
 }  // namespace eh

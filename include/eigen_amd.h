@@ -24,6 +24,9 @@
  *   eh_gemm_bf16
  *       the sbgemm redirect of bfloat16 products
  *       (Eigen/src/Core/products/GeneralMatrixMatrix_BLAS.h:120-202).
+ *   eh_gemm_f16
+ *       the product of Eigen::half matrices (GEBP on half scalars,
+ *       GeneralBlockPanelKernel.h with packet_traits<half>).
  *   eh_gemm_strided
  *       the same driver with a result increment (resIncr,
  *       GeneralMatrixMatrix.h:98-137) for destinations with an inner stride.
@@ -171,6 +174,22 @@ eh_status eh_gemm_bf16(eh_op opa, eh_op opb, int64_t m, int64_t n, int64_t k, co
                        int64_t lda, const void* B, int64_t ldb, const float* beta, float* C, int64_t ldc,
                        void* stream);
 
+/* IEEE half (binary16) inputs, FP32 result: C = alpha*op(A)*op(B) + beta*C
+ * with A and B half bit patterns (column-major, any element alignment) and
+ * alpha, beta, C float.  Products of half values are exact in FP32 and are
+ * accumulated in FP32 on the FP16 matrix cores (v_mfma_f32_32x32x16_f16), on
+ * the same packed planes and tiles as eh_gemm_bf16.  Replaces the product of
+ * Eigen::half matrices (GEBP over half scalars, exercised by
+ * test/product_large.cpp:366), which on hosts without native half arithmetic
+ * rounds every multiply-add to half (pmadd<Packet8h>,
+ * arch/AVX/PacketMath.h:2328); here only the final FP32 result is rounded,
+ * and a caller that wants half output converts it (Eigen::half(float),
+ * Half.h:683).  beta == 0 does not read C; alpha == 0 or k == 0 does not read
+ * A or B.  Operands must span less than 4 GB (EH_ERR_UNSUPPORTED otherwise). */
+eh_status eh_gemm_f16(eh_op opa, eh_op opb, int64_t m, int64_t n, int64_t k, const float* alpha, const void* A,
+                      int64_t lda, const void* B, int64_t ldb, const float* beta, float* C, int64_t ldc,
+                      void* stream);
+
 /* Complex product C = alpha*op(A)*op(B) + beta*C on interleaved complex
  * storage (std::complex<float> for EH_F32, std::complex<double> for EH_F64;
  * leading dimensions in complex elements; alpha and beta point to one host
@@ -284,6 +303,60 @@ eh_status eh_trmm(eh_dtype dtype, eh_side side, eh_uplo uplo, eh_op transa, eh_d
  * reference, it produces infinities. */
 eh_status eh_trsm(eh_dtype dtype, eh_side side, eh_uplo uplo, eh_op transa, eh_diag diag, int64_t m, int64_t n,
                   const void* alpha, const void* A, int64_t lda, void* B, int64_t ldb, void* stream);
+
+/* ---- complex level-3 routines ---------------------------------------------
+ * The complex counterparts of the routines above, on interleaved complex
+ * storage (std::complex<float> for EH_F32, std::complex<double> for EH_F64;
+ * leading dimensions in complex elements; alpha and beta point to one host
+ * complex scalar -- two reals -- unless stated "real").  Replace Eigen's
+ * complex BLAS instantiations of blas/level3_impl.h (?trsm_ :228-364, ?trmm_
+ * :366-503, ?symm_ :505-592, ?syrk_ :594-689, ?syr2k_ :690-758, ?gemmtr_
+ * :98-228, ?hemm_ :760-833, ?herk_ :835-912, ?her2k_ :915-986) and the
+ * Eigen products they run (triangularView<>().solve(), selfadjointView<>()
+ * products and rank updates on complex matrices).  Every product runs as one
+ * real product over 2x2 real blocks on the FP32 / FP64 MFMA kernels
+ * (eh_gemm_complex); op C conjugates.  Same device / stream / temporary
+ * rules as the real routines; dimensions below 2^30.
+ *   eh_trsm_complex, eh_trmm_complex: as eh_trsm / eh_trmm (transa C: op(A)
+ *     = A^H); the TRSM leaves invert 32 x 32 diagonal blocks.
+ *   eh_symm_complex: A complex symmetric (mirror not conjugated).
+ *   eh_hemm: A Hermitian, stored in its uplo triangle (mirror conjugated,
+ *     the imaginary part of its diagonal not read).
+ *   eh_syrk_complex:  C = alpha*op(A)*op(A)^T + beta*C, trans N or T.
+ *   eh_herk:   C = alpha*op(A)*op(A)^H + beta*C, trans N or C, alpha and
+ *     beta REAL (one host real each); the imaginary parts of C's diagonal
+ *     are set to zero unless the call leaves C unchanged.
+ *   eh_syr2k_complex: C = alpha*op(A)*op(B)^T + alpha*op(B)*op(A)^T + beta*C,
+ *     trans N or T.
+ *   eh_her2k: C = alpha*op(A)*op(B)^H + conj(alpha)*op(B)*op(A)^H + beta*C,
+ *     trans N or C, beta REAL; diagonal imaginary parts as eh_herk.
+ *   eh_gemmtr_complex: C = alpha*op(A)*op(B) + beta*C on one triangle.
+ * The rank updates and eh_gemmtr_complex form the full n x n product in a
+ * stream-ordered temporary and combine its uplo triangle into C (one pass);
+ * k == 0 or alpha == 0 scales the triangle without reading A or B. */
+eh_status eh_trsm_complex(eh_dtype dtype, eh_side side, eh_uplo uplo, eh_op transa, eh_diag diag, int64_t m,
+                          int64_t n, const void* alpha, const void* A, int64_t lda, void* B, int64_t ldb,
+                          void* stream);
+eh_status eh_trmm_complex(eh_dtype dtype, eh_side side, eh_uplo uplo, eh_op transa, eh_diag diag, int64_t m,
+                          int64_t n, const void* alpha, const void* A, int64_t lda, void* B, int64_t ldb,
+                          void* stream);
+eh_status eh_symm_complex(eh_dtype dtype, eh_side side, eh_uplo uplo, int64_t m, int64_t n, const void* alpha,
+                          const void* A, int64_t lda, const void* B, int64_t ldb, const void* beta, void* C,
+                          int64_t ldc, void* stream);
+eh_status eh_hemm(eh_dtype dtype, eh_side side, eh_uplo uplo, int64_t m, int64_t n, const void* alpha, const void* A,
+                  int64_t lda, const void* B, int64_t ldb, const void* beta, void* C, int64_t ldc, void* stream);
+eh_status eh_syrk_complex(eh_dtype dtype, eh_uplo uplo, eh_op trans, int64_t n, int64_t k, const void* alpha,
+                          const void* A, int64_t lda, const void* beta, void* C, int64_t ldc, void* stream);
+eh_status eh_herk(eh_dtype dtype, eh_uplo uplo, eh_op trans, int64_t n, int64_t k, const void* alpha, const void* A,
+                  int64_t lda, const void* beta, void* C, int64_t ldc, void* stream);
+eh_status eh_syr2k_complex(eh_dtype dtype, eh_uplo uplo, eh_op trans, int64_t n, int64_t k, const void* alpha,
+                           const void* A, int64_t lda, const void* B, int64_t ldb, const void* beta, void* C,
+                           int64_t ldc, void* stream);
+eh_status eh_her2k(eh_dtype dtype, eh_uplo uplo, eh_op trans, int64_t n, int64_t k, const void* alpha, const void* A,
+                   int64_t lda, const void* B, int64_t ldb, const void* beta, void* C, int64_t ldc, void* stream);
+eh_status eh_gemmtr_complex(eh_dtype dtype, eh_uplo uplo, eh_op opa, eh_op opb, int64_t n, int64_t k,
+                            const void* alpha, const void* A, int64_t lda, const void* B, int64_t ldb,
+                            const void* beta, void* C, int64_t ldc, void* stream);
 
 /* ---- planning ------------------------------------------------------------ */
 

@@ -99,6 +99,56 @@ void cgemm_(const char* transa, const char* transb, const int* m, const int* n, 
             const void* a, const int* lda, const void* b, const int* ldb, const void* beta, void* c, const int* ldc);
 void zgemm_(const char* transa, const char* transb, const int* m, const int* n, const int* k, const void* alpha,
             const void* a, const int* lda, const void* b, const int* ldb, const void* beta, void* c, const int* ldc);
+/* Complex level-3 routines (Fortran COMPLEX / COMPLEX*16 arguments as
+ * pairs of float / double), with the symbols, arguments, validation order,
+ * info codes and quick returns of Eigen's complex BLAS (blas/level3_impl.h):
+ *   c/z trsm_, trmm_, symm_, syr2k_, gemmtr_: as the real routines;
+ *   c/z syrk_ (:594-689): trans 'C' is rejected (info 2), as in the reference;
+ *   c/z hemm_ (:760-833): info 1 side, 2 uplo, 3 m, 4 n, 7 lda, 9 ldb, 12 ldc;
+ *   c/z herk_ (:835-912): alpha and beta REAL; trans 'T' rejected (info 2);
+ *       info 1 uplo, 3 n, 4 k, 7 lda, 10 ldc;
+ *   c/z her2k_ (:915-986): beta REAL; trans 'T' rejected; info 1 uplo, 2 trans,
+ *       3 n, 4 k, 7 lda, 9 ldb, 12 ldc.
+ * Device or host pointers as for ?gemm_; the call returns when the output
+ * is complete. */
+void ctrsm_(const char* side, const char* uplo, const char* transa, const char* diag, const int* m, const int* n,
+            const void* alpha, const void* a, const int* lda, void* b, const int* ldb);
+void ztrsm_(const char* side, const char* uplo, const char* transa, const char* diag, const int* m, const int* n,
+            const void* alpha, const void* a, const int* lda, void* b, const int* ldb);
+void ctrmm_(const char* side, const char* uplo, const char* transa, const char* diag, const int* m, const int* n,
+            const void* alpha, const void* a, const int* lda, void* b, const int* ldb);
+void ztrmm_(const char* side, const char* uplo, const char* transa, const char* diag, const int* m, const int* n,
+            const void* alpha, const void* a, const int* lda, void* b, const int* ldb);
+void csymm_(const char* side, const char* uplo, const int* m, const int* n, const void* alpha, const void* a,
+            const int* lda, const void* b, const int* ldb, const void* beta, void* c, const int* ldc);
+void zsymm_(const char* side, const char* uplo, const int* m, const int* n, const void* alpha, const void* a,
+            const int* lda, const void* b, const int* ldb, const void* beta, void* c, const int* ldc);
+void chemm_(const char* side, const char* uplo, const int* m, const int* n, const void* alpha, const void* a,
+            const int* lda, const void* b, const int* ldb, const void* beta, void* c, const int* ldc);
+void zhemm_(const char* side, const char* uplo, const int* m, const int* n, const void* alpha, const void* a,
+            const int* lda, const void* b, const int* ldb, const void* beta, void* c, const int* ldc);
+void csyrk_(const char* uplo, const char* trans, const int* n, const int* k, const void* alpha, const void* a,
+            const int* lda, const void* beta, void* c, const int* ldc);
+void zsyrk_(const char* uplo, const char* trans, const int* n, const int* k, const void* alpha, const void* a,
+            const int* lda, const void* beta, void* c, const int* ldc);
+void cherk_(const char* uplo, const char* trans, const int* n, const int* k, const float* alpha, const void* a,
+            const int* lda, const float* beta, void* c, const int* ldc);
+void zherk_(const char* uplo, const char* trans, const int* n, const int* k, const double* alpha, const void* a,
+            const int* lda, const double* beta, void* c, const int* ldc);
+void csyr2k_(const char* uplo, const char* trans, const int* n, const int* k, const void* alpha, const void* a,
+             const int* lda, const void* b, const int* ldb, const void* beta, void* c, const int* ldc);
+void zsyr2k_(const char* uplo, const char* trans, const int* n, const int* k, const void* alpha, const void* a,
+             const int* lda, const void* b, const int* ldb, const void* beta, void* c, const int* ldc);
+void cher2k_(const char* uplo, const char* trans, const int* n, const int* k, const void* alpha, const void* a,
+             const int* lda, const void* b, const int* ldb, const float* beta, void* c, const int* ldc);
+void zher2k_(const char* uplo, const char* trans, const int* n, const int* k, const void* alpha, const void* a,
+             const int* lda, const void* b, const int* ldb, const double* beta, void* c, const int* ldc);
+void cgemmtr_(const char* uplo, const char* transa, const char* transb, const int* n, const int* k,
+              const void* alpha, const void* a, const int* lda, const void* b, const int* ldb, const void* beta,
+              void* c, const int* ldc);
+void zgemmtr_(const char* uplo, const char* transa, const char* transb, const int* n, const int* k,
+              const void* alpha, const void* a, const int* lda, const void* b, const int* ldb, const void* beta,
+              void* c, const int* ldc);
 /* Complex matrix-vector product (the ?gemv_ contract on complex elements,
  * names "CGEMV " / "ZGEMV "); op 'C' conjugates. */
 void cgemv_(const char* trans, const int* m, const int* n, const void* alpha, const void* a, const int* lda,

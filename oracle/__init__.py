@@ -88,6 +88,8 @@ def lib():
                                          vp, i64, vp]
         L.orc_gemm_split_pairs_ref.argtypes = [i32, i32, i32, i32, i64, i64, i64, f64, vp, i64, vp, i64, f64, vp,
                                                i64, vp, i64, vp]
+        L.orc_half_rtne_n.argtypes = [vp, i64, vp]
+        L.orc_half_to_f32_n.argtypes = [vp, i64, vp]
         L.orc_split_kept_pairs.restype = i32
         L.orc_split_kept_pairs.argtypes = [i32]
         _lib = L
@@ -117,6 +119,8 @@ def ref():
         L.ref_bf16_rtne.restype = C.c_uint16
         L.ref_bf16_rtne.argtypes = [C.c_float]
         L.ref_bf16_rtne_n.argtypes = [vp, i64, vp]
+        L.ref_half_rtne_n.argtypes = [vp, i64, vp]
+        L.ref_gemm_half.argtypes = [lg, lg, lg, vp, vp, vp]
         L.ref_max_threads.restype = i32
         L.ref_get_threads.restype = i32
         L.ref_set_threads.argtypes = [i32]
@@ -213,6 +217,21 @@ def bf16_rtne(x: np.ndarray) -> np.ndarray:
     return t0
 
 
+def half_rtne(x: np.ndarray) -> np.ndarray:
+    """IEEE binary16 bits of float32 values, round to nearest even."""
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    out = np.empty(x.shape, np.uint16)
+    lib().orc_half_rtne_n(_ptr(x), x.size, _ptr(out))
+    return out
+
+
+def half_to_f32(h: np.ndarray) -> np.ndarray:
+    h = np.ascontiguousarray(h, dtype=np.uint16)
+    out = np.empty(h.shape, np.float32)
+    lib().orc_half_to_f32_n(_ptr(h), h.size, _ptr(out))
+    return out
+
+
 def bf16_split(x: np.ndarray, terms: int):
     x = np.ascontiguousarray(x, dtype=np.float32)
     ts = [np.zeros(x.shape, np.uint16) for _ in range(3)]
@@ -274,55 +293,138 @@ def higham_mary_bound(absprod: np.ndarray, k: int, u: float, lam: float = 5.0, n
 
 
 # --------------------------------------------------------------------------
-# Other real level-3 routines (restated in float64 numpy; for FP64 inputs the
-# float64 result carries rounding of the same order as the device's, so
-# tests compare against it with normwise bounds, not bit for bit).
+# Other level-3 routines, real and complex (restated in numpy: float64 for
+# real inputs, extended-precision complex for complex ones; tests compare the
+# device with normwise bounds, not bit for bit).
 # --------------------------------------------------------------------------
+def _w(x) -> np.ndarray:
+    """Working copy: float64 for real arrays, clongdouble for complex ones."""
+    x = np.asarray(x)
+    return np.array(x, np.clongdouble if np.iscomplexobj(x) else np.float64)
+
+
 def _tri_dense(A: np.ndarray, n: int, lower: bool, unit: bool) -> np.ndarray:
-    """The n x n triangle of A (column-major) as a dense float64 matrix; the
-    other triangle is never read, unit puts ones on the diagonal."""
-    T = np.tril(np.asarray(A[:n, :n], np.float64)) if lower else np.triu(np.asarray(A[:n, :n], np.float64))
+    """The n x n triangle of A (column-major) as a dense matrix; the other
+    triangle is never read, unit puts ones on the diagonal."""
+    T = _w(A[:n, :n])
+    T = np.tril(T) if lower else np.triu(T)
     if unit:
         np.fill_diagonal(T, 1.0)
     return T
 
 
 def _sym_dense(A: np.ndarray, n: int, lower: bool) -> np.ndarray:
-    """Symmetric matrix from the stored triangle (blas/level3_impl.h:565-571 form)."""
+    """Symmetric matrix from the stored triangle (blas/level3_impl.h:550-557
+    form: the mirror is the plain transpose, also for complex scalars)."""
     T = _tri_dense(A, n, lower, False)
     return T + np.tril(T, -1).T if lower else T + np.triu(T, 1).T
 
 
+def _herm_dense(A: np.ndarray, n: int, lower: bool) -> np.ndarray:
+    """Hermitian matrix from the stored triangle (?hemm_, blas/level3_impl.h:760-833
+    through selfadjointView): the mirror is the conjugate transpose and only the
+    real part of the diagonal is read."""
+    T = _tri_dense(A, n, lower, False)
+    np.fill_diagonal(T, np.real(np.diag(T)))
+    return T + np.tril(T, -1).conj().T if lower else T + np.triu(T, 1).conj().T
+
+
 def _op(x: np.ndarray, trans: str) -> np.ndarray:
-    return x.T if trans.upper() in "TC" else x
+    t = trans.upper()
+    return x if t == "N" else (x.T if t == "T" else x.conj().T)
+
+
+def _tri_mask(n: int, uplo: str) -> np.ndarray:
+    return np.tril(np.ones((n, n), bool)) if uplo.upper() == "L" else np.triu(np.ones((n, n), bool))
 
 
 def gemmtr_ref(uplo: str, ta: str, tb: str, alpha, A, B, beta, C0) -> np.ndarray:
     """?gemmtr_ (blas/level3_impl.h:198-227): beta-scale the uplo triangle
     (beta == 0 sets it to zero), then add alpha*op(A)*op(B) to it; the other
     triangle of C is returned unchanged."""
-    C = np.array(C0, np.float64, order="F")
+    C = _w(C0)
     n = C.shape[0]
-    P = alpha * (_op(np.asarray(A, np.float64), ta) @ _op(np.asarray(B, np.float64), tb))
-    mask = np.tril(np.ones((n, n), bool)) if uplo.upper() == "L" else np.triu(np.ones((n, n), bool))
+    P = alpha * (_op(_w(A), ta) @ _op(_w(B), tb))
+    mask = _tri_mask(n, uplo)
     scaled = np.zeros_like(C) if beta == 0 else beta * C
     C[mask] = (scaled + P)[mask]
     return C
 
 
-def syr2k_ref(uplo: str, trans: str, alpha, A, B, beta, C0) -> np.ndarray:
-    """?syr2k_ (blas/level3_impl.h:718-757): beta-scale the uplo triangle, then
-    add alpha*(A*B^T + B*A^T) (trans N) or alpha*(A^T*B + B^T*A) (trans T/C)."""
-    A64, B64 = np.asarray(A, np.float64), np.asarray(B, np.float64)
-    if trans.upper() == "N":
-        P = A64 @ B64.T + B64 @ A64.T
-    else:
-        P = A64.T @ B64 + B64.T @ A64
-    C = np.array(C0, np.float64, order="F")
-    n = C.shape[0]
-    mask = np.tril(np.ones((n, n), bool)) if uplo.upper() == "L" else np.triu(np.ones((n, n), bool))
+def syrk_ref(uplo: str, trans: str, alpha, A, beta, C0) -> np.ndarray:
+    """?syrk_ (blas/level3_impl.h:651-689): beta-scale the uplo triangle, then add
+    alpha*A*A^T (trans N) or alpha*A^T*A (trans T; also C for real scalars)."""
+    A_ = _w(A)
+    P = A_ @ A_.T if trans.upper() == "N" else A_.T @ A_
+    C = _w(C0)
+    mask = _tri_mask(C.shape[0], uplo)
     scaled = np.zeros_like(C) if beta == 0 else beta * C
     C[mask] = (scaled + alpha * P)[mask]
+    return C
+
+
+def syr2k_ref(uplo: str, trans: str, alpha, A, B, beta, C0) -> np.ndarray:
+    """?syr2k_ (blas/level3_impl.h:718-757): beta-scale the uplo triangle, then
+    add alpha*(A*B^T + B*A^T) (trans N) or alpha*(A^T*B + B^T*A) (trans T;
+    also C for real scalars)."""
+    A_, B_ = _w(A), _w(B)
+    if trans.upper() == "N":
+        P = A_ @ B_.T + B_ @ A_.T
+    else:
+        P = A_.T @ B_ + B_.T @ A_
+    C = _w(C0)
+    mask = _tri_mask(C.shape[0], uplo)
+    scaled = np.zeros_like(C) if beta == 0 else beta * C
+    C[mask] = (scaled + alpha * P)[mask]
+    return C
+
+
+def herk_ref(uplo: str, trans: str, alpha: float, A, beta: float, C0) -> np.ndarray:
+    """?herk_ (blas/level3_impl.h:886-911), alpha and beta real: when beta != 1
+    the uplo triangle is scaled (beta == 0 sets it to zero) and the diagonal's
+    imaginary part cleared; when k > 0 and alpha != 0, alpha*A*A^H (trans N) or
+    alpha*A^H*A (trans C) is added and the diagonal's imaginary part cleared."""
+    A_ = _w(A)
+    C = _w(C0)
+    n = C.shape[0]
+    k = A_.shape[1] if trans.upper() == "N" else A_.shape[0]
+    mask = _tri_mask(n, uplo)
+    d = np.arange(n)
+    if beta != 1:
+        C[mask] = 0 if beta == 0 else (beta * C)[mask]
+        C[d, d] = C[d, d].real
+    if k > 0 and alpha != 0:
+        P = A_ @ A_.conj().T if trans.upper() == "N" else A_.conj().T @ A_
+        C[mask] = (C + alpha * P)[mask]
+        C[d, d] = C[d, d].real
+    return C
+
+
+def her2k_ref(uplo: str, trans: str, alpha, A, B, beta: float, C0) -> np.ndarray:
+    """?her2k_ (blas/level3_impl.h:944-985), beta real: when beta != 1 the uplo
+    triangle is scaled (zero for beta == 0) and the diagonal's imaginary part
+    cleared, else (k > 0, alpha != 0) only the latter; then, for k > 0, the
+    triangle gets alpha*A*B^H + conj(alpha)*B*A^H (trans N) or
+    alpha*A^H*B + conj(alpha)*B^H*A (trans C)."""
+    A_, B_ = _w(A), _w(B)
+    C = _w(C0)
+    n = C.shape[0]
+    k = A_.shape[1] if trans.upper() == "N" else A_.shape[0]
+    mask = _tri_mask(n, uplo)
+    d = np.arange(n)
+    if beta != 1:
+        C[mask] = 0 if beta == 0 else (beta * C)[mask]
+        C[d, d] = C[d, d].real
+    elif k > 0 and alpha != 0:
+        C[d, d] = C[d, d].real
+    if k == 0:
+        return C
+    ca = np.conj(alpha)
+    if trans.upper() == "N":
+        P = alpha * (A_ @ B_.conj().T) + ca * (B_ @ A_.conj().T)
+    else:
+        P = alpha * (A_.conj().T @ B_) + ca * (B_.conj().T @ A_)
+    C[mask] = (C + P)[mask]
     return C
 
 
@@ -333,33 +435,45 @@ def symm_ref(side: str, uplo: str, alpha, A, B, beta, C0) -> np.ndarray:
     m, n = C0.shape
     ka = m if side.upper() == "L" else n
     S = _sym_dense(A, ka, uplo.upper() == "L")
-    B64 = np.asarray(B, np.float64)[:m, :n]
-    C = np.zeros((m, n)) if beta == 0 else beta * np.asarray(C0, np.float64)
-    return C + alpha * (S @ B64 if side.upper() == "L" else B64 @ S)
+    B_ = _w(B)[:m, :n]
+    C = np.zeros_like(_w(C0)) if beta == 0 else beta * _w(C0)
+    return C + alpha * (S @ B_ if side.upper() == "L" else B_ @ S)
+
+
+def hemm_ref(side: str, uplo: str, alpha, A, B, beta, C0) -> np.ndarray:
+    """?hemm_ (blas/level3_impl.h:760-833): as symm_ref with the Hermitian
+    matrix of A's uplo triangle."""
+    m, n = C0.shape
+    ka = m if side.upper() == "L" else n
+    H = _herm_dense(A, ka, uplo.upper() == "L")
+    B_ = _w(B)[:m, :n]
+    C = np.zeros_like(_w(C0)) if beta == 0 else beta * _w(C0)
+    return C + alpha * (H @ B_ if side.upper() == "L" else B_ @ H)
 
 
 def trmm_ref(side: str, uplo: str, trans: str, diag: str, alpha, A, B0) -> np.ndarray:
     """?trmm_ (blas/level3_impl.h:482-502): B = alpha*op(T)*B (side L) or
     alpha*B*op(T) (side R), T the uplo triangle of A (unit diagonal when diag
-    is U)."""
+    is U); op C is the conjugate transpose."""
     m, n = B0.shape
     ka = m if side.upper() == "L" else n
     T = _op(_tri_dense(A, ka, uplo.upper() == "L", diag.upper() == "U"), trans)
-    B64 = np.asarray(B0, np.float64)
-    return alpha * (T @ B64 if side.upper() == "L" else B64 @ T)
+    B_ = _w(B0)
+    return alpha * (T @ B_ if side.upper() == "L" else B_ @ T)
 
 
 def trsm_ref(side: str, uplo: str, trans: str, diag: str, alpha, A, B0) -> np.ndarray:
     """?trsm_ (blas/level3_impl.h:339-364): X with op(T)*X = B (side L) or
     X*op(T) = B (side R), then scaled by alpha (the reference's order); T the
-    uplo triangle of A.  Substitution in float64, the textbook recurrence
-    over the rows of X (side R: over the rows of X^T, solving op(T)^T X^T = B^T)."""
+    uplo triangle of A.  Substitution (float64, or extended complex), the
+    textbook recurrence over the rows of X (side R: over the rows of X^T,
+    solving op(T)^T X^T = B^T)."""
     m, n = B0.shape
     left = side.upper() == "L"
     ka = m if left else n
     T = _op(_tri_dense(A, ka, uplo.upper() == "L", diag.upper() == "U"), trans)
     lower = (uplo.upper() == "L") != (trans.upper() in "TC")  # triangle of op(T)
-    Bt = np.array(B0, np.float64)
+    Bt = _w(B0)
     if not left:
         T, Bt, lower = T.T, Bt.T.copy(), not lower
     X = np.zeros_like(Bt)

@@ -131,6 +131,23 @@ __attribute__((visibility("default"))) void ref_bf16_rtne_n(const float* x, int6
   for (int64_t i = 0; i < n; ++i) out[i] = Eigen::numext::bit_cast<uint16_t>(Eigen::bfloat16(x[i]));
 }
 
+// Eigen's own float -> half conversion (Eigen::half(float), float_to_half_rtne).
+__attribute__((visibility("default"))) void ref_half_rtne_n(const float* x, int64_t n, uint16_t* out) {
+  for (int64_t i = 0; i < n; ++i) out[i] = Eigen::numext::bit_cast<uint16_t>(Eigen::half(x[i]));
+}
+
+// Eigen's own half product C = A*B (Matrix<half>, GEBP on half scalars) on
+// host column-major arrays of half bits: the reference for what Eigen's CPU
+// product of half matrices returns.
+__attribute__((visibility("default"))) void ref_gemm_half(long m, long n, long k, const uint16_t* A, const uint16_t* B,
+                                                          uint16_t* C) {
+  using HM = Eigen::Matrix<Eigen::half, Eigen::Dynamic, Eigen::Dynamic>;
+  Eigen::Map<const HM> a(reinterpret_cast<const Eigen::half*>(A), m, k);
+  Eigen::Map<const HM> b(reinterpret_cast<const Eigen::half*>(B), k, n);
+  Eigen::Map<HM> c(reinterpret_cast<Eigen::half*>(C), m, n);
+  c.noalias() = a * b;
+}
+
 // Eigen's blocking choice for (m,n,k) on this host (informational).
 __attribute__((visibility("default"))) void ref_blocking_f32(long m, long n, long k, long* kc, long* mc, long* nc) {
   Eigen::Index K = k, M = m, N = n;

@@ -346,3 +346,62 @@ ORC_API void orc_gemm_split_ref(int terms, int ta, int tb, int64_t m, int64_t n,
 ORC_API void orc_bf16_rtne_n(const float* x, int64_t n, uint16_t* out) {
   for (int64_t i = 0; i < n; ++i) out[i] = orc_bf16_rtne(x[i]);
 }
+
+/* ------------------------------------------------------------------------ */
+/* IEEE binary16: float -> half round to nearest, ties to even (the           */
+/* conversion Eigen::half(float) performs, float_to_half_rtne in               */
+/* Eigen/src/Core/arch/Default/Half.h:683), restated from the IEEE 754         */
+/* format: 5 exponent bits (bias 15), 10 fraction bits, subnormals down to     */
+/* 2^-24, overflow to infinity at and above 65520 (the tie between 65504 and   */
+/* 2^16 rounds to the even 2^16), NaN kept quiet with its top fraction bits.   */
+/* ------------------------------------------------------------------------ */
+ORC_API uint16_t orc_half_rtne(float f) {
+  uint32_t u;
+  memcpy(&u, &f, 4);
+  const uint16_t sign = (uint16_t)((u >> 16) & 0x8000u);
+  const uint32_t a = u & 0x7fffffffu;
+  if (a > 0x7f800000u) return (uint16_t)(sign | 0x7e00u | ((a >> 13) & 0x3ffu)); /* NaN */
+  if (a >= 0x477ff000u) return (uint16_t)(sign | 0x7c00u);                         /* inf or overflow */
+  const uint32_t e = a >> 23;                                                       /* biased float exponent */
+  uint32_t q, rest, half_ulp;
+  if (e >= 113) { /* normal half: exponent e - 112, fraction = top 10 of 23 bits */
+    q = ((e - 112) << 10) | ((a >> 13) & 0x3ffu);
+    rest = a & 0x1fffu;
+    half_ulp = 0x1000u;
+  } else { /* subnormal half (or zero): count units of 2^-24 */
+    if (e < 102) return sign; /* below 2^-25: rounds to zero (2^-25 itself is e = 102) */
+    const uint32_t m = (a & 0x7fffffu) | 0x800000u;
+    const uint32_t shift = 126 - e; /* value = m * 2^(e - 150) = (m >> shift) units of 2^-24 */
+    q = m >> shift;
+    rest = m & ((1u << shift) - 1u);
+    half_ulp = 1u << (shift - 1);
+  }
+  if (rest > half_ulp || (rest == half_ulp && (q & 1u))) ++q; /* may carry into the exponent */
+  return (uint16_t)(sign | q);
+}
+
+ORC_API float orc_half_to_f32(uint16_t h) {
+  const uint32_t sign = (uint32_t)(h & 0x8000u) << 16;
+  const uint32_t e = (h >> 10) & 0x1fu, frac = h & 0x3ffu;
+  uint32_t u;
+  if (e == 0x1f) {
+    u = sign | 0x7f800000u | (frac << 13);
+  } else if (e != 0) {
+    u = sign | ((e + 112) << 23) | (frac << 13);
+  } else {
+    float v = (float)frac * 5.9604644775390625e-08f; /* frac * 2^-24, exact */
+    memcpy(&u, &v, 4);
+    u |= sign;
+  }
+  float f;
+  memcpy(&f, &u, 4);
+  return f;
+}
+
+ORC_API void orc_half_rtne_n(const float* x, int64_t n, uint16_t* out) {
+  for (int64_t i = 0; i < n; ++i) out[i] = orc_half_rtne(x[i]);
+}
+
+ORC_API void orc_half_to_f32_n(const uint16_t* h, int64_t n, float* out) {
+  for (int64_t i = 0; i < n; ++i) out[i] = orc_half_to_f32(h[i]);
+}

@@ -1,6 +1,6 @@
-"""Netlib level-3 known-answer testers (blas/testing/sblat3.f, dblat3.f of the
-reference, compiled into oracle/_ref and linked against the library's BLAS),
-and cblat3.f / zblat3.f for the complex GEMM.
+"""Netlib level-3 known-answer testers (blas/testing/sblat3.f, dblat3.f,
+cblat3.f, zblat3.f of the reference, compiled into oracle/_ref and linked
+against the library's BLAS), every routine of every precision switched on.
 
 The input deck mirrors the reference's sblat3.dat / dblat3.dat (N in
 {0,1,2,3,5,9}, alpha in {0,1,0.7}, beta in {0,1,1.3}, threshold 16) with the
@@ -26,9 +26,10 @@ ROUTINES = [("GEMM ", True), ("SYMM ", True), ("TRMM ", True), ("TRSM ", True), 
 NAMES = ["GEMM", "SYMM", "TRMM", "TRSM", "SYRK", "SYR2K"]
 
 
-# complex decks (cblat3.dat / zblat3.dat): the library provides GEMM only
-ROUTINES_COMPLEX = [("GEMM ", True), ("HEMM ", False), ("SYMM ", False), ("TRMM ", False), ("TRSM ", False),
-                    ("HERK ", False), ("SYRK ", False), ("HER2K", False), ("SYR2K", False)]
+# complex decks (cblat3.dat / zblat3.dat): all nine complex routines
+ROUTINES_COMPLEX = [("GEMM ", True), ("HEMM ", True), ("SYMM ", True), ("TRMM ", True), ("TRSM ", True),
+                    ("HERK ", True), ("SYRK ", True), ("HER2K", True), ("SYR2K", True)]
+NAMES_COMPLEX = ["GEMM", "HEMM", "SYMM", "TRMM", "TRSM", "HERK", "SYRK", "HER2K", "SYR2K"]
 
 
 def deck(prec: str, ns, error_exits=True) -> str:
@@ -89,7 +90,9 @@ def test_computational(gpu, prec):
 @pytest.mark.parametrize("prec", ["c", "z"])
 def test_complex_error_exits(prec):
     out = run_tester(prec, [0])
-    assert re.search(rf"{prec.upper()}GEMM\s+PASSED THE TESTS OF ERROR-EXITS", out), out
+    for name in NAMES_COMPLEX:
+        assert re.search(rf"{prec.upper()}{name}\s+PASSED THE TESTS OF ERROR-EXITS", out), (name, out)
+        assert re.search(rf"{prec.upper()}{name}\s+PASSED THE COMPUTATIONAL TESTS \(\s*\d+ CALLS\)", out), (name, out)
     assert f"{prec.upper()}GEMM  PASSED THE COMPUTATIONAL TESTS (    81 CALLS)" in out, out
     assert "FAILED" not in out and "SUSPECT" not in out, out
 
@@ -97,9 +100,11 @@ def test_complex_error_exits(prec):
 @pytest.mark.gpu
 @pytest.mark.parametrize("prec", ["c", "z"])
 def test_complex_computational(gpu, prec):
-    """CGEMM / ZGEMM: every op combination (N, T, C), complex alpha and beta."""
+    """All nine complex routines: every op / side / uplo / diag combination,
+    complex alpha and beta (real for HERK, HER2K's beta)."""
     out = run_tester(prec, [0, 1, 2, 3, 5, 9])
-    assert re.search(rf"{prec.upper()}GEMM\s+PASSED THE TESTS OF ERROR-EXITS", out), out
-    m = re.search(rf"{prec.upper()}GEMM\s+PASSED THE COMPUTATIONAL TESTS \(\s*(\d+) CALLS\)", out)
-    assert m and int(m.group(1)) > 10000, out
+    for name in NAMES_COMPLEX:
+        assert re.search(rf"{prec.upper()}{name}\s+PASSED THE TESTS OF ERROR-EXITS", out), (name, out)
+        m = re.search(rf"{prec.upper()}{name}\s+PASSED THE COMPUTATIONAL TESTS \(\s*(\d+) CALLS\)", out)
+        assert m and int(m.group(1)) > 1000, (name, out)
     assert "FAILED" not in out and "SUSPECT" not in out, out

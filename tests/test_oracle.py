@@ -83,6 +83,54 @@ def test_bf16_split_is_exact():
     assert [int(v) for v in inf3[0]] == [0x7F80, 0xFF80] and not inf3[1].any() and not inf3[2].any()
 
 
+# ---------------------------------------------------------------- IEEE half
+def test_half_golden_vectors():
+    # test/half_float.cpp:33-52 (conversion from float, denormals, ties to even)
+    cases = [(1.0, 0x3C00), (0.5, 0x3800), (0.33333, 0x3555), (0.0, 0x0000), (-0.0, 0x8000), (65504.0, 0x7BFF),
+             (65536.0, 0x7C00), (-5.96046e-08, 0x8001), (5.96046e-08, 0x0001), (1.19209e-07, 0x0002),
+             (-1.0, 0xBC00), (2.0, 0x4000), (3.0, 0x4200)]
+    got = O.half_rtne(np.array([c[0] for c in cases], np.float32))
+    assert [int(g) for g in got] == [c[1] for c in cases]
+    v = O.half_to_f32(np.array([0x3C00, 0x3C01, 0x3C02], np.uint16))
+    mids = np.array([0.5 * (v[0] + v[1]), 0.5 * (v[1] + v[2])], np.float32)
+    assert [int(g) for g in O.half_rtne(mids)] == [0x3C00, 0x3C02]
+    # widening: test/half_float.cpp:67-73
+    w = O.half_to_f32(np.array([0x0000, 0x3C00, 0x8001, 0x0001, 0x0002, 0x7BFF, 0x7C00, 0xFC00], np.uint16))
+    assert list(w[:2]) == [0.0, 1.0] and w[2] == -(2.0 ** -24) and w[3] == 2.0 ** -24 and w[4] == 2.0 ** -23
+    assert w[5] == 65504.0 and np.isposinf(w[6]) and np.isneginf(w[7])
+
+
+def test_half_matches_ieee_conversion_sweep():
+    # every 4099th float bit pattern and all half values' neighbourhoods, against
+    # numpy's IEEE float32 -> float16 conversion (round to nearest even)
+    bits = np.arange(0, 2 ** 32, 4099, dtype=np.uint64).astype(np.uint32)
+    x = bits.view(np.float32)
+    with np.errstate(over="ignore", invalid="ignore"):
+        ref = x.astype(np.float16).view(np.uint16)
+    ours = O.half_rtne(x)
+    fin = ~np.isnan(x)
+    assert np.array_equal(ours[fin], ref[fin])
+    assert np.all((ours[~fin] & 0x7E00) == 0x7E00)  # NaN stays NaN (quiet)
+    # widening is exact and inverts the rounding on every half value
+    h = np.arange(0, 2 ** 16, dtype=np.uint32).astype(np.uint16)
+    f = O.half_to_f32(h)
+    fin = ~np.isnan(f)
+    assert np.array_equal(f[fin], h[fin].view(np.float16).astype(np.float32))
+    assert np.array_equal(O.half_rtne(f[fin]), h[fin])
+
+
+@needs_ref
+def test_half_matches_eigen_sweep():
+    # Eigen's own Eigen::half(float) on a strided sweep of all float bit patterns
+    bits = (np.arange(0, 2 ** 32, 4099, dtype=np.uint64)).astype(np.uint32)
+    x = np.ascontiguousarray(bits.view(np.float32))
+    ref = np.empty(x.shape, np.uint16)
+    O.ref().ref_half_rtne_n(O._ptr(x), x.size, O._ptr(ref))
+    ours = O.half_rtne(x)
+    fin = ~np.isnan(x)
+    assert np.array_equal(ours[fin], ref[fin])
+
+
 # ---------------------------------------------------------------- split product value
 def test_split_kept_pairs_are_the_published_sets():
     # bf16x3 keeps hi*hi, hi*mid, mid*hi; bf16x6 adds hi*lo, lo*hi, mid*mid
