@@ -33,8 +33,20 @@ inline int64_t grid_tiles(const GemmParams<T>& p) {
   return tri_compact<BM, BN>(p) ? ntm * (ntm + 1) / 2 : ntm * ntn;
 }
 
+// LDS bytes of one tile body: two stages of (BM + BN) rows of 128 bytes.
+template <int BM, int BN>
+constexpr int tile_smem_bytes() {
+  return 2 * (BM + BN) * 128;
+}
+
+// One BM x BN tile of C at (m0, n0) over k in [kbeg, kend): the whole body of
+// a product workgroup, shared by the one-product kernel below and the
+// grouped kernel's LDS-staged units (grouped.hip).  z / partial: split-K
+// slice and whether the result goes to the partial buffer p.W.  smem holds
+// tile_smem_bytes<BM, BN>(); the caller synchronises before reusing it.
 template <typename T, int BM, int BN, int WGM, int WGN, bool TA, bool TB, int RS>
-__global__ void __launch_bounds__(64 * WGM * WGN) gemm_mfma_kernel(GemmParams<T> p) {
+__device__ __forceinline__ void gemm_tile(GemmParams<T> p, int m0, int n0, int kbeg, int kend, int z,
+                                          bool partial, unsigned char* __restrict__ smem) {
   using F = Mfma<T>;
   constexpr int NT = 64 * WGM * WGN;
   constexpr int ROWB = 128;                // bytes of k per LDS row
@@ -51,30 +63,10 @@ __global__ void __launch_bounds__(64 * WGM * WGN) gemm_mfma_kernel(GemmParams<T>
   static_assert(CPL * EPC == KPL, "lane k range must be whole chunks");
   static_assert(RS == 1 || RS == 2, "one or two register stages");
 
-  __shared__ __attribute__((aligned(16))) unsigned char smem[2 * (BM + BN) * ROWB];
-
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wid = tid >> 6;
   const int wm = wid % WGM, wn = wid / WGM;
-
-  const int ntm = (p.M + BM - 1) / BM, ntn = (p.N + BN - 1) / BN;
-  int tm, tn;
-  if (tri_compact<BM, BN>(p)) {
-    // the grid holds only the tiles that meet the triangle (lower: tn <= tm)
-    int r, c;
-    tri_tile_coords(xcd_remap(blockIdx.x, gridDim.x), ntm, 8, r, c);
-    tm = p.tri == 1 ? r : c;
-    tn = p.tri == 1 ? c : r;
-  } else {
-    tile_coords(xcd_remap(blockIdx.x, gridDim.x), ntm, ntn, 8, tm, tn);
-  }
-  const int m0 = tm * BM, n0 = tn * BN;
-  // a tile with no element in the updated triangle has nothing to do
-  if ((p.tri == 1 && m0 + BM <= n0) || (p.tri == 2 && n0 + BN <= m0)) return;
-  const int z = blockIdx.y;
-  const int kbeg = z * p.k_per_split;
-  const int kend = min(p.K, kbeg + p.k_per_split);
   const int nkt = kend > kbeg ? (kend - kbeg + BK - 1) / BK : 0;
 
   constexpr uint32_t SZ = sizeof(T);
@@ -258,7 +250,6 @@ __global__ void __launch_bounds__(64 * WGM * WGN) gemm_mfma_kernel(GemmParams<T>
   }
 
   // ---- epilogue -------------------------------------------------------------
-  const bool partial = gridDim.y > 1;
 #pragma unroll
   for (int i = 0; i < RM; ++i)
 #pragma unroll
@@ -278,6 +269,29 @@ __global__ void __launch_bounds__(64 * WGM * WGN) gemm_mfma_kernel(GemmParams<T>
         }
       }
     }
+}
+
+template <typename T, int BM, int BN, int WGM, int WGN, bool TA, bool TB, int RS>
+__global__ void __launch_bounds__(64 * WGM * WGN) gemm_mfma_kernel(GemmParams<T> p) {
+  __shared__ __attribute__((aligned(16))) unsigned char smem[tile_smem_bytes<BM, BN>()];
+  const int ntm = (p.M + BM - 1) / BM, ntn = (p.N + BN - 1) / BN;
+  int tm, tn;
+  if (tri_compact<BM, BN>(p)) {
+    // the grid holds only the tiles that meet the triangle (lower: tn <= tm)
+    int r, c;
+    tri_tile_coords(xcd_remap(blockIdx.x, gridDim.x), ntm, 8, r, c);
+    tm = p.tri == 1 ? r : c;
+    tn = p.tri == 1 ? c : r;
+  } else {
+    tile_coords(xcd_remap(blockIdx.x, gridDim.x), ntm, ntn, 8, tm, tn);
+  }
+  const int m0 = tm * BM, n0 = tn * BN;
+  // a tile with no element in the updated triangle has nothing to do
+  if ((p.tri == 1 && m0 + BM <= n0) || (p.tri == 2 && n0 + BN <= m0)) return;
+  const int z = blockIdx.y;
+  const int kbeg = z * p.k_per_split;
+  const int kend = min(p.K, kbeg + p.k_per_split);
+  gemm_tile<T, BM, BN, WGM, WGN, TA, TB, RS>(p, m0, n0, kbeg, kend, z, gridDim.y > 1, smem);
 }
 
 // Deterministic split-K reduction: C = alpha * sum_z W[z] + beta*C.

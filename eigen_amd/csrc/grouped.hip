@@ -7,24 +7,37 @@
 // in one persistent kernel that walks a work list:
 //
 //   1. eh_grouped_create builds the work list once, on the host: one
-//      descriptor per product, the first 32x32 output tile of each product
-//      (prefix sums of tile counts), and a partition of the tile sequence
-//      into one contiguous run per resident wave, cut so that every run
-//      carries the same estimated cost (k steps per tile, FP64 steps counted
-//      twice, plus a per-tile overhead).  Runs of neighbouring waves on one
-//      XCD are neighbours in the tile sequence, so the tiles of one product
-//      share that XCD's L2.
-//   2. the kernel starts exactly the waves that fit on the chip at once;
-//      each wave reads its run bounds and walks its tiles in order, advancing
-//      a product cursor (all wave-uniform, scalar loads), and computes each
-//      32x32 tile over the full depth with MFMA (v_mfma_f32_32x32x2_f32 for
-//      FP32, a 2x2 block of v_mfma_f64_16x16x4_f64 for FP64).  Operands are
-//      read with raw buffer loads (reads past the end of an operand return
-//      zero, so no load can fault), double-buffered in registers so the next
-//      16-deep k step is in flight while the current one is multiplied; the
-//      output tile is produced transposed (C^T = op(B)^T op(A)^T) so that
-//      consecutive lanes store consecutive rows of the column-major C, with
-//      alpha/beta fused into the store.
+//      descriptor per product, grouped by operand type (dtype, op(A), op(B))
+//      and deepest first within a type, the first 32x32 output tile of each
+//      product (prefix sums of tile counts), and a partition of the tile
+//      sequence into contiguous runs, one per resident workgroup.  Each
+//      type gets runs in proportion to its estimated cost (k steps per tile
+//      plus a per-tile overhead) and its runs carry equal cost, so every run
+//      holds one operand type.  Within a product the tiles are numbered in
+//      bands of kBand tile columns, row by row inside a band, so any
+//      kWavesPerGroup consecutive tiles form a block of about
+//      kWavesPerGroup / kBand rows by kBand columns of tiles.
+//   2. the kernel starts exactly the workgroups that fit on the chip at once;
+//      each picks the instantiation for its run's operand type once, and
+//      wave w takes tiles w, w + G, w + 2G, ... of the run
+//      (G = kWavesPerGroup), so the G waves of a workgroup compute one block
+//      of a product at the same time: each op(A) row panel and op(B) column
+//      panel of the block is read from HBM once and shared through the CU's
+//      and the XCD's caches (a wave walking a product alone finds its panels
+//      evicted by the time it comes back to them: this layout halved the
+//      kernel's HBM reads and cut its time by a fifth).  A barrier after
+//      every block keeps the waves in step (without it they drift apart and
+//      lose the sharing).  Each wave advances a product cursor
+//      (wave-uniform, scalar loads) and computes each 32x32 tile over the
+//      full depth with MFMA (v_mfma_f32_32x32x2_f32 for FP32, a 2x2 block of
+//      v_mfma_f64_16x16x4_f64 for FP64).  Operands are read with raw buffer
+//      loads (reads past the end of an operand return zero, so no load can
+//      fault), double-buffered in registers so the next k step is in flight
+//      while the current one is multiplied; the final step skips the MFMAs
+//      whose k values all lie past K.  The output tile is produced
+//      transposed (C^T = op(B)^T op(A)^T) so that consecutive lanes store
+//      consecutive rows of the column-major C, with alpha/beta fused into
+//      the store.
 //   3. products whose operands span 2^31 bytes or more (32-bit buffer
 //      offsets) run after the kernel through the single-product path.
 //
@@ -53,292 +66,357 @@ struct GroupedDesc {
   uint32_t lda, ldb;        // clamped to 2^31 - 1: a leading dimension is only used when it is below that
   int32_t dtype, ta, tb;
   int32_t m, n, k;
-  int32_t tiles_m;
+  int32_t tiles_m, tiles_n;
 };
 
-constexpr int kWavesPerGroup = 4;
+constexpr int kWavesPerGroup = 16;
+constexpr int kBand = 4;  // tile columns of a band (block width)
 // k depth of one step: 16 for FP32 (8 MFMAs), 8 for FP64 (8 MFMAs of a 2x2
-// block), so both dtypes keep two steps of operands in flight per wave
-// within 128 registers (four waves per SIMD).
+// block); two steps of operands per wave stay within 128 registers (four
+// waves per SIMD).
 constexpr int kStepF32 = 16, kStepF64 = 8;
+constexpr int kLaneF32 = kStepF32 / 2, kLaneF64 = kStepF64 / 4;  // consecutive k per lane
 
 // ---- slab loads -------------------------------------------------------------
-// A slab is 32 rows (of op(A), or of op(B)^T) by 16 k.  KC: the operand is
-// k-contiguous in memory (X[k + r*ld]); otherwise its rows are contiguous
+// A slab is 32 rows (of op(A), or of op(B)^T) by one k step.  KC: the operand
+// is k-contiguous in memory (X[k + r*ld]); otherwise its rows are contiguous
 // (X[r + k*ld]).  `vo` is the lane's byte offset at k0 = 0; elements with
 // r >= rows only reach outputs that are never stored, elements with k >= K
 // are zeroed by tail_* on the last step.
 
-// FP32: lane (i = lane%32, g = lane/32) holds v[s] = X(r0 + i, k0 + 8g + s).
+// FP32: lane (i = lane%32, g = lane/32) holds v[s] = X(r0 + i, k0 + 16g + s).
 template <bool KC>
 __device__ inline uint32_t lane_offset_f32(int r0, int lane, uint32_t ld) {
   const uint32_t r = (uint32_t)(r0 + (lane & 31)), g = (uint32_t)(lane >> 5);
-  return KC ? (r * ld + 8u * g) * 4u : (r + 8u * g * ld) * 4u;
+  return KC ? (r * ld + kLaneF32 * g) * 4u : (r + kLaneF32 * g * ld) * 4u;
 }
 
 template <bool KC>
-__device__ inline void load_f32(__amdgpu_buffer_rsrc_t rs, uint32_t vo, int k0, uint32_t ld, float (&v)[8]) {
+__device__ inline void load_f32(__amdgpu_buffer_rsrc_t rs, uint32_t vo, int k0, uint32_t ld, float (&v)[kLaneF32]) {
   if constexpr (KC) {
     const uint32_t so = (uint32_t)k0 * 4u;
-    const u32x4 x0 = buf_load16(rs, vo, so), x1 = buf_load16(rs, vo + 16u, so);
 #pragma unroll
-    for (int s = 0; s < 4; ++s) {
-      v[s] = __uint_as_float(x0[s]);
-      v[s + 4] = __uint_as_float(x1[s]);
+    for (int q = 0; q < kLaneF32 / 4; ++q) {
+      const u32x4 x = buf_load16(rs, vo + 16u * q, so);
+#pragma unroll
+      for (int s = 0; s < 4; ++s) v[4 * q + s] = __uint_as_float(x[s]);
     }
   } else {
+    // one load per k, the k stride added to a running scalar offset; the
+    // empty asm keeps the compiler from precomputing all kLaneF32 offsets
+    // into scalar registers ahead of the loop
+    uint32_t so = (uint32_t)k0 * ld * 4u;
 #pragma unroll
-    for (int s = 0; s < 8; ++s) v[s] = buf_load(rs, vo, (uint32_t)(k0 + s) * ld * 4u, (float*)nullptr);
+    for (int s = 0; s < kLaneF32; ++s) {
+      v[s] = buf_load(rs, vo, so, (float*)nullptr);
+      so += ld * 4u;
+      asm volatile("" : "+s"(so));
+    }
   }
 }
 
-__device__ inline void tail_f32(float (&v)[8], int k0, int K, int lane) {
-  const int kb = k0 + 8 * (lane >> 5);
+__device__ inline void tail_f32(float (&v)[kLaneF32], int k0, int K, int lane) {
+  const int kb = k0 + kLaneF32 * (lane >> 5);
 #pragma unroll
-  for (int s = 0; s < 8; ++s)
+  for (int s = 0; s < kLaneF32; ++s)
     if (kb + s >= K) v[s] = 0.0f;
 }
 
-// FP64: lane (i = lane%16, g = lane/16) holds v[x][s] = X(r0 + 16x + i, k0 + 2g + s).
+// FP64: lane (i = lane%16, g = lane/16) holds v[x][s] = X(r0 + 16x + i, k0 + 4g + s).
 template <bool KC>
 __device__ inline uint32_t lane_offset_f64(int r0, int lane, uint32_t ld) {
   const uint32_t r = (uint32_t)(r0 + (lane & 15)), g = (uint32_t)(lane >> 4);
-  return KC ? (r * ld + 2u * g) * 8u : (r + 2u * g * ld) * 8u;
+  return KC ? (r * ld + kLaneF64 * g) * 8u : (r + kLaneF64 * g * ld) * 8u;
 }
 
 template <bool KC>
-__device__ inline void load_f64(__amdgpu_buffer_rsrc_t rs, uint32_t vo, int k0, uint32_t ld, double (&v)[2][2]) {
+__device__ inline void load_f64(__amdgpu_buffer_rsrc_t rs, uint32_t vo, int k0, uint32_t ld,
+                                double (&v)[2][kLaneF64]) {
 #pragma unroll
   for (int x = 0; x < 2; ++x) {
     if constexpr (KC) {
       const uint32_t so = (uint32_t)k0 * 8u + (uint32_t)x * 16u * ld * 8u;
-      const u32x4 q = buf_load16(rs, vo, so);
-      v[x][0] = __hiloint2double((int)q[1], (int)q[0]);
-      v[x][1] = __hiloint2double((int)q[3], (int)q[2]);
+#pragma unroll
+      for (int q = 0; q < kLaneF64 / 2; ++q) {
+        const u32x4 w = buf_load16(rs, vo + 16u * q, so);
+        v[x][2 * q] = __hiloint2double((int)w[1], (int)w[0]);
+        v[x][2 * q + 1] = __hiloint2double((int)w[3], (int)w[2]);
+      }
     } else {
 #pragma unroll
-      for (int s = 0; s < 2; ++s)
+      for (int s = 0; s < kLaneF64; ++s)
         v[x][s] = buf_load(rs, vo + (uint32_t)x * 128u, (uint32_t)(k0 + s) * ld * 8u, (double*)nullptr);
     }
   }
 }
 
-__device__ inline void tail_f64(double (&v)[2][2], int k0, int K, int lane) {
-  const int kb = k0 + 2 * (lane >> 4);
+__device__ inline void tail_f64(double (&v)[2][kLaneF64], int k0, int K, int lane) {
+  const int kb = k0 + kLaneF64 * (lane >> 4);
 #pragma unroll
   for (int x = 0; x < 2; ++x)
 #pragma unroll
-    for (int s = 0; s < 2; ++s)
+    for (int s = 0; s < kLaneF64; ++s)
       if (kb + s >= K) v[x][s] = 0.0;
 }
 
-// ---- tiles ------------------------------------------------------------------
-// One 32x32 FP32 tile of C = alpha*op(A)*op(B) + beta*C, computed as C^T:
-// the MFMA's first operand is the op(B)^T slab (32 columns of C), its second
-// the op(A) slab (32 rows of C), so accumulator r of lane l holds
-// C(m0 + l%32, n0 + row(r, l)).
-template <bool TA, bool TB>
-__device__ void grouped_tile_f32(const GroupedDesc& g, int tm, int tn, int lane) {
-  const __amdgpu_buffer_rsrc_t rA = make_rsrc(g.A, g.bytesA), rB = make_rsrc(g.B, g.bytesB);
-  const uint32_t vA = lane_offset_f32<TA>(tm * 32, lane, g.lda);
-  const uint32_t vB = lane_offset_f32<!TB>(tn * 32, lane, g.ldb);
-  const int K = g.k;
-  floatx16 acc;
-#pragma unroll
-  for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
-  if (K > 0) {
-    constexpr int kStep = kStepF32;
-    float a0[8], b0[8], a1[8], b1[8];
-    auto mma = [&](float (&a)[8], float (&b)[8]) {
-#pragma unroll
-      for (int s = 0; s < 8; ++s) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(b[s], a[s], acc, 0, 0, 0);
-    };
-    auto load = [&](int k0, float (&a)[8], float (&b)[8]) {
-      load_f32<TA>(rA, vA, k0, g.lda, a);
-      load_f32<!TB>(rB, vB, k0, g.ldb, b);
-    };
-    // Full steps in pairs with unconditional prefetch (a load past K stays
-    // inside the operand's buffer range or reads zero), so the loop has no
-    // branch and the wait before each step's MFMAs covers exactly that
-    // step's loads; the partial last step is masked after the loop.
-    const int nfull = K / kStep;
-    const bool tail = K % kStep != 0;
-    load(0, a0, b0);
-    int i = 0;
-    for (; i + 2 <= nfull; i += 2) {
-      load((i + 1) * kStep, a1, b1);
-      mma(a0, b0);
-      load((i + 2) * kStep, a0, b0);
-      mma(a1, b1);
-    }
-    if (i < nfull) {  // one full step left, then the partial one
-      if (tail) load((i + 1) * kStep, a1, b1);
-      mma(a0, b0);
-      if (tail) {
-        tail_f32(a1, (i + 1) * kStep, K, lane);
-        tail_f32(b1, (i + 1) * kStep, K, lane);
-        mma(a1, b1);
-      }
-    } else if (tail) {
-      tail_f32(a0, i * kStep, K, lane);
-      tail_f32(b0, i * kStep, K, lane);
-      mma(a0, b0);
-    }
-  }
-  const int m = tm * 32 + (lane & 31);
-  if (m >= g.m) return;
-  const float alpha = (float)g.alpha, beta = (float)g.beta;
-  float* C = static_cast<float*>(g.C) + m;
-  if (beta != 0.0f) {  // read every C value of the lane before the first store
-    float c[16];
-#pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      const int n = tn * 32 + Mfma<float>::row(r, lane);
-      c[r] = n < g.n ? C[(int64_t)n * g.ldc] : 0.0f;
-    }
-#pragma unroll
-    for (int r = 0; r < 16; ++r) acc[r] = __builtin_fmaf(alpha, acc[r], beta * c[r]);
+// Tile `l` of a product in band order: bands of kBand tile columns (the last
+// band holds the remaining columns), each band walked row by row.
+__device__ inline void band_tile(const GroupedDesc& g, int l, int& tm, int& tn) {
+  const int band_tiles = g.tiles_m * kBand, full = g.tiles_n / kBand;
+  const int b = l / band_tiles;
+  if (b < full) {
+    const int r = l - b * band_tiles;
+    tm = r / kBand;
+    tn = b * kBand + r % kBand;
   } else {
-#pragma unroll
-    for (int r = 0; r < 16; ++r) acc[r] *= alpha;
-  }
-#pragma unroll
-  for (int r = 0; r < 16; ++r) {
-    const int n = tn * 32 + Mfma<float>::row(r, lane);
-    if (n < g.n) C[(int64_t)n * g.ldc] = acc[r];
+    const int w = g.tiles_n - full * kBand, r = l - full * band_tiles;
+    tm = r / w;
+    tn = full * kBand + r % w;
   }
 }
 
-// One 32x32 FP64 tile as a 2x2 block of 16x16 MFMA tiles, also as C^T:
-// acc[y][x] of lane l, entry r, holds C(m0 + 16x + l%16, n0 + 16y + l/16 + 4r).
+// ---- tiles ------------------------------------------------------------------
+// Tile<T, TA, TB> holds one 32x32 output tile of a product (its descriptor,
+// tile coordinates, buffer resources and lane offsets, all wave-uniform
+// except the offsets) and the operations on it: load k step `st` into a
+// fragment, multiply a fragment into the accumulator, and store the tile.
+// The tile is computed as C^T = op(B)^T op(A)^T: the MFMA's first operand is
+// the op(B)^T slab (32 columns of C), its second the op(A) slab (32 rows of
+// C), so consecutive lanes store consecutive rows of the column-major C.
+template <typename T, bool TA, bool TB>
+struct Tile;
+
+// FP32: v_mfma_f32_32x32x2_f32; accumulator r of lane l holds
+// C(m0 + l%32, n0 + row(r, l)).
 template <bool TA, bool TB>
-__device__ void grouped_tile_f64(const GroupedDesc& g, int tm, int tn, int lane) {
-  const __amdgpu_buffer_rsrc_t rA = make_rsrc(g.A, g.bytesA), rB = make_rsrc(g.B, g.bytesB);
-  const uint32_t vA = lane_offset_f64<TA>(tm * 32, lane, g.lda);
-  const uint32_t vB = lane_offset_f64<!TB>(tn * 32, lane, g.ldb);
-  const int K = g.k;
-  doublex4 acc[2][2];
+struct Tile<float, TA, TB> {
+  static constexpr int kStep = kStepF32;
+  struct Frag {
+    float a[kLaneF32], b[kLaneF32];
+  };
+  using Acc = floatx16;
+  GroupedDesc g;
+  int tm, tn, steps;
+  __amdgpu_buffer_rsrc_t rA, rB;
+  uint32_t vA, vB;
+
+  __device__ void init(const GroupedDesc& d, int l, int lane) {
+    g = d;
+    band_tile(g, l, tm, tn);
+    steps = (g.k + kStep - 1) / kStep;
+    rA = make_rsrc(g.A, g.bytesA);
+    rB = make_rsrc(g.B, g.bytesB);
+    vA = lane_offset_f32<TA>(tm * 32, lane, g.lda);
+    vB = lane_offset_f32<!TB>(tn * 32, lane, g.ldb);
+  }
+  __device__ void load(int st, Frag& f) const {
+    load_f32<TA>(rA, vA, st * kStep, g.lda, f.a);
+    load_f32<!TB>(rB, vB, st * kStep, g.ldb, f.b);
+  }
+  __device__ static void zero(Acc& acc) {
 #pragma unroll
-  for (int y = 0; y < 2; ++y)
+    for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
+  }
+  // The final step (possibly partial) masks the k values at or past K and
+  // skips the MFMAs whose k values all lie there: MFMA s multiplies k0 + s
+  // and k0 + kLaneF32 + s, so with r = K - k0 <= kLaneF32 only s < r remain.
+  __device__ void mma(int st, Frag& f, Acc& acc, int lane) const {
+    int live = kLaneF32;
+    if (st == steps - 1) {
+      tail_f32(f.a, st * kStep, g.k, lane);
+      tail_f32(f.b, st * kStep, g.k, lane);
+      live = min(kLaneF32, g.k - st * kStep);
+    }
 #pragma unroll
-    for (int x = 0; x < 2; ++x)
+    for (int s = 0; s < kLaneF32; ++s)
+      if (s < live) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(f.b[s], f.a[s], acc, 0, 0, 0);
+  }
+  __device__ void store(Acc& acc, int lane) const {
+    const int m = tm * 32 + (lane & 31);
+    if (m >= g.m) return;
+    const float alpha = (float)g.alpha, beta = (float)g.beta;
+    float* C = static_cast<float*>(g.C) + m;
+    if (beta != 0.0f) {  // read every C value of the lane before the first store
+      float c[16];
 #pragma unroll
-      for (int r = 0; r < 4; ++r) acc[y][x][r] = 0.0;
-  if (K > 0) {
-    constexpr int kStep = kStepF64;
-    double a0[2][2], b0[2][2], a1[2][2], b1[2][2];
-    auto mma = [&](double (&a)[2][2], double (&b)[2][2]) {
+      for (int r = 0; r < 16; ++r) {
+        const int n = tn * 32 + Mfma<float>::row(r, lane);
+        c[r] = n < g.n ? C[(int64_t)n * g.ldc] : 0.0f;
+      }
 #pragma unroll
-      for (int s = 0; s < 2; ++s)
+      for (int r = 0; r < 16; ++r) acc[r] = __builtin_fmaf(alpha, acc[r], beta * c[r]);
+    } else {
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[r] *= alpha;
+    }
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int n = tn * 32 + Mfma<float>::row(r, lane);
+      if (n < g.n) C[(int64_t)n * g.ldc] = acc[r];
+    }
+  }
+};
+
+// FP64: a 2x2 block of v_mfma_f64_16x16x4_f64 tiles; acc.v[y][x] of lane l,
+// entry r, holds C(m0 + 16x + l%16, n0 + 16y + l/16 + 4r).
+template <bool TA, bool TB>
+struct Tile<double, TA, TB> {
+  static constexpr int kStep = kStepF64;
+  struct Frag {
+    double a[2][kLaneF64], b[2][kLaneF64];
+  };
+  struct Acc {
+    doublex4 v[2][2];
+  };
+  GroupedDesc g;
+  int tm, tn, steps;
+  __amdgpu_buffer_rsrc_t rA, rB;
+  uint32_t vA, vB;
+
+  __device__ void init(const GroupedDesc& d, int l, int lane) {
+    g = d;
+    band_tile(g, l, tm, tn);
+    steps = (g.k + kStep - 1) / kStep;
+    rA = make_rsrc(g.A, g.bytesA);
+    rB = make_rsrc(g.B, g.bytesB);
+    vA = lane_offset_f64<TA>(tm * 32, lane, g.lda);
+    vB = lane_offset_f64<!TB>(tn * 32, lane, g.ldb);
+  }
+  __device__ void load(int st, Frag& f) const {
+    load_f64<TA>(rA, vA, st * kStep, g.lda, f.a);
+    load_f64<!TB>(rB, vB, st * kStep, g.ldb, f.b);
+  }
+  __device__ static void zero(Acc& acc) {
+#pragma unroll
+    for (int y = 0; y < 2; ++y)
+#pragma unroll
+      for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc.v[y][x][r] = 0.0;
+  }
+  // as for FP32: MFMA s multiplies k0 + 4g + s (g = 0..3), so with
+  // r = K - k0 <= kLaneF64 only s < r remain on the final step
+  __device__ void mma(int st, Frag& f, Acc& acc, int lane) const {
+    int live = kLaneF64;
+    if (st == steps - 1) {
+      tail_f64(f.a, st * kStep, g.k, lane);
+      tail_f64(f.b, st * kStep, g.k, lane);
+      live = min(kLaneF64, g.k - st * kStep);
+    }
+#pragma unroll
+    for (int s = 0; s < kLaneF64; ++s) {
+      if (s < live) {
 #pragma unroll
         for (int y = 0; y < 2; ++y)
 #pragma unroll
           for (int x = 0; x < 2; ++x)
-            acc[y][x] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[y][s], a[x][s], acc[y][x], 0, 0, 0);
-    };
-    auto load = [&](int k0, double (&a)[2][2], double (&b)[2][2]) {
-      load_f64<TA>(rA, vA, k0, g.lda, a);
-      load_f64<!TB>(rB, vB, k0, g.ldb, b);
-    };
-    const int nfull = K / kStep;
-    const bool tail = K % kStep != 0;
-    load(0, a0, b0);
-    int i = 0;
-    for (; i + 2 <= nfull; i += 2) {
-      load((i + 1) * kStep, a1, b1);
-      mma(a0, b0);
-      load((i + 2) * kStep, a0, b0);
-      mma(a1, b1);
-    }
-    if (i < nfull) {
-      if (tail) load((i + 1) * kStep, a1, b1);
-      mma(a0, b0);
-      if (tail) {
-        tail_f64(a1, (i + 1) * kStep, K, lane);
-        tail_f64(b1, (i + 1) * kStep, K, lane);
-        mma(a1, b1);
+            acc.v[y][x] = __builtin_amdgcn_mfma_f64_16x16x4f64(f.b[y][s], f.a[x][s], acc.v[y][x], 0, 0, 0);
       }
-    } else if (tail) {
-      tail_f64(a0, i * kStep, K, lane);
-      tail_f64(b0, i * kStep, K, lane);
-      mma(a0, b0);
     }
   }
-  double* C = static_cast<double*>(g.C);
-  if (g.beta != 0.0) {  // read every C value of the lane before the first store
-    double c[2][2][4];
+  __device__ void store(Acc& acc, int lane) const {
+    double* C = static_cast<double*>(g.C);
+    // one 16-row half of the tile at a time (x), its C values read first
 #pragma unroll
-    for (int x = 0; x < 2; ++x)
+    for (int x = 0; x < 2; ++x) {
+      const int m = tm * 32 + 16 * x + (lane & 15);
+      if (m >= g.m) continue;
+      if (g.beta != 0.0) {
+        double c[2][4];
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int n = tn * 32 + 16 * y + (lane >> 4) + 4 * r;
+            c[y][r] = n < g.n ? C[m + (int64_t)n * g.ldc] : 0.0;
+          }
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) acc.v[y][x][r] = __builtin_fma(g.alpha, acc.v[y][x][r], g.beta * c[y][r]);
+      } else {
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) acc.v[y][x][r] *= g.alpha;
+      }
 #pragma unroll
       for (int y = 0; y < 2; ++y)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int m = tm * 32 + 16 * x + (lane & 15), n = tn * 32 + 16 * y + (lane >> 4) + 4 * r;
-          c[x][y][r] = (m < g.m && n < g.n) ? C[m + (int64_t)n * g.ldc] : 0.0;
+          const int n = tn * 32 + 16 * y + (lane >> 4) + 4 * r;
+          if (n < g.n) C[m + (int64_t)n * g.ldc] = acc.v[y][x][r];
         }
-#pragma unroll
-    for (int x = 0; x < 2; ++x)
-#pragma unroll
-      for (int y = 0; y < 2; ++y)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) acc[y][x][r] = __builtin_fma(g.alpha, acc[y][x][r], g.beta * c[x][y][r]);
-  } else {
-#pragma unroll
-    for (int x = 0; x < 2; ++x)
-#pragma unroll
-      for (int y = 0; y < 2; ++y)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) acc[y][x][r] *= g.alpha;
+    }
   }
-#pragma unroll
-  for (int x = 0; x < 2; ++x) {
-    const int m = tm * 32 + 16 * x + (lane & 15);
-    if (m >= g.m) continue;
-#pragma unroll
-    for (int y = 0; y < 2; ++y)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int n = tn * 32 + 16 * y + (lane >> 4) + 4 * r;
-        if (n < g.n) C[m + (int64_t)n * g.ldc] = acc[y][x][r];
+};
+
+// The tiles of one workgroup run, all of one operand type <T, TA, TB>.  Wave
+// w computes tiles tbeg + w, tbeg + w + G, ... (G = kWavesPerGroup); per tile
+// the k steps go through two register fragments, one step in flight while
+// the other is multiplied.
+template <typename T, bool TA, bool TB>
+__device__ void run_tiles(const GroupedDesc* __restrict__ d, const int64_t* __restrict__ tile_begin, int64_t tbeg,
+                          int64_t tend, int p, int w, int lane) {
+  using TL = Tile<T, TA, TB>;
+  for (int64_t t0 = tbeg, t = tbeg + w; t0 < tend; t0 += kWavesPerGroup, t += kWavesPerGroup) {
+    if (t < tend) {
+      while (tile_begin[p + 1] <= t) ++p;
+      TL cur;
+      cur.init(d[p], (int)(t - tile_begin[p]), lane);
+      typename TL::Frag f0, f1;
+      typename TL::Acc acc;
+      TL::zero(acc);
+      const int n = cur.steps;
+      if (n > 0) cur.load(0, f0);
+      if (n > 1) cur.load(1, f1);
+      for (int i = 0; i < n; i += 2) {
+        cur.mma(i, f0, acc, lane);
+        if (i + 2 < n) cur.load(i + 2, f0);
+        if (i + 1 < n) {
+          cur.mma(i + 1, f1, acc, lane);
+          if (i + 3 < n) cur.load(i + 3, f1);
+        }
       }
+      cur.store(acc, lane);
+    }
+    __syncthreads();  // every wave runs the same number of blocks
   }
 }
 
-// Persistent kernel: virtual wave v (XCD-major numbering of the resident
-// waves) computes tiles wave_tile[v] .. wave_tile[v+1]-1 of the work list.
-__global__ void __launch_bounds__(256) grouped_gemm_kernel(const GroupedDesc* __restrict__ d,
-                                                           const int64_t* __restrict__ tile_begin,
-                                                           const int64_t* __restrict__ wave_tile,
-                                                           const int32_t* __restrict__ wave_prod) {
+// Persistent kernel: workgroup v (XCD-major numbering of the resident
+// workgroups) computes tiles run[v] .. run[v+1]-1 of the work list; the host
+// cuts runs where the operand type changes, so one instantiation of
+// run_tiles serves the whole run.
+__global__ void __launch_bounds__(64 * kWavesPerGroup) grouped_gemm_kernel(const GroupedDesc* __restrict__ d,
+                                                                           const int64_t* __restrict__ tile_begin,
+                                                                           const int64_t* __restrict__ run,
+                                                                           const int32_t* __restrict__ run_prod) {
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int v = xcd_remap(blockIdx.x, gridDim.x) * kWavesPerGroup + w;
-  const int64_t tend = wave_tile[v + 1];
-  int p = wave_prod[v];
-  for (int64_t t = wave_tile[v]; t < tend; ++t) {
-    while (tile_begin[p + 1] <= t) ++p;
-    const GroupedDesc g = d[p];
-    const int local = (int)(t - tile_begin[p]);
-    const int tm = local % g.tiles_m, tn = local / g.tiles_m;
-    if (g.dtype == EH_F32) {
-      if (!g.ta && !g.tb)
-        grouped_tile_f32<false, false>(g, tm, tn, lane);
-      else if (g.ta && !g.tb)
-        grouped_tile_f32<true, false>(g, tm, tn, lane);
-      else if (!g.ta && g.tb)
-        grouped_tile_f32<false, true>(g, tm, tn, lane);
-      else
-        grouped_tile_f32<true, true>(g, tm, tn, lane);
-    } else {
-      if (!g.ta && !g.tb)
-        grouped_tile_f64<false, false>(g, tm, tn, lane);
-      else if (g.ta && !g.tb)
-        grouped_tile_f64<true, false>(g, tm, tn, lane);
-      else if (!g.ta && g.tb)
-        grouped_tile_f64<false, true>(g, tm, tn, lane);
-      else
-        grouped_tile_f64<true, true>(g, tm, tn, lane);
-    }
+  const int v = xcd_remap(blockIdx.x, gridDim.x);
+  const int64_t tbeg = run[v], tend = run[v + 1];
+  if (tbeg >= tend) return;
+  const int p = run_prod[v];
+  const GroupedDesc& g = d[p];
+  if (g.dtype == EH_F32) {
+    if (!g.ta && !g.tb)
+      run_tiles<float, false, false>(d, tile_begin, tbeg, tend, p, w, lane);
+    else if (g.ta && !g.tb)
+      run_tiles<float, true, false>(d, tile_begin, tbeg, tend, p, w, lane);
+    else if (!g.ta && g.tb)
+      run_tiles<float, false, true>(d, tile_begin, tbeg, tend, p, w, lane);
+    else
+      run_tiles<float, true, true>(d, tile_begin, tbeg, tend, p, w, lane);
+  } else {
+    if (!g.ta && !g.tb)
+      run_tiles<double, false, false>(d, tile_begin, tbeg, tend, p, w, lane);
+    else if (g.ta && !g.tb)
+      run_tiles<double, true, false>(d, tile_begin, tbeg, tend, p, w, lane);
+    else if (!g.ta && g.tb)
+      run_tiles<double, false, true>(d, tile_begin, tbeg, tend, p, w, lane);
+    else
+      run_tiles<double, true, true>(d, tile_begin, tbeg, tend, p, w, lane);
   }
 }
 
@@ -358,11 +436,11 @@ static int64_t tile_cost(const GroupedDesc& d) {
 
 // Work list of `count` products of one shape at fixed byte strides, built on
 // the device: descriptor p is `base` with its operands advanced by p strides,
-// product p owns tiles [p*T, (p+1)*T), and with equal tile costs the W waves'
-// equal-cost runs start at tile floor(v * count * T / W).
+// product p owns tiles [p*T, (p+1)*T), and with equal tile costs the W
+// workgroups' equal-cost runs start at tile floor(v * count * T / W).
 __global__ void __launch_bounds__(256) strided_build_kernel(GroupedDesc base, int64_t sa, int64_t sb, int64_t sc,
                                                             int64_t count, int64_t T, int64_t W, GroupedDesc* d,
-                                                            int64_t* begin, int64_t* wave_tile, int32_t* wave_prod) {
+                                                            int64_t* begin, int64_t* run, int32_t* run_prod) {
   const int64_t total = count * T;
   const int64_t n = max(count + 1, W + 1);
   for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
@@ -374,10 +452,10 @@ __global__ void __launch_bounds__(256) strided_build_kernel(GroupedDesc base, in
       d[i] = g;
     }
     if (i <= count) begin[i] = i * T;
-    if (i <= W) wave_tile[i] = i == W ? total : (int64_t)((__int128)total * i / W);
+    if (i <= W) run[i] = i == W ? total : (int64_t)((__int128)total * i / W);
     if (i < W) {
       const int64_t t = (int64_t)((__int128)total * i / W);
-      wave_prod[i] = (int32_t)min(t / T, count - 1);
+      run_prod[i] = (int32_t)min(t / T, count - 1);
     }
   }
 }
@@ -499,6 +577,7 @@ extern "C" eh_status eh_gemm_batched_strided(eh_dtype dtype, eh_op opa, eh_op op
   base.n = (int32_t)n;
   base.k = reads ? (int32_t)k : 0;
   base.tiles_m = (int32_t)tm;
+  base.tiles_n = (int32_t)tn;
   const int64_t T = tm * tn, tiles = count * T;
   // resident workgroups, as for eh_grouped_create (queried once per process;
   // the initialisation of a function-local static is thread-safe)
@@ -517,8 +596,8 @@ extern "C" eh_status eh_gemm_batched_strided(eh_dtype dtype, eh_op opa, eh_op op
   int64_t groups = std::min<int64_t>(resident, (tiles + kWavesPerGroup - 1) / kWavesPerGroup);
   if (groups > 8) groups = (groups + 7) / 8 * 8;
   groups = std::max<int64_t>(groups, 1);
-  const int64_t W = groups * kWavesPerGroup;
-  // work list in the per-stream workspace: descriptors, tile starts, wave runs
+  const int64_t W = groups;
+  // work list in the per-stream workspace: descriptors, tile starts, workgroup runs
   const size_t bd = (size_t)count * sizeof(GroupedDesc), bb = (size_t)(count + 1) * sizeof(int64_t),
                bw = (size_t)(W + 1) * sizeof(int64_t), bp = (size_t)W * sizeof(int32_t);
   void* ws = nullptr;
@@ -528,17 +607,17 @@ extern "C" eh_status eh_gemm_batched_strided(eh_dtype dtype, eh_op opa, eh_op op
   char* p = static_cast<char*>(ws);
   GroupedDesc* d = reinterpret_cast<GroupedDesc*>(p);
   int64_t* begin = reinterpret_cast<int64_t*>(p + bd);
-  int64_t* wave_tile = reinterpret_cast<int64_t*>(p + bd + bb);
-  int32_t* wave_prod = reinterpret_cast<int32_t*>(p + bd + bb + bw);
+  int64_t* run = reinterpret_cast<int64_t*>(p + bd + bb);
+  int32_t* run_prod = reinterpret_cast<int32_t*>(p + bd + bb + bw);
   const int64_t nb = std::min<int64_t>((std::max(count, W) + 256) / 256, 65536);
   hipLaunchKernelGGL(strided_build_kernel, dim3((unsigned)nb), dim3(256), 0, s, base, stride_a * es, stride_b * es,
-                     stride_c * es, count, T, W, d, begin, wave_tile, wave_prod);
+                     stride_c * es, count, T, W, d, begin, run, run_prod);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return hip_fail(e, "batched build launch");
   {
     KernelTimer t(EH_KC_GROUPED, s);
-    hipLaunchKernelGGL(grouped_gemm_kernel, dim3((unsigned)groups), dim3(64 * kWavesPerGroup), 0, s, d, begin,
-                       wave_tile, wave_prod);
+    hipLaunchKernelGGL(grouped_gemm_kernel, dim3((unsigned)groups), dim3(64 * kWavesPerGroup), 0, s, d, begin, run,
+                       run_prod);
     e = hipGetLastError();
   }
   return e == hipSuccess ? EH_SUCCESS : hip_fail(e, "grouped kernel launch");
@@ -549,8 +628,8 @@ struct eh_grouped {
   std::vector<eh_gemm_problem> large;  // products past 32-bit buffer offsets
   GroupedDesc* d_desc = nullptr;
   int64_t* d_begin = nullptr;
-  int64_t* d_wave_tile = nullptr;
-  int32_t* d_wave_prod = nullptr;
+  int64_t* d_run = nullptr;
+  int32_t* d_run_prod = nullptr;
   int64_t count = 0;   // products in the kernel's work list
   int64_t tiles = 0;
   int64_t groups = 0;  // workgroups launched (kWavesPerGroup waves each)
@@ -563,7 +642,7 @@ static eh_status grouped_enqueue(eh_grouped* g, hipStream_t s) {
   if (g->count > 0) {
     KernelTimer t(EH_KC_GROUPED, s);
     hipLaunchKernelGGL(grouped_gemm_kernel, dim3((unsigned)g->groups), dim3(64 * kWavesPerGroup), 0, s, g->d_desc,
-                       g->d_begin, g->d_wave_tile, g->d_wave_prod);
+                       g->d_begin, g->d_run, g->d_run_prod);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "grouped kernel launch");
   }
@@ -625,11 +704,13 @@ eh_status eh_grouped_create(const eh_gemm_problem* problems, int64_t count, eh_g
     d.n = (int32_t)q.n;
     d.k = reads ? (int32_t)q.k : 0;
     d.tiles_m = (int32_t)tm;
+    d.tiles_n = (int32_t)tn;
     desc.push_back(d);
   }
-  // FP32 products first, then FP64; each group deepest first
-  std::stable_sort(desc.begin(), desc.end(), [](const GroupedDesc& x, const GroupedDesc& y) {
-    if (x.dtype != y.dtype) return x.dtype < y.dtype;
+  // grouped by operand type (FP32 before FP64), each type deepest first
+  auto type_of = [](const GroupedDesc& x) { return x.dtype * 4 + x.ta * 2 + x.tb; };
+  std::stable_sort(desc.begin(), desc.end(), [&](const GroupedDesc& x, const GroupedDesc& y) {
+    if (type_of(x) != type_of(y)) return type_of(x) < type_of(y);
     return x.k > y.k;
   });
   auto* g = new eh_grouped();
@@ -646,13 +727,13 @@ eh_status eh_grouped_create(const eh_gemm_problem* problems, int64_t count, eh_g
   std::vector<int64_t> cost_begin(g->count + 1);
   begin[0] = cost_begin[0] = 0;
   for (int64_t p = 0; p < g->count; ++p) {
-    const int64_t tiles = (int64_t)desc[p].tiles_m * ((desc[p].n + 31) / 32);
+    const int64_t tiles = (int64_t)desc[p].tiles_m * desc[p].tiles_n;
     begin[p + 1] = begin[p] + tiles;
     cost_begin[p + 1] = cost_begin[p] + tiles * tile_cost(desc[p]);
   }
   g->tiles = begin[g->count];
 
-  // resident waves: as many workgroups as fit on every CU at once
+  // resident workgroups: as many as fit on every CU at once
   int dev = 0, cus = 256, per_cu = 0;
   hipError_t e = hipGetDevice(&dev);
   if (e == hipSuccess) {
@@ -665,41 +746,58 @@ eh_status eh_grouped_create(const eh_gemm_problem* problems, int64_t count, eh_g
       per_cu < 1)
     per_cu = 1;
   int64_t groups = (int64_t)cus * per_cu;
-  groups = std::min<int64_t>(groups, (g->tiles + kWavesPerGroup - 1) / kWavesPerGroup);
-  if (groups > 8) groups = (groups + 7) / 8 * 8;  // whole XCD rows for the remap
-  g->groups = std::max<int64_t>(groups, 1);
-  const int64_t W = g->groups * kWavesPerGroup;
+  groups = std::max<int64_t>(1, std::min<int64_t>(groups, (g->tiles + kWavesPerGroup - 1) / kWavesPerGroup));
 
-  // partition the tile sequence into W runs of equal cost: run v starts at
-  // the first tile whose starting cost reaches v * total / W
-  std::vector<int64_t> wave_tile(W + 1);
-  std::vector<int32_t> wave_prod(W);
+  // Runs: every run holds one operand type.  Each type's products get runs
+  // in proportion to their cost (at least one), cut so that the runs of a
+  // type carry equal cost: run v starts at the first tile whose starting
+  // cost reaches the run's share.
+  std::vector<int64_t> seg{0};
+  for (int64_t p = 1; p < g->count; ++p)
+    if (type_of(desc[p]) != type_of(desc[p - 1])) seg.push_back(p);
+  seg.push_back(g->count);
   const int64_t total_cost = cost_begin[g->count];
-  int64_t p = 0;
-  for (int64_t v = 0; v < W; ++v) {
-    const int64_t target = (int64_t)((__int128)total_cost * v / W);
-    while (p < g->count - 1 && cost_begin[p + 1] <= target) ++p;
-    const int64_t c = tile_cost(desc[p]);
-    int64_t t = begin[p] + (target - cost_begin[p] + c - 1) / c;
-    t = std::min(t, begin[p + 1]);
-    if (v > 0) t = std::max(t, wave_tile[v - 1]);
-    wave_tile[v] = t;
-    // product holding tile t (the next product when t ends this one)
-    int64_t q = p;
-    while (q < g->count - 1 && begin[q + 1] <= t) ++q;
-    wave_prod[v] = (int32_t)q;
+  std::vector<int64_t> seg_runs(seg.size() - 1);
+  int64_t W = 0;
+  for (size_t i = 0; i + 1 < seg.size(); ++i) {
+    const int64_t c = cost_begin[seg[i + 1]] - cost_begin[seg[i]];
+    seg_runs[i] = std::max<int64_t>(1, (int64_t)((__int128)groups * c / total_cost));
+    W += seg_runs[i];
   }
-  wave_tile[W] = g->tiles;
+  // launched workgroups: whole XCD rows for the remap; the extra runs are empty
+  const int64_t WL = W > 8 ? (W + 7) / 8 * 8 : W;
+  g->groups = WL;
+  std::vector<int64_t> run(WL + 1, g->tiles);
+  std::vector<int32_t> run_prod(WL, (int32_t)(g->count - 1));
+  int64_t v = 0;
+  for (size_t i = 0; i + 1 < seg.size(); ++i) {
+    const int64_t ps = seg[i], pe = seg[i + 1], cb = cost_begin[ps], ce = cost_begin[pe], R = seg_runs[i];
+    int64_t p = ps;
+    for (int64_t r = 0; r < R; ++r, ++v) {
+      const int64_t target = cb + (int64_t)((__int128)(ce - cb) * r / R);
+      while (p < pe - 1 && cost_begin[p + 1] <= target) ++p;
+      const int64_t c = tile_cost(desc[p]);
+      int64_t t = begin[p] + (target - cost_begin[p] + c - 1) / c;
+      t = std::min(t, begin[p + 1]);
+      if (r > 0) t = std::max(t, run[v - 1]);
+      run[v] = t;
+      // product holding tile t (the next product of the type when t ends
+      // this one; a run starting at the type's end is empty)
+      int64_t q = p;
+      while (q < pe - 1 && begin[q + 1] <= t) ++q;
+      run_prod[v] = (int32_t)q;
+    }
+  }
 
   e = hipMalloc(&g->d_desc, sizeof(GroupedDesc) * g->count);
   if (e == hipSuccess) e = hipMalloc(&g->d_begin, sizeof(int64_t) * (g->count + 1));
-  if (e == hipSuccess) e = hipMalloc(&g->d_wave_tile, sizeof(int64_t) * (W + 1));
-  if (e == hipSuccess) e = hipMalloc(&g->d_wave_prod, sizeof(int32_t) * W);
+  if (e == hipSuccess) e = hipMalloc(&g->d_run, sizeof(int64_t) * (WL + 1));
+  if (e == hipSuccess) e = hipMalloc(&g->d_run_prod, sizeof(int32_t) * WL);
   if (e == hipSuccess) e = hipMemcpy(g->d_desc, desc.data(), sizeof(GroupedDesc) * g->count, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(g->d_begin, begin.data(), sizeof(int64_t) * (g->count + 1), hipMemcpyHostToDevice);
   if (e == hipSuccess)
-    e = hipMemcpy(g->d_wave_tile, wave_tile.data(), sizeof(int64_t) * (W + 1), hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(g->d_wave_prod, wave_prod.data(), sizeof(int32_t) * W, hipMemcpyHostToDevice);
+    e = hipMemcpy(g->d_run, run.data(), sizeof(int64_t) * (WL + 1), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(g->d_run_prod, run_prod.data(), sizeof(int32_t) * WL, hipMemcpyHostToDevice);
   if (e != hipSuccess) {
     eh_grouped_destroy(g);
     return fail(EH_ERR_ALLOC, std::string("grouped setup: ") + hipGetErrorString(e));
@@ -779,8 +877,8 @@ eh_status eh_grouped_destroy(eh_grouped* g) {
   if (g->graph) (void)hipGraphDestroy(g->graph);
   if (g->d_desc) (void)hipFree(g->d_desc);
   if (g->d_begin) (void)hipFree(g->d_begin);
-  if (g->d_wave_tile) (void)hipFree(g->d_wave_tile);
-  if (g->d_wave_prod) (void)hipFree(g->d_wave_prod);
+  if (g->d_run) (void)hipFree(g->d_run);
+  if (g->d_run_prod) (void)hipFree(g->d_run_prod);
   delete g;
   return EH_SUCCESS;
 }

@@ -90,6 +90,43 @@ def test_grouped_alpha_beta_and_transposes(gpu):
     g.close()
 
 
+def test_grouped_all_operand_types_many_runs_exact(gpu):
+    """1,600 products over all eight operand types (dtype x op(A) x op(B)),
+    interleaved in the input order and with K = 0 and alpha = 0 products among
+    them, so that every type gets several workgroup runs; integer inputs make
+    every result exact.  Checks every 5th product."""
+    import torch
+    rng = np.random.default_rng(21)
+    probs, host, bufs = [], [], []
+    for p in range(1600):
+        dt = E.F32 if p % 2 else E.F64
+        npdt = np.float32 if dt == E.F32 else np.float64
+        m, n = (int(x) for x in rng.integers(1, 100, 2))
+        k = 0 if p % 97 == 0 else int(rng.integers(1, 90))
+        ta, tb = bool(p & 2), bool(p & 4)
+        alpha, beta = (0.0, 2.0) if p % 89 == 0 else ((1.0, 0.0) if p % 3 else (-1.0, 0.5))
+        A = O.fill((max(k, 1), m) if ta else (m, max(k, 1)), npdt, 1000 + p, O.INT, -8, 8)
+        B = O.fill((n, max(k, 1)) if tb else (max(k, 1), n), npdt, 5000 + p, O.INT, -8, 8)
+        C0 = O.fill((m, n), npdt, 9000 + p, O.INT, -8, 8)
+        dA, dB, dC = (torch.from_numpy(x.reshape(-1, order="F").copy()).cuda() for x in (A, B, C0))
+        bufs += [dA, dB, dC]
+        probs.append(E.GemmProblem(dtype=dt, opa=int(ta), opb=int(tb), m=m, n=n, k=k, lda=A.shape[0],
+                                   ldb=B.shape[0], ldc=m, A=dA.data_ptr(), B=dB.data_ptr(), C=dC.data_ptr(),
+                                   alpha=alpha, beta=beta))
+        if p % 5 == 0:
+            Ak = A[:, :k] if not ta else A[:k, :]
+            Bk = B[:k, :] if not tb else B[:, :k]
+            host.append((np.asfortranarray(Ak), np.asfortranarray(Bk), C0, ta, tb, alpha, beta, dC, (m, n)))
+    g = E.Grouped(probs)
+    g.graph_launch()
+    torch.cuda.synchronize()
+    for A, B, C0, ta, tb, alpha, beta, dC, (m, n) in host:
+        want = O.gemm_ref(ta, tb, alpha, A, B, beta, C0)
+        got = dC.cpu().numpy().reshape((n, m)).T
+        assert np.array_equal(got.astype(np.float64), want)
+    g.close()
+
+
 def test_grouped_random_within_bound(gpu):
     import torch
     b = uneven_batch(count=400, seed=11)  # the batched workload's distribution, smaller count
