@@ -84,7 +84,7 @@ constexpr int kLaneF32 = kStepF32 / 2, kLaneF64 = kStepF64 / 4;  // consecutive 
 // r >= rows only reach outputs that are never stored, elements with k >= K
 // are zeroed by tail_* on the last step.
 
-// FP32: lane (i = lane%32, g = lane/32) holds v[s] = X(r0 + i, k0 + 16g + s).
+// FP32: lane (i = lane%32, g = lane/32) holds v[s] = X(r0 + i, k0 + kLaneF32*g + s).
 template <bool KC>
 __device__ inline uint32_t lane_offset_f32(int r0, int lane, uint32_t ld) {
   const uint32_t r = (uint32_t)(r0 + (lane & 31)), g = (uint32_t)(lane >> 5);
@@ -122,7 +122,7 @@ __device__ inline void tail_f32(float (&v)[kLaneF32], int k0, int K, int lane) {
     if (kb + s >= K) v[s] = 0.0f;
 }
 
-// FP64: lane (i = lane%16, g = lane/16) holds v[x][s] = X(r0 + 16x + i, k0 + 4g + s).
+// FP64: lane (i = lane%16, g = lane/16) holds v[x][s] = X(r0 + 16x + i, k0 + kLaneF64*g + s).
 template <bool KC>
 __device__ inline uint32_t lane_offset_f64(int r0, int lane, uint32_t ld) {
   const uint32_t r = (uint32_t)(r0 + (lane & 15)), g = (uint32_t)(lane >> 4);
@@ -221,12 +221,14 @@ struct Tile<float, TA, TB> {
   // skips the MFMAs whose k values all lie there: MFMA s multiplies k0 + s
   // and k0 + kLaneF32 + s, so with r = K - k0 <= kLaneF32 only s < r remain.
   __device__ void mma(int st, Frag& f, Acc& acc, int lane) const {
-    int live = kLaneF32;
-    if (st == steps - 1) {
-      tail_f32(f.a, st * kStep, g.k, lane);
-      tail_f32(f.b, st * kStep, g.k, lane);
-      live = min(kLaneF32, g.k - st * kStep);
+    if (st < steps - 1) {
+#pragma unroll
+      for (int s = 0; s < kLaneF32; ++s) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(f.b[s], f.a[s], acc, 0, 0, 0);
+      return;
     }
+    tail_f32(f.a, st * kStep, g.k, lane);
+    tail_f32(f.b, st * kStep, g.k, lane);
+    const int live = min(kLaneF32, g.k - st * kStep);
 #pragma unroll
     for (int s = 0; s < kLaneF32; ++s)
       if (s < live) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(f.b[s], f.a[s], acc, 0, 0, 0);
@@ -294,25 +296,27 @@ struct Tile<double, TA, TB> {
 #pragma unroll
         for (int r = 0; r < 4; ++r) acc.v[y][x][r] = 0.0;
   }
-  // as for FP32: MFMA s multiplies k0 + 4g + s (g = 0..3), so with
+  // as for FP32: MFMA s multiplies k0 + kLaneF64*g + s (g = 0..3), so with
   // r = K - k0 <= kLaneF64 only s < r remain on the final step
   __device__ void mma(int st, Frag& f, Acc& acc, int lane) const {
-    int live = kLaneF64;
-    if (st == steps - 1) {
-      tail_f64(f.a, st * kStep, g.k, lane);
-      tail_f64(f.b, st * kStep, g.k, lane);
-      live = min(kLaneF64, g.k - st * kStep);
+    auto k_slice = [&](int s) {
+#pragma unroll
+      for (int y = 0; y < 2; ++y)
+#pragma unroll
+        for (int x = 0; x < 2; ++x)
+          acc.v[y][x] = __builtin_amdgcn_mfma_f64_16x16x4f64(f.b[y][s], f.a[x][s], acc.v[y][x], 0, 0, 0);
+    };
+    if (st < steps - 1) {
+#pragma unroll
+      for (int s = 0; s < kLaneF64; ++s) k_slice(s);
+      return;
     }
+    tail_f64(f.a, st * kStep, g.k, lane);
+    tail_f64(f.b, st * kStep, g.k, lane);
+    const int live = min(kLaneF64, g.k - st * kStep);
 #pragma unroll
-    for (int s = 0; s < kLaneF64; ++s) {
-      if (s < live) {
-#pragma unroll
-        for (int y = 0; y < 2; ++y)
-#pragma unroll
-          for (int x = 0; x < 2; ++x)
-            acc.v[y][x] = __builtin_amdgcn_mfma_f64_16x16x4f64(f.b[y][s], f.a[x][s], acc.v[y][x], 0, 0, 0);
-      }
-    }
+    for (int s = 0; s < kLaneF64; ++s)
+      if (s < live) k_slice(s);
   }
   __device__ void store(Acc& acc, int lane) const {
     double* C = static_cast<double*>(g.C);
