@@ -1,0 +1,57 @@
+"""Compile-time guards of the Eigen drop-in (include/eigen_amd_eigen.h).
+
+Expressions that Eigen would evaluate on the host over device memory must not
+compile: assigning a non-product expression to an eigen_amd::DeviceMap, and a
+conjugated device operand that has no BLAS op (conj(A) without a transpose).
+Each snippet is compiled with g++ -fsyntax-only against the Eigen headers of
+the reference tree; the failing ones must fail with the header's own message,
+and a control snippet with the supported forms must compile.  Skipped when
+the Eigen headers are absent."""
+import os
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EIGEN = "/root/reference"
+
+HEAD = """
+#include <complex>
+#include "eigen_amd_eigen.h"
+using M = Eigen::MatrixXf;
+using CM = Eigen::MatrixXcf;
+void f(float* a, float* b, float* c, std::complex<float>* x, std::complex<float>* y, std::complex<float>* z) {
+  eigen_amd::DeviceMap<M> A(a, 4, 4), B(b, 4, 4), C(c, 4, 4);
+  eigen_amd::DeviceMap<CM> X(x, 4, 4), Y(y, 4, 4), Z(z, 4, 4);
+  (void)A; (void)B; (void)C; (void)X; (void)Y; (void)Z;
+"""
+
+
+def _compile(body: str):
+    src = HEAD + body + "\n}\n"
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-I" + EIGEN, "-I" + os.path.join(ROOT, "include"),
+                        "-x", "c++", "-"], input=src, capture_output=True, text=True, timeout=300)
+    return r.returncode, r.stderr
+
+
+pytestmark = pytest.mark.skipif(not os.path.isdir(os.path.join(EIGEN, "Eigen")), reason="Eigen headers absent")
+
+
+def test_supported_forms_compile():
+    rc, err = _compile("""
+  C = A * B; C.noalias() = A * B; C += 2.0f * A.transpose() * B; C -= A * B.transpose();
+  Z = X.adjoint() * Y; Z.noalias() -= std::complex<float>(1, 2) * X * Y.adjoint();
+""")
+    assert rc == 0, err[-3000:]
+
+
+@pytest.mark.parametrize("body,message", [
+    ("C = A + B;", "a DeviceMap can only be assigned a product"),
+    ("C += A.cwiseProduct(B);", "a DeviceMap can only be updated by a product"),
+    ("Z = X.conjugate() * Y;", "a conjugated device operand must enter the product transposed"),
+    ("C = A;", "deleted function"),
+])
+def test_unsafe_forms_rejected(body, message):
+    rc, err = _compile(body)
+    assert rc != 0
+    assert message in err, err[-3000:]

@@ -2,6 +2,10 @@
 
 * test_device_glue    -- C.device(dev).noalias() {=,+=,-=} expr on device Maps
   through Eigen's AssignmentWithDevice seam (include/eigen_amd_eigen.h);
+* test_device_map     -- C = A*B, C.noalias() {=,+=,-=} expr on
+  eigen_amd::DeviceMap with no .device(): matrix, vector, row-vector, tiny
+  (2x3x1, 1x1) and row-major shapes, real and complex scalars (adjoint
+  operands, complex alpha), and complex products through the .device() seam;
 * test_blas_redirect  -- unchanged C.noalias() += alpha*A*B with
   EIGEN_USE_BLAS routed to libeigen_amd_blas (host and device Maps);
 * test_blas_level3_redirect -- unchanged matrix-vector, triangular solve,
@@ -21,7 +25,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 BUILD = os.path.join(HERE, "eigen_glue", "_build")
 
 
-@pytest.mark.parametrize("prog", ["test_device_glue", "test_blas_redirect", "test_blas_level3_redirect"])
+@pytest.mark.parametrize("prog", ["test_device_glue", "test_device_map", "test_blas_redirect",
+                                  "test_blas_level3_redirect"])
 def test_eigen_program(gpu, prog):
     path = os.path.join(BUILD, prog)
     if not os.path.exists(path):
