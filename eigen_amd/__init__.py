@@ -189,6 +189,11 @@ def _declare_extra(L) -> None:
         "eh_tuner_save": (i32, [C.c_char_p]),
         "eh_tuner_count": (i64, []),
         "eh_tuner_clear": (i32, []),
+        "eh_num_packed_configs": (i32, [i32]),
+        "eh_packed_config_info": (i32, [i32, i32, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)]),
+        "eh_plan_packed": (i32, [i32, i32, i32, i64, i64, i64, C.POINTER(i32)]),
+        "eh_time_config_packed": (i32, [i32, i32, i32, i64, i64, i64, vp, vp, vp, i32, i32, C.POINTER(f32)]),
+        "eh_tuner_record_packed": (i32, [i32, i32, i32, i64, i64, i64, i32]),
         "eh_mlp_create": (i32, [i32, i32, i32, i64, C.POINTER(vp)]),
         "eh_mlp_destroy": (i32, [vp]),
         "eh_mlp_set_params": (i32, [vp, vp]),
@@ -435,6 +440,53 @@ def plan(dtype: int, opa: int, opb: int, m: int, n: int, k: int) -> int:
     c = C.c_int()
     check(lib().eh_plan(dtype, opa, opb, m, n, k, C.byref(c)), "eh_plan")
     return c.value
+
+
+PACKED_SPLIT2, PACKED_SPLIT3, PACKED_BF16, PACKED_F16 = 2, 3, 4, 5
+
+
+def num_packed_configs(kind: int) -> int:
+    """Configurations of a packed product kind (split-BF16 2/3 terms, bf16 / half inputs)."""
+    return lib().eh_num_packed_configs(kind)
+
+
+def plan_packed(kind: int, opa: int, opb: int, m: int, n: int, k: int) -> int:
+    c = C.c_int()
+    check(lib().eh_plan_packed(kind, opa, opb, m, n, k, C.byref(c)), "eh_plan_packed")
+    return c.value
+
+
+def time_config(dtype: int, opa: int, opb: int, m: int, n: int, k: int, A, B, Cm, config: int, reps: int = 3) -> float:
+    """Median ms of `reps` runs of a plain configuration on the caller's buffers (eh_time_config)."""
+    ms = C.c_float()
+    check(lib().eh_time_config(dtype, opa, opb, m, n, k, _ptr(A), _ptr(B), _ptr(Cm), config, reps, C.byref(ms)),
+          "eh_time_config")
+    return ms.value
+
+
+def time_config_packed(kind: int, opa: int, opb: int, m: int, n: int, k: int, A, B, Cm, config: int,
+                       reps: int = 3) -> float:
+    """Median ms of `reps` runs of a packed configuration (eh_time_config_packed)."""
+    ms = C.c_float()
+    check(lib().eh_time_config_packed(kind, opa, opb, m, n, k, _ptr(A), _ptr(B), _ptr(Cm), config, reps,
+                                      C.byref(ms)), "eh_time_config_packed")
+    return ms.value
+
+
+def tuner_count() -> int:
+    return lib().eh_tuner_count()
+
+
+def tuner_clear() -> None:
+    check(lib().eh_tuner_clear(), "eh_tuner_clear")
+
+
+def tuner_save(path: str) -> None:
+    check(lib().eh_tuner_save(path.encode()), "eh_tuner_save")
+
+
+def tuner_load(path: str) -> None:
+    check(lib().eh_tuner_load(path.encode()), "eh_tuner_load")
 
 
 def set_tune_mode(mode: int) -> None:

@@ -83,6 +83,47 @@ int heuristic_config(int dtype, int64_t m, int64_t n, int64_t k) {
   return tile * kNumSplits + split_index(s);
 }
 
+// ---- packed products: configurations ----------------------------------------
+// The split-BF16 FP32 products and the bf16 / half input products run the
+// packed kernel (launch_gemm_split_packed), whose configurations are
+// tile * kNumSplits + split index like the plain ones, over its own tiles:
+// split terms 0 = 128x128, 1 = 256x128 (4 waves), 2 = 256x128 (8 waves);
+// one term 0 = 128x128, 1 = 256x256.
+static bool packed_one_term(int kind) { return kind == EH_PACKED_BF16 || kind == EH_PACKED_F16; }
+
+int packed_num_configs(int kind) { return (packed_one_term(kind) ? 2 : 3) * kNumSplits; }
+
+bool packed_kind_valid(int kind) {
+  return kind == EH_PACKED_SPLIT2 || kind == EH_PACKED_SPLIT3 || kind == EH_PACKED_BF16 || kind == EH_PACKED_F16;
+}
+
+// The fixed rule: the large tile (256x256 for one term, 256x128 with 8
+// waves for the split terms) once it gives one wave of workgroups over the
+// 256 CUs, 128x128 below; then split K while the grid stays under one wave
+// and each split keeps at least 512 of depth.
+int packed_rule_config(int kind, int64_t m, int64_t n, int64_t k) {
+  const bool one = packed_one_term(kind);
+  const int64_t t_large = ((m + 255) / 256) * ((n + (one ? 255 : 127)) / (one ? 256 : 128));
+  const int tile = t_large >= 256 ? (one ? 1 : 2) : 0;
+  const int64_t tiles = tile ? t_large : ((m + 127) / 128) * ((n + 127) / 128);
+  int s = 1;
+  while (s < 8 && tiles * s < 256 && k / (2 * s) >= 512) s *= 2;
+  return tile * kNumSplits + split_index(s);
+}
+
+int table_plan_packed(int kind, int opa, int opb, int64_t m, int64_t n, int64_t k);
+
+// Configuration of a packed product under the current tuning mode: the
+// timing table (tuned on first use) in EH_TUNE_TABLE, the rule otherwise
+// (the learned selector is trained on the plain kernels only).
+int plan_packed(int kind, int opa, int opb, int64_t m, int64_t n, int64_t k) {
+  if (tune_mode() == EH_TUNE_TABLE) {
+    const int c = table_plan_packed(kind, opa, opb, m, n, k);
+    if (c >= 0 && c < packed_num_configs(kind)) return c;
+  }
+  return packed_rule_config(kind, m, n, k);
+}
+
 bool mlp_has_default();
 
 static std::mutex g_mutex;
@@ -258,19 +299,11 @@ static eh_status launch_split_product(const GemmParams<float>& g, bool ta, bool 
                                       hipStream_t stream) {
   const int m = g.M, n = g.N, k = g.K;
   const int KS = split_kslices(k);
-  // 256x128 tiles with 8 waves (two per SIMD) once the grid fills the chip,
-  // 128x128 below that
-  int64_t t256 = ((int64_t)(m + 255) / 256) * ((n + 127) / 128);
-  int tile = t256 >= 256 ? 2 : 0;
-  int64_t tiles = tile ? t256 : ((int64_t)(m + 127) / 128) * ((n + 127) / 128);
-  int splits = 1;
-  if (config >= 0) {
-    int ct = 0;
-    config_decode(config, &ct, &splits);
-    tile = ct >= 5 ? 1 : (ct == 4 ? 2 : 0);
-  } else {
-    while (splits < 8 && tiles * splits < 256 && k / (2 * splits) >= 512) splits *= 2;
-  }
+  const int kind = terms == 3 ? EH_PACKED_SPLIT3 : EH_PACKED_SPLIT2;
+  if (config < 0) config = plan_packed(kind, ta, tb, m, n, k);
+  if (config >= packed_num_configs(kind)) return fail(EH_ERR_INVALID, "packed config out of range");
+  int tile = 0, splits = 1;
+  config_decode(config, &tile, &splits);
   const int per = (KS + splits - 1) / splits;
   splits = (KS + per - 1) / per;
   const size_t bytesA = split_packed_bytes(m, k, terms), bytesB = split_packed_bytes(n, k, terms);
@@ -325,18 +358,17 @@ static eh_status launch_split_product(const GemmParams<float>& g, bool ta, bool 
 // (32-bit buffer offsets).
 static eh_status launch_bf16_product(bool ta, bool tb, int m, int n, int k, float alpha, const unsigned short* A,
                                      int64_t lda, const unsigned short* B, int64_t ldb, float beta, float* C,
-                                     int64_t ldc, hipStream_t stream, bool f16 = false) {
+                                     int64_t ldc, hipStream_t stream, bool f16 = false, int config = -1) {
   const uint64_t extA = ta ? (uint64_t)(m - 1) * lda + k : (uint64_t)(k - 1) * lda + m;
   const uint64_t extB = !tb ? (uint64_t)(n - 1) * ldb + k : (uint64_t)(k - 1) * ldb + n;
   if ((extA + 64) * 2 >= kMaxExtent || (extB + 64) * 2 >= kMaxExtent)
     return fail(EH_ERR_UNSUPPORTED, "16-bit operands must span less than 4 GB");
   const int KS = split_kslices(k);
-  // 256x256 tiles (tile 1) once they fill the chip, 128x128 (tile 0) below
-  const int64_t t256 = ((int64_t)(m + 255) / 256) * ((n + 255) / 256);
-  const int tile = t256 >= 256 ? 1 : 0;
-  const int64_t tiles = tile ? t256 : ((int64_t)(m + 127) / 128) * ((n + 127) / 128);
-  int splits = 1;
-  while (splits < 8 && tiles * splits < 256 && k / (2 * splits) >= 512) splits *= 2;
+  const int kind = f16 ? EH_PACKED_F16 : EH_PACKED_BF16;
+  if (config < 0) config = plan_packed(kind, ta, tb, m, n, k);
+  if (config >= packed_num_configs(kind)) return fail(EH_ERR_INVALID, "packed config out of range");
+  int tile = 0, splits = 1;
+  config_decode(config, &tile, &splits);
   const int per = (KS + splits - 1) / splits;
   splits = (KS + per - 1) / per;
   const size_t bytesA = split_packed_bytes(m, k, 1), bytesB = split_packed_bytes(n, k, 1);
@@ -551,6 +583,8 @@ static eh_status run_gemm(int opa, int opb, int64_t m, int64_t n, int64_t k, T a
   } else if (prec != EH_PREC_AUTO && prec != EH_PREC_PLAIN) {
     return fail(EH_ERR_UNSUPPORTED, "FP64 products run on FP64 MFMA only (precision AUTO or PLAIN)");
   }
+  if (split_prec && config >= packed_num_configs(terms == 3 ? EH_PACKED_SPLIT3 : EH_PACKED_SPLIT2))
+    return fail(EH_ERR_INVALID, "config out of range for the split-BF16 product");
   if (!split_prec && config < 0 && (n == 1 || m == 1)) {
     // HBM-bound matrix-vector product (a split precision keeps its own
     // arithmetic and runs the split product)
@@ -626,6 +660,25 @@ static eh_status gemv_run(bool trans, int64_t rows, int64_t k, T alpha, const T*
   hipError_t e = launch_gemv<T>(g, trans, (int)rows, (int)k, alpha, A, lda, (uint32_t)(extent - 64 * sizeof(T)), x, incx,
                                 beta, y, incy, ws, s);
   return e == hipSuccess ? EH_SUCCESS : hip_fail(e, "gemv launch");
+}
+
+// C = op(A)*op(B) (alpha 1, beta 0, leading dimensions the operands' rows)
+// as the packed product of `kind` with configuration `config`: the tuner's
+// timed call.  A and B are FP32 for the split kinds, 16-bit otherwise.
+eh_status packed_product(int kind, int opa, int opb, int64_t m, int64_t n, int64_t k, const void* A, const void* B,
+                         void* C, int config, hipStream_t s) {
+  const int64_t lda = opa == EH_OP_N ? std::max<int64_t>(m, 1) : std::max<int64_t>(k, 1);
+  const int64_t ldb = opb == EH_OP_N ? std::max<int64_t>(k, 1) : std::max<int64_t>(n, 1);
+  const float one = 1.0f, zero = 0.0f;
+  if (!packed_one_term(kind))
+    return gemm_dispatch(EH_F32, opa, opb, m, n, k, &one, A, lda, B, ldb, &zero, C, std::max<int64_t>(m, 1),
+                         kind == EH_PACKED_SPLIT3 ? EH_PREC_SPLIT3 : EH_PREC_SPLIT2, config, s);
+  eh_status st = check_gemm_args(EH_F32, opa, opb, m, n, k, &one, A, lda, B, ldb, &zero, C, std::max<int64_t>(m, 1));
+  if (st != EH_SUCCESS) return st;
+  if (m == 0 || n == 0 || k == 0) return EH_SUCCESS;
+  return launch_bf16_product(opa != EH_OP_N, opb != EH_OP_N, (int)m, (int)n, (int)k, 1.0f,
+                             static_cast<const unsigned short*>(A), lda, static_cast<const unsigned short*>(B), ldb,
+                             0.0f, static_cast<float*>(C), std::max<int64_t>(m, 1), s, kind == EH_PACKED_F16, config);
 }
 
 eh_status gemm_dispatch(int dtype, int opa, int opb, int64_t m, int64_t n, int64_t k, const void* alpha,

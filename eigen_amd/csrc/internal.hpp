@@ -28,6 +28,16 @@ eh_tune_mode tune_mode();
 // Planner (planner.cpp): configuration for a shape under the current mode.
 eh_status plan_config(int dtype, int opa, int opb, int64_t m, int64_t n, int64_t k, int* config);
 
+// Packed products (capi.cpp): the split-BF16 FP32 products and the bf16 /
+// half input products, kind = eh_packed_kind.  Their configurations are
+// tile * kNumSplits + split index over the packed kernel's tiles.
+bool packed_kind_valid(int kind);
+int packed_num_configs(int kind);
+int packed_rule_config(int kind, int64_t m, int64_t n, int64_t k);
+int plan_packed(int kind, int opa, int opb, int64_t m, int64_t n, int64_t k);
+eh_status packed_product(int kind, int opa, int opb, int64_t m, int64_t n, int64_t k, const void* A, const void* B,
+                         void* C, int config, hipStream_t s);
+
 // Per-stream device workspace (capi.cpp).  A lease holds the stream's
 // workspace lock and a reference to its buffer: hold it until every launch
 // of the call that uses the buffer is queued, so that another thread's call

@@ -129,8 +129,10 @@ eh_status eh_gemm_noalloc(eh_dtype dtype, eh_op opa, eh_op opb, int64_t m, int64
                           const void* A, int64_t lda, const void* B, int64_t ldb, const void* beta, void* C,
                           int64_t ldc, eh_precision prec, void* stream);
 
-/* Same, with an explicit configuration index in [0, eh_num_configs()) for
- * the plain kernels (config < 0 = let the planner choose). */
+/* Same, with an explicit configuration index (config < 0 = let the planner
+ * choose): in [0, eh_num_configs()) for the plain kernels, in
+ * [0, eh_num_packed_configs(EH_PACKED_SPLIT2 / SPLIT3)) when prec selects
+ * the split-BF16 product. */
 eh_status eh_gemm_config(eh_dtype dtype, eh_op opa, eh_op opb, int64_t m, int64_t n, int64_t k, const void* alpha,
                          const void* A, int64_t lda, const void* B, int64_t ldb, const void* beta, void* C,
                          int64_t ldc, eh_precision prec, int config, void* stream);
@@ -435,9 +437,40 @@ eh_status eh_time_config(int dtype, int opa, int opb, int64_t m, int64_t n, int6
 eh_status eh_tuner_record(int dtype, int opa, int opb, int64_t m, int64_t n, int64_t k, int config);
 int64_t eh_tuner_count(void);
 eh_status eh_tuner_clear(void);
-/* Text file, one line per shape: "dtype opa opb m n k config". */
+/* Text file: a header "# eigen_amd tuning table v2 arch=<gcnArchName>", then
+ * one line per shape "kind dtype opa opb m n k config" (kind 0 = plain
+ * kernels, else an eh_packed_kind).  Loading refuses a table made for
+ * another architecture; v1 files (lines "dtype opa opb m n k config", plain
+ * kernels only) still load. */
 eh_status eh_tuner_save(const char* path);
 eh_status eh_tuner_load(const char* path);
+
+/* Packed products -- the split-BF16 FP32 products (eh_gemm with
+ * EH_PREC_SPLIT2 / SPLIT3) and the 16-bit input products (eh_gemm_bf16,
+ * eh_gemm_f16) -- have their own configurations: tile * 4 + split-K index
+ * (splits {1, 2, 4, 8}); tiles for the split kinds 0 = 128x128, 1 = 256x128
+ * (4 waves), 2 = 256x128 (8 waves); for the 16-bit kinds 0 = 128x128,
+ * 1 = 256x256.  EH_TUNE_TABLE times every configuration of a new shape on
+ * first use (scratch operands) and remembers the fastest; the other modes
+ * use the fixed rule (large tile once it fills the chip, split K below one
+ * wave of workgroups).  No counterpart in the reference: Eigen's blocking
+ * choice (GeneralBlockPanelKernel.h:201-452) has no per-kernel tuning. */
+typedef enum eh_packed_kind {
+  EH_PACKED_SPLIT2 = 2,
+  EH_PACKED_SPLIT3 = 3,
+  EH_PACKED_BF16 = 4,
+  EH_PACKED_F16 = 5
+} eh_packed_kind;
+int eh_num_packed_configs(eh_packed_kind kind); /* 0 for an unknown kind */
+eh_status eh_packed_config_info(eh_packed_kind kind, int config, int* tile_m, int* tile_n, int* split_k);
+/* Configuration the planner would use (may tune, synchronously, in TABLE mode). */
+eh_status eh_plan_packed(eh_packed_kind kind, eh_op opa, eh_op opb, int64_t m, int64_t n, int64_t k, int* config);
+/* As eh_time_config for a packed product: A and B are FP32 for the split
+ * kinds, 16-bit (bf16 / IEEE half bits) for EH_PACKED_BF16 / F16; C FP32. */
+eh_status eh_time_config_packed(eh_packed_kind kind, eh_op opa, eh_op opb, int64_t m, int64_t n, int64_t k,
+                                const void* A, const void* B, void* C, int config, int reps, float* ms);
+eh_status eh_tuner_record_packed(eh_packed_kind kind, eh_op opa, eh_op opb, int64_t m, int64_t n, int64_t k,
+                                 int config);
 
 /* ---- learned selector ------------------------------------------------------ */
 

@@ -67,6 +67,43 @@ def test_heuristic_plan_is_valid():
         E.set_tune_mode(prev)
 
 
+def test_packed_configs_and_rule():
+    """Packed products (split-BF16, bf16 / half inputs) have their own
+    configuration spaces; outside the table mode the planner uses the fixed
+    rule: the large tile once it fills the chip (256 workgroups), else
+    128x128, then split K while the grid stays under one wave and each split
+    keeps 512 of depth."""
+    assert E.num_packed_configs(E.PACKED_SPLIT2) == E.num_packed_configs(E.PACKED_SPLIT3) == 12
+    assert E.num_packed_configs(E.PACKED_BF16) == E.num_packed_configs(E.PACKED_F16) == 8
+    assert E.num_packed_configs(7) == 0
+    info = lambda kind, c: tuple(_packed_info(kind, c))
+    assert info(E.PACKED_SPLIT3, 0) == (128, 128, 1) and info(E.PACKED_SPLIT3, 8) == (256, 128, 1)
+    assert info(E.PACKED_BF16, 4) == (256, 256, 1) and info(E.PACKED_F16, 3) == (128, 128, 8)
+    prev = E.lib().eh_get_tune_mode()
+    E.set_tune_mode(E.TUNE_HEURISTIC)
+    try:
+        # 8192^2: 256 tiles of 256x128 fill the chip -> tile 2, no split
+        assert E.plan_packed(E.PACKED_SPLIT3, 0, 0, 8192, 8192, 8192) == 8
+        # 1024^2 x 8192: 64 tiles of 128x128 -> split 4 (64*4 = 256)
+        assert E.plan_packed(E.PACKED_SPLIT2, 0, 0, 1024, 1024, 8192) == 2
+        # bf16 8192^3: 1024 tiles of 256x256 -> tile 1
+        assert E.plan_packed(E.PACKED_BF16, 0, 0, 8192, 8192, 8192) == 4
+        # 4096^3 bf16: 256 tiles of 256x256 -> tile 1; 2048^3: 64 -> 128x128 (256 tiles), no split
+        assert E.plan_packed(E.PACKED_F16, 0, 0, 4096, 4096, 4096) == 4
+        assert E.plan_packed(E.PACKED_F16, 0, 0, 2048, 2048, 2048) == 0
+    finally:
+        E.set_tune_mode(prev)
+    with pytest.raises(E.EigenAmdError):
+        E.plan_packed(9, 0, 0, 8, 8, 8)
+
+
+def _packed_info(kind, c):
+    import ctypes as C
+    tm, tn, sk = C.c_int(), C.c_int(), C.c_int()
+    E.check(E.lib().eh_packed_config_info(kind, c, C.byref(tm), C.byref(tn), C.byref(sk)), "eh_packed_config_info")
+    return tm.value, tn.value, sk.value
+
+
 def test_compiled_in_selector_is_the_trained_network():
     """The planner's default network (compiled into the library) is the one
     trained on the device's timing table: its host-side choices equal the

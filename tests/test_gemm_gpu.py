@@ -182,9 +182,10 @@ def test_device_split_matches_oracle(gpu):
     assert (int(ts[0][0].item()) & 0x7FFF) > 0x7F80
 
 
-# configurations of the split path: planner default, 128x128 (3 stages),
-# 256x128 with 8 waves, 256x128 with 4 waves, and that with split K
-SPLIT_CONFIGS = [None, 12, 16, 20, 21]
+# packed configurations of the split path (tile * 4 + split index): planner
+# default, 128x128 (3 stages), 256x128 with 8 waves, 256x128 with 4 waves,
+# and that with split K 2
+SPLIT_CONFIGS = [None, 0, 8, 4, 5]
 
 
 _SPLIT_CASES = {}
