@@ -63,6 +63,14 @@ class Grouped:
         check(lib().eh_grouped_run_per_product(self.handle, current_stream() if stream is None else stream),
               "eh_grouped_run_per_product")
 
+    def set_schedule(self, schedule: int) -> None:
+        check(lib().eh_grouped_set_schedule(self.handle, schedule), "eh_grouped_set_schedule")
+
+    def schedule(self) -> int:
+        v = C.c_int()
+        check(lib().eh_grouped_get_schedule(self.handle, C.byref(v)), "eh_grouped_get_schedule")
+        return v.value
+
     def info(self):
         t, p = C.c_int64(), C.c_int64()
         check(lib().eh_grouped_info(self.handle, C.byref(t), C.byref(p)), "eh_grouped_info")
@@ -183,6 +191,10 @@ def _declare_extra(L) -> None:
         "eh_grouped_run_per_product": (i32, [vp, vp]),
         "eh_grouped_destroy": (i32, [vp]),
         "eh_grouped_info": (i32, [vp, C.POINTER(i64), C.POINTER(i64)]),
+        "eh_grouped_num_schedules": (i32, []),
+        "eh_grouped_schedule_info": (i32, [i32, C.POINTER(i32), C.POINTER(i32)]),
+        "eh_grouped_set_schedule": (i32, [vp, i32]),
+        "eh_grouped_get_schedule": (i32, [vp, C.POINTER(i32)]),
         "eh_time_config": (i32, [i32, i32, i32, i64, i64, i64, vp, vp, vp, i32, i32, C.POINTER(f32)]),
         "eh_tuner_record": (i32, [i32, i32, i32, i64, i64, i64, i32]),
         "eh_tuner_load": (i32, [C.c_char_p]),

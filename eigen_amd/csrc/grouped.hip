@@ -69,8 +69,12 @@ struct GroupedDesc {
   int32_t tiles_m, tiles_n;
 };
 
-constexpr int kWavesPerGroup = 16;
-constexpr int kBand = 4;  // tile columns of a band (block width)
+// Schedules: waves per workgroup G and band width (tile columns of a block).
+// Schedule 0 (16 waves, 4x4-tile blocks) is the default; EH_TUNE_TABLE
+// times all of them on each new batch (eh_grouped_create).
+constexpr int kNumSchedules = 3;
+constexpr int kSchedWaves[kNumSchedules] = {16, 8, 4};
+constexpr int kSchedBand[kNumSchedules] = {4, 2, 2};
 // k depth of one step: 16 for FP32 (8 MFMAs), 8 for FP64 (8 MFMAs of a 2x2
 // block); two steps of operands per wave stay within 128 registers (four
 // waves per SIMD).
@@ -159,19 +163,20 @@ __device__ inline void tail_f64(double (&v)[2][kLaneF64], int k0, int K, int lan
       if (kb + s >= K) v[x][s] = 0.0;
 }
 
-// Tile `l` of a product in band order: bands of kBand tile columns (the last
+// Tile `l` of a product in band order: bands of Band tile columns (the last
 // band holds the remaining columns), each band walked row by row.
+template <int Band>
 __device__ inline void band_tile(const GroupedDesc& g, int l, int& tm, int& tn) {
-  const int band_tiles = g.tiles_m * kBand, full = g.tiles_n / kBand;
+  const int band_tiles = g.tiles_m * Band, full = g.tiles_n / Band;
   const int b = l / band_tiles;
   if (b < full) {
     const int r = l - b * band_tiles;
-    tm = r / kBand;
-    tn = b * kBand + r % kBand;
+    tm = r / Band;
+    tn = b * Band + r % Band;
   } else {
-    const int w = g.tiles_n - full * kBand, r = l - full * band_tiles;
+    const int w = g.tiles_n - full * Band, r = l - full * band_tiles;
     tm = r / w;
-    tn = full * kBand + r % w;
+    tn = full * Band + r % w;
   }
 }
 
@@ -200,9 +205,10 @@ struct Tile<float, TA, TB> {
   __amdgpu_buffer_rsrc_t rA, rB;
   uint32_t vA, vB;
 
-  __device__ void init(const GroupedDesc& d, int l, int lane) {
+  __device__ void init(const GroupedDesc& d, int tile_m, int tile_n, int lane) {
     g = d;
-    band_tile(g, l, tm, tn);
+    tm = tile_m;
+    tn = tile_n;
     steps = (g.k + kStep - 1) / kStep;
     rA = make_rsrc(g.A, g.bytesA);
     rB = make_rsrc(g.B, g.bytesB);
@@ -275,9 +281,10 @@ struct Tile<double, TA, TB> {
   __amdgpu_buffer_rsrc_t rA, rB;
   uint32_t vA, vB;
 
-  __device__ void init(const GroupedDesc& d, int l, int lane) {
+  __device__ void init(const GroupedDesc& d, int tile_m, int tile_n, int lane) {
     g = d;
-    band_tile(g, l, tm, tn);
+    tm = tile_m;
+    tn = tile_n;
     steps = (g.k + kStep - 1) / kStep;
     rA = make_rsrc(g.A, g.bytesA);
     rB = make_rsrc(g.B, g.bytesB);
@@ -356,18 +363,21 @@ struct Tile<double, TA, TB> {
 };
 
 // The tiles of one workgroup run, all of one operand type <T, TA, TB>.  Wave
-// w computes tiles tbeg + w, tbeg + w + G, ... (G = kWavesPerGroup); per tile
-// the k steps go through two register fragments, one step in flight while
-// the other is multiplied.
-template <typename T, bool TA, bool TB>
+// w computes tiles tbeg + w, tbeg + w + G, ...; per tile the k steps go
+// through two register fragments, one step in flight while the other is
+// multiplied.
+template <typename T, bool TA, bool TB, int G, int Band>
 __device__ void run_tiles(const GroupedDesc* __restrict__ d, const int64_t* __restrict__ tile_begin, int64_t tbeg,
                           int64_t tend, int p, int w, int lane) {
   using TL = Tile<T, TA, TB>;
-  for (int64_t t0 = tbeg, t = tbeg + w; t0 < tend; t0 += kWavesPerGroup, t += kWavesPerGroup) {
+  for (int64_t t0 = tbeg, t = tbeg + w; t0 < tend; t0 += G, t += G) {
     if (t < tend) {
       while (tile_begin[p + 1] <= t) ++p;
+      const GroupedDesc& g = d[p];
+      int tm, tn;
+      band_tile<Band>(g, (int)(t - tile_begin[p]), tm, tn);
       TL cur;
-      cur.init(d[p], (int)(t - tile_begin[p]), lane);
+      cur.init(g, tm, tn, lane);
       typename TL::Frag f0, f1;
       typename TL::Acc acc;
       TL::zero(acc);
@@ -388,14 +398,16 @@ __device__ void run_tiles(const GroupedDesc* __restrict__ d, const int64_t* __re
   }
 }
 
-// Persistent kernel: workgroup v (XCD-major numbering of the resident
-// workgroups) computes tiles run[v] .. run[v+1]-1 of the work list; the host
-// cuts runs where the operand type changes, so one instantiation of
-// run_tiles serves the whole run.
-__global__ void __launch_bounds__(64 * kWavesPerGroup) grouped_gemm_kernel(const GroupedDesc* __restrict__ d,
-                                                                           const int64_t* __restrict__ tile_begin,
-                                                                           const int64_t* __restrict__ run,
-                                                                           const int32_t* __restrict__ run_prod) {
+// Persistent kernel of schedule (G, Band), four waves per SIMD (at most 128
+// registers) for every G: workgroup v (XCD-major numbering
+// of the resident workgroups) computes tiles run[v] .. run[v+1]-1 of the
+// work list; the host cuts runs where the operand type changes, so one
+// instantiation of run_tiles serves the whole run.
+template <int G, int Band>
+__global__ void __launch_bounds__(64 * G, 4) grouped_gemm_kernel(const GroupedDesc* __restrict__ d,
+                                                              const int64_t* __restrict__ tile_begin,
+                                                              const int64_t* __restrict__ run,
+                                                              const int32_t* __restrict__ run_prod) {
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int v = xcd_remap(blockIdx.x, gridDim.x);
@@ -405,23 +417,57 @@ __global__ void __launch_bounds__(64 * kWavesPerGroup) grouped_gemm_kernel(const
   const GroupedDesc& g = d[p];
   if (g.dtype == EH_F32) {
     if (!g.ta && !g.tb)
-      run_tiles<float, false, false>(d, tile_begin, tbeg, tend, p, w, lane);
+      run_tiles<float, false, false, G, Band>(d, tile_begin, tbeg, tend, p, w, lane);
     else if (g.ta && !g.tb)
-      run_tiles<float, true, false>(d, tile_begin, tbeg, tend, p, w, lane);
+      run_tiles<float, true, false, G, Band>(d, tile_begin, tbeg, tend, p, w, lane);
     else if (!g.ta && g.tb)
-      run_tiles<float, false, true>(d, tile_begin, tbeg, tend, p, w, lane);
+      run_tiles<float, false, true, G, Band>(d, tile_begin, tbeg, tend, p, w, lane);
     else
-      run_tiles<float, true, true>(d, tile_begin, tbeg, tend, p, w, lane);
+      run_tiles<float, true, true, G, Band>(d, tile_begin, tbeg, tend, p, w, lane);
   } else {
     if (!g.ta && !g.tb)
-      run_tiles<double, false, false>(d, tile_begin, tbeg, tend, p, w, lane);
+      run_tiles<double, false, false, G, Band>(d, tile_begin, tbeg, tend, p, w, lane);
     else if (g.ta && !g.tb)
-      run_tiles<double, true, false>(d, tile_begin, tbeg, tend, p, w, lane);
+      run_tiles<double, true, false, G, Band>(d, tile_begin, tbeg, tend, p, w, lane);
     else if (!g.ta && g.tb)
-      run_tiles<double, false, true>(d, tile_begin, tbeg, tend, p, w, lane);
+      run_tiles<double, false, true, G, Band>(d, tile_begin, tbeg, tend, p, w, lane);
     else
-      run_tiles<double, true, true>(d, tile_begin, tbeg, tend, p, w, lane);
+      run_tiles<double, true, true, G, Band>(d, tile_begin, tbeg, tend, p, w, lane);
   }
+}
+
+using GroupedKernel = void (*)(const GroupedDesc*, const int64_t*, const int64_t*, const int32_t*);
+static const GroupedKernel kSchedKernel[kNumSchedules] = {grouped_gemm_kernel<16, 4>, grouped_gemm_kernel<8, 2>,
+                                                          grouped_gemm_kernel<4, 2>};
+static_assert(kSchedWaves[0] == 16 && kSchedBand[0] == 4 && kSchedWaves[1] == 8 && kSchedBand[1] == 2 &&
+                  kSchedWaves[2] == 4 && kSchedBand[2] == 2,
+              "kernel table and schedule constants agree");
+
+// Resident workgroups of schedule s on the current device (queried once per
+// process and schedule; function-local statics initialise thread-safely).
+static int64_t resident_groups(int s) {
+  static int64_t cache[kNumSchedules] = {};
+  static std::once_flag once[kNumSchedules];
+  std::call_once(once[s], [s] {
+    int dev = 0, cus = 256, per_cu = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess &&
+        prop.multiProcessorCount > 0)
+      cus = prop.multiProcessorCount;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kSchedKernel[s], 64 * kSchedWaves[s], 0) !=
+            hipSuccess ||
+        per_cu < 1)
+      per_cu = 1;
+    cache[s] = (int64_t)cus * per_cu;
+  });
+  return cache[s];
+}
+
+static hipError_t launch_grouped(int s, int64_t groups, const GroupedDesc* d, const int64_t* begin,
+                                 const int64_t* run, const int32_t* run_prod, hipStream_t stream) {
+  hipLaunchKernelGGL(kSchedKernel[s], dim3((unsigned)groups), dim3(64 * kSchedWaves[s]), 0, stream, d, begin, run,
+                     run_prod);
+  return hipGetLastError();
 }
 
 // Bytes spanned by a column-major operand of `rows` x `cols` (leading
@@ -583,21 +629,9 @@ extern "C" eh_status eh_gemm_batched_strided(eh_dtype dtype, eh_op opa, eh_op op
   base.tiles_m = (int32_t)tm;
   base.tiles_n = (int32_t)tn;
   const int64_t T = tm * tn, tiles = count * T;
-  // resident workgroups, as for eh_grouped_create (queried once per process;
-  // the initialisation of a function-local static is thread-safe)
-  static const int64_t resident = [] {
-    int dev = 0, cus = 256, per_cu = 0;
-    hipDeviceProp_t prop;
-    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess &&
-        prop.multiProcessorCount > 0)
-      cus = prop.multiProcessorCount;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, grouped_gemm_kernel, 64 * kWavesPerGroup, 0) !=
-            hipSuccess ||
-        per_cu < 1)
-      per_cu = 1;
-    return (int64_t)cus * per_cu;
-  }();
-  int64_t groups = std::min<int64_t>(resident, (tiles + kWavesPerGroup - 1) / kWavesPerGroup);
+  // the default schedule, resident workgroups as for eh_grouped_create
+  constexpr int sched = 0;
+  int64_t groups = std::min<int64_t>(resident_groups(sched), (tiles + kSchedWaves[sched] - 1) / kSchedWaves[sched]);
   if (groups > 8) groups = (groups + 7) / 8 * 8;
   groups = std::max<int64_t>(groups, 1);
   const int64_t W = groups;
@@ -620,9 +654,7 @@ extern "C" eh_status eh_gemm_batched_strided(eh_dtype dtype, eh_op opa, eh_op op
   if (e != hipSuccess) return hip_fail(e, "batched build launch");
   {
     KernelTimer t(EH_KC_GROUPED, s);
-    hipLaunchKernelGGL(grouped_gemm_kernel, dim3((unsigned)groups), dim3(64 * kWavesPerGroup), 0, s, d, begin, run,
-                       run_prod);
-    e = hipGetLastError();
+    e = launch_grouped(sched, groups, d, begin, run, run_prod, s);
   }
   return e == hipSuccess ? EH_SUCCESS : hip_fail(e, "grouped kernel launch");
 }
@@ -630,13 +662,16 @@ extern "C" eh_status eh_gemm_batched_strided(eh_dtype dtype, eh_op opa, eh_op op
 struct eh_grouped {
   std::vector<eh_gemm_problem> host;   // as given (for the per-product path)
   std::vector<eh_gemm_problem> large;  // products past 32-bit buffer offsets
+  std::vector<GroupedDesc> desc;       // the kernel's work list (host copy)
+  std::vector<int64_t> begin, cost_begin;
   GroupedDesc* d_desc = nullptr;
   int64_t* d_begin = nullptr;
   int64_t* d_run = nullptr;
   int32_t* d_run_prod = nullptr;
   int64_t count = 0;   // products in the kernel's work list
   int64_t tiles = 0;
-  int64_t groups = 0;  // workgroups launched (kWavesPerGroup waves each)
+  int schedule = 0;
+  int64_t groups = 0;  // workgroups launched (kSchedWaves[schedule] waves each)
   hipGraphExec_t exec = nullptr;
   hipGraph_t graph = nullptr;
   std::mutex mu;
@@ -645,9 +680,7 @@ struct eh_grouped {
 static eh_status grouped_enqueue(eh_grouped* g, hipStream_t s) {
   if (g->count > 0) {
     KernelTimer t(EH_KC_GROUPED, s);
-    hipLaunchKernelGGL(grouped_gemm_kernel, dim3((unsigned)g->groups), dim3(64 * kWavesPerGroup), 0, s, g->d_desc,
-                       g->d_begin, g->d_run, g->d_run_prod);
-    hipError_t e = hipGetLastError();
+    hipError_t e = launch_grouped(g->schedule, g->groups, g->d_desc, g->d_begin, g->d_run, g->d_run_prod, s);
     if (e != hipSuccess) return hip_fail(e, "grouped kernel launch");
   }
   for (const eh_gemm_problem& q : g->large) {
@@ -661,6 +694,134 @@ static eh_status grouped_enqueue(eh_grouped* g, hipStream_t s) {
     if (st != EH_SUCCESS) return st;
   }
   return EH_SUCCESS;
+}
+
+// Runs of schedule `sched` for g's work list, uploaded to the device.  Every
+// run holds one operand type: each type's products get runs in proportion
+// to their cost (at least one), cut so that the runs of a type carry equal
+// cost (run v starts at the first tile whose starting cost reaches the
+// run's share).  A captured graph of the previous runs is dropped.
+static eh_status grouped_build_runs(eh_grouped* g, int sched) {
+  const auto& desc = g->desc;
+  const auto& begin = g->begin;
+  const auto& cost_begin = g->cost_begin;
+  const int G = kSchedWaves[sched];
+  const int64_t groups = std::max<int64_t>(1, std::min<int64_t>(resident_groups(sched), (g->tiles + G - 1) / G));
+  auto type_of = [](const GroupedDesc& x) { return x.dtype * 4 + x.ta * 2 + x.tb; };
+  std::vector<int64_t> seg{0};
+  for (int64_t p = 1; p < g->count; ++p)
+    if (type_of(desc[p]) != type_of(desc[p - 1])) seg.push_back(p);
+  seg.push_back(g->count);
+  const int64_t total_cost = cost_begin[g->count];
+  std::vector<int64_t> seg_runs(seg.size() - 1);
+  int64_t W = 0;
+  for (size_t i = 0; i + 1 < seg.size(); ++i) {
+    const int64_t c = cost_begin[seg[i + 1]] - cost_begin[seg[i]];
+    seg_runs[i] = std::max<int64_t>(1, (int64_t)((__int128)groups * c / std::max<int64_t>(total_cost, 1)));
+    W += seg_runs[i];
+  }
+  // launched workgroups: whole XCD rows for the remap; the extra runs are empty
+  const int64_t WL = W > 8 ? (W + 7) / 8 * 8 : W;
+  std::vector<int64_t> run(WL + 1, g->tiles);
+  std::vector<int32_t> run_prod(WL, (int32_t)(g->count - 1));
+  int64_t v = 0;
+  for (size_t i = 0; i + 1 < seg.size(); ++i) {
+    const int64_t ps = seg[i], pe = seg[i + 1], cb = cost_begin[ps], ce = cost_begin[pe], R = seg_runs[i];
+    int64_t p = ps;
+    for (int64_t r = 0; r < R; ++r, ++v) {
+      const int64_t target = cb + (int64_t)((__int128)(ce - cb) * r / R);
+      while (p < pe - 1 && cost_begin[p + 1] <= target) ++p;
+      const int64_t c = tile_cost(desc[p]);
+      int64_t t = begin[p] + (target - cost_begin[p] + c - 1) / c;
+      t = std::min(t, begin[p + 1]);
+      if (r > 0) t = std::max(t, run[v - 1]);
+      run[v] = t;
+      // product holding tile t (the next product of the type when t ends
+      // this one; a run starting at the type's end is empty)
+      int64_t q = p;
+      while (q < pe - 1 && begin[q + 1] <= t) ++q;
+      run_prod[v] = (int32_t)q;
+    }
+  }
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  g->exec = nullptr;
+  g->graph = nullptr;
+  if (g->d_run) (void)hipFree(g->d_run);
+  if (g->d_run_prod) (void)hipFree(g->d_run_prod);
+  g->d_run = nullptr;
+  g->d_run_prod = nullptr;
+  hipError_t e = hipMalloc(&g->d_run, sizeof(int64_t) * (WL + 1));
+  if (e == hipSuccess) e = hipMalloc(&g->d_run_prod, sizeof(int32_t) * WL);
+  if (e == hipSuccess) e = hipMemcpy(g->d_run, run.data(), sizeof(int64_t) * (WL + 1), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(g->d_run_prod, run_prod.data(), sizeof(int32_t) * WL, hipMemcpyHostToDevice);
+  if (e != hipSuccess) return fail(EH_ERR_ALLOC, std::string("grouped runs: ") + hipGetErrorString(e));
+  g->schedule = sched;
+  g->groups = WL;
+  return EH_SUCCESS;
+}
+
+// Time every schedule on this batch and keep the fastest.  The timed runs
+// write to a scratch copy of the results (each product's C as its own m x n
+// block, beta = 0), so the caller's C is untouched; batches whose results
+// exceed kTuneScratchMax keep the default schedule.
+static eh_status grouped_tune(eh_grouped* g) {
+  constexpr size_t kTuneScratchMax = size_t(1) << 30;
+  std::vector<GroupedDesc> td = g->desc;
+  size_t bytes = 0;
+  for (auto& x : td) {
+    const size_t es = x.dtype == EH_F64 ? 8 : 4;
+    bytes = (bytes + 255) / 256 * 256;
+    x.C = reinterpret_cast<void*>(bytes);  // offset, rebased below
+    x.ldc = x.m;
+    x.beta = 0.0;
+    bytes += (size_t)x.m * x.n * es;
+  }
+  if (bytes > kTuneScratchMax) return EH_SUCCESS;
+  char* scratch = nullptr;
+  GroupedDesc* d_td = nullptr;
+  hipStream_t s = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t e = hipMalloc(&scratch, std::max<size_t>(bytes, 256));
+  if (e == hipSuccess) e = hipMalloc(&d_td, sizeof(GroupedDesc) * td.size());
+  if (e == hipSuccess) {
+    for (auto& x : td) x.C = scratch + reinterpret_cast<size_t>(x.C);
+    e = hipMemcpy(d_td, td.data(), sizeof(GroupedDesc) * td.size(), hipMemcpyHostToDevice);
+  }
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  eh_status st = e == hipSuccess ? EH_SUCCESS : hip_fail(e, "grouped tuning setup");
+  int best = 0;
+  float best_ms = 0.0f;
+  for (int sched = 0; sched < kNumSchedules && st == EH_SUCCESS; ++sched) {
+    st = grouped_build_runs(g, sched);
+    std::vector<float> t;
+    for (int r = 0; r < 4 && st == EH_SUCCESS; ++r) {  // one warm-up, median of three
+      (void)hipEventRecord(e0, s);
+      e = launch_grouped(sched, g->groups, d_td, g->d_begin, g->d_run, g->d_run_prod, s);
+      if (e == hipSuccess) e = hipEventRecord(e1, s);
+      if (e == hipSuccess) e = hipEventSynchronize(e1);
+      float ms = 0.0f;
+      if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+      if (e != hipSuccess) st = hip_fail(e, "grouped tuning run");
+      if (r > 0) t.push_back(ms);
+    }
+    if (st != EH_SUCCESS) break;
+    std::sort(t.begin(), t.end());
+    if (sched == 0 || t[1] < best_ms) {
+      best = sched;
+      best_ms = t[1];
+    }
+  }
+  if (e1) (void)hipEventDestroy(e1);
+  if (e0) (void)hipEventDestroy(e0);
+  if (s) (void)hipStreamDestroy(s);
+  if (d_td) (void)hipFree(d_td);
+  if (scratch) (void)hipFree(scratch);
+  if (st != EH_SUCCESS) return st;
+  return grouped_build_runs(g, best);
 }
 
 extern "C" {
@@ -727,84 +888,31 @@ eh_status eh_grouped_create(const eh_gemm_problem* problems, int64_t count, eh_g
   }
 
   // work list: first tile of each product and cumulative cost
-  std::vector<int64_t> begin(g->count + 1);
-  std::vector<int64_t> cost_begin(g->count + 1);
-  begin[0] = cost_begin[0] = 0;
+  g->desc = std::move(desc);
+  g->begin.assign(g->count + 1, 0);
+  g->cost_begin.assign(g->count + 1, 0);
   for (int64_t p = 0; p < g->count; ++p) {
-    const int64_t tiles = (int64_t)desc[p].tiles_m * desc[p].tiles_n;
-    begin[p + 1] = begin[p] + tiles;
-    cost_begin[p + 1] = cost_begin[p] + tiles * tile_cost(desc[p]);
+    const int64_t tiles = (int64_t)g->desc[p].tiles_m * g->desc[p].tiles_n;
+    g->begin[p + 1] = g->begin[p] + tiles;
+    g->cost_begin[p + 1] = g->cost_begin[p] + tiles * tile_cost(g->desc[p]);
   }
-  g->tiles = begin[g->count];
+  g->tiles = g->begin[g->count];
 
-  // resident workgroups: as many as fit on every CU at once
-  int dev = 0, cus = 256, per_cu = 0;
-  hipError_t e = hipGetDevice(&dev);
-  if (e == hipSuccess) {
-    hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0)
-      cus = prop.multiProcessorCount;
-  }
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, grouped_gemm_kernel, 64 * kWavesPerGroup, 0) !=
-          hipSuccess ||
-      per_cu < 1)
-    per_cu = 1;
-  int64_t groups = (int64_t)cus * per_cu;
-  groups = std::max<int64_t>(1, std::min<int64_t>(groups, (g->tiles + kWavesPerGroup - 1) / kWavesPerGroup));
-
-  // Runs: every run holds one operand type.  Each type's products get runs
-  // in proportion to their cost (at least one), cut so that the runs of a
-  // type carry equal cost: run v starts at the first tile whose starting
-  // cost reaches the run's share.
-  std::vector<int64_t> seg{0};
-  for (int64_t p = 1; p < g->count; ++p)
-    if (type_of(desc[p]) != type_of(desc[p - 1])) seg.push_back(p);
-  seg.push_back(g->count);
-  const int64_t total_cost = cost_begin[g->count];
-  std::vector<int64_t> seg_runs(seg.size() - 1);
-  int64_t W = 0;
-  for (size_t i = 0; i + 1 < seg.size(); ++i) {
-    const int64_t c = cost_begin[seg[i + 1]] - cost_begin[seg[i]];
-    seg_runs[i] = std::max<int64_t>(1, (int64_t)((__int128)groups * c / total_cost));
-    W += seg_runs[i];
-  }
-  // launched workgroups: whole XCD rows for the remap; the extra runs are empty
-  const int64_t WL = W > 8 ? (W + 7) / 8 * 8 : W;
-  g->groups = WL;
-  std::vector<int64_t> run(WL + 1, g->tiles);
-  std::vector<int32_t> run_prod(WL, (int32_t)(g->count - 1));
-  int64_t v = 0;
-  for (size_t i = 0; i + 1 < seg.size(); ++i) {
-    const int64_t ps = seg[i], pe = seg[i + 1], cb = cost_begin[ps], ce = cost_begin[pe], R = seg_runs[i];
-    int64_t p = ps;
-    for (int64_t r = 0; r < R; ++r, ++v) {
-      const int64_t target = cb + (int64_t)((__int128)(ce - cb) * r / R);
-      while (p < pe - 1 && cost_begin[p + 1] <= target) ++p;
-      const int64_t c = tile_cost(desc[p]);
-      int64_t t = begin[p] + (target - cost_begin[p] + c - 1) / c;
-      t = std::min(t, begin[p + 1]);
-      if (r > 0) t = std::max(t, run[v - 1]);
-      run[v] = t;
-      // product holding tile t (the next product of the type when t ends
-      // this one; a run starting at the type's end is empty)
-      int64_t q = p;
-      while (q < pe - 1 && begin[q + 1] <= t) ++q;
-      run_prod[v] = (int32_t)q;
-    }
-  }
-
-  e = hipMalloc(&g->d_desc, sizeof(GroupedDesc) * g->count);
+  hipError_t e = hipMalloc(&g->d_desc, sizeof(GroupedDesc) * g->count);
   if (e == hipSuccess) e = hipMalloc(&g->d_begin, sizeof(int64_t) * (g->count + 1));
-  if (e == hipSuccess) e = hipMalloc(&g->d_run, sizeof(int64_t) * (WL + 1));
-  if (e == hipSuccess) e = hipMalloc(&g->d_run_prod, sizeof(int32_t) * WL);
-  if (e == hipSuccess) e = hipMemcpy(g->d_desc, desc.data(), sizeof(GroupedDesc) * g->count, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(g->d_begin, begin.data(), sizeof(int64_t) * (g->count + 1), hipMemcpyHostToDevice);
   if (e == hipSuccess)
-    e = hipMemcpy(g->d_run, run.data(), sizeof(int64_t) * (WL + 1), hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(g->d_run_prod, run_prod.data(), sizeof(int32_t) * WL, hipMemcpyHostToDevice);
+    e = hipMemcpy(g->d_desc, g->desc.data(), sizeof(GroupedDesc) * g->count, hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(g->d_begin, g->begin.data(), sizeof(int64_t) * (g->count + 1), hipMemcpyHostToDevice);
   if (e != hipSuccess) {
     eh_grouped_destroy(g);
     return fail(EH_ERR_ALLOC, std::string("grouped setup: ") + hipGetErrorString(e));
+  }
+  eh_status st = grouped_build_runs(g, 0);
+  if (st == EH_SUCCESS && tune_mode() == EH_TUNE_TABLE && !fixed_scratch_active()) st = grouped_tune(g);
+  if (st != EH_SUCCESS) {
+    eh_grouped_destroy(g);
+    return st;
   }
   *out = g;
   return EH_SUCCESS;
@@ -872,6 +980,32 @@ eh_status eh_grouped_info(eh_grouped* g, int64_t* tiles, int64_t* products) {
   if (!g) return fail(EH_ERR_INVALID, "NULL handle");
   if (tiles) *tiles = g->tiles;
   if (products) *products = g->count + (int64_t)g->large.size();
+  return EH_SUCCESS;
+}
+
+int eh_grouped_num_schedules(void) { return kNumSchedules; }
+
+eh_status eh_grouped_schedule_info(int schedule, int* waves_per_group, int* band) {
+  if (schedule < 0 || schedule >= kNumSchedules) return fail(EH_ERR_INVALID, "schedule out of range");
+  if (waves_per_group) *waves_per_group = kSchedWaves[schedule];
+  if (band) *band = kSchedBand[schedule];
+  return EH_SUCCESS;
+}
+
+eh_status eh_grouped_set_schedule(eh_grouped* g, int schedule) {
+  if (!g) return fail(EH_ERR_INVALID, "NULL handle");
+  if (schedule < 0 || schedule >= kNumSchedules) return fail(EH_ERR_INVALID, "schedule out of range");
+  std::lock_guard<std::mutex> lk(g->mu);
+  if (g->count == 0) {
+    g->schedule = schedule;
+    return EH_SUCCESS;
+  }
+  return grouped_build_runs(g, schedule);
+}
+
+eh_status eh_grouped_get_schedule(eh_grouped* g, int* schedule) {
+  if (!g || !schedule) return fail(EH_ERR_INVALID, "NULL argument");
+  *schedule = g->schedule;
   return EH_SUCCESS;
 }
 

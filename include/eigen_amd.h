@@ -392,11 +392,14 @@ typedef struct eh_gemm_problem {
 typedef struct eh_grouped eh_grouped; /* opaque batch handle */
 
 /* Validates `count` descriptors (host array), builds the work list on the
- * host once -- one descriptor per product, the first output tile of each
- * product, and one [begin, end) run of tiles of equal estimated cost per
- * resident wave -- and uploads it to the device.  Products whose operands
- * span 2^31 bytes or more are kept aside for the single-product path.
- * Products must not write overlapping C. */
+ * host once -- one descriptor per product grouped by operand type, the first
+ * output tile of each product, and one [begin, end) run of tiles of one
+ * operand type and equal estimated cost per resident workgroup -- and
+ * uploads it to the device.  Products whose operands span 2^31 bytes or
+ * more are kept aside for the single-product path.  Products must not write
+ * overlapping C.  Under EH_TUNE_TABLE the call also times every schedule
+ * (below) on the batch -- reading A and B, writing a scratch copy of the
+ * results, never C -- and keeps the fastest; otherwise schedule 0. */
 eh_status eh_grouped_create(const eh_gemm_problem* problems, int64_t count, eh_grouped** out);
 /* All products of the work list in one launch of the persistent grouped
  * kernel, then the set-aside large products one eh_gemm each. */
@@ -408,6 +411,15 @@ eh_status eh_grouped_graph_launch(eh_grouped* g, void* stream);
 eh_status eh_grouped_run_per_product(eh_grouped* g, void* stream);
 /* Number of 32x32 output tiles and of non-empty products. */
 eh_status eh_grouped_info(eh_grouped* g, int64_t* tiles, int64_t* products);
+/* Schedules of the grouped kernel: `waves_per_group` waves per workgroup
+ * compute a block of one product's 32x32 tiles at a time, `band` tile
+ * columns wide (0: 16 waves, 4; 1: 8 waves, 2; 2: 4 waves, 2).  Setting a
+ * schedule rebuilds the runs (and drops a captured graph); results are
+ * bit-identical across schedules. */
+int eh_grouped_num_schedules(void);
+eh_status eh_grouped_schedule_info(int schedule, int* waves_per_group, int* band);
+eh_status eh_grouped_set_schedule(eh_grouped* g, int schedule);
+eh_status eh_grouped_get_schedule(eh_grouped* g, int* schedule);
 eh_status eh_grouped_destroy(eh_grouped* g);
 
 /* `count` products of one shape with operands at fixed element strides

@@ -127,6 +127,62 @@ def test_grouped_all_operand_types_many_runs_exact(gpu):
     g.close()
 
 
+def test_grouped_schedules_bit_identical(gpu):
+    """Every schedule (waves per workgroup x band width) computes each tile with
+    the same code, so all give the same bits; checked on a mixed uneven batch
+    (FP32 and FP64, random inputs), with a captured graph dropped and rebuilt
+    on every change."""
+    import torch
+    b = uneven_batch(count=600, seed=13)
+    g = E.Grouped(b.problems)
+    ns = E.lib().eh_grouped_num_schedules()
+    assert ns == 3
+    results = []
+    for sched in range(ns):
+        g.set_schedule(sched)
+        assert g.schedule() == sched
+        g.graph_launch()
+        torch.cuda.synchronize()
+        results.append([t.clone() for t in b.buffers.values()])
+    for r in results[1:]:
+        for x, y in zip(results[0], r):
+            assert torch.equal(x, y)
+    with pytest.raises(E.EigenAmdError):
+        g.set_schedule(ns)
+    g.close()
+
+
+def test_grouped_table_mode_tunes_schedule_without_touching_c(gpu):
+    """Under EH_TUNE_TABLE, eh_grouped_create times every schedule on the batch
+    (scratch results): C holds its old values until the first run, and the
+    run under the chosen schedule matches schedule 0 bit for bit."""
+    import torch
+    b = uneven_batch(count=500, seed=17, beta=0.5)
+    before = [t.clone() for t in b.buffers.values()]
+    prev = E.lib().eh_get_tune_mode()
+    E.set_tune_mode(E.TUNE_TABLE)
+    try:
+        g = E.Grouped(b.problems)
+    finally:
+        E.set_tune_mode(prev)
+    torch.cuda.synchronize()
+    for x, y in zip(before, b.buffers.values()):
+        assert torch.equal(x, y)  # tuning never wrote C (nor A, B)
+    chosen = g.schedule()
+    assert 0 <= chosen < E.lib().eh_grouped_num_schedules()
+    g.run()
+    torch.cuda.synchronize()
+    tuned = [t.clone() for t in b.buffers.values()]
+    for t, x in zip(b.buffers.values(), before):  # restore C and run the default schedule
+        t.copy_(x)
+    g.set_schedule(0)
+    g.run()
+    torch.cuda.synchronize()
+    for x, y in zip(tuned, b.buffers.values()):
+        assert torch.equal(x, y)
+    g.close()
+
+
 def test_grouped_random_within_bound(gpu):
     import torch
     b = uneven_batch(count=400, seed=11)  # the batched workload's distribution, smaller count
