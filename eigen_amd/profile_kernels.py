@@ -6,7 +6,8 @@
 `run` launches one workload `warmup` times (default 2), then `reps` times
 with the library's kernel event timing on (HIP events around each launch of
 the dominant kernel class, on its stream), and prints the mean event-timed
-launch of the dominant class ("event_ms ..."): the in-process figure the
+launch of the dominant class ("event_ms ...") and the mean event-timed step
+("step_ms ...", events around each whole step): the in-process figures the
 profiler's kernel trace of the same command is compared with.  It is the
 program under `rocprofv3 --kernel-trace --stats` or `rocprofv3 --pmc`:
 
@@ -21,7 +22,8 @@ program under `rocprofv3 --kernel-trace --stats` or `rocprofv3 --pmc`:
 
 `summarize` reads directories named trace_<workload> (kernel-trace csv; with
 --skip N the first N dispatches of every kernel -- the warm-up launches --
-are left out of the per-dispatch averages),
+are left out of the per-dispatch averages; a stdout.txt there holding the
+run's own output adds its event-timed kernel and step figures),
 pmc_<workload>_<FETCH_SIZE|WRITE_SIZE> and util_<workload> (counter csv of
 one pass over UTIL_COUNTERS) under the given root and
 prints per-kernel duration tables and counter readings as markdown, plus the
@@ -61,13 +63,19 @@ def run(what: str, reps: int, warmup: int = 2) -> None:
     cls = getattr(E, KCLASS[what]) if what in KCLASS else None
     if cls is not None:
         E.kernel_timing(True)
-    for _ in range(reps):
+    stream = torch.cuda.current_stream()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
+    for e0, e1 in ev:
+        e0.record(stream)
         step()
+        e1.record(stream)
     torch.cuda.synchronize()
+    step_ms = sum(e0.elapsed_time(e1) for e0, e1 in ev) / max(reps, 1)
     if cls is not None:
         ms, n = E.kernel_timing_read(cls)
         E.kernel_timing(False)
         print(f"event_ms {what} {KCLASS[what]} {ms / max(n, 1):.4f} over {n} launches", flush=True)
+    print(f"step_ms {what} {step_ms:.4f} over {reps} steps (HIP events around each step on its stream)", flush=True)
     print("done", what, flush=True)
 
 
@@ -214,6 +222,13 @@ def summarize(root: str, skip: int = 0) -> dict:
         table, _ = kernel_table(d, skip)
         if table:
             print(f"## {os.path.basename(d)}\n\n{table}\n")
+            # the same process's own event timings (printed by `run` into stdout.txt)
+            log = os.path.join(d, "stdout.txt")
+            if os.path.exists(log):
+                for line in open(log):
+                    if line.startswith(("event_ms", "step_ms")):
+                        print(f"- {line.strip()}")
+                print()
     workloads = sorted({os.path.basename(p).split("_")[1] for p in glob.glob(os.path.join(root, "pmc_*"))})
     for w in workloads:
         if w == "copy":
