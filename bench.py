@@ -202,7 +202,8 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the product path here has no multi-GPU mode "
-              "(sharding is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+              "(one process per GPU would each run this path on its block of the product); run with --gpus 1",
+              file=sys.stderr)
         return 2
 
     import torch

@@ -14,27 +14,26 @@
 //      type gets runs in proportion to its estimated cost (k steps per tile
 //      plus a per-tile overhead) and its runs carry equal cost, so every run
 //      holds one operand type.  Within a product the tiles are numbered in
-//      bands of kBand tile columns, row by row inside a band, so any
-//      kWavesPerGroup consecutive tiles form a block of about
-//      kWavesPerGroup / kBand rows by kBand columns of tiles.
+//      bands of Band tile columns, row by row inside a band, so any G
+//      consecutive tiles form a block of about G / Band rows by Band columns
+//      of tiles (schedule (G, Band): 16 x 4 by default, 8 x 2 or 4 x 2).
 //   2. the kernel starts exactly the workgroups that fit on the chip at once;
 //      each picks the instantiation for its run's operand type once, and
-//      wave w takes tiles w, w + G, w + 2G, ... of the run
-//      (G = kWavesPerGroup), so the G waves of a workgroup compute one block
-//      of a product at the same time: each op(A) row panel and op(B) column
-//      panel of the block is read from HBM once and shared through the CU's
-//      and the XCD's caches (a wave walking a product alone finds its panels
-//      evicted by the time it comes back to them: this layout halved the
-//      kernel's HBM reads and cut its time by a fifth).  A barrier after
-//      every block keeps the waves in step (without it they drift apart and
-//      lose the sharing).  Each wave advances a product cursor
-//      (wave-uniform, scalar loads) and computes each 32x32 tile over the
-//      full depth with MFMA (v_mfma_f32_32x32x2_f32 for FP32, a 2x2 block of
-//      v_mfma_f64_16x16x4_f64 for FP64).  Operands are read with raw buffer
-//      loads (reads past the end of an operand return zero, so no load can
-//      fault), double-buffered in registers so the next k step is in flight
-//      while the current one is multiplied; the final step skips the MFMAs
-//      whose k values all lie past K.  The output tile is produced
+//      wave w takes tiles w, w + G, w + 2G, ... of the run, so the G waves of
+//      a workgroup compute one block of a product at the same time: each
+//      op(A) row panel and op(B) column panel of the block is read from HBM
+//      once and shared through the CU's and the XCD's caches (a wave walking
+//      a product alone finds its panels evicted by the time it comes back to
+//      them: this layout halved the kernel's HBM reads and cut its time by a
+//      fifth).  A barrier after every block keeps the waves in step (without
+//      it they drift apart and lose the sharing).  Each wave advances a
+//      product cursor (wave-uniform, scalar loads) and computes each 32x32
+//      tile over the full depth with MFMA (v_mfma_f32_32x32x2_f32 for FP32, a
+//      2x2 block of v_mfma_f64_16x16x4_f64 for FP64).  Operands are read with
+//      raw buffer loads (reads past the end of an operand return zero, so no
+//      load can fault), double-buffered in registers so the next k step is in
+//      flight while the current one is multiplied; the final step skips the
+//      MFMAs whose k values all lie past K.  The output tile is produced
 //      transposed (C^T = op(B)^T op(A)^T) so that consecutive lanes store
 //      consecutive rows of the column-major C, with alpha/beta fused into
 //      the store.
@@ -629,8 +628,12 @@ extern "C" eh_status eh_gemm_batched_strided(eh_dtype dtype, eh_op opa, eh_op op
   base.tiles_m = (int32_t)tm;
   base.tiles_n = (int32_t)tn;
   const int64_t T = tm * tn, tiles = count * T;
-  // the default schedule, resident workgroups as for eh_grouped_create
-  constexpr int sched = 0;
+  // Schedule by tiles per product: products of a few tiles share no panels
+  // across a block, so smaller workgroups (less waiting at the block
+  // barrier) win.  Measured on batches of one shape: 4 waves from 16^3 to
+  // 64^3 (1 to 4 tiles, 36 vs 40 us for 20,000 FP64 16^3), 8 waves at 100^3
+  // (16 tiles), 16 waves at 200^3 (49 tiles).
+  const int sched = T < 8 ? 2 : (T < 32 ? 1 : 0);
   int64_t groups = std::min<int64_t>(resident_groups(sched), (tiles + kSchedWaves[sched] - 1) / kSchedWaves[sched]);
   if (groups > 8) groups = (groups + 7) / 8 * 8;
   groups = std::max<int64_t>(groups, 1);

@@ -9,7 +9,7 @@
 // kind 0 is the plain FP32 / FP64 kernels (24 configurations), kinds 2..5
 // the packed products (split-BF16 with 2 or 3 terms, bf16 and half inputs;
 // eh_packed_kind).  The table can be saved to / loaded from a text file
-// stamped with the GPU architecture.  It is also the judge the learned
+// stamped with the GPU architecture.  It is also the yardstick the learned
 // selector is measured against (regret = t(choice)/t(best) - 1).
 #include <hip/hip_runtime.h>
 

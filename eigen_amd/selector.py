@@ -8,7 +8,7 @@ autotuner's timing table and measures it:
 * targets: soft labels q_c proportional to exp(-(t_c / t_best - 1) / tau)
   over the 24 measured configurations of each shape;
 * regret of a choice c on a shape: t_c / t_best - 1 against the exhaustive
-  sweep (the judge);
+  sweep (the yardstick);
 * the method without the network is the library's closed-form heuristic
   (eh_plan in TUNE_HEURISTIC mode).
 

@@ -4,7 +4,7 @@ Trains the 7-64-24 network on the device (the library's own HIP forward /
 backward / Adam kernels) from the committed autotuner table: 4,000 shapes,
 M, N, K log-uniform in [64, 16384], all 24 configurations timed.  On the 500
 held-out shapes the regret of its choice against exhaustive tuning
-(t(choice) / t(best) - 1, the table is the judge) must beat the closed-form
+(t(choice) / t(best) - 1, measured against the table) must beat the closed-form
 heuristic's."""
 import os
 
