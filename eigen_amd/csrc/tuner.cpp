@@ -15,6 +15,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -29,6 +30,10 @@
 namespace eh {
 
 constexpr int kPlainKind = 0;
+// Version of the configuration sets the table indexes (the plain kernels'
+// 24, the packed kernels' tiles and splits): a table saved under another
+// version names other kernels and is refused.
+constexpr int kKernelSetVersion = 2;
 using ShapeKey = std::tuple<int, int, int, int, int64_t, int64_t, int64_t>;
 static std::mutex g_tune_mutex;
 static std::map<ShapeKey, int> g_table;
@@ -251,7 +256,8 @@ eh_status eh_tuner_save(const char* path) {
   std::lock_guard<std::mutex> lk(g_tune_mutex);
   FILE* f = std::fopen(path, "w");
   if (!f) return fail(EH_ERR_INVALID, std::string("cannot write ") + path);
-  std::fprintf(f, "# eigen_amd tuning table v2 arch=%s\n# kind dtype opa opb m n k config\n", arch.c_str());
+  std::fprintf(f, "# eigen_amd tuning table v2 arch=%s kernels=%d\n# kind dtype opa opb m n k config\n", arch.c_str(),
+               kKernelSetVersion);
   for (const auto& kv : g_table)
     std::fprintf(f, "%d %d %d %d %lld %lld %lld %d\n", std::get<0>(kv.first), std::get<1>(kv.first),
                  std::get<2>(kv.first), std::get<3>(kv.first), (long long)std::get<4>(kv.first),
@@ -274,11 +280,16 @@ eh_status eh_tuner_load(const char* path) {
   while (std::fgets(line, sizeof line, f)) {
     ++lineno;
     if (line[0] == '#') {
-      const char* a = std::strstr(line, "arch=");
-      if (a) {
+      if (const char* a = std::strstr(line, "arch=")) {
         std::string arch(a + 5);
-        while (!arch.empty() && (arch.back() == '\n' || arch.back() == '\r' || arch.back() == ' ')) arch.pop_back();
+        arch = arch.substr(0, arch.find_first_of(" \r\n"));
         if (arch != device_arch()) return bad("table made for " + arch + ", this device is " + device_arch());
+      }
+      if (const char* kv = std::strstr(line, "kernels=")) {
+        const int v = std::atoi(kv + 8);
+        if (v != kKernelSetVersion)
+          return bad("table made for kernel set " + std::to_string(v) + ", this library has " +
+                     std::to_string(kKernelSetVersion));
       }
       continue;
     }

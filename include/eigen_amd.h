@@ -414,8 +414,9 @@ eh_status eh_grouped_info(eh_grouped* g, int64_t* tiles, int64_t* products);
 /* Schedules of the grouped kernel: `waves_per_group` waves per workgroup
  * compute a block of one product's 32x32 tiles at a time, `band` tile
  * columns wide (0: 16 waves, 4; 1: 8 waves, 2; 2: 4 waves, 2).  Setting a
- * schedule rebuilds the runs (and drops a captured graph); results are
- * bit-identical across schedules. */
+ * schedule rebuilds the runs (and drops a captured graph), so it must not
+ * overlap a run of the same handle; results are bit-identical across
+ * schedules. */
 int eh_grouped_num_schedules(void);
 eh_status eh_grouped_schedule_info(int schedule, int* waves_per_group, int* band);
 eh_status eh_grouped_set_schedule(eh_grouped* g, int schedule);
@@ -449,11 +450,12 @@ eh_status eh_time_config(int dtype, int opa, int opb, int64_t m, int64_t n, int6
 eh_status eh_tuner_record(int dtype, int opa, int opb, int64_t m, int64_t n, int64_t k, int config);
 int64_t eh_tuner_count(void);
 eh_status eh_tuner_clear(void);
-/* Text file: a header "# eigen_amd tuning table v2 arch=<gcnArchName>", then
- * one line per shape "kind dtype opa opb m n k config" (kind 0 = plain
- * kernels, else an eh_packed_kind).  Loading refuses a table made for
- * another architecture; v1 files (lines "dtype opa opb m n k config", plain
- * kernels only) still load. */
+/* Text file: a header "# eigen_amd tuning table v2 arch=<gcnArchName>
+ * kernels=<version of the configuration sets>", then one line per shape
+ * "kind dtype opa opb m n k config" (kind 0 = plain kernels, else an
+ * eh_packed_kind).  Loading refuses a table made for another architecture or
+ * kernel set; v1 files (lines "dtype opa opb m n k config", plain kernels
+ * only, no stamp) still load. */
 eh_status eh_tuner_save(const char* path);
 eh_status eh_tuner_load(const char* path);
 

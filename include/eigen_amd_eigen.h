@@ -251,6 +251,16 @@ class DeviceMap : public Eigen::Map<Plain, 0, Eigen::OuterStride<>> {
   }
   DeviceMap& operator=(const DeviceMap&) = delete;  // an element copy on the host
 
+  // A sub-block stays a DeviceMap (same storage order and leading dimension),
+  // so products assigned into it run on the device too.  (Eigen's own
+  // block(), col() and the like return plain Eigen blocks, whose assignments
+  // would run on the host.)
+  DeviceMap block(Index i, Index j, Index rows, Index cols) {
+    eigen_assert(i >= 0 && j >= 0 && rows >= 0 && cols >= 0 && i + rows <= this->rows() && j + cols <= this->cols());
+    Scalar* p = this->data() + (Plain::IsRowMajor ? i * this->outerStride() + j : i + j * this->outerStride());
+    return DeviceMap(p, rows, cols, this->outerStride());
+  }
+
   // C.noalias() = / += / -= A * B: the same device products (they never
   // evaluate into a temporary)
   struct NoAlias {

@@ -140,6 +140,20 @@ int run_all(const char* tname) {
     Mat want2 = u.host.transpose() * v.host;
     fails += report(tag("tiny_1x1"), s.fetch() == want2);
   }
+  {  // a product into a sub-block of a DeviceMap (column- and row-major)
+    Dev<Mat> A(m, k, 37), B(k, n, 38), C(m, n, 39, 2);
+    auto c = C.map();
+    c.block(3, 4, 40, 30) = A.map().block(5, 2, 40, 20) * B.map().block(1, 6, 20, 30);
+    Mat want = C.host;
+    want.block(3, 4, 40, 30) = A.host.block(5, 2, 40, 20) * B.host.block(1, 6, 20, 30);
+    fails += report(tag("block_assign"), C.fetch() == want);
+    Dev<RowMat> R(m, n, 40, 3);
+    auto r = R.map();
+    r.block(7, 1, 20, 40) -= A.map().block(0, 0, 20, 10) * B.map().block(0, 0, 10, 40);
+    RowMat want2 = R.host;
+    want2.block(7, 1, 20, 40) -= A.host.block(0, 0, 20, 10) * B.host.block(0, 0, 10, 40);
+    fails += report(tag("block_sub_row_major"), R.fetch() == want2);
+  }
   {  // row-major destination and operand
     Dev<RowMat> A(m, k, 25), C(m, n, 27, 2);
     Dev<Mat> B(k, n, 26);

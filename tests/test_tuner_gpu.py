@@ -110,7 +110,7 @@ def test_table_save_load_round_trip(gpu, table_mode, tmp_path):
     path = str(tmp_path / "table.txt")
     E.tuner_save(path)
     text = open(path).read().splitlines()
-    assert text[0].startswith("# eigen_amd tuning table v2 arch=gfx950")
+    assert text[0].startswith("# eigen_amd tuning table v2 arch=gfx950 kernels=")
     rows = [line.split() for line in text if not line.startswith("#")]
     assert sorted(rows) == sorted([["0", "1", "0", "1", "100", "200", "300", "5"],
                                    ["3", "0", "1", "0", "400", "500", "600", "9"]])
@@ -131,4 +131,11 @@ def test_table_save_load_round_trip(gpu, table_mode, tmp_path):
         f.write("# eigen_amd tuning table v2 arch=gfx942\n0 0 0 0 64 64 64 3\n")
     with pytest.raises(E.EigenAmdError):
         E.tuner_load(other)
+    # and one made for another kernel set
+    stamp = text[0].split("kernels=")[1].split()[0]
+    stale = str(tmp_path / "stale.txt")
+    with open(stale, "w") as f:
+        f.write(f"# eigen_amd tuning table v2 arch=gfx950 kernels={int(stamp) + 1}\n0 0 0 0 64 64 64 3\n")
+    with pytest.raises(E.EigenAmdError):
+        E.tuner_load(stale)
     assert E.plan(E.F32, 0, 0, 64, 64, 64) == 7
