@@ -185,6 +185,11 @@ def emit_inc(params, path: str, source: str, tuned=()) -> None:
         f.write(f"static const int kNumTunedShapes = {len(tuned)};\n")
 
 
+def _device_arch() -> str:
+    import torch
+    return torch.cuda.get_device_properties(0).gcnArchName.split(":")[0] if torch.cuda.is_available() else "unknown"
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--db")
@@ -205,6 +210,10 @@ def main(argv=None) -> int:
     if not args.db or not args.out:
         ap.error("--db and --out are required for training")
     sel, res = train_and_evaluate(args.db, args.epochs, args.lr, args.wd)
+    # what the network's outputs mean: the device it was timed on and the
+    # (tile_m, tile_n, split_k) of each configuration it chooses among
+    from . import config_info, num_configs
+    res = {"arch": _device_arch(), "configs": [list(config_info(c)) for c in range(num_configs())], **res}
     res["params"] = [float(v) for v in sel.params()]
     with open(args.out, "w") as f:
         json.dump(res, f)

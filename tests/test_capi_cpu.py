@@ -116,6 +116,10 @@ def test_compiled_in_selector_is_the_trained_network():
     path = os.path.join(os.path.dirname(E.__file__), "data", "selector_gfx950.json")
     res = json.load(open(path))
     assert res["mlp_regret"]["mean"] < res["heuristic_regret"]["mean"]
+    # the network chooses among the library's configurations as they were when
+    # it was trained (a changed configuration table needs a retrained network)
+    assert res["arch"] == "gfx950"
+    assert [tuple(c) for c in res["configs"]] == [E.config_info(c) for c in range(E.num_configs())]
     P = np.array(res["params"], dtype=np.float64)
     assert E.lib().eh_get_tune_mode() == E.TUNE_MLP  # the default when a network is compiled in
     shapes, _ = load_db(os.path.join(os.path.dirname(E.__file__), "data", "tune_gfx950.csv"))
