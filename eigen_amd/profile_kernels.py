@@ -179,25 +179,32 @@ XCDS, SIMDS = 8, 1024  # MI355X: GRBM_GUI_ACTIVE is summed over the 8 XCDs; 256 
 def utilisation(d: str, workload: str):
     """Clock and MFMA utilisation of the dominant kernel from one UTIL_COUNTERS pass:
     clock = GRBM_GUI_ACTIVE / XCDS / duration, MFMA busy = SQ_VALU_MFMA_BUSY_CYCLES /
-    (GRBM_GUI_ACTIVE / XCDS * SIMDS), and the wave-cycle fractions spent waiting."""
+    (GRBM_GUI_ACTIVE / XCDS * SIMDS), and the wave-cycle fractions spent waiting.
+    Counters and durations are summed over the dispatches before dividing, so
+    a workload of many launches is weighted by time (the mean of per-dispatch
+    ratios would let the shortest launches, whose counters barely resolve,
+    count as much as the longest)."""
     rows = _one(d, "*counter_collection.csv") or []
     per, dur = {}, {}
     for r in rows:
         if DOMINANT[workload] not in r["Kernel_Name"]:
             continue
-        per.setdefault(r["Dispatch_Id"], {})[r["Counter_Name"]] = float(r["Counter_Value"])
+        c = per.setdefault(r["Dispatch_Id"], {})
+        c[r["Counter_Name"]] = c.get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
         dur[r["Dispatch_Id"]] = int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
     if not per:
         return None
-    out = []
-    for k, v in per.items():
-        gui = v["GRBM_GUI_ACTIVE"] / XCDS
-        out.append({"duration_us": dur[k] / 1e3, "clock_mhz": gui / dur[k] * 1e3,
-                    "mfma_busy": v["SQ_VALU_MFMA_BUSY_CYCLES"] / (gui * SIMDS),
-                    "wait_any": v["SQ_WAIT_ANY"] / v["SQ_WAVE_CYCLES"],
-                    "wait_inst_any": v["SQ_WAIT_INST_ANY"] / v["SQ_WAVE_CYCLES"],
-                    "active_inst_any": v["SQ_ACTIVE_INST_ANY"] / v["SQ_WAVE_CYCLES"]})
-    return {key: sum(o[key] for o in out) / len(out) for key in out[0]}
+    tot = {}
+    for v in per.values():
+        for k, x in v.items():
+            tot[k] = tot.get(k, 0.0) + x
+    ns = sum(dur.values())
+    gui = tot["GRBM_GUI_ACTIVE"] / XCDS
+    return {"duration_us": ns / 1e3 / len(per), "dispatches": len(per), "clock_mhz": gui / ns * 1e3,
+            "mfma_busy": tot["SQ_VALU_MFMA_BUSY_CYCLES"] / (gui * SIMDS),
+            "wait_any": tot["SQ_WAIT_ANY"] / tot["SQ_WAVE_CYCLES"],
+            "wait_inst_any": tot["SQ_WAIT_INST_ANY"] / tot["SQ_WAVE_CYCLES"],
+            "active_inst_any": tot["SQ_ACTIVE_INST_ANY"] / tot["SQ_WAVE_CYCLES"]}
 
 
 def _dominant(values: dict, workload: str):
@@ -252,11 +259,11 @@ def summarize(root: str, skip: int = 0) -> dict:
         u = utilisation(d, w)
         if u:
             print(f"## Clock and MFMA utilisation: {w}\n")
-            print("| duration (us) | clock (MHz) | MFMA busy | waves waiting (waitcnt/barrier) | waves waiting to issue | "
-                  "waves issuing |")
-            print("|---|---|---|---|---|---|")
-            print(f"| {u['duration_us']:.0f} | {u['clock_mhz']:.0f} | {u['mfma_busy']:.1%} | {u['wait_any']:.1%} | "
-                  f"{u['wait_inst_any']:.1%} | {u['active_inst_any']:.1%} |\n")
+            print("| dispatches | mean duration (us) | clock (MHz) | MFMA busy | waves waiting (waitcnt/barrier) | "
+                  "waves waiting to issue | waves issuing |")
+            print("|---|---|---|---|---|---|---|")
+            print(f"| {u['dispatches']} | {u['duration_us']:.0f} | {u['clock_mhz']:.0f} | {u['mfma_busy']:.1%} | "
+                  f"{u['wait_any']:.1%} | {u['wait_inst_any']:.1%} | {u['active_inst_any']:.1%} |\n")
     return traffic
 
 
