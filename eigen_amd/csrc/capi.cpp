@@ -87,25 +87,33 @@ int heuristic_config(int dtype, int64_t m, int64_t n, int64_t k) {
 // The split-BF16 FP32 products and the bf16 / half input products run the
 // packed kernel (launch_gemm_split_packed), whose configurations are
 // tile * kNumSplits + split index like the plain ones, over its own tiles:
-// split terms 0 = 128x128, 1 = 256x128 (4 waves), 2 = 256x128 (8 waves);
-// one term 0 = 128x128, 1 = 256x256.
+// split terms 0 = 128x128, 1 = 256x128 (4 waves), 2 = 256x128 (8 waves),
+// 3 = 256x256 (8 waves, 16-deep stages); one term 0 = 128x128, 1 = 256x256.
 static bool packed_one_term(int kind) { return kind == EH_PACKED_BF16 || kind == EH_PACKED_F16; }
 
-int packed_num_configs(int kind) { return (packed_one_term(kind) ? 2 : 3) * kNumSplits; }
+int packed_num_configs(int kind) { return (packed_one_term(kind) ? 2 : 4) * kNumSplits; }
 
 bool packed_kind_valid(int kind) {
   return kind == EH_PACKED_SPLIT2 || kind == EH_PACKED_SPLIT3 || kind == EH_PACKED_BF16 || kind == EH_PACKED_F16;
 }
 
-// The fixed rule: the large tile (256x256 for one term, 256x128 with 8
-// waves for the split terms) once it gives one wave of workgroups over the
-// 256 CUs, 128x128 below; then split K while the grid stays under one wave
-// and each split keeps at least 512 of depth.
+// The fixed rule: the largest tile that gives one wave of workgroups over
+// the 256 CUs (256x256, then for the split terms 256x128 with 8 waves),
+// 128x128 below; then split K while the grid stays under one wave and each
+// split keeps at least 512 of depth.
 int packed_rule_config(int kind, int64_t m, int64_t n, int64_t k) {
   const bool one = packed_one_term(kind);
-  const int64_t t_large = ((m + 255) / 256) * ((n + (one ? 255 : 127)) / (one ? 256 : 128));
-  const int tile = t_large >= 256 ? (one ? 1 : 2) : 0;
-  const int64_t tiles = tile ? t_large : ((m + 127) / 128) * ((n + 127) / 128);
+  const int64_t t256 = ((m + 255) / 256) * ((n + 255) / 256);
+  const int64_t t256x128 = ((m + 255) / 256) * ((n + 127) / 128);
+  int tile = 0;
+  int64_t tiles = ((m + 127) / 128) * ((n + 127) / 128);
+  if (t256 >= 256) {
+    tile = one ? 1 : 3;
+    tiles = t256;
+  } else if (!one && t256x128 >= 256) {
+    tile = 2;
+    tiles = t256x128;
+  }
   int s = 1;
   while (s < 8 && tiles * s < 256 && k / (2 * s) >= 512) s *= 2;
   return tile * kNumSplits + split_index(s);

@@ -127,6 +127,17 @@ __global__ void __launch_bounds__(256) pack_split_kernel(const float* __restrict
 }
 
 __device__ inline void wait_vmcnt0() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+
+// Barrier between LDS stages that leaves later stages' direct loads in
+// flight.  __syncthreads' release fence waits for every outstanding vector
+// memory operation (vmcnt(0)), including the direct-to-LDS loads of stages
+// still ahead, which cuts the pipeline to one stage in flight.  Here each
+// wave has already waited (vmcnt) for its own loads of the stage about to be
+// read, so after the barrier every wave's part of that stage is in LDS; its
+// own LDS reads of the previous stage are complete (lgkmcnt(0)) before it
+// arrives, so the buffer refilled next is free.  The asm's memory clobber
+// keeps the compiler from moving LDS reads or loads across it.
+__device__ inline void stage_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 template <int N>
 __device__ inline void wait_vmcnt() {
   // gfx9 encoding: vmcnt[3:0] | expcnt[6:4] | lgkmcnt[11:8] | vmcnt[5:4] << 14
@@ -135,24 +146,38 @@ __device__ inline void wait_vmcnt() {
 }
 
 // ---------------------------------------------------------------------------
-// Product over packed planes.  Each stage of LDS holds one 32-deep k slice:
-// TERMS planes of [BM rows of A | BN rows of B] x 64 B, XOR-swizzled by
-// 16-byte chunk.  Stages are filled by global_load_lds_dwordx4: every wave
-// instruction moves one 1 KB block (16 rows) straight from HBM into LDS,
+// Product over packed planes.  Each stage of LDS holds KD of k (a whole
+// 32-deep packed slice, or half of one): TERMS planes of [BM rows of A | BN
+// rows of B] x 2*KD bytes.  Stages are filled by global_load_lds_dwordx4:
+// every wave instruction moves one 1 KB block straight from HBM into LDS,
 // each lane fetching the global chunk that belongs at its LDS position.
-// NSTAGE - 1 slices are in flight while one is consumed.
+// NSTAGE - 1 stages are in flight while one is consumed.
+//   KD = 32: 64-byte rows stored in pairs under swz8 (lds_pair_offset).
+//   KD = 16: per 16 plane rows and 16-byte chunk h, one 256-byte run of the
+//            16 rows' chunks in row order (lds_half_offset), so the 8 lanes
+//            a ds_read_b128 serves together read 128 contiguous bytes.
+// Half stages let a 256x256 tile with three terms keep three stages in LDS
+// (48 KB each): half the L2-to-LDS bytes and 3/4 of the LDS reads per MFMA
+// of the 256x128 tile.
 // ---------------------------------------------------------------------------
 // F16: the planes hold IEEE binary16 operands (one term, no split) and the
 // product runs on v_mfma_f32_32x32x16_f16 instead of the BF16 instruction.
-template <int BM, int BN, int WGM, int WGN, int TERMS, int NSTAGE, bool F16 = false>
+__device__ __host__ inline int lds_half_offset(int row, int h) { return ((row >> 4) * 2 + h) * 256 + (row & 15) * 16; }
+
+template <int BM, int BN, int WGM, int WGN, int TERMS, int NSTAGE, bool F16 = false, int KD = 32>
 __global__ void __launch_bounds__(64 * WGM * WGN) gemm_split_packed_kernel(SplitParams p) {
+  static_assert(KD == 32 || KD == 16, "stage depth");
   constexpr int NW = WGM * WGN;
-  constexpr int ROWB = 64;
+  constexpr int ROWB = 64;                   // bytes per row of a packed 32-deep slice
+  constexpr int RB = 2 * KD;                 // LDS bytes per plane row in a stage
+  constexpr int SPS = 32 / KD;               // stages per packed slice
+  constexpr int RPB = 1024 / RB;             // plane rows per 1 KB block
   constexpr int PROWS = BM + BN;             // rows of one plane in a stage
-  constexpr int PLANE = PROWS * ROWB;
+  constexpr int PLANE = PROWS * RB;
   constexpr int STAGE = TERMS * PLANE;
-  constexpr int BLOCKS = TERMS * PROWS / 16;  // 1 KB blocks per stage
-  constexpr int LPW = BLOCKS / NW;            // direct loads per wave per stage
+  constexpr int BLOCKS = STAGE / 1024;       // 1 KB blocks per stage
+  constexpr int LPW = BLOCKS / NW;           // direct loads per wave per stage
+  static_assert(PROWS % RPB == 0, "blocks must not straddle planes");
   constexpr int WTM = BM / WGM, WTN = BN / WGN;
   constexpr int RM = WTM / 32, RN = WTN / 32;
   static_assert(BLOCKS % NW == 0, "blocks must divide over waves");
@@ -171,15 +196,18 @@ __global__ void __launch_bounds__(64 * WGM * WGN) gemm_split_packed_kernel(Split
   const int m0 = tm * BM, n0 = tn * BN;
   const int z = blockIdx.y;
   const int ks0 = z * p.ks_per_split;
-  const int nkt = min(p.KS, ks0 + p.ks_per_split) - ks0;
+  const int nkt = SPS * (min(p.KS, ks0 + p.ks_per_split) - ks0);  // stages
 
-  // Direct loads: block b of a stage covers 16 plane rows [blk0, blk0 + 16)
-  // of one plane, i.e. 8 LDS rows of 128 bytes (plane rows 2R and 2R + 1
-  // side by side, see lds_pair_offset); lane l fills LDS slot l of the block:
-  // LDS row blk0/2 + l/8, physical chunk l&7, which holds logical chunk
-  // V = (l&7) ^ swizzle(row), i.e. plane row 2R + V/4, 16-byte chunk V&3.
+  // Direct loads: block b of a stage covers RPB plane rows [blk0, blk0 + RPB)
+  // of one plane, and lane l fills LDS slot l of the block.
+  //   KD = 32: 8 LDS rows of 128 bytes (plane rows 2R and 2R + 1 side by
+  //   side, see lds_pair_offset): LDS row blk0/2 + l/8, physical chunk l&7,
+  //   which holds logical chunk V = (l&7) ^ swizzle(row), i.e. plane row
+  //   2R + V/4, 16-byte chunk V&3.
+  //   KD = 16: four 256-byte runs (lds_half_offset): plane row
+  //   blk0 + 16*(l/32) + (l&15), chunk (l/16)&1 of the stage's half slice.
   // The per-lane global offset is computed per block once; the block bases
-  // are wave-uniform and advance by one slice per stage.
+  // are wave-uniform and advance by one slice per SPS stages.
   const unsigned char* srcA = reinterpret_cast<const unsigned char*>(p.Ap);
   const unsigned char* srcB = reinterpret_cast<const unsigned char*>(p.Bp);
   const int64_t sliceA = (int64_t)TERMS * p.Mp * ROWB, sliceB = (int64_t)TERMS * p.Np * ROWB;
@@ -189,11 +217,15 @@ __global__ void __launch_bounds__(64 * WGM * WGN) gemm_split_packed_kernel(Split
 #pragma unroll
   for (int i = 0; i < LPW; ++i) {
     const int b = wid + NW * i;                // block index within the stage (wave-uniform)
-    const int t = b / (PROWS / 16);            // plane
-    const int blk0 = (b % (PROWS / 16)) * 16;  // first plane row of the block
-    const int R = (blk0 >> 1) + (lane >> 3);   // LDS row (pair of plane rows)
-    const int V = swz8(R, lane & 7);           // logical chunk stored at this LDS slot
-    loff[i] = (uint32_t)((2 * (lane >> 3) + (V >> 2)) * ROWB + (V & 3) * 16);
+    const int t = b / (PROWS / RPB);             // plane
+    const int blk0 = (b % (PROWS / RPB)) * RPB;  // first plane row of the block
+    if constexpr (KD == 32) {
+      const int R = (blk0 >> 1) + (lane >> 3);   // LDS row (pair of plane rows)
+      const int V = swz8(R, lane & 7);           // logical chunk stored at this LDS slot
+      loff[i] = (uint32_t)((2 * (lane >> 3) + (V >> 2)) * ROWB + (V & 3) * 16);
+    } else {
+      loff[i] = (uint32_t)((16 * (lane >> 5) + (lane & 15)) * ROWB + ((lane >> 4) & 1) * 16);
+    }
     if (blk0 < BM) {
       gbase[i] = srcA + (int64_t)ks0 * sliceA + ((int64_t)t * p.Mp + m0 + blk0) * ROWB;
       gstride[i] = sliceA;
@@ -208,7 +240,8 @@ __global__ void __launch_bounds__(64 * WGM * WGN) gemm_split_packed_kernel(Split
       const int b = wid + NW * i;
       __attribute__((address_space(3))) void* l =
           (__attribute__((address_space(3))) void*)(smem + buf * STAGE + b * 1024);
-      __builtin_amdgcn_global_load_lds((const void*)(gbase[i] + kt * gstride[i] + loff[i]), l, 16, 0, 0);
+      const int64_t g = (SPS == 1 ? kt : kt / SPS) * gstride[i] + (SPS == 1 ? 0 : (kt % SPS) * 32);
+      __builtin_amdgcn_global_load_lds((const void*)(gbase[i] + g + loff[i]), l, 16, 0, 0);
     }
   };
 
@@ -236,28 +269,29 @@ __global__ void __launch_bounds__(64 * WGM * WGN) gemm_split_packed_kernel(Split
     } else {
       wait_vmcnt0();
     }
-    __syncthreads();
+    stage_barrier();
     if (kt + NSTAGE - 1 < nkt) issue(kt + NSTAGE - 1, (kt + NSTAGE - 1) % NSTAGE);
     const unsigned char* base = smem + (kt % NSTAGE) * STAGE;
     // Fragments travel as 32-bit integer vectors (bf16 vectors crossing
     // control flow are legalised element by element) and are bit-cast at the
     // MFMA.  (Forcing all reads of both 16-deep steps ahead of the MFMAs
     // measured slower than the compiler's interleaving.)
-    u32x4 a[2][TERMS][RM], b[2][TERMS][RN];
+    constexpr int NSTEP = KD / 16;
+    auto loc = [&](int r, int s) { return KD == 32 ? lds_pair_offset(r, 2 * s + lg) : lds_half_offset(r, lg); };
+    u32x4 a[NSTEP][TERMS][RM], b[NSTEP][TERMS][RN];
 #pragma unroll
-    for (int s = 0; s < 2; ++s) {
-      const int c = 2 * s + lg;
+    for (int s = 0; s < NSTEP; ++s) {
 #pragma unroll
       for (int t = 0; t < TERMS; ++t) {
 #pragma unroll
         for (int i = 0; i < RM; ++i) {
           const int r = wm * WTM + i * 32 + li;
-          a[s][t][i] = *reinterpret_cast<const u32x4*>(base + t * PLANE + lds_pair_offset(r, c));
+          a[s][t][i] = *reinterpret_cast<const u32x4*>(base + t * PLANE + loc(r, s));
         }
 #pragma unroll
         for (int j = 0; j < RN; ++j) {
           const int r = BM + wn * WTN + j * 32 + li;
-          b[s][t][j] = *reinterpret_cast<const u32x4*>(base + t * PLANE + lds_pair_offset(r, c));
+          b[s][t][j] = *reinterpret_cast<const u32x4*>(base + t * PLANE + loc(r, s));
         }
       }
     }
@@ -268,7 +302,7 @@ __global__ void __launch_bounds__(64 * WGM * WGN) gemm_split_packed_kernel(Split
     constexpr int PA[6] = {TERMS == 3 ? 2 : (TERMS == 2 ? 1 : 0), 0, TERMS == 3 ? 1 : 0, 1, 0, 0};
     constexpr int PB[6] = {0, TERMS == 3 ? 2 : (TERMS == 2 ? 1 : 0), TERMS == 3 ? 1 : 0, 0, 1, 0};
 #pragma unroll
-    for (int s = 0; s < 2; ++s)
+    for (int s = 0; s < NSTEP; ++s)
 #pragma unroll
       for (int pq = 0; pq < NPROD; ++pq)
 #pragma unroll
@@ -428,10 +462,10 @@ hipError_t launch_pack_split(const float* X, int64_t ld, bool kcont, int rows, i
   return hipErrorInvalidValue;
 }
 
-template <int BM, int BN, int WGM, int WGN, int TERMS, int NSTAGE, bool F16 = false>
+template <int BM, int BN, int WGM, int WGN, int TERMS, int NSTAGE, bool F16 = false, int KD = 32>
 static hipError_t launch_packed_t(const SplitParams& p, int splits, hipStream_t s) {
   const int ntm = (p.M + BM - 1) / BM, ntn = (p.N + BN - 1) / BN;
-  hipLaunchKernelGGL((gemm_split_packed_kernel<BM, BN, WGM, WGN, TERMS, NSTAGE, F16>), dim3(ntm * ntn, splits),
+  hipLaunchKernelGGL((gemm_split_packed_kernel<BM, BN, WGM, WGN, TERMS, NSTAGE, F16, KD>), dim3(ntm * ntn, splits),
                      dim3(64 * WGM * WGN), 0, s, p);
   return hipGetLastError();
 }
@@ -443,15 +477,18 @@ hipError_t launch_gemm_split_packed(const SplitParams& p, int terms, int tile, i
     return launch_packed_t<256, 256, 2, 4, 1, 2, true>(p, splits, s);
   }
   // tile 0: 128x128, 4 waves (3 stages); 1: 256x128, 4 waves of 128x64 (2 stages);
-  // 2: 256x128, 8 waves of 64x64 -- two waves per SIMD (2 stages)
+  // 2: 256x128, 8 waves of 64x64 -- two waves per SIMD (2 stages);
+  // 3: 256x256, 8 waves of 128x64, 16-deep half-slice stages (3 stages)
   if (terms == 3) {
     if (tile == 0) return launch_packed_t<128, 128, 2, 2, 3, 3>(p, splits, s);
     if (tile == 2) return launch_packed_t<256, 128, 4, 2, 3, 2>(p, splits, s);
+    if (tile == 3) return launch_packed_t<256, 256, 2, 4, 3, 3, false, 16>(p, splits, s);
     return launch_packed_t<256, 128, 2, 2, 3, 2>(p, splits, s);
   }
   if (terms == 2) {
     if (tile == 0) return launch_packed_t<128, 128, 2, 2, 2, 3>(p, splits, s);
     if (tile == 2) return launch_packed_t<256, 128, 4, 2, 2, 2>(p, splits, s);
+    if (tile == 3) return launch_packed_t<256, 256, 2, 4, 2, 4, false, 16>(p, splits, s);
     return launch_packed_t<256, 128, 2, 2, 2, 2>(p, splits, s);
   }
   if (terms == 1) {

@@ -33,7 +33,7 @@ constexpr int kPlainKind = 0;
 // Version of the configuration sets the table indexes (the plain kernels'
 // 24, the packed kernels' tiles and splits): a table saved under another
 // version names other kernels and is refused.
-constexpr int kKernelSetVersion = 2;
+constexpr int kKernelSetVersion = 3;
 using ShapeKey = std::tuple<int, int, int, int, int64_t, int64_t, int64_t>;
 static std::mutex g_tune_mutex;
 static std::map<ShapeKey, int> g_table;
@@ -210,7 +210,7 @@ eh_status eh_packed_config_info(eh_packed_kind kind, int config, int* tile_m, in
   config_decode(config, &tile, &s);
   const bool one = kind == EH_PACKED_BF16 || kind == EH_PACKED_F16;
   if (tile_m) *tile_m = tile == 0 ? 128 : 256;
-  if (tile_n) *tile_n = tile == 0 ? 128 : (one ? 256 : 128);
+  if (tile_n) *tile_n = tile == 0 ? 128 : (one || tile == 3 ? 256 : 128);
   if (split_k) *split_k = s;
   return EH_SUCCESS;
 }

@@ -73,17 +73,20 @@ def test_packed_configs_and_rule():
     rule: the large tile once it fills the chip (256 workgroups), else
     128x128, then split K while the grid stays under one wave and each split
     keeps 512 of depth."""
-    assert E.num_packed_configs(E.PACKED_SPLIT2) == E.num_packed_configs(E.PACKED_SPLIT3) == 12
+    assert E.num_packed_configs(E.PACKED_SPLIT2) == E.num_packed_configs(E.PACKED_SPLIT3) == 16
     assert E.num_packed_configs(E.PACKED_BF16) == E.num_packed_configs(E.PACKED_F16) == 8
     assert E.num_packed_configs(7) == 0
     info = lambda kind, c: tuple(_packed_info(kind, c))
     assert info(E.PACKED_SPLIT3, 0) == (128, 128, 1) and info(E.PACKED_SPLIT3, 8) == (256, 128, 1)
+    assert info(E.PACKED_SPLIT2, 13) == (256, 256, 2)
     assert info(E.PACKED_BF16, 4) == (256, 256, 1) and info(E.PACKED_F16, 3) == (128, 128, 8)
     prev = E.lib().eh_get_tune_mode()
     E.set_tune_mode(E.TUNE_HEURISTIC)
     try:
-        # 8192^2: 256 tiles of 256x128 fill the chip -> tile 2, no split
-        assert E.plan_packed(E.PACKED_SPLIT3, 0, 0, 8192, 8192, 8192) == 8
+        # 8192^2: 1024 tiles of 256x256 fill the chip -> tile 3, no split;
+        # 4096 x 2048: 128 of 256x256 but 256 of 256x128 -> tile 2
+        assert E.plan_packed(E.PACKED_SPLIT3, 0, 0, 8192, 8192, 8192) == 12
+        assert E.plan_packed(E.PACKED_SPLIT3, 0, 0, 4096, 2048, 4096) == 8
         # 1024^2 x 8192: 64 tiles of 128x128 -> split 4 (64*4 = 256)
         assert E.plan_packed(E.PACKED_SPLIT2, 0, 0, 1024, 1024, 8192) == 2
         # bf16 8192^3: 1024 tiles of 256x256 -> tile 1

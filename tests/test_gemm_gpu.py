@@ -184,8 +184,8 @@ def test_device_split_matches_oracle(gpu):
 
 # packed configurations of the split path (tile * 4 + split index): planner
 # default, 128x128 (3 stages), 256x128 with 8 waves, 256x128 with 4 waves,
-# and that with split K 2
-SPLIT_CONFIGS = [None, 0, 8, 4, 5]
+# and that with split K 2, 256x256 with 16-deep stages, and that with split K 2
+SPLIT_CONFIGS = [None, 0, 8, 4, 5, 12, 13]
 
 
 _SPLIT_CASES = {}

@@ -56,7 +56,7 @@ def test_time_config_every_packed_config(gpu, kind):
     dC = to_dev(np.full((m, n), np.nan, np.float32))
     want = A.astype(np.float64) @ B.astype(np.float64)
     nc = E.num_packed_configs(kind)
-    assert nc == (8 if kind in (E.PACKED_BF16, E.PACKED_F16) else 12)
+    assert nc == (8 if kind in (E.PACKED_BF16, E.PACKED_F16) else 16)
     for c in range(nc):
         ms = E.time_config_packed(kind, 0, 0, m, n, k, dA, dB, dC, c, reps=2)
         assert 0.0 < ms < 1000.0, (kind, c, ms)
