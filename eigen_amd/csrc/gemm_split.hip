@@ -135,9 +135,16 @@ __device__ inline void wait_vmcnt0() { asm volatile("s_waitcnt vmcnt(0)" ::: "me
 // wave has already waited (vmcnt) for its own loads of the stage about to be
 // read, so after the barrier every wave's part of that stage is in LDS; its
 // own LDS reads of the previous stage are complete (lgkmcnt(0)) before it
-// arrives, so the buffer refilled next is free.  The asm's memory clobber
-// keeps the compiler from moving LDS reads or loads across it.
-__device__ inline void stage_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+// arrives, so the buffer refilled next is free.  The empty asm statements'
+// memory clobbers keep the compiler from moving LDS reads or loads across
+// it; the wait is the builtin so that the compiler's own wait insertion
+// sees it.
+__device__ inline void stage_barrier() {
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0); vmcnt, expcnt unconstrained
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+}
 template <int N>
 __device__ inline void wait_vmcnt() {
   // gfx9 encoding: vmcnt[3:0] | expcnt[6:4] | lgkmcnt[11:8] | vmcnt[5:4] << 14
@@ -163,6 +170,20 @@ __device__ inline void wait_vmcnt() {
 // F16: the planes hold IEEE binary16 operands (one term, no split) and the
 // product runs on v_mfma_f32_32x32x16_f16 instead of the BF16 instruction.
 __device__ __host__ inline int lds_half_offset(int row, int h) { return ((row >> 4) * 2 + h) * 256 + (row & 15) * 16; }
+
+// Wait until at most `ahead` stages of LPW direct loads each are still in
+// flight (capped at A, the most a loop ever has ahead of the stage it reads).
+template <int A, int LPW>
+__device__ inline void wait_stages_behind(int ahead) {
+  if constexpr (A <= 0) {
+    wait_vmcnt0();
+  } else {
+    if (ahead >= A)
+      wait_vmcnt<A * LPW>();
+    else
+      wait_stages_behind<A - 1, LPW>(ahead);
+  }
+}
 
 template <int BM, int BN, int WGM, int WGN, int TERMS, int NSTAGE, bool F16 = false, int KD = 32>
 __global__ void __launch_bounds__(64 * WGM * WGN) gemm_split_packed_kernel(SplitParams p) {
@@ -256,68 +277,73 @@ __global__ void __launch_bounds__(64 * WGM * WGN) gemm_split_packed_kernel(Split
   const int li = lane & 31;
   const int lg = lane >> 5;
 
+  // Fragments travel as 32-bit integer vectors (bf16 vectors crossing
+  // control flow are legalised element by element) and are bit-cast at the
+  // MFMA.
+  constexpr int NSTEP = KD / 16;  // 16-deep MFMA steps per stage
+  auto read_step = [&](const unsigned char* base, int s, u32x4(&fa)[TERMS][RM], u32x4(&fb)[TERMS][RN]) {
 #pragma unroll
-  for (int s = 0; s < NSTAGE - 1; ++s)
-    if (s < nkt) issue(s, s);
-
-  for (int kt = 0; kt < nkt; ++kt) {
-    if constexpr (NSTAGE > 2) {
-      if (kt + 1 < nkt)
-        wait_vmcnt<(NSTAGE - 2) * LPW>();
-      else
-        wait_vmcnt0();
-    } else {
-      wait_vmcnt0();
-    }
-    stage_barrier();
-    if (kt + NSTAGE - 1 < nkt) issue(kt + NSTAGE - 1, (kt + NSTAGE - 1) % NSTAGE);
-    const unsigned char* base = smem + (kt % NSTAGE) * STAGE;
-    // Fragments travel as 32-bit integer vectors (bf16 vectors crossing
-    // control flow are legalised element by element) and are bit-cast at the
-    // MFMA.  (Forcing all reads of both 16-deep steps ahead of the MFMAs
-    // measured slower than the compiler's interleaving.)
-    constexpr int NSTEP = KD / 16;
-    auto loc = [&](int r, int s) { return KD == 32 ? lds_pair_offset(r, 2 * s + lg) : lds_half_offset(r, lg); };
-    u32x4 a[NSTEP][TERMS][RM], b[NSTEP][TERMS][RN];
+    for (int t = 0; t < TERMS; ++t) {
 #pragma unroll
-    for (int s = 0; s < NSTEP; ++s) {
+      for (int i = 0; i < RM; ++i) {
+        const int r = wm * WTM + i * 32 + li;
+        const int off = KD == 32 ? lds_pair_offset(r, 2 * s + lg) : lds_half_offset(r, lg);
+        fa[t][i] = *reinterpret_cast<const u32x4*>(base + t * PLANE + off);
+      }
 #pragma unroll
-      for (int t = 0; t < TERMS; ++t) {
-#pragma unroll
-        for (int i = 0; i < RM; ++i) {
-          const int r = wm * WTM + i * 32 + li;
-          a[s][t][i] = *reinterpret_cast<const u32x4*>(base + t * PLANE + loc(r, s));
-        }
-#pragma unroll
-        for (int j = 0; j < RN; ++j) {
-          const int r = BM + wn * WTN + j * 32 + li;
-          b[s][t][j] = *reinterpret_cast<const u32x4*>(base + t * PLANE + loc(r, s));
-        }
+      for (int j = 0; j < RN; ++j) {
+        const int r = BM + wn * WTN + j * 32 + li;
+        const int off = KD == 32 ? lds_pair_offset(r, 2 * s + lg) : lds_half_offset(r, lg);
+        fb[t][j] = *reinterpret_cast<const u32x4*>(base + t * PLANE + off);
       }
     }
-    // cross products with p + q < TERMS, smallest first; consecutive MFMAs go
-    // to different accumulators
-    // (one term: the single product of bf16 inputs)
+  };
+  // cross products with p + q < TERMS, smallest first; consecutive MFMAs go
+  // to different accumulators (one term: the single product of bf16 inputs)
+  auto mma_step = [&](const u32x4(&fa)[TERMS][RM], const u32x4(&fb)[TERMS][RN]) {
     constexpr int NPROD = TERMS == 3 ? 6 : (TERMS == 2 ? 3 : 1);
     constexpr int PA[6] = {TERMS == 3 ? 2 : (TERMS == 2 ? 1 : 0), 0, TERMS == 3 ? 1 : 0, 1, 0, 0};
     constexpr int PB[6] = {0, TERMS == 3 ? 2 : (TERMS == 2 ? 1 : 0), TERMS == 3 ? 1 : 0, 0, 1, 0};
 #pragma unroll
-    for (int s = 0; s < NSTEP; ++s)
+    for (int pq = 0; pq < NPROD; ++pq)
 #pragma unroll
-      for (int pq = 0; pq < NPROD; ++pq)
+      for (int i = 0; i < RM; ++i)
 #pragma unroll
-        for (int i = 0; i < RM; ++i)
+        for (int j = 0; j < RN; ++j) {
+          if constexpr (F16)
+            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(f16x8, fa[PA[pq]][i]),
+                                                               __builtin_bit_cast(f16x8, fb[PB[pq]][j]), acc[i][j], 0,
+                                                               0, 0);
+          else
+            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, fa[PA[pq]][i]),
+                                                                __builtin_bit_cast(bf16x8, fb[PB[pq]][j]), acc[i][j],
+                                                                0, 0, 0);
+        }
+  };
+
+  {
+    // Per stage: wait for it, barrier, refill the buffer freed a stage ago,
+    // read all its fragments, multiply.  (Forcing all reads of both 16-deep
+    // steps ahead of the MFMAs measured slower than the compiler's
+    // interleaving, and so did reading the next step's fragments into a
+    // second register set during the MFMAs: within 2% of this loop for the
+    // bf16 256x256 tile, and no room in registers for the split tile.)
 #pragma unroll
-          for (int j = 0; j < RN; ++j) {
-            if constexpr (F16)
-              acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(f16x8, a[s][PA[pq]][i]),
-                                                                 __builtin_bit_cast(f16x8, b[s][PB[pq]][j]),
-                                                                 acc[i][j], 0, 0, 0);
-            else
-              acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a[s][PA[pq]][i]),
-                                                                  __builtin_bit_cast(bf16x8, b[s][PB[pq]][j]),
-                                                                  acc[i][j], 0, 0, 0);
-          }
+    for (int s = 0; s < NSTAGE - 1; ++s)
+      if (s < nkt) issue(s, s);
+    for (int kt = 0; kt < nkt; ++kt) {
+      // stage kt must have landed; the stages issued after it (at most
+      // NSTAGE - 2, fewer near the end) may still be in flight
+      wait_stages_behind<NSTAGE - 2, LPW>(nkt - 1 - kt);
+      stage_barrier();
+      if (kt + NSTAGE - 1 < nkt) issue(kt + NSTAGE - 1, (kt + NSTAGE - 1) % NSTAGE);
+      const unsigned char* base = smem + (kt % NSTAGE) * STAGE;
+      u32x4 a[NSTEP][TERMS][RM], b[NSTEP][TERMS][RN];
+#pragma unroll
+      for (int s = 0; s < NSTEP; ++s) read_step(base, s, a[s], b[s]);
+#pragma unroll
+      for (int s = 0; s < NSTEP; ++s) mma_step(a[s], b[s]);
+    }
   }
 
   const bool partial = gridDim.y > 1;
