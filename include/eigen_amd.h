@@ -463,11 +463,11 @@ eh_status eh_tuner_load(const char* path);
  * EH_PREC_SPLIT2 / SPLIT3) and the 16-bit input products (eh_gemm_bf16,
  * eh_gemm_f16) -- have their own configurations: tile * 4 + split-K index
  * (splits {1, 2, 4, 8}); tiles for the split kinds 0 = 128x128, 1 = 256x128
- * (4 waves), 2 = 256x128 (8 waves); for the 16-bit kinds 0 = 128x128,
- * 1 = 256x256.  EH_TUNE_TABLE times every configuration of a new shape on
- * first use (scratch operands) and remembers the fastest; the other modes
- * use the fixed rule (large tile once it fills the chip, split K below one
- * wave of workgroups).  No counterpart in the reference: Eigen's blocking
+ * (4 waves), 2 = 256x128 (8 waves), 3 = 256x256 (8 waves, 16-deep stages);
+ * for the 16-bit kinds 0 = 128x128, 1 = 256x256.  EH_TUNE_TABLE times every
+ * configuration of a new shape on first use (scratch operands) and
+ * remembers the fastest; the other modes use the fixed rule (the largest
+ * tile that fills the chip, split K below one wave of workgroups).  No counterpart in the reference: Eigen's blocking
  * choice (GeneralBlockPanelKernel.h:201-452) has no per-kernel tuning. */
 typedef enum eh_packed_kind {
   EH_PACKED_SPLIT2 = 2,
