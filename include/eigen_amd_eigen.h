@@ -125,10 +125,25 @@ struct operand_op {
   static constexpr eh_op op = !transposed ? EH_OP_N : (Conj ? EH_OP_C : EH_OP_T);
 };
 
-// dst (=, +=, -=) lhs * rhs on the device; Lhs / Rhs are the product's
-// operand expressions (possibly transposed, scaled, negated, conjugated).
+// The arguments of one column-major BLAS product for dst (=, +=, -=)
+// lhs * rhs; Lhs / Rhs are the product's operand expressions (possibly
+// transposed, scaled, negated, conjugated).  A row-major destination is
+// computed as the transposed product into its column-major view (`flip`).
+template <typename Scalar>
+struct GemmArgs {
+  int64_t m, n, k;
+  const void* A;
+  int64_t lda;
+  eh_op opa;
+  const void* B;
+  int64_t ldb;
+  eh_op opb;
+  Scalar alpha, beta;
+  bool flip;
+};
+
 template <typename Dst, typename Lhs, typename Rhs>
-void device_product(Dst& dst, const Lhs& a_lhs, const Rhs& a_rhs, Kind kind, HipDevice& dev) {
+GemmArgs<typename Dst::Scalar> gemm_args(const Dst& dst, const Lhs& a_lhs, const Rhs& a_rhs, Kind kind) {
   using Scalar = typename Dst::Scalar;
   using Info = scalar_info<Scalar>;
   using LhsBlasTraits = Eigen::internal::blas_traits<Lhs>;
@@ -155,52 +170,99 @@ void device_product(Dst& dst, const Lhs& a_lhs, const Rhs& a_rhs, Kind kind, Hip
   eigen_assert(dst.rows() == a_lhs.rows() && dst.cols() == a_rhs.cols());
   const ActualLhs& lhs = LhsBlasTraits::extract(a_lhs);
   const ActualRhs& rhs = RhsBlasTraits::extract(a_rhs);
-  Scalar alpha = Eigen::internal::combine_scalar_factors(Scalar(1), a_lhs, a_rhs);
-  if (kind == Kind::Sub) alpha = -alpha;
-  const Scalar beta = kind == Kind::Assign ? Scalar(0) : Scalar(1);
+  GemmArgs<Scalar> g;
+  g.alpha = Eigen::internal::combine_scalar_factors(Scalar(1), a_lhs, a_rhs);
+  if (kind == Kind::Sub) g.alpha = -g.alpha;
+  g.beta = kind == Kind::Assign ? Scalar(0) : Scalar(1);
   eigen_assert(lhs.innerStride() == 1 && rhs.innerStride() == 1);
 
   // Column-major view of each operand: a row-major m x k matrix with outer
   // stride s is the column-major k x m matrix X^T with leading dimension s.
-  int64_t m = dst.rows(), n = dst.cols(), k = a_lhs.cols();
-  const void* A = lhs.data();
-  const void* B = rhs.data();
-  int64_t lda = lhs.outerStride(), ldb = rhs.outerStride();
-  eh_op opa = OpA::op, opb = OpB::op;
+  g.m = dst.rows();
+  g.n = dst.cols();
+  g.k = a_lhs.cols();
+  g.A = lhs.data();
+  g.B = rhs.data();
+  g.lda = lhs.outerStride();
+  g.ldb = rhs.outerStride();
+  g.opa = OpA::op;
+  g.opb = OpB::op;
+  g.flip = flip;
   if (flip) {  // C^T = op(B)^T op(A)^T, computed into the column-major view of C
-    std::swap(m, n);
-    std::swap(A, B);
-    std::swap(lda, ldb);
-    std::swap(opa, opb);
+    std::swap(g.m, g.n);
+    std::swap(g.A, g.B);
+    std::swap(g.lda, g.ldb);
+    std::swap(g.opa, g.opb);
   }
   // leading dimensions of empty / degenerate operands
-  if (lda < 1) lda = 1;
-  if (ldb < 1) ldb = 1;
+  if (g.lda < 1) g.lda = 1;
+  if (g.ldb < 1) g.ldb = 1;
+  return g;
+}
+
+// dst (=, +=, -=) lhs * rhs on the device.
+template <typename Dst, typename Lhs, typename Rhs>
+void device_product(Dst& dst, const Lhs& a_lhs, const Rhs& a_rhs, Kind kind, HipDevice& dev) {
+  using Scalar = typename Dst::Scalar;
+  using Info = scalar_info<Scalar>;
+  const GemmArgs<Scalar> g = gemm_args(dst, a_lhs, a_rhs, kind);
   const int64_t ldc = dst.outerStride() < 1 ? 1 : dst.outerStride();
-  if (m == 0 || n == 0) return;
+  if (g.m == 0 || g.n == 0) return;
   const eh_dtype dt = Info::dtype;
   const eh_precision prec = dt == EH_F32 ? dev.precision : EH_PREC_AUTO;
   // a destination with an inner stride keeps it in either storage order:
   // element (i, j) of the column-major view is at i*inner + j*outer
   const int64_t incc = dst.innerStride();
   if (Info::complex) {
-    dev.status = incc == 1 ? eh_gemm_complex(dt, opa, opb, m, n, k, &alpha, A, lda, B, ldb, &beta, dst.data(), ldc,
-                                             prec, dev.stream)
+    dev.status = incc == 1 ? eh_gemm_complex(dt, g.opa, g.opb, g.m, g.n, g.k, &g.alpha, g.A, g.lda, g.B, g.ldb,
+                                             &g.beta, dst.data(), ldc, prec, dev.stream)
                            : EH_ERR_UNSUPPORTED;  // no strided complex destination
   } else if (incc == 1) {
-    dev.status = eh_gemm(dt, opa, opb, m, n, k, &alpha, A, lda, B, ldb, &beta, dst.data(), ldc, prec, dev.stream);
+    dev.status = eh_gemm(dt, g.opa, g.opb, g.m, g.n, g.k, &g.alpha, g.A, g.lda, g.B, g.ldb, &g.beta, dst.data(), ldc,
+                         prec, dev.stream);
   } else {
-    dev.status = eh_gemm_strided(dt, opa, opb, m, n, k, &alpha, A, lda, B, ldb, &beta, dst.data(), incc, ldc, prec,
-                                 dev.stream);
+    dev.status = eh_gemm_strided(dt, g.opa, g.opb, g.m, g.n, g.k, &g.alpha, g.A, g.lda, g.B, g.ldb, &g.beta,
+                                 dst.data(), incc, ldc, prec, dev.stream);
   }
   if (dev.status == EH_SUCCESS && dev.synchronize) dev.status = eh_synchronize(dev.stream);
   eigen_assert(dev.status == EH_SUCCESS && "eigen_amd device product failed (see eh_last_error)");
+}
+
+// Finish a level-3 call: optional synchronisation, status kept on the device.
+inline void finish(eh_status st, HipDevice& dev) {
+  dev.status = st;
+  if (dev.status == EH_SUCCESS && dev.synchronize) dev.status = eh_synchronize(dev.stream);
+  eigen_assert(dev.status == EH_SUCCESS && "eigen_amd device level-3 call failed (see eh_last_error)");
 }
 
 template <typename T>
 struct always_false : std::false_type {};
 
 }  // namespace detail
+
+namespace detail {
+// BLAS op codes of the level-3 views: 0 = N, 1 = T, 2 = C (conjugate
+// transpose), 3 = the conjugate alone (no BLAS op; rejected where used).
+constexpr int op_transpose(int o) { return o == 0 ? 1 : o == 1 ? 0 : o == 2 ? 3 : 2; }
+constexpr int op_adjoint(int o, bool cplx) {
+  return !cplx ? op_transpose(o) : (o == 0 ? 2 : o == 2 ? 0 : o == 1 ? 3 : 1);
+}
+constexpr eh_op eh_op_of(int o) { return o == 0 ? EH_OP_N : o == 1 ? EH_OP_T : EH_OP_C; }
+constexpr eh_uplo uplo_of(int mode, bool flip) {
+  return ((mode & Eigen::Lower) != 0) != flip ? EH_LOWER : EH_UPPER;
+}
+}  // namespace detail
+
+template <typename Plain>
+class DeviceMap;
+template <typename Plain, int Mode, int Op = 0>
+class DeviceTriangularView;
+template <typename Plain, int Mode>
+class DeviceSelfAdjointView;
+template <typename View, typename P2, int Side>
+struct DeviceTriangularProduct;
+template <typename View, typename P2, int Side>
+struct DeviceSelfAdjointProduct;
 
 // An Eigen::Map of device memory whose assignments are device products.
 template <typename Plain>
@@ -261,6 +323,47 @@ class DeviceMap : public Eigen::Map<Plain, 0, Eigen::OuterStride<>> {
     return DeviceMap(p, rows, cols, this->outerStride());
   }
 
+  // Level-3 views (see DeviceTriangularView / DeviceSelfAdjointView):
+  //   A.triangularView<Lower>().solveInPlace(X)       -> eh_trsm
+  //   X = A.triangularView<Lower>() * X                -> eh_trmm (in place)
+  //   C.triangularView<Lower>() = / += / -= A * B      -> eh_gemmtr
+  //   C = / += / -= A.selfadjointView<Lower>() * B     -> eh_symm / eh_hemm
+  //   C.selfadjointView<Lower>().rankUpdate(U, a)      -> eh_syrk / eh_herk
+  //   C.selfadjointView<Lower>().rankUpdate(U, V, a)   -> eh_syr2k / eh_her2k
+  // They hide Map's own triangularView() / selfadjointView(), whose
+  // evaluation would run on the host.
+  template <unsigned int Mode>
+  DeviceTriangularView<Plain, int(Mode)> triangularView() const {
+    return DeviceTriangularView<Plain, int(Mode)>(*this);
+  }
+  template <unsigned int Mode>
+  DeviceSelfAdjointView<Plain, int(Mode)> selfadjointView() const {
+    return DeviceSelfAdjointView<Plain, int(Mode)>(*this);
+  }
+  // X = T * X, X = X * T: triangular product in place (BLAS trmm semantics:
+  // the right-hand operand must be this same matrix)
+  template <typename View, typename P2, int Side>
+  DeviceMap& operator=(const DeviceTriangularProduct<View, P2, Side>& p) {
+    p.run_into(*this);
+    return *this;
+  }
+  // C = / += / -= S * B, B * S with S self-adjoint
+  template <typename View, typename P2, int Side>
+  DeviceMap& operator=(const DeviceSelfAdjointProduct<View, P2, Side>& p) {
+    p.run_into(*this, detail::Kind::Assign);
+    return *this;
+  }
+  template <typename View, typename P2, int Side>
+  DeviceMap& operator+=(const DeviceSelfAdjointProduct<View, P2, Side>& p) {
+    p.run_into(*this, detail::Kind::Add);
+    return *this;
+  }
+  template <typename View, typename P2, int Side>
+  DeviceMap& operator-=(const DeviceSelfAdjointProduct<View, P2, Side>& p) {
+    p.run_into(*this, detail::Kind::Sub);
+    return *this;
+  }
+
   // C.noalias() = / += / -= A * B: the same device products (they never
   // evaluate into a temporary)
   struct NoAlias {
@@ -288,6 +391,346 @@ class DeviceMap : public Eigen::Map<Plain, 0, Eigen::OuterStride<>> {
     return *this;
   }
 };
+
+// ---- level-3 views on device maps -------------------------------------------
+// Each view is a handle (a DeviceMap, a scalar factor, an op) whose
+// operations are one call of the C-ABI level-3 routines, on the stream of
+// eigen_amd::default_device(); the status lands in its `status`.  They stand
+// for Eigen's TriangularView and SelfAdjointView on the same expressions,
+// whose evaluation reaches triangular_solve_matrix
+// (TriangularSolverMatrix.h:20-330), product_triangular_matrix_matrix
+// (TriangularMatrixMatrix.h:84-420), general_matrix_matrix_triangular_product
+// (GeneralMatrixMatrixTriangular.h:38-330) and product_selfadjoint_matrix
+// (SelfadjointMatrixMatrix.h:323-520) -- the routines that
+// TriangularSolverMatrix_BLAS.h, TriangularMatrixMatrix_BLAS.h,
+// GeneralMatrixMatrixTriangular_BLAS.h and SelfadjointMatrixMatrix_BLAS.h
+// redirect to ?trsm_, ?trmm_, ?syrk_ / ?gemmtr_ and ?symm_ / ?hemm_ under
+// EIGEN_USE_BLAS.  Here no device data is touched on the host.
+//
+// A row-major matrix is handled through its column-major view (the
+// transpose): its stored triangle flips and its op transposes.
+
+// op(tri(A)) scaled by a factor: Mode Lower / Upper / UnitLower / UnitUpper
+// of the DeviceMap's matrix, Op 0 = as is, 1 = transposed, 2 = adjoint.
+template <typename Plain, int Mode, int Op>
+class DeviceTriangularView {
+ public:
+  using Scalar = typename Plain::Scalar;
+  using Index = Eigen::Index;
+  static constexpr bool kComplex = Eigen::NumTraits<Scalar>::IsComplex;
+  static_assert(detail::scalar_info<Scalar>::supported,
+                "device views support float, double, std::complex<float> and std::complex<double>");
+  static_assert(Mode == Eigen::Lower || Mode == Eigen::Upper || Mode == Eigen::UnitLower || Mode == Eigen::UnitUpper,
+                "device triangular views: Lower, Upper, UnitLower or UnitUpper");
+  // the triangle as the BLAS routines see it: the column-major matrix in
+  // memory, its stored triangle and the op on it
+  static constexpr bool kRowMajor = Plain::IsRowMajor;
+  static constexpr int kStoredOp = kRowMajor ? detail::op_transpose(Op) : Op;
+  static constexpr eh_uplo kStoredUplo = detail::uplo_of(Mode, kRowMajor);
+  static constexpr eh_diag kDiag = (Mode & Eigen::UnitDiag) ? EH_UNIT : EH_NON_UNIT;
+
+  explicit DeviceTriangularView(const DeviceMap<Plain>& m, Scalar factor = Scalar(1)) : m_(m), factor_(factor) {}
+  const DeviceMap<Plain>& nestedExpression() const { return m_; }
+  Scalar factor() const { return factor_; }
+  Index rows() const { return m_.rows(); }
+  Index cols() const { return m_.cols(); }
+
+  DeviceTriangularView<Plain, Mode, detail::op_transpose(Op)> transpose() const {
+    return DeviceTriangularView<Plain, Mode, detail::op_transpose(Op)>(m_, factor_);
+  }
+  DeviceTriangularView<Plain, Mode, detail::op_adjoint(Op, kComplex)> adjoint() const {
+    return DeviceTriangularView<Plain, Mode, detail::op_adjoint(Op, kComplex)>(m_, Eigen::numext::conj(factor_));
+  }
+
+  // op(T) X = X_old (OnTheLeft) or X op(T) = X_old (OnTheRight); X
+  // overwrites the right-hand sides (eh_trsm).  A factor s solves with s*T.
+  template <int Side = Eigen::OnTheLeft, typename P2>
+  void solveInPlace(const DeviceMap<P2>& X) const {
+    static_assert(std::is_same<typename P2::Scalar, Scalar>::value, "mixed scalar types are not supported");
+    static_assert(Side == Eigen::OnTheLeft || Side == Eigen::OnTheRight, "OnTheLeft or OnTheRight");
+    // a row-major X is solved through its column-major view X^T:
+    // op(T) X = B  <=>  X^T op(T)^T = B^T
+    constexpr bool xrow = P2::IsRowMajor;
+    constexpr int op = xrow ? detail::op_transpose(kStoredOp) : kStoredOp;
+    static_assert(op != 3, "this combination of storage orders and adjoint needs a conjugate without transpose, "
+                           "which no BLAS op expresses");
+    eigen_assert(rows() == cols());
+    eigen_assert((Side == Eigen::OnTheLeft ? X.rows() : X.cols()) == rows());
+    const eh_side side = (Side == Eigen::OnTheLeft) != xrow ? EH_LEFT : EH_RIGHT;
+    const int64_t m = xrow ? X.cols() : X.rows(), n = xrow ? X.rows() : X.cols();
+    const Scalar alpha = Scalar(1) / factor_;
+    HipDevice& dev = default_device();
+    if (m == 0 || n == 0) return detail::finish(EH_SUCCESS, dev);
+    const int64_t lda = m_.outerStride() < 1 ? 1 : m_.outerStride();
+    const int64_t ldx = X.outerStride() < 1 ? 1 : X.outerStride();
+    void* x = const_cast<Scalar*>(X.data());
+    constexpr eh_dtype dt = detail::scalar_info<Scalar>::dtype;
+    detail::finish(kComplex ? eh_trsm_complex(dt, side, kStoredUplo, detail::eh_op_of(op), kDiag, m, n, &alpha,
+                                              m_.data(), lda, x, ldx, dev.stream)
+                            : eh_trsm(dt, side, kStoredUplo, detail::eh_op_of(op), kDiag, m, n, &alpha, m_.data(),
+                                      lda, x, ldx, dev.stream),
+                   dev);
+  }
+
+  // C.triangularView<Mode>() = / += / -= A * B: the product on the Mode
+  // triangle of C only (eh_gemmtr); the rest of C is not touched.
+  template <typename L, typename R, int O>
+  DeviceTriangularView& operator=(const Eigen::Product<L, R, O>& p) {
+    return assign_product(p, detail::Kind::Assign);
+  }
+  template <typename L, typename R, int O>
+  DeviceTriangularView& operator+=(const Eigen::Product<L, R, O>& p) {
+    return assign_product(p, detail::Kind::Add);
+  }
+  template <typename L, typename R, int O>
+  DeviceTriangularView& operator-=(const Eigen::Product<L, R, O>& p) {
+    return assign_product(p, detail::Kind::Sub);
+  }
+
+ private:
+  template <typename L, typename R, int O>
+  DeviceTriangularView& assign_product(const Eigen::Product<L, R, O>& p, detail::Kind kind) {
+    static_assert(Op == 0 && (Mode == Eigen::Lower || Mode == Eigen::Upper),
+                  "assign to the Lower or Upper triangle of the matrix itself");
+    eigen_assert(rows() == cols());
+    eigen_assert(factor_ == Scalar(1));
+    const typename DeviceMap<Plain>::Base& dst = m_;
+    const detail::GemmArgs<Scalar> g = detail::gemm_args(dst, p.lhs(), p.rhs(), kind);
+    HipDevice& dev = default_device();
+    if (g.m == 0) return detail::finish(EH_SUCCESS, dev), *this;
+    const int64_t ldc = m_.outerStride() < 1 ? 1 : m_.outerStride();
+    const eh_uplo uplo = detail::uplo_of(Mode, g.flip);
+    void* c = const_cast<Scalar*>(m_.data());
+    constexpr eh_dtype dt = detail::scalar_info<Scalar>::dtype;
+    detail::finish(kComplex ? eh_gemmtr_complex(dt, uplo, g.opa, g.opb, g.m, g.k, &g.alpha, g.A, g.lda, g.B, g.ldb,
+                                                &g.beta, c, ldc, dev.stream)
+                            : eh_gemmtr(dt, uplo, g.opa, g.opb, g.m, g.k, &g.alpha, g.A, g.lda, g.B, g.ldb, &g.beta,
+                                        c, ldc, dev.stream),
+                   dev);
+    return *this;
+  }
+
+  DeviceMap<Plain> m_;
+  Scalar factor_;
+};
+
+// s * T: the view scaled (for products; a solve divides by it)
+template <typename Plain, int Mode, int Op>
+DeviceTriangularView<Plain, Mode, Op> operator*(const typename Plain::Scalar& s,
+                                                const DeviceTriangularView<Plain, Mode, Op>& t) {
+  return DeviceTriangularView<Plain, Mode, Op>(t.nestedExpression(), s * t.factor());
+}
+
+// T * X (Side OnTheLeft) or X * T (OnTheRight), to be assigned to X itself.
+template <typename View, typename P2, int Side>
+struct DeviceTriangularProduct {
+  View tri;
+  DeviceMap<P2> x;
+  template <typename Dst>
+  void run_into(Dst& dst) const {
+    using Scalar = typename View::Scalar;
+    static_assert(std::is_same<typename P2::Scalar, Scalar>::value, "mixed scalar types are not supported");
+    static_assert(std::is_same<typename Dst::Scalar, Scalar>::value && bool(Dst::IsRowMajor) == bool(P2::IsRowMajor),
+                  "the destination must be the product's other operand");
+    // in place, as BLAS trmm: the destination is the other operand
+    eigen_assert(dst.data() == x.data() && dst.rows() == x.rows() && dst.cols() == x.cols() &&
+                 dst.outerStride() == x.outerStride() &&
+                 "X = T * X needs the same X on both sides (copy B into C first for C = T * B)");
+    constexpr bool xrow = P2::IsRowMajor;
+    constexpr int op = xrow ? detail::op_transpose(View::kStoredOp) : View::kStoredOp;
+    static_assert(op != 3, "this combination of storage orders and adjoint needs a conjugate without transpose, "
+                           "which no BLAS op expresses");
+    eigen_assert(tri.rows() == tri.cols());
+    eigen_assert((Side == Eigen::OnTheLeft ? x.rows() : x.cols()) == tri.rows());
+    const eh_side side = (Side == Eigen::OnTheLeft) != xrow ? EH_LEFT : EH_RIGHT;
+    const int64_t m = xrow ? x.cols() : x.rows(), n = xrow ? x.rows() : x.cols();
+    const Scalar alpha = tri.factor();
+    HipDevice& dev = default_device();
+    if (m == 0 || n == 0) return detail::finish(EH_SUCCESS, dev);
+    const auto& A = tri.nestedExpression();
+    const int64_t lda = A.outerStride() < 1 ? 1 : A.outerStride();
+    const int64_t ldx = x.outerStride() < 1 ? 1 : x.outerStride();
+    void* xp = const_cast<Scalar*>(x.data());
+    constexpr eh_dtype dt = detail::scalar_info<Scalar>::dtype;
+    detail::finish(View::kComplex ? eh_trmm_complex(dt, side, View::kStoredUplo, detail::eh_op_of(op), View::kDiag, m,
+                                                    n, &alpha, A.data(), lda, xp, ldx, dev.stream)
+                                  : eh_trmm(dt, side, View::kStoredUplo, detail::eh_op_of(op), View::kDiag, m, n,
+                                            &alpha, A.data(), lda, xp, ldx, dev.stream),
+                   dev);
+  }
+};
+
+template <typename Plain, int Mode, int Op, typename P2>
+DeviceTriangularProduct<DeviceTriangularView<Plain, Mode, Op>, P2, Eigen::OnTheLeft> operator*(
+    const DeviceTriangularView<Plain, Mode, Op>& t, const DeviceMap<P2>& x) {
+  return {t, x};
+}
+template <typename Plain, int Mode, int Op, typename P2>
+DeviceTriangularProduct<DeviceTriangularView<Plain, Mode, Op>, P2, Eigen::OnTheRight> operator*(
+    const DeviceMap<P2>& x, const DeviceTriangularView<Plain, Mode, Op>& t) {
+  return {t, x};
+}
+
+// The self-adjoint matrix (symmetric for real scalars, Hermitian for
+// complex ones) stored in the Mode triangle of the DeviceMap's matrix,
+// scaled by a factor (products), or the destination of rank updates.
+template <typename Plain, int Mode>
+class DeviceSelfAdjointView {
+ public:
+  using Scalar = typename Plain::Scalar;
+  using RealScalar = typename Eigen::NumTraits<Scalar>::Real;
+  using Index = Eigen::Index;
+  static constexpr bool kComplex = Eigen::NumTraits<Scalar>::IsComplex;
+  static_assert(detail::scalar_info<Scalar>::supported,
+                "device views support float, double, std::complex<float> and std::complex<double>");
+  static_assert(Mode == Eigen::Lower || Mode == Eigen::Upper, "device self-adjoint views: Lower or Upper");
+  // A row-major matrix's column-major view is its transpose, self-adjoint
+  // too (the conjugate, for a Hermitian one), stored in the other triangle.
+  static constexpr bool kRowMajor = Plain::IsRowMajor;
+  static constexpr eh_uplo kStoredUplo = detail::uplo_of(Mode, kRowMajor);
+
+  explicit DeviceSelfAdjointView(const DeviceMap<Plain>& m, Scalar factor = Scalar(1)) : m_(m), factor_(factor) {}
+  const DeviceMap<Plain>& nestedExpression() const { return m_; }
+  Scalar factor() const { return factor_; }
+  Index rows() const { return m_.rows(); }
+  Index cols() const { return m_.cols(); }
+
+  // C = C + alpha * U * U^*  on the stored triangle (eh_syrk / eh_herk)
+  template <typename U>
+  DeviceSelfAdjointView& rankUpdate(const Eigen::MatrixBase<U>& u, const RealScalar& alpha = RealScalar(1)) {
+    using Traits = Eigen::internal::blas_traits<U>;
+    using Actual = Eigen::internal::remove_all_t<typename Traits::DirectLinearAccessType>;
+    static_assert(std::is_same<typename U::Scalar, Scalar>::value, "mixed scalar types are not supported");
+    static_assert(bool(Traits::HasUsableDirectAccess), "rank-update operands need direct access with unit inner stride");
+    static_assert(!kComplex || !kRowMajor, "a Hermitian rank update of a row-major matrix is not supported");
+    using Op = detail::operand_op<detail::row_major<Actual>(), bool(Traits::NeedToConjugate), false>;
+    static_assert(Op::valid && (!kComplex || Op::op != EH_OP_T),
+                  "the rank-update operand must be U, U^T (real) or U^H (complex) of a stored matrix");
+    eigen_assert(rows() == cols() && u.rows() == rows());
+    const Actual& a = Traits::extract(u.derived());
+    // a scalar factor f of U enters as |f|^2
+    const RealScalar f2 = Eigen::numext::abs2(Traits::extractScalarFactor(u.derived()));
+    const int64_t n = rows(), k = u.cols();
+    const int64_t lda = a.outerStride() < 1 ? 1 : a.outerStride();
+    const int64_t ldc = m_.outerStride() < 1 ? 1 : m_.outerStride();
+    void* c = const_cast<Scalar*>(m_.data());
+    constexpr eh_dtype dt = detail::scalar_info<Scalar>::dtype;
+    HipDevice& dev = default_device();
+    if (n == 0) return detail::finish(EH_SUCCESS, dev), *this;
+    if (kComplex) {
+      const RealScalar ar = alpha * f2, br = RealScalar(1);
+      detail::finish(eh_herk(dt, kStoredUplo, Op::op, n, k, &ar, a.data(), lda, &br, c, ldc, dev.stream), dev);
+    } else {
+      const Scalar as = Scalar(alpha * f2), bs = Scalar(1);
+      detail::finish(eh_syrk(dt, kStoredUplo, Op::op, n, k, &as, a.data(), lda, &bs, c, ldc, dev.stream), dev);
+    }
+    return *this;
+  }
+
+  // C = C + alpha * U * V^* + conj(alpha) * V * U^*  (eh_syr2k / eh_her2k).
+  // For vectors U, V this is Eigen's rank-2 update; n x k factors give the
+  // rank-2k update (Eigen's own rankUpdate(u, v) takes vectors only).
+  template <typename U, typename V>
+  DeviceSelfAdjointView& rankUpdate(const Eigen::MatrixBase<U>& u, const Eigen::MatrixBase<V>& v,
+                                    const Scalar& alpha = Scalar(1)) {
+    using TU = Eigen::internal::blas_traits<U>;
+    using TV = Eigen::internal::blas_traits<V>;
+    using AU = Eigen::internal::remove_all_t<typename TU::DirectLinearAccessType>;
+    using AV = Eigen::internal::remove_all_t<typename TV::DirectLinearAccessType>;
+    static_assert(std::is_same<typename U::Scalar, Scalar>::value && std::is_same<typename V::Scalar, Scalar>::value,
+                  "mixed scalar types are not supported");
+    static_assert(bool(TU::HasUsableDirectAccess) && bool(TV::HasUsableDirectAccess),
+                  "rank-update operands need direct access with unit inner stride");
+    static_assert(!kComplex || !kRowMajor, "a Hermitian rank update of a row-major matrix is not supported");
+    using OpU = detail::operand_op<detail::row_major<AU>(), bool(TU::NeedToConjugate), false>;
+    using OpV = detail::operand_op<detail::row_major<AV>(), bool(TV::NeedToConjugate), false>;
+    static_assert(OpU::valid && OpV::valid && OpU::op == OpV::op && (!kComplex || OpU::op != EH_OP_T),
+                  "U and V must both be stored matrices, both transposed (real) or both adjoints (complex)");
+    eigen_assert(rows() == cols() && u.rows() == rows() && v.rows() == rows() && u.cols() == v.cols());
+    const AU& a = TU::extract(u.derived());
+    const AV& b = TV::extract(v.derived());
+    // scalar factors of U and V fold into alpha: (fu U)(fv V)^* = fu conj(fv) U V^*
+    const Scalar al = alpha * TU::extractScalarFactor(u.derived()) *
+                      Eigen::numext::conj(TV::extractScalarFactor(v.derived()));
+    const int64_t n = rows(), k = u.cols();
+    const int64_t lda = a.outerStride() < 1 ? 1 : a.outerStride();
+    const int64_t ldb = b.outerStride() < 1 ? 1 : b.outerStride();
+    const int64_t ldc = m_.outerStride() < 1 ? 1 : m_.outerStride();
+    void* c = const_cast<Scalar*>(m_.data());
+    constexpr eh_dtype dt = detail::scalar_info<Scalar>::dtype;
+    HipDevice& dev = default_device();
+    if (n == 0) return detail::finish(EH_SUCCESS, dev), *this;
+    if (kComplex) {
+      const RealScalar br = RealScalar(1);
+      detail::finish(
+          eh_her2k(dt, kStoredUplo, OpU::op, n, k, &al, a.data(), lda, b.data(), ldb, &br, c, ldc, dev.stream), dev);
+    } else {
+      const Scalar bs = Scalar(1);
+      detail::finish(
+          eh_syr2k(dt, kStoredUplo, OpU::op, n, k, &al, a.data(), lda, b.data(), ldb, &bs, c, ldc, dev.stream), dev);
+    }
+    return *this;
+  }
+
+ private:
+  DeviceMap<Plain> m_;
+  Scalar factor_;
+};
+
+template <typename Plain, int Mode>
+DeviceSelfAdjointView<Plain, Mode> operator*(const typename Plain::Scalar& s, const DeviceSelfAdjointView<Plain, Mode>& a) {
+  return DeviceSelfAdjointView<Plain, Mode>(a.nestedExpression(), s * a.factor());
+}
+
+// S * B (Side OnTheLeft) or B * S (OnTheRight), assigned (=, +=, -=) to a
+// DeviceMap C of the same storage order as B (eh_symm / eh_hemm).
+template <typename View, typename P2, int Side>
+struct DeviceSelfAdjointProduct {
+  View sa;
+  DeviceMap<P2> b;
+  template <typename Dst>
+  void run_into(Dst& dst, detail::Kind kind) const {
+    using Scalar = typename View::Scalar;
+    static_assert(std::is_same<typename P2::Scalar, Scalar>::value && std::is_same<typename Dst::Scalar, Scalar>::value,
+                  "mixed scalar types are not supported");
+    // C and B in the same storage order: a row-major pair is the transposed
+    // product of their column-major views, C^T = B^T S^T, with S^T the
+    // column-major view of S's storage (self-adjoint, other triangle)
+    static_assert(bool(Dst::IsRowMajor) == bool(P2::IsRowMajor) && bool(P2::IsRowMajor) == View::kRowMajor,
+                  "the self-adjoint matrix, B and C must share one storage order");
+    constexpr bool flip = P2::IsRowMajor;
+    eigen_assert(sa.rows() == sa.cols());
+    eigen_assert(dst.rows() == b.rows() && dst.cols() == b.cols());
+    eigen_assert((Side == Eigen::OnTheLeft ? b.rows() : b.cols()) == sa.rows());
+    const eh_side side = (Side == Eigen::OnTheLeft) != flip ? EH_LEFT : EH_RIGHT;
+    const int64_t m = flip ? dst.cols() : dst.rows(), n = flip ? dst.rows() : dst.cols();
+    const Scalar alpha = kind == detail::Kind::Sub ? -sa.factor() : sa.factor();
+    const Scalar beta = kind == detail::Kind::Assign ? Scalar(0) : Scalar(1);
+    HipDevice& dev = default_device();
+    if (m == 0 || n == 0) return detail::finish(EH_SUCCESS, dev);
+    const auto& A = sa.nestedExpression();
+    const int64_t lda = A.outerStride() < 1 ? 1 : A.outerStride();
+    const int64_t ldb = b.outerStride() < 1 ? 1 : b.outerStride();
+    const int64_t ldc = dst.outerStride() < 1 ? 1 : dst.outerStride();
+    constexpr eh_dtype dt = detail::scalar_info<Scalar>::dtype;
+    detail::finish(View::kComplex ? eh_hemm(dt, side, View::kStoredUplo, m, n, &alpha, A.data(), lda, b.data(), ldb,
+                                            &beta, dst.data(), ldc, dev.stream)
+                                  : eh_symm(dt, side, View::kStoredUplo, m, n, &alpha, A.data(), lda, b.data(), ldb,
+                                            &beta, dst.data(), ldc, dev.stream),
+                   dev);
+  }
+};
+
+template <typename Plain, int Mode, typename P2>
+DeviceSelfAdjointProduct<DeviceSelfAdjointView<Plain, Mode>, P2, Eigen::OnTheLeft> operator*(
+    const DeviceSelfAdjointView<Plain, Mode>& a, const DeviceMap<P2>& b) {
+  return {a, b};
+}
+template <typename Plain, int Mode, typename P2>
+DeviceSelfAdjointProduct<DeviceSelfAdjointView<Plain, Mode>, P2, Eigen::OnTheRight> operator*(
+    const DeviceMap<P2>& b, const DeviceSelfAdjointView<Plain, Mode>& a) {
+  return {a, b};
+}
 
 }  // namespace eigen_amd
 

@@ -39,7 +39,10 @@ void fill_int(Mat& m, unsigned seed) {
     }
 }
 
+// One line per case, flushed, so that the cases run so far are on record
+// even if the program dies.
 inline int report(const char* name, bool ok) {
   std::printf("%s %s\n", ok ? "PASS" : "FAIL", name);
+  std::fflush(stdout);
   return ok ? 0 : 1;
 }

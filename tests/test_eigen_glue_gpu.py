@@ -6,6 +6,10 @@
   eigen_amd::DeviceMap with no .device(): matrix, vector, row-vector, tiny
   (2x3x1, 1x1) and row-major shapes, real and complex scalars (adjoint
   operands, complex alpha), and complex products through the .device() seam;
+* test_device_level3  -- the level-3 views on DeviceMap: triangular solves
+  and in-place products, products into a triangle, self-adjoint products
+  and rank updates (transposed / adjoint / scaled views, unit diagonals,
+  row-major storage, real and complex scalars);
 * test_blas_redirect  -- unchanged C.noalias() += alpha*A*B with
   EIGEN_USE_BLAS routed to libeigen_amd_blas (host and device Maps);
 * test_blas_level3_redirect -- unchanged matrix-vector, triangular solve,
@@ -25,7 +29,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 BUILD = os.path.join(HERE, "eigen_glue", "_build")
 
 
-@pytest.mark.parametrize("prog", ["test_device_glue", "test_device_map", "test_blas_redirect",
+@pytest.mark.parametrize("prog", ["test_device_glue", "test_device_map", "test_device_level3", "test_blas_redirect",
                                   "test_blas_level3_redirect"])
 def test_eigen_program(gpu, prog):
     path = os.path.join(BUILD, prog)
