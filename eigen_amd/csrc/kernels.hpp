@@ -83,6 +83,11 @@ struct TrtriParams {
 };
 template <typename T>
 hipError_t launch_trtri(const TrtriParams<T>& p, hipStream_t s);
+// D_p = [S_2p 0; 0 S_2p+1] (blocks of order L into blocks of order 2L,
+// off-diagonal blocks zero), p < count: the first step of doubling the
+// order of the inverted diagonal blocks.
+template <typename T>
+hipError_t launch_inverse_place(const T* S, T* D, int L, int64_t count, hipStream_t s);
 
 // Triangle of a complex rank update (level3_kernels.hip): on the lower
 // (upper) triangle, C = alpha*W + alpha2*f(W^T) + beta*C with f = conj when

@@ -25,6 +25,7 @@
 
 #include <algorithm>
 #include <string>
+#include <type_traits>
 
 #include "../../include/eigen_amd.h"
 #include "internal.hpp"
@@ -104,20 +105,63 @@ int64_t half(int64_t sz, int64_t leaf) {
 }
 
 // ---- TRSM --------------------------------------------------------------------
-// The diagonal blocks of order trsm_leaf<T>() are inverted up front (one launch);
-// a leaf of the recursion multiplies its right-hand sides by the inverse, on
-// the MFMA kernels, from a copy of them.
+// The diagonal blocks of order trsm_leaf<T>() are inverted up front (one
+// launch), and for real types their order is then doubled up to
+// kTrsmLeafMax while it divides the order of the triangle: for every pair of
+// consecutive blocks at once (two batched products of the current order),
+//   lower:  inv([E11 0; E21 E22]) = [I1 0; -I2 E21 I1, I2]
+//   upper:  inv([E11 E12; 0 E22]) = [I1 -I1 E12 I2; 0 I2].
+// A leaf of the recursion multiplies its right-hand sides by the inverse of
+// its block, on the MFMA kernels, from a copy of them.  Larger leaves halve
+// the recursion's smallest and least efficient launches at each doubling
+// (a 64-row leaf multiplies 64 x n by 64 x 64).
+constexpr int64_t kTrsmLeafMax = 256;
+
+template <typename T>
+struct IsCplx : std::false_type {};
+template <typename R>
+struct IsCplx<cplx<R>> : std::true_type {};
+
 template <typename T>
 struct TrsmWork {
   const T* inv;  // inverse of diagonal block b at inv + b*leaf^2 (ld leaf)
   T* copy;       // leaf x n (left) or m x leaf (right)
+  int64_t leaf;  // order of the inverted blocks
 };
+
+template <typename T>
+constexpr eh_dtype real_dtype() {
+  return std::is_same<T, double>::value ? EH_F64 : EH_F32;
+}
+
+// Inverses of order 2L in D (count = ka / 2L blocks) from those of order L in
+// S; tmp holds count blocks of order L.
+template <typename T>
+eh_status double_inverses(const Tri<T>& E, int64_t ka, int64_t L, const T* S, T* D, T* tmp, hipStream_t s) {
+  const int64_t count = ka / (2 * L), L2 = 2 * L;
+  hipError_t e = launch_inverse_place<T>(S, D, (int)L, count, s);
+  if (e != hipSuccess) return hip_fail(e, "inverse placement launch");
+  const int64_t sd = L2 * L2, st = L * L, se = L2 * (1 + E.lda);  // strides between pairs
+  const T one(1), zero(0), minus(-1);
+  constexpr eh_dtype dt = real_dtype<T>();
+  if (E.lower) {  // tmp = E21 I1; D(L:, :L) = -I2 tmp
+    EH_TRY(eh_gemm_batched_strided(dt, (eh_op)E.op(), EH_OP_N, L, L, L, &one, E.block(L, 0), E.lda, se, D, L2, sd,
+                                   &zero, tmp, L, st, count, s));
+    return eh_gemm_batched_strided(dt, EH_OP_N, EH_OP_N, L, L, L, &minus, D + L + L * L2, L2, sd, tmp, L, st, &zero,
+                                   D + L, L2, sd, count, s);
+  }
+  // tmp = E12 I2; D(:L, L:) = -I1 tmp
+  EH_TRY(eh_gemm_batched_strided(dt, (eh_op)E.op(), EH_OP_N, L, L, L, &one, E.block(0, L), E.lda, se, D + L + L * L2,
+                                 L2, sd, &zero, tmp, L, st, count, s));
+  return eh_gemm_batched_strided(dt, EH_OP_N, EH_OP_N, L, L, L, &minus, D, L2, sd, tmp, L, st, &zero, D + L * L2, L2,
+                                 sd, count, s);
+}
 
 // E[d:d+sz, d:d+sz] X = alpha*B, B = rows d.. of the right-hand sides (sz x n).
 template <typename T>
 eh_status trsm_left(const Tri<T>& E, int64_t d, int64_t sz, T* B, int64_t ldb, int64_t n, T alpha,
                     const TrsmWork<T>& w, hipStream_t s) {
-  constexpr int64_t leaf = trsm_leaf<T>();
+  const int64_t leaf = w.leaf;
   if (sz <= leaf) {  // X = alpha * inv(E_dd) * B
     hipError_t e = hipMemcpy2DAsync(w.copy, sz * sizeof(T), B, ldb * sizeof(T), sz * sizeof(T), n,
                                     hipMemcpyDeviceToDevice, s);
@@ -145,7 +189,7 @@ eh_status trsm_left(const Tri<T>& E, int64_t d, int64_t sz, T* B, int64_t ldb, i
 template <typename T>
 eh_status trsm_right(const Tri<T>& E, int64_t d, int64_t sz, T* B, int64_t ldb, int64_t m, T alpha,
                      const TrsmWork<T>& w, hipStream_t s) {
-  constexpr int64_t leaf = trsm_leaf<T>();
+  const int64_t leaf = w.leaf;
   if (sz <= leaf) {  // X = alpha * B * inv(E_dd)
     hipError_t e = hipMemcpy2DAsync(w.copy, m * sizeof(T), B, ldb * sizeof(T), m * sizeof(T), sz,
                                     hipMemcpyDeviceToDevice, s);
@@ -264,17 +308,35 @@ eh_status trsm_t(int side, int uplo, int transa, int diag, int64_t m, int64_t n,
   const bool ta = transa != EH_OP_N;
   const Tri<T> E{A, lda, ta, (uplo == EH_LOWER) != ta, diag == EH_UNIT, transa == EH_OP_C};
   const int64_t ka = side == EH_LEFT ? m : n;
-  constexpr int64_t leaf = trsm_leaf<T>();
-  const int64_t blocks = (ka + leaf - 1) / leaf;
-  const int64_t inv = blocks * leaf * leaf;
+  constexpr int64_t leaf0 = trsm_leaf<T>();
+  // doubled leaf order (real types: the doubling runs on the batched
+  // product, which takes real operands)
+  int64_t leaf = leaf0;
+  if constexpr (!IsCplx<T>::value)
+    while (leaf < kTrsmLeafMax && ka % (2 * leaf) == 0) leaf *= 2;
+  const int64_t blocks0 = (ka + leaf0 - 1) / leaf0;
+  const int64_t inv0 = blocks0 * leaf0 * leaf0;                 // the inversion's blocks
+  const int64_t invd = leaf > leaf0 ? ka * leaf : 0;           // each doubled set (ka / L blocks of L^2)
+  const int64_t tmp = leaf > leaf0 ? ka * leaf / 4 : 0;        // (ka / 2L) blocks of L^2, L <= leaf / 2
   const int64_t copy = side == EH_LEFT ? leaf * n : m * leaf;
   StreamTemp t;
-  EH_TRY(t.get((size_t)(inv + copy) * sizeof(T), s, "level-3 temporary"));
-  const TrsmWork<T> w{static_cast<T*>(t.ptr), static_cast<T*>(t.ptr) + inv};
-  TrtriParams<T> tp{A, lda, (int)E.ta, (int)E.lower, (int)E.unit, (int)ka, static_cast<T*>(t.ptr)};
+  EH_TRY(t.get((size_t)(inv0 + 2 * invd + tmp + copy) * sizeof(T), s, "level-3 temporary"));
+  T* base = static_cast<T*>(t.ptr);
+  T* bufs[2] = {base + inv0, base + inv0 + invd};
+  T* tbuf = base + inv0 + 2 * invd;
+  TrtriParams<T> tp{A, lda, (int)E.ta, (int)E.lower, (int)E.unit, (int)ka, base};
   tp.conj = E.conj;
   hipError_t e = launch_trtri<T>(tp, s);
   if (e != hipSuccess) return hip_fail(e, "diagonal block inversion launch");
+  const T* inv = base;
+  int which = 0;
+  if constexpr (!IsCplx<T>::value) {
+    for (int64_t L = leaf0; L < leaf; L *= 2, which ^= 1) {
+      EH_TRY(double_inverses(E, ka, L, inv, bufs[which], tbuf, s));
+      inv = bufs[which];
+    }
+  }
+  const TrsmWork<T> w{inv, tbuf + tmp, leaf};
   return side == EH_LEFT ? trsm_left<T>(E, 0, m, B, ldb, n, alpha, w, s)
                          : trsm_right<T>(E, 0, n, B, ldb, m, alpha, w, s);
 }

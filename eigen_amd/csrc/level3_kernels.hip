@@ -214,10 +214,37 @@ hipError_t launch_trtri(const TrtriParams<T>& p, hipStream_t s) {
   return hipGetLastError();
 }
 
+// D_p = [S_2p 0; 0 S_2p+1] for p < count: S holds 2*count blocks of order
+// L (L x L, column-major, consecutive), D count blocks of order 2L; the
+// off-diagonal blocks are zeroed (one of them is then overwritten by the
+// doubling product).  One thread per element of D.
+template <typename T>
+__global__ void __launch_bounds__(256) inverse_place_kernel(const T* __restrict__ S, T* __restrict__ D, int L,
+                                                            int64_t total) {
+  const int64_t idx = blockIdx.x * 256ll + threadIdx.x;
+  if (idx >= total) return;
+  const int64_t blk = 4ll * L * L;
+  const int64_t p = idx / blk, r = idx % blk;
+  const int j = (int)(r / (2 * L)), i = (int)(r % (2 * L));
+  T v = T(0);
+  if (i < L && j < L) v = S[(2 * p) * L * L + i + (int64_t)j * L];
+  if (i >= L && j >= L) v = S[(2 * p + 1) * L * L + (i - L) + (int64_t)(j - L) * L];
+  D[idx] = v;
+}
+
+template <typename T>
+hipError_t launch_inverse_place(const T* S, T* D, int L, int64_t count, hipStream_t s) {
+  const int64_t total = count * 4ll * L * L;
+  if (total <= 0) return hipSuccess;
+  hipLaunchKernelGGL(inverse_place_kernel<T>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, S, D, L, total);
+  return hipGetLastError();
+}
+
 #define EH_L3_INST(T)                                                                                       \
   template hipError_t launch_tri_dense<T>(const T*, int64_t, int, T*, int64_t, bool, bool, bool, hipStream_t, \
                                           bool, bool);                                                      \
-  template hipError_t launch_trtri<T>(const TrtriParams<T>&, hipStream_t);
+  template hipError_t launch_trtri<T>(const TrtriParams<T>&, hipStream_t);                                  \
+  template hipError_t launch_inverse_place<T>(const T*, T*, int, int64_t, hipStream_t);
 EH_L3_INST(float)
 EH_L3_INST(double)
 EH_L3_INST(cplx<float>)

@@ -297,8 +297,10 @@ eh_status eh_trmm(eh_dtype dtype, eh_side side, eh_uplo uplo, eh_op transa, eh_d
  * from the remaining right-hand sides by a product on the MFMA kernels.  The
  * 64 x 64 diagonal blocks are inverted up front by one batched launch
  * (substitution, multiplying by the reciprocal of the diagonal as the
- * reference does) and a leaf multiplies its right-hand sides by the inverse
- * on the MFMA kernels, from a copy of them.
+ * reference does); for real types the inverses are then doubled to order
+ * 128 and 256 while that order divides the triangle's (two batched products
+ * per doubling), and a leaf multiplies its right-hand sides by the inverse
+ * of its block on the MFMA kernels, from a copy of them.
  * alpha is applied to the right-hand sides before the solve (the reference
  * scales the solution afterwards: equal up to rounding); alpha == 0 sets B
  * to zero without reading A.  No check for a zero diagonal: as in the
