@@ -14,7 +14,8 @@
 //    Opt>, Func, Device, Dense2Dense> (Eigen/src/Core/DeviceWrapper.h:59-70),
 //    reached from DeviceWrapper::noalias() (DeviceWrapper.h:46-48) through
 //    call_assignment_no_alias (DeviceWrapper.h:94-117); this header
-//    specialises it for eigen_amd::HipDevice.
+//    specialises it for eigen_amd::HipDevice.  A product with a
+//    selfAdjointView<>() factor (S * B or B * S) runs as eh_symm / eh_hemm.
 //
 // 2. A device map type, with no `.device()` at the call site:
 //
@@ -228,19 +229,6 @@ void device_product(Dst& dst, const Lhs& a_lhs, const Rhs& a_rhs, Kind kind, Hip
   eigen_assert(dev.status == EH_SUCCESS && "eigen_amd device product failed (see eh_last_error)");
 }
 
-// Finish a level-3 call: optional synchronisation, status kept on the device.
-inline void finish(eh_status st, HipDevice& dev) {
-  dev.status = st;
-  if (dev.status == EH_SUCCESS && dev.synchronize) dev.status = eh_synchronize(dev.stream);
-  eigen_assert(dev.status == EH_SUCCESS && "eigen_amd device level-3 call failed (see eh_last_error)");
-}
-
-template <typename T>
-struct always_false : std::false_type {};
-
-}  // namespace detail
-
-namespace detail {
 // BLAS op codes of the level-3 views: 0 = N, 1 = T, 2 = C (conjugate
 // transpose), 3 = the conjugate alone (no BLAS op; rejected where used).
 constexpr int op_transpose(int o) { return o == 0 ? 1 : o == 1 ? 0 : o == 2 ? 3 : 2; }
@@ -248,9 +236,76 @@ constexpr int op_adjoint(int o, bool cplx) {
   return !cplx ? op_transpose(o) : (o == 0 ? 2 : o == 2 ? 0 : o == 1 ? 3 : 1);
 }
 constexpr eh_op eh_op_of(int o) { return o == 0 ? EH_OP_N : o == 1 ? EH_OP_T : EH_OP_C; }
+// BLAS triangle of a Mode (Lower / Upper) view, flipped for a row-major
+// matrix's column-major view
 constexpr eh_uplo uplo_of(int mode, bool flip) {
   return ((mode & Eigen::Lower) != 0) != flip ? EH_LOWER : EH_UPPER;
 }
+
+// Finish a level-3 call: optional synchronisation, status kept on the device.
+inline void finish(eh_status st, HipDevice& dev) {
+  dev.status = st;
+  if (dev.status == EH_SUCCESS && dev.synchronize) dev.status = eh_synchronize(dev.stream);
+  eigen_assert(dev.status == EH_SUCCESS && "eigen_amd device level-3 call failed (see eh_last_error)");
+}
+
+// Eigen::SelfAdjointView<M, Mode> (any constness of M)
+template <typename X>
+struct sa_view : std::false_type {};
+template <typename M, unsigned int Mode>
+struct sa_view<Eigen::SelfAdjointView<M, Mode>> : std::true_type {
+  static constexpr int mode = int(Mode);
+};
+
+// dst (=, +=, -=) factor * S * other (left) or factor * other * S (right),
+// S the self-adjoint matrix (symmetric real / Hermitian complex) stored in
+// the Mode triangle of `a` (eh_symm / eh_hemm).  `a`, `other` (after its
+// scalar factors) and dst are stored matrices of one storage order: a
+// row-major triple is the transposed product of their column-major views,
+// C^T = B^T S^T, where S^T is the column-major view of S's storage (itself
+// self-adjoint, stored in the other triangle).
+template <int Mode, typename Dst, typename AX, typename Other>
+void device_symm(Dst& dst, const AX& a, const Other& a_other, bool left, typename Dst::Scalar factor, Kind kind,
+                 HipDevice& dev) {
+  using Scalar = typename Dst::Scalar;
+  using Info = scalar_info<Scalar>;
+  using Traits = Eigen::internal::blas_traits<Other>;
+  using ActualOther = Eigen::internal::remove_all_t<typename Traits::DirectLinearAccessType>;
+  static_assert(Info::supported,
+                "eigen_amd device products support float, double, std::complex<float> and std::complex<double>");
+  static_assert(Mode == Eigen::Lower || Mode == Eigen::Upper, "a self-adjoint view is Lower or Upper");
+  static_assert(std::is_same<typename AX::Scalar, Scalar>::value && std::is_same<typename Other::Scalar, Scalar>::value,
+                "mixed scalar types are not supported");
+  static_assert((int(Eigen::internal::traits<AX>::Flags) & Eigen::DirectAccessBit) != 0 &&
+                    bool(Traits::HasUsableDirectAccess) && !bool(Traits::NeedToConjugate),
+                "a device self-adjoint product needs stored operands (no conjugate)");
+  static_assert(row_major<Dst>() == row_major<AX>() && row_major<Dst>() == row_major<ActualOther>(),
+                "the self-adjoint matrix, B and C must share one storage order");
+  constexpr bool flip = row_major<Dst>();
+  const ActualOther& b = Traits::extract(a_other);
+  const Scalar f = factor * Traits::extractScalarFactor(a_other);
+  eigen_assert(a.rows() == a.cols() && a.innerStride() == 1 && b.innerStride() == 1 && dst.innerStride() == 1);
+  eigen_assert(dst.rows() == (left ? a.rows() : b.rows()) && dst.cols() == (left ? b.cols() : a.cols()));
+  const eh_side side = left != flip ? EH_LEFT : EH_RIGHT;
+  const eh_uplo uplo = uplo_of(Mode, flip);
+  const int64_t m = flip ? dst.cols() : dst.rows(), n = flip ? dst.rows() : dst.cols();
+  const Scalar alpha = kind == Kind::Sub ? -f : f;
+  const Scalar beta = kind == Kind::Assign ? Scalar(0) : Scalar(1);
+  if (m == 0 || n == 0) return finish(EH_SUCCESS, dev);
+  const int64_t lda = a.outerStride() < 1 ? 1 : a.outerStride();
+  const int64_t ldb = b.outerStride() < 1 ? 1 : b.outerStride();
+  const int64_t ldc = dst.outerStride() < 1 ? 1 : dst.outerStride();
+  constexpr eh_dtype dt = Info::dtype;
+  finish(Info::complex ? eh_hemm(dt, side, uplo, m, n, &alpha, a.data(), lda, b.data(), ldb, &beta, dst.data(), ldc,
+                                 dev.stream)
+                       : eh_symm(dt, side, uplo, m, n, &alpha, a.data(), lda, b.data(), ldb, &beta, dst.data(), ldc,
+                                 dev.stream),
+         dev);
+}
+
+template <typename T>
+struct always_false : std::false_type {};
+
 }  // namespace detail
 
 template <typename Plain>
@@ -588,6 +643,7 @@ class DeviceSelfAdjointView {
   // too (the conjugate, for a Hermitian one), stored in the other triangle.
   static constexpr bool kRowMajor = Plain::IsRowMajor;
   static constexpr eh_uplo kStoredUplo = detail::uplo_of(Mode, kRowMajor);
+  static constexpr int kMode = Mode;
 
   explicit DeviceSelfAdjointView(const DeviceMap<Plain>& m, Scalar factor = Scalar(1)) : m_(m), factor_(factor) {}
   const DeviceMap<Plain>& nestedExpression() const { return m_; }
@@ -690,34 +746,12 @@ struct DeviceSelfAdjointProduct {
   DeviceMap<P2> b;
   template <typename Dst>
   void run_into(Dst& dst, detail::Kind kind) const {
-    using Scalar = typename View::Scalar;
-    static_assert(std::is_same<typename P2::Scalar, Scalar>::value && std::is_same<typename Dst::Scalar, Scalar>::value,
-                  "mixed scalar types are not supported");
-    // C and B in the same storage order: a row-major pair is the transposed
-    // product of their column-major views, C^T = B^T S^T, with S^T the
-    // column-major view of S's storage (self-adjoint, other triangle)
-    static_assert(bool(Dst::IsRowMajor) == bool(P2::IsRowMajor) && bool(P2::IsRowMajor) == View::kRowMajor,
-                  "the self-adjoint matrix, B and C must share one storage order");
-    constexpr bool flip = P2::IsRowMajor;
-    eigen_assert(sa.rows() == sa.cols());
-    eigen_assert(dst.rows() == b.rows() && dst.cols() == b.cols());
-    eigen_assert((Side == Eigen::OnTheLeft ? b.rows() : b.cols()) == sa.rows());
-    const eh_side side = (Side == Eigen::OnTheLeft) != flip ? EH_LEFT : EH_RIGHT;
-    const int64_t m = flip ? dst.cols() : dst.rows(), n = flip ? dst.rows() : dst.cols();
-    const Scalar alpha = kind == detail::Kind::Sub ? -sa.factor() : sa.factor();
-    const Scalar beta = kind == detail::Kind::Assign ? Scalar(0) : Scalar(1);
-    HipDevice& dev = default_device();
-    if (m == 0 || n == 0) return detail::finish(EH_SUCCESS, dev);
-    const auto& A = sa.nestedExpression();
-    const int64_t lda = A.outerStride() < 1 ? 1 : A.outerStride();
-    const int64_t ldb = b.outerStride() < 1 ? 1 : b.outerStride();
-    const int64_t ldc = dst.outerStride() < 1 ? 1 : dst.outerStride();
-    constexpr eh_dtype dt = detail::scalar_info<Scalar>::dtype;
-    detail::finish(View::kComplex ? eh_hemm(dt, side, View::kStoredUplo, m, n, &alpha, A.data(), lda, b.data(), ldb,
-                                            &beta, dst.data(), ldc, dev.stream)
-                                  : eh_symm(dt, side, View::kStoredUplo, m, n, &alpha, A.data(), lda, b.data(), ldb,
-                                            &beta, dst.data(), ldc, dev.stream),
-                   dev);
+    // as plain Maps (Eigen's traits are defined for Map, not for DeviceMap)
+    using AMap = typename std::decay_t<decltype(sa.nestedExpression())>::Base;
+    const AMap& a = sa.nestedExpression();
+    const typename DeviceMap<P2>::Base& bm = b;
+    typename Dst::Base& d = dst;
+    detail::device_symm<View::kMode>(d, a, bm, Side == Eigen::OnTheLeft, sa.factor(), kind, default_device());
   }
 };
 
@@ -749,8 +783,17 @@ struct AssignmentWithDevice<DstXprType, Product<Lhs, Rhs, Options>, Functor, eig
 
   static void run(DstXprType& dst, const SrcXprType& src, const Functor&, eigen_amd::HipDevice& dev) {
     using eigen_amd::detail::Kind;
-    eigen_amd::detail::device_product(dst, src.lhs(), src.rhs(), kAssign ? Kind::Assign : (kAdd ? Kind::Add : Kind::Sub),
-                                      dev);
+    using LV = eigen_amd::detail::sa_view<remove_all_t<Lhs>>;
+    using RV = eigen_amd::detail::sa_view<remove_all_t<Rhs>>;
+    const Kind kind = kAssign ? Kind::Assign : (kAdd ? Kind::Add : Kind::Sub);
+    if constexpr (LV::value)  // S * B: eh_symm / eh_hemm, left
+      eigen_amd::detail::device_symm<LV::mode>(dst, src.lhs().nestedExpression(), src.rhs(), true, Scalar(1), kind,
+                                               dev);
+    else if constexpr (RV::value)  // B * S: right
+      eigen_amd::detail::device_symm<RV::mode>(dst, src.rhs().nestedExpression(), src.lhs(), false, Scalar(1), kind,
+                                               dev);
+    else
+      eigen_amd::detail::device_product(dst, src.lhs(), src.rhs(), kind, dev);
   }
 };
 

@@ -161,6 +161,23 @@ int run_all(const char* tname) {
       fails += report(tag("split3"), C.fetch() == want);
     }
   }
+  {  // self-adjoint products through the seam (eh_symm): S * B, B * S, row-major
+    Dev<T, 0> S(m, m, 31, 2), B(m, n, 32), C(m, n, 33);
+    auto c = C.map();
+    c.device(dev).noalias() = S.map().template selfadjointView<Eigen::Lower>() * B.map();
+    Mat<T> want = S.host.template selfadjointView<Eigen::Lower>() * B.host;
+    fails += report(tag("symm_lower_left"), C.fetch() == want && dev.status == EH_SUCCESS);
+    Dev<T, 0> S2(n, n, 34), B2(m, n, 35), C2(m, n, 36);
+    auto c2 = C2.map();
+    c2.device(dev).noalias() -= B2.map() * T(2) * S2.map().template selfadjointView<Eigen::Upper>();
+    Mat<T> want2 = C2.host - B2.host * T(2) * S2.host.template selfadjointView<Eigen::Upper>();
+    fails += report(tag("symm_upper_right_scaled_sub"), C2.fetch() == want2);
+    Dev<T, Eigen::RowMajor> S3(m, m, 37, 1), B3(m, n, 38, 3), C3(m, n, 39, 2);
+    auto c3 = C3.map();
+    c3.device(dev).noalias() += S3.map().template selfadjointView<Eigen::Upper>() * B3.map();
+    Mat<T, Eigen::RowMajor> want3 = C3.host + S3.host.template selfadjointView<Eigen::Upper>() * B3.host;
+    fails += report(tag("symm_row_major_add"), C3.fetch() == want3);
+  }
   return fails;
 }
 

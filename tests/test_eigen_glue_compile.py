@@ -25,7 +25,8 @@ using M = Eigen::MatrixXf;
 using CM = Eigen::MatrixXcf;
 using RM = Eigen::Matrix<float, Eigen::Dynamic, Eigen::Dynamic, Eigen::RowMajor>;
 using RCM = Eigen::Matrix<std::complex<float>, Eigen::Dynamic, Eigen::Dynamic, Eigen::RowMajor>;
-void f(float* a, float* b, float* c, std::complex<float>* x, std::complex<float>* y, std::complex<float>* z) {
+void f(float* a, float* b, float* c, std::complex<float>* x, std::complex<float>* y, std::complex<float>* z,
+       float* a_, float* b_, float* c_) {
   eigen_amd::DeviceMap<M> A(a, 4, 4), B(b, 4, 4), C(c, 4, 4);
   eigen_amd::DeviceMap<CM> X(x, 4, 4), Y(y, 4, 4), Z(z, 4, 4);
   eigen_amd::DeviceMap<RM> R(c, 4, 4);
@@ -56,6 +57,8 @@ def test_supported_forms_compile():
   R.selfadjointView<Eigen::Lower>().rankUpdate(A);
   X.triangularView<Eigen::Upper>().adjoint().solveInPlace(Y); Z.selfadjointView<Eigen::Lower>().rankUpdate(X.adjoint(), 1.0f);
   Z.selfadjointView<Eigen::Upper>().rankUpdate(X, Y, std::complex<float>(0, 1)); Z = X.selfadjointView<Eigen::Lower>() * Y;
+  eigen_amd::HipDevice dev; Eigen::Map<M> ma(a_, 4, 4), mb(b_, 4, 4), mc(c_, 4, 4);
+  mc.device(dev).noalias() += ma.selfadjointView<Eigen::Lower>() * mb; mc.device(dev).noalias() = 2.0f * mb * ma.selfadjointView<Eigen::Upper>();
 """)
     assert rc == 0, err[-3000:]
 
@@ -70,6 +73,9 @@ def test_supported_forms_compile():
     ("Z.selfadjointView<Eigen::Lower>().rankUpdate(X.transpose());", "U^H (complex)"),
     ("C = R.selfadjointView<Eigen::Lower>() * B;", "must share one storage order"),
     ("C.triangularView<Eigen::Lower>().transpose() = A * B;", "assign to the Lower or Upper triangle"),
+    ("eigen_amd::HipDevice d; Eigen::Map<M> ma(a_, 4, 4), mb(b_, 4, 4), mc(c_, 4, 4);"
+     " mc.device(d).noalias() = ma.selfadjointView<Eigen::Lower>() * mb.transpose();",
+     "must share one storage order"),
 ])
 def test_unsafe_forms_rejected(body, message):
     rc, err = _compile(body)

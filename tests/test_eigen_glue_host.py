@@ -17,7 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 BUILD = os.path.join(HERE, "eigen_glue", "_build")
 
 
-@pytest.mark.parametrize("prog", ["test_device_map_host", "test_device_level3_host"])
+@pytest.mark.parametrize("prog", ["test_device_glue_host", "test_device_map_host", "test_device_level3_host"])
 def test_eigen_program_on_host_stand_ins(prog):
     path = os.path.join(BUILD, prog)
     if not os.path.exists(path):
