@@ -185,7 +185,20 @@ __device__ inline void wait_stages_behind(int ahead) {
   }
 }
 
-template <int BM, int BN, int WGM, int WGN, int TERMS, int NSTAGE, bool F16 = false, int KD = 32>
+// Scheduling groups of one pipelined step: one MFMA, then one LDS read of
+// the next step, until the NREAD reads are placed, then the other MFMAs.
+template <int NREAD, int NMMA, int Q = 0>
+__device__ inline void interleave_order() {
+  if constexpr (Q < NREAD) {
+    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
+    __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // LDS read
+    interleave_order<NREAD, NMMA, Q + 1>();
+  } else {
+    __builtin_amdgcn_sched_group_barrier(0x008, NMMA - Q, 0);
+  }
+}
+
+template <int BM, int BN, int WGM, int WGN, int TERMS, int NSTAGE, bool F16 = false, int KD = 32, bool PIPE = false>
 __global__ void __launch_bounds__(64 * WGM * WGN) gemm_split_packed_kernel(SplitParams p) {
   static_assert(KD == 32 || KD == 16, "stage depth");
   constexpr int NW = WGM * WGN;
@@ -321,13 +334,54 @@ __global__ void __launch_bounds__(64 * WGM * WGN) gemm_split_packed_kernel(Split
         }
   };
 
-  {
+  if constexpr (PIPE) {
+    // Software-pipelined over the two 16-deep steps of each stage: the
+    // fragments of the next step are read from LDS into a second register
+    // set while the MFMAs of the current one run, one read after each MFMA
+    // (scheduling groups).  Entering stage st + 1, a wave waits for its own
+    // loads of it and meets the others at the barrier; by then every wave's
+    // reads of stage st have returned, so its buffer is refilled NSTAGE
+    // stages ahead.  Fills past the last stage reload the last stage (the
+    // same bytes) into a free buffer, so every transition waits for the same
+    // count of loads and the loop has no branch: a branch inside it makes the
+    // compiler's LDS waits before the MFMAs cover the reads just issued.
+    static_assert(NSTEP == 2, "the pipelined loop takes two steps per stage");
+    auto fill = [&](int st, int buf) { issue(st < nkt ? st : nkt - 1, buf); };
+    if (nkt > 0) {
+#pragma unroll
+      for (int s = 0; s < NSTAGE; ++s) fill(s, s);
+      wait_vmcnt<(NSTAGE - 1) * LPW>();
+      stage_barrier();
+      u32x4 fa0[TERMS][RM], fb0[TERMS][RN], fa1[TERMS][RM], fb1[TERMS][RN];
+      read_step(smem, 0, fa0, fb0);
+      constexpr int NREAD = TERMS * (RM + RN), NMMA = (TERMS == 3 ? 6 : (TERMS == 2 ? 3 : 1)) * RM * RN;
+      static_assert(NMMA >= NREAD, "not enough MFMAs to interleave the reads with");
+      for (int st = 0; st < nkt; ++st) {
+        read_step(smem + (st % NSTAGE) * STAGE, 1, fa1, fb1);
+        mma_step(fa0, fb0);
+        interleave_order<NREAD, NMMA>();
+        __builtin_amdgcn_sched_barrier(0);
+        // stage st + 1 must have landed; NSTAGE - 2 fills after it may not
+        wait_vmcnt<(NSTAGE - 2) * LPW>();
+        stage_barrier();
+        fill(st + NSTAGE, st % NSTAGE);
+        // no LDS read is outstanding here (the barrier waited for them): this
+        // wait costs nothing and keeps the compiler from counting the direct
+        // loads just issued against the reads below
+        __builtin_amdgcn_s_waitcnt(0xC07F);
+        const int stn = st + 1 < nkt ? st + 1 : st;  // past the end: an unused re-read
+        read_step(smem + (stn % NSTAGE) * STAGE, 0, fa0, fb0);
+        mma_step(fa1, fb1);
+        interleave_order<NREAD, NMMA>();
+        __builtin_amdgcn_sched_barrier(0);
+      }
+      wait_vmcnt0();  // no direct load may still target LDS when the waves end
+    }
+  } else {
     // Per stage: wait for it, barrier, refill the buffer freed a stage ago,
     // read all its fragments, multiply.  (Forcing all reads of both 16-deep
     // steps ahead of the MFMAs measured slower than the compiler's
-    // interleaving, and so did reading the next step's fragments into a
-    // second register set during the MFMAs: within 2% of this loop for the
-    // bf16 256x256 tile, and no room in registers for the split tile.)
+    // interleaving.)
 #pragma unroll
     for (int s = 0; s < NSTAGE - 1; ++s)
       if (s < nkt) issue(s, s);
@@ -488,10 +542,11 @@ hipError_t launch_pack_split(const float* X, int64_t ld, bool kcont, int rows, i
   return hipErrorInvalidValue;
 }
 
-template <int BM, int BN, int WGM, int WGN, int TERMS, int NSTAGE, bool F16 = false, int KD = 32>
+template <int BM, int BN, int WGM, int WGN, int TERMS, int NSTAGE, bool F16 = false, int KD = 32, bool PIPE = false>
 static hipError_t launch_packed_t(const SplitParams& p, int splits, hipStream_t s) {
   const int ntm = (p.M + BM - 1) / BM, ntn = (p.N + BN - 1) / BN;
-  hipLaunchKernelGGL((gemm_split_packed_kernel<BM, BN, WGM, WGN, TERMS, NSTAGE, F16, KD>), dim3(ntm * ntn, splits),
+  hipLaunchKernelGGL((gemm_split_packed_kernel<BM, BN, WGM, WGN, TERMS, NSTAGE, F16, KD, PIPE>),
+                     dim3(ntm * ntn, splits),
                      dim3(64 * WGM * WGN), 0, s, p);
   return hipGetLastError();
 }
@@ -500,7 +555,7 @@ hipError_t launch_gemm_split_packed(const SplitParams& p, int terms, int tile, i
   if (f16) {  // IEEE half operands: the one-term tiles on the FP16 instruction
     if (terms != 1) return hipErrorInvalidValue;
     if (tile == 0) return launch_packed_t<128, 128, 2, 2, 1, 4, true>(p, splits, s);
-    return launch_packed_t<256, 256, 2, 4, 1, 2, true>(p, splits, s);
+    return launch_packed_t<256, 256, 2, 4, 1, 3, true, 32, true>(p, splits, s);
   }
   // tile 0: 128x128, 4 waves (3 stages); 1: 256x128, 4 waves of 128x64 (2 stages);
   // 2: 256x128, 8 waves of 64x64 -- two waves per SIMD (2 stages);
@@ -520,13 +575,15 @@ hipError_t launch_gemm_split_packed(const SplitParams& p, int terms, int tile, i
   if (terms == 1) {
     // bf16 inputs: one product per slice, so the LDS reads per MFMA decide
     // the rate.  tile 0: 128x128, 4 waves of 64x64 (4 stages); tile 1:
-    // 256x256, 8 waves of 128x64 (2 stages, two waves per SIMD): 3/4 of the
-    // LDS reads per MFMA of 64x64 wave tiles, 903 TFLOP/s at 8192^3 against
-    // 637 for the 256x128 8-wave tile and 866 for 4 waves of 128x128 (one
-    // wave per SIMD, accumulators in AGPRs); 814 vs 776 at 4096^3.  Below
-    // 256 tiles of 256x256 the 128x128 tile wins.
+    // 256x256, 8 waves of 128x64 (two waves per SIMD): 3/4 of the LDS reads
+    // per MFMA of 64x64 wave tiles, 903 TFLOP/s at 8192^3 against 637 for
+    // the 256x128 8-wave tile and 866 for 4 waves of 128x128 (one wave per
+    // SIMD, accumulators in AGPRs); 814 vs 776 at 4096^3.  Its loop is the
+    // software-pipelined one with three stages: 4% faster than the plain
+    // loop (1.075-1.079 ms against 1.114-1.128 ms at 8192^3, alternating
+    // on one box).  Below 256 tiles of 256x256 the 128x128 tile wins.
     if (tile == 0) return launch_packed_t<128, 128, 2, 2, 1, 4>(p, splits, s);
-    return launch_packed_t<256, 256, 2, 4, 1, 2>(p, splits, s);
+    return launch_packed_t<256, 256, 2, 4, 1, 3, false, 32, true>(p, splits, s);
   }
   return hipErrorInvalidValue;
 }
