@@ -554,7 +554,7 @@ static hipError_t launch_packed_t(const SplitParams& p, int splits, hipStream_t 
 hipError_t launch_gemm_split_packed(const SplitParams& p, int terms, int tile, int splits, hipStream_t s, bool f16) {
   if (f16) {  // IEEE half operands: the one-term tiles on the FP16 instruction
     if (terms != 1) return hipErrorInvalidValue;
-    if (tile == 0) return launch_packed_t<128, 128, 2, 2, 1, 4, true>(p, splits, s);
+    if (tile == 0) return launch_packed_t<128, 128, 2, 2, 1, 4, true, 32, true>(p, splits, s);
     return launch_packed_t<256, 256, 2, 4, 1, 3, true, 32, true>(p, splits, s);
   }
   // tile 0: 128x128, 4 waves (3 stages); 1: 256x128, 4 waves of 128x64 (2 stages);
@@ -581,8 +581,10 @@ hipError_t launch_gemm_split_packed(const SplitParams& p, int terms, int tile, i
     // SIMD, accumulators in AGPRs); 814 vs 776 at 4096^3.  Its loop is the
     // software-pipelined one with three stages: 4% faster than the plain
     // loop (1.075-1.079 ms against 1.114-1.128 ms at 8192^3, alternating
-    // on one box).  Below 256 tiles of 256x256 the 128x128 tile wins.
-    if (tile == 0) return launch_packed_t<128, 128, 2, 2, 1, 4>(p, splits, s);
+    // on one box).  Below 256 tiles of 256x256 the 128x128 tile wins; its
+    // pipelined loop is 6-9% faster at 2048^3 and 3072^3 (47-49 against 52
+    // us, 114 against 121 us), 2% at 1024^3.
+    if (tile == 0) return launch_packed_t<128, 128, 2, 2, 1, 4, false, 32, true>(p, splits, s);
     return launch_packed_t<256, 256, 2, 4, 1, 3, false, 32, true>(p, splits, s);
   }
   return hipErrorInvalidValue;

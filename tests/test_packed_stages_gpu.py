@@ -5,12 +5,12 @@ before reading a stage, waits until only the stages issued after it may
 still be loading.  Near the end of the k loop fewer stages are issued after
 the one being read, so the wait must shrink with them; a wait that assumes a
 full pipeline there lets a wave read a stage that has not landed.  (The
-software-pipelined loop of the bf16 / half 256x256 tile instead reloads the
-last stage past the end, so its waits stay constant.)  These cases run every
-configuration of the four-stage tiles (128x128 for bf16 and half inputs,
-256x256 with 16-deep stages for two split terms), the three-stage ones and
-the pipelined one over k of one to seven 32-deep slices, so every position
-of each loop's tail is hit, on integer-valued inputs whose products are exact in
+software-pipelined loop of the bf16 / half tiles instead reloads the last
+stage past the end, so its waits stay constant.)  These cases run every
+configuration of the split tiles (256x256 with four 16-deep stages for two
+terms, three-stage ones for three) and of the pipelined bf16 / half tiles
+over k of one to seven 32-deep slices, so every position of each loop's
+tail is hit, on integer-valued inputs whose products are exact in
 every configuration: the result must equal the exact product bit for bit,
 on every repetition."""
 import numpy as np
@@ -40,7 +40,7 @@ def _operands(kind, A, B):
 
 
 @pytest.mark.parametrize("kind,configs", [
-    (E.PACKED_BF16, range(0, 8)),    # 128x128 (four stages), 256x256 (three, pipelined loop)
+    (E.PACKED_BF16, range(0, 8)),    # 128x128 (four stages), 256x256 (three), both the pipelined loop
     (E.PACKED_F16, range(0, 8)),     # the same tiles on the FP16 instruction
     (E.PACKED_SPLIT2, range(12, 16)),  # 256x256, 16-deep stages (four)
     (E.PACKED_SPLIT3, range(0, 4)),  # 128x128 (three stages)
