@@ -5,11 +5,12 @@ before reading a stage, waits until only the stages issued after it may
 still be loading.  Near the end of the k loop fewer stages are issued after
 the one being read, so the wait must shrink with them; a wait that assumes a
 full pipeline there lets a wave read a stage that has not landed.  (The
-software-pipelined loop of the bf16 / half tiles instead reloads the last
+software-pipelined loop of the bf16 / half tiles and of two split tiles reloads the last
 stage past the end, so its waits stay constant.)  These cases run every
-configuration of the split tiles (256x256 with four 16-deep stages for two
-terms, three-stage ones for three) and of the pipelined bf16 / half tiles
-over k of one to seven 32-deep slices, so every position of each loop's
+configuration of the four split tiles (the pipelined 128x128 and 8-wave
+256x128 ones, the plain 4-wave 256x128 one, and 256x256 with 16-deep
+stages) and of the pipelined bf16 / half tiles over k of one to seven
+32-deep slices, so every position of each loop's
 tail is hit, on integer-valued inputs whose products are exact in
 every configuration: the result must equal the exact product bit for bit,
 on every repetition."""
@@ -42,9 +43,8 @@ def _operands(kind, A, B):
 @pytest.mark.parametrize("kind,configs", [
     (E.PACKED_BF16, range(0, 8)),    # 128x128 (four stages), 256x256 (three), both the pipelined loop
     (E.PACKED_F16, range(0, 8)),     # the same tiles on the FP16 instruction
-    (E.PACKED_SPLIT2, range(12, 16)),  # 256x256, 16-deep stages (four)
-    (E.PACKED_SPLIT3, range(0, 4)),  # 128x128 (three stages)
-    (E.PACKED_SPLIT3, range(12, 16)),  # 256x256, 16-deep stages (three)
+    (E.PACKED_SPLIT2, range(0, 16)),  # 128x128, 256x128 (two layouts), 256x256 with 16-deep stages
+    (E.PACKED_SPLIT3, range(0, 16)),  # the same four tiles with three terms
 ])
 def test_stage_pipeline_tail_exact(gpu, kind, configs):
     m, n = 300, 270
