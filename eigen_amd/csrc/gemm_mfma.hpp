@@ -40,8 +40,7 @@ constexpr int tile_smem_bytes() {
 }
 
 // One BM x BN tile of C at (m0, n0) over k in [kbeg, kend): the whole body of
-// a product workgroup, shared by the one-product kernel below and the
-// grouped kernel's LDS-staged units (grouped.hip).  z / partial: split-K
+// a product workgroup (gemm_mfma_kernel below).  z / partial: split-K
 // slice and whether the result goes to the partial buffer p.W.  smem holds
 // tile_smem_bytes<BM, BN>(); the caller synchronises before reusing it.
 template <typename T, int BM, int BN, int WGM, int WGN, bool TA, bool TB, int RS>
