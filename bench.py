@@ -409,7 +409,9 @@ def bench_level3(E, torch, dev, args):
 def bench_batched_strided(E, torch, dev, args):
     """Arrays of fixed-size matrices through eh_gemm_batched_strided: 100,000
     products of 4x4 FP32 and 20,000 of 16x16 FP64 (C = A*B, U[-1,1]);
-    algorithmic bytes = A, B read and C written once.  Error: the largest
+    algorithmic bytes = A, B read and C written once; `ms` is the wall time
+    per call (for such small batches mostly the host's call and launch),
+    `kernel_ms` the product kernel's own HIP-event time.  Error: the largest
     normwise error over every 100th product against the exact product."""
     out = {}
     for name, dtype, tdt, es, size, count in (("f32_4x4x4", E.F32, torch.float32, 4, 4, 100000),
@@ -420,7 +422,8 @@ def bench_batched_strided(E, torch, dev, args):
         Cm = torch.zeros(e * count, dtype=tdt, device=dev)
         fn = lambda: E.gemm_batched_strided(dtype, 0, 0, size, size, size, 1.0, A, size, e, B, size, e, 0.0, Cm, size,
                                             e, count)
-        wall, kern = timed(fn, max(10, args.steps), args.warmup)
+        wall, kern, kt = timed(fn, max(10, args.steps), args.warmup, (E.KC_GROUPED,))
+        kms = kt[E.KC_GROUPED][0]
         flop, nbytes = 2.0 * size ** 3 * count, 3.0 * e * es * count
         u = float(np.finfo(np.float32 if dtype == E.F32 else np.float64).eps) / 2
         Ah = A.view(count, size, size).transpose(1, 2).cpu().numpy()
@@ -430,6 +433,8 @@ def bench_batched_strided(E, torch, dev, args):
                 for p in range(0, count, 100)]
         out[name] = {"products": count, "ms": wall, "gflops": flop / (wall * 1e-3) / 1e9,
                      "gbs": nbytes / (wall * 1e-3) / 1e9, "frac_of_hbm": nbytes / (wall * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "kernel_ms": kms, "kernel_gbs": nbytes / (kms * 1e-3) / 1e9 if kms > 0 else None,
+                     "kernel_frac_of_hbm": nbytes / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS if kms > 0 else None,
                      "normwise_err_max": float(np.max(errs))}
         del A, B, Cm
     log(f"batched_strided: {out}")

@@ -547,7 +547,7 @@ typedef enum eh_kernel_class {
   EH_KC_PRODUCT = 0,        /* plain FP32 / FP64 MFMA product kernels */
   EH_KC_PACKED_PRODUCT = 1, /* split-BF16 and bf16 products over packed planes */
   EH_KC_PACK = 2,           /* split / bf16 operand packing */
-  EH_KC_GROUPED = 3,        /* persistent grouped kernel */
+  EH_KC_GROUPED = 3,        /* persistent grouped kernel, fixed-size batched kernel */
   EH_KC_GEMV = 4,           /* matrix-vector kernels */
   EH_KC_COUNT = 5
 } eh_kernel_class;

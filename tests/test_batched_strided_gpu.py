@@ -77,6 +77,35 @@ def test_batched_strided_exact(gpu, dt, ta, tb, m, n, k, count):
 
 
 @pytest.mark.parametrize("dt", [np.float32, np.float64])
+@pytest.mark.parametrize("stride,beta", [(16, 0.0), (16, -1.0), (20, 0.5), (17, -1.0)])
+def test_batched_strided_dense4_exact(gpu, dt, stride, beta):
+    """Arrays of dense 4x4 matrices (leading dimension 4): packed (stride 16)
+    and spaced (20) arrays of 16-byte aligned matrices, and stride 17, whose
+    matrices are not all 16-byte aligned; beta == 0 with NaN in C."""
+    rng = np.random.default_rng(stride)
+    count = 30000
+    A, am = _batch(rng, count, 4, 4, 4, stride, dt)
+    B, bm = _batch(rng, count, 4, 4, 4, stride, dt)
+    C0, cm = _batch(rng, count, 4, 4, 4, stride, dt)
+    if beta == 0.0:
+        for p in range(count):
+            C0[p * stride: p * stride + 16] = np.nan
+    dC = _dev(C0)
+    E.gemm_batched_strided(ED[dt], 0, 0, 4, 4, 4, 3.0, _dev(A), 4, stride, _dev(B), 4, stride, beta, dC, 4, stride,
+                           count)
+    got = _host(dC)
+    gaps = np.ones(len(C0), bool)
+    for p in range(count):
+        gaps[p * stride: p * stride + 16] = False
+    assert np.array_equal(got[gaps], C0[gaps], equal_nan=True), "elements between the products written"
+    for p in range(0, count, 97):
+        want = 3 * (am[p].astype(np.int64) @ bm[p].astype(np.int64))
+        if beta != 0.0:
+            want = want + beta * cm[p].astype(np.float64)
+        assert np.array_equal(_unpack(got, p, 4, 4, 4, stride), want.astype(dt)), p
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
 @pytest.mark.parametrize("m,n,k,count", [(20, 12, 9, 30), (4, 3, 2, 300)])  # grouped kernel; one thread per product
 def test_batched_strided_alpha_beta_zero(gpu, dt, m, n, k, count):
     rng = np.random.default_rng(3)
