@@ -360,9 +360,9 @@ static eh_status launch_split_product(const GemmParams<float>& g, bool ta, bool 
 }
 
 // bf16 (or, with f16, IEEE half) inputs, FP32 accumulation and result:
-// both operands are repacked into the one-term plane layout, then the packed
-// BF16 (FP16) MFMA product runs with a single product per k slice (and the
-// split-K reduction when K is cut).  Operand extents must stay below 4 GB
+// the operands are repacked into the one-term plane layout (or read in
+// place, below), then the packed BF16 (FP16) MFMA product runs with a single
+// product per k slice (and the split-K reduction when K is cut).  Operand extents must stay below 4 GB
 // (32-bit buffer offsets).
 static eh_status launch_bf16_product(bool ta, bool tb, int m, int n, int k, float alpha, const unsigned short* A,
                                      int64_t lda, const unsigned short* B, int64_t ldb, float beta, float* C,
@@ -379,26 +379,35 @@ static eh_status launch_bf16_product(bool ta, bool tb, int m, int n, int k, floa
   config_decode(config, &tile, &splits);
   const int per = (KS + splits - 1) / splits;
   splits = (KS + per - 1) / per;
-  const size_t bytesA = split_packed_bytes(m, k, 1), bytesB = split_packed_bytes(n, k, 1);
+  // An operand whose op(X) rows are k-contiguous (A^T, or B), with K a
+  // multiple of 32 and 16-byte aligned rows, is read in place: its rows
+  // already hold the packed layout's 64-byte slices.
+  auto in_place = [k](bool kcont, const unsigned short* X, int64_t ld) {
+    return kcont && k % 32 == 0 && ld % 8 == 0 && reinterpret_cast<uintptr_t>(X) % 16 == 0;
+  };
+  const bool dA = in_place(ta, A, lda), dB = in_place(!tb, B, ldb);
+  const size_t bytesA = dA ? 0 : split_packed_bytes(m, k, 1), bytesB = dB ? 0 : split_packed_bytes(n, k, 1);
   const size_t bytesW = splits > 1 ? (size_t)splits * m * n * sizeof(float) : 0;
   void* ws = nullptr;
   WorkspaceLease lease;
   eh_status st = workspace_get(stream, bytesA + bytesB + bytesW, &lease, &ws);
   if (st != EH_SUCCESS) return st;
   unsigned char* base = static_cast<unsigned char*>(ws);
-  hipError_t e;
-  {
+  hipError_t e = hipSuccess;
+  if (!dA) {
     KernelTimer t(EH_KC_PACK, stream);
     e = launch_pack_bf16(A, lda, ta, m, k, base, stream);
   }
-  if (e == hipSuccess) {
+  if (e == hipSuccess && !dB) {
     KernelTimer t(EH_KC_PACK, stream);
     e = launch_pack_bf16(B, ldb, !tb, n, k, base + bytesA, stream);
   }
   if (e != hipSuccess) return hip_fail(e, "bf16 pack launch");
   SplitParams sp{};
-  sp.Ap = base;
-  sp.Bp = base + bytesA;
+  sp.Ap = dA ? static_cast<const void*>(A) : base;
+  sp.Bp = dB ? static_cast<const void*>(B) : base + bytesA;
+  sp.ldAd = dA ? lda : 0;
+  sp.ldBd = dB ? ldb : 0;
   sp.C = C;
   sp.W = splits > 1 ? reinterpret_cast<float*>(base + bytesA + bytesB) : nullptr;
   sp.ldc = ldc;

@@ -241,7 +241,10 @@ __global__ void __launch_bounds__(64 * WGM * WGN) gemm_split_packed_kernel(Split
   //   KD = 16: four 256-byte runs (lds_half_offset): plane row
   //   blk0 + 16*(l/32) + (l&15), chunk (l/16)&1 of the stage's half slice.
   // The per-lane global offset is computed per block once; the block bases
-  // are wave-uniform and advance by one slice per SPS stages.
+  // are wave-uniform and advance by one slice per SPS stages.  An operand
+  // read in place (one term, ldAd / ldBd set) has the same rows and chunks
+  // at its own addresses: row r of op(X) at r * ld elements, a slice 64
+  // bytes further along it; rows past the matrix re-read its last row.
   const unsigned char* srcA = reinterpret_cast<const unsigned char*>(p.Ap);
   const unsigned char* srcB = reinterpret_cast<const unsigned char*>(p.Bp);
   const int64_t sliceA = (int64_t)TERMS * p.Mp * ROWB, sliceB = (int64_t)TERMS * p.Np * ROWB;
@@ -253,19 +256,32 @@ __global__ void __launch_bounds__(64 * WGM * WGN) gemm_split_packed_kernel(Split
     const int b = wid + NW * i;                // block index within the stage (wave-uniform)
     const int t = b / (PROWS / RPB);             // plane
     const int blk0 = (b % (PROWS / RPB)) * RPB;  // first plane row of the block
+    int prow, pchunk;                            // plane row within the block and 16-byte chunk of this lane
     if constexpr (KD == 32) {
       const int R = (blk0 >> 1) + (lane >> 3);   // LDS row (pair of plane rows)
       const int V = swz8(R, lane & 7);           // logical chunk stored at this LDS slot
-      loff[i] = (uint32_t)((2 * (lane >> 3) + (V >> 2)) * ROWB + (V & 3) * 16);
+      prow = 2 * (lane >> 3) + (V >> 2);
+      pchunk = V & 3;
     } else {
-      loff[i] = (uint32_t)((16 * (lane >> 5) + (lane & 15)) * ROWB + ((lane >> 4) & 1) * 16);
+      prow = 16 * (lane >> 5) + (lane & 15);
+      pchunk = (lane >> 4) & 1;
     }
-    if (blk0 < BM) {
-      gbase[i] = srcA + (int64_t)ks0 * sliceA + ((int64_t)t * p.Mp + m0 + blk0) * ROWB;
-      gstride[i] = sliceA;
+    const bool isA = blk0 < BM;
+    const int64_t ldd = TERMS == 1 ? (isA ? p.ldAd : p.ldBd) : 0;
+    if (ldd != 0) {
+      const int row = min((isA ? m0 + blk0 : n0 + blk0 - BM) + prow, (isA ? p.M : p.N) - 1);
+      gbase[i] = (isA ? srcA : srcB) + (int64_t)ks0 * ROWB;
+      loff[i] = (uint32_t)((int64_t)row * ldd * 2 + pchunk * 16);
+      gstride[i] = ROWB;
     } else {
-      gbase[i] = srcB + (int64_t)ks0 * sliceB + ((int64_t)t * p.Np + n0 + blk0 - BM) * ROWB;
-      gstride[i] = sliceB;
+      loff[i] = (uint32_t)(prow * ROWB + pchunk * 16);
+      if (isA) {
+        gbase[i] = srcA + (int64_t)ks0 * sliceA + ((int64_t)t * p.Mp + m0 + blk0) * ROWB;
+        gstride[i] = sliceA;
+      } else {
+        gbase[i] = srcB + (int64_t)ks0 * sliceB + ((int64_t)t * p.Np + n0 + blk0 - BM) * ROWB;
+        gstride[i] = sliceB;
+      }
     }
   }
   auto issue = [&](int kt, int buf) {

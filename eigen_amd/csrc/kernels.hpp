@@ -137,6 +137,10 @@ struct SplitParams {
   int M, N, KS, Mp, Np;
   int ks_per_split;
   float alpha, beta;
+  // One-term products only: a nonzero value is the leading dimension (in
+  // elements) of an operand read in place instead of from packed planes --
+  // its op(X) rows k-contiguous, K a multiple of 32, 16-byte aligned rows.
+  int64_t ldAd, ldBd;
 };
 hipError_t launch_pack_split(const float* X, int64_t ld, bool kcont, int rows, int K, int terms, void* P,
                              hipStream_t s);

@@ -171,7 +171,10 @@ eh_status eh_gemv(eh_dtype dtype, eh_op trans, int64_t m, int64_t n, const void*
  * EIGEN_USE_OPENBLAS_BFLOAT16 (GeneralMatrixMatrix_BLAS.h:120-202), which
  * then rounds the FP32 result to bfloat16 itself; beta == 0 does not read C,
  * alpha == 0 or k == 0 does not read A or B.  Operands must span less than
- * 4 GB (EH_ERR_UNSUPPORTED otherwise). */
+ * 4 GB (EH_ERR_UNSUPPORTED otherwise).  Operands are repacked into a
+ * stream workspace, except those whose op(X) rows are k-contiguous (A with
+ * opa T, B with opb N) when k is a multiple of 32 and the rows are 16-byte
+ * aligned (8 | ld, 16-byte aligned base): those are read in place. */
 eh_status eh_gemm_bf16(eh_op opa, eh_op opb, int64_t m, int64_t n, int64_t k, const float* alpha, const void* A,
                        int64_t lda, const void* B, int64_t ldb, const float* beta, float* C, int64_t ldc,
                        void* stream);
@@ -187,7 +190,8 @@ eh_status eh_gemm_bf16(eh_op opa, eh_op opb, int64_t m, int64_t n, int64_t k, co
  * arch/AVX/PacketMath.h:2328); here only the final FP32 result is rounded,
  * and a caller that wants half output converts it (Eigen::half(float),
  * Half.h:683).  beta == 0 does not read C; alpha == 0 or k == 0 does not read
- * A or B.  Operands must span less than 4 GB (EH_ERR_UNSUPPORTED otherwise). */
+ * A or B.  Operands must span less than 4 GB (EH_ERR_UNSUPPORTED otherwise);
+ * they are repacked or read in place as for eh_gemm_bf16. */
 eh_status eh_gemm_f16(eh_op opa, eh_op opb, int64_t m, int64_t n, int64_t k, const float* alpha, const void* A,
                       int64_t lda, const void* B, int64_t ldb, const float* beta, float* C, int64_t ldc,
                       void* stream);

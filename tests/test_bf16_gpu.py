@@ -97,6 +97,32 @@ def test_bf16_integer_inputs_exact(gpu, ta, tb):
     assert np.array_equal(got, (a @ b).astype(np.float32))
 
 
+@pytest.mark.parametrize("ta,tb", list(itertools.product((0, 1), (0, 1))))
+@pytest.mark.parametrize("m,n,k,pad,offset", [(300, 270, 64, 8, 0), (129, 513, 1024, 16, 8), (64, 64, 4096, 8, 0),
+                                              (300, 270, 64, 8, 1), (200, 130, 96, 3, 0)])
+def test_bf16_in_place_operands_exact(gpu, ta, tb, m, n, k, pad, offset):
+    """Operands whose op(X) rows are k-contiguous (A^T, B) are read in place
+    when k is a multiple of 32 and the rows are 16-byte aligned (leading
+    dimension a multiple of 8, base aligned); the others, and these with an
+    odd padding or a 2-byte shifted base, go through the packed layout.
+    Integer inputs (exact result), NaN in every padding element, ragged m
+    and n (rows past the matrix), k = 4096 on a 64 x 64 result (split k)."""
+    rng = np.random.default_rng(m + n + k + pad + offset)
+    sa = (k, m) if ta else (m, k)
+    sb = (n, k) if tb else (k, n)
+    A = _bf16(rng, *sa, sa[0] + pad, ints=True)
+    B = _bf16(rng, *sb, sb[0] + pad, ints=True)
+    tA, pA = _dev16(A, offset=offset)
+    tB, pB = _dev16(B, offset=offset)
+    dC = _dev32(np.full((m, n), np.nan, np.float32))
+    E.gemm_bf16(ta, tb, m, n, k, 1.0, pA, A.shape[0], pB, B.shape[0], 0.0, dC, m)
+    got = from_dev(dC, (m, n))
+    a = _widen(A[: sa[0], : sa[1]]).reshape(sa).astype(np.int64)
+    b = _widen(B[: sb[0], : sb[1]]).reshape(sb).astype(np.int64)
+    a, b = (a.T if ta else a), (b.T if tb else b)
+    assert np.array_equal(got, (a @ b).astype(np.float32))
+
+
 def test_bf16_alpha_zero_and_empty(gpu):
     rng = np.random.default_rng(6)
     m, n, k = 33, 17, 40

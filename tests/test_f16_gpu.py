@@ -104,6 +104,28 @@ def test_f16_integer_inputs_exact(gpu, ta, tb, scale):
     assert np.array_equal(got, (a @ b).astype(np.float64) * scale)
 
 
+@pytest.mark.parametrize("ta,tb", list(itertools.product((0, 1), (0, 1))))
+@pytest.mark.parametrize("m,n,k,pad", [(300, 270, 64, 8), (64, 64, 4096, 8), (200, 130, 96, 3)])
+def test_f16_in_place_operands_exact(gpu, ta, tb, m, n, k, pad):
+    """As for bf16: k-contiguous operands with 16-byte aligned rows and k a
+    multiple of 32 are read in place, the rest packed; integer inputs, NaN
+    padding, ragged m and n, k = 4096 on a 64 x 64 result."""
+    rng = np.random.default_rng(m + n + k + pad)
+    sa = (k, m) if ta else (m, k)
+    sb = (n, k) if tb else (k, n)
+    A = _f16(rng, *sa, sa[0] + pad, ints=True)
+    B = _f16(rng, *sb, sb[0] + pad, ints=True)
+    tA, pA = _dev16(A)
+    tB, pB = _dev16(B)
+    dC = _dev32(np.full((m, n), np.nan, np.float32))
+    E.gemm_f16(ta, tb, m, n, k, 1.0, pA, A.shape[0], pB, B.shape[0], 0.0, dC, m)
+    got = from_dev(dC, (m, n))
+    a = _widen(A[: sa[0], : sa[1]]).reshape(sa).astype(np.int64)
+    b = _widen(B[: sb[0], : sb[1]]).reshape(sb).astype(np.int64)
+    a, b = (a.T if ta else a), (b.T if tb else b)
+    assert np.array_equal(got, (a @ b).astype(np.float32))
+
+
 def test_f16_alpha_zero_and_empty(gpu):
     rng = np.random.default_rng(6)
     m, n, k = 33, 17, 40
