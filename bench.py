@@ -30,23 +30,32 @@ sys.path.insert(0, ROOT)
 
 PEAK = {"bf16": 2500.0, "fp16": 2500.0, "fp32": 157.3, "fp64": 78.6}  # dense MFMA TFLOP/s, MI355X
 HBM_PEAK_GBS = 8000.0
+WARM_MS = 100.0  # untimed warm-up of each leg after the headline (at least, beyond --warmup steps)
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def timed(fn, steps: int, warmup: int, kernels=()):
+def timed(fn, steps: int, warmup: int, kernels=(), warm_ms: float = 0.0):
     """Wall-clock ms per step and HIP-event ms per step on the launch stream.
 
     `kernels`: kernel classes (eigen_amd.KC_*) whose launches inside the timed
     steps are bracketed by HIP events on their own stream; their mean launch
-    time in ms is returned as a dict (warm-up launches are not recorded)."""
+    time in ms is returned as a dict (warm-up launches are not recorded).
+    `warm_ms`: after the `warmup` steps, keep running untimed steps until this
+    much time has passed (the legs after the headline start from a GPU left
+    idle by the previous leg's host-side error check, and a short leg would
+    otherwise time the clock ramp)."""
     import torch
     import eigen_amd as E
+    t_warm = time.perf_counter()
     for _ in range(warmup):
         fn()
     torch.cuda.synchronize()
+    while (time.perf_counter() - t_warm) * 1e3 < warm_ms:
+        fn()
+        torch.cuda.synchronize()
     if kernels:
         E.kernel_timing(True)
     ev0 = torch.cuda.Event(enable_timing=True)
@@ -311,7 +320,7 @@ def bench_dgemm(E, torch, dev, args):
         E.fill(E.F64, dB, k * n, 12, E.DIST_NORMAL, 0, 1)
         E.fill(E.F64, dC, m * n, 13, E.DIST_NORMAL, 0, 1)
         fn = lambda: E.gemm(E.F64, E.OP_N, E.OP_N, m, n, k, 1.5, dA, m, dB, k, 0.5, dC, m)
-        wall, kern, kt = timed(fn, max(3, args.steps // 2), args.warmup, (E.KC_PRODUCT,))
+        wall, kern, kt = timed(fn, max(3, args.steps // 2), args.warmup, (E.KC_PRODUCT,), warm_ms=WARM_MS)
         tf = 2.0 * m * n * k / (wall * 1e-3) / 1e12
         cfg = E.plan(E.F64, 0, 0, m, n, k)
         out[name] = {"ms": wall, "event_ms": kern, "product_kernel_ms": kt[E.KC_PRODUCT][0], "tflops": tf,
@@ -368,7 +377,7 @@ def bench_level3(E, torch, dev, args):
     host = lambda t: t.view(n, n).T.cpu().numpy()  # column-major n x n
     out = {}
     for name, (flop, fn) in cases.items():
-        wall, kern = timed(fn, 3, 1)
+        wall, kern = timed(fn, 3, 1, warm_ms=WARM_MS)
         tf = flop / (wall * 1e-3) / 1e12
         out[name] = {"ms": wall, "tflops": tf, "frac_of_fp64_peak": tf / PEAK["fp64"]}
         fresh()
@@ -422,7 +431,7 @@ def bench_batched_strided(E, torch, dev, args):
         Cm = torch.zeros(e * count, dtype=tdt, device=dev)
         fn = lambda: E.gemm_batched_strided(dtype, 0, 0, size, size, size, 1.0, A, size, e, B, size, e, 0.0, Cm, size,
                                             e, count)
-        wall, kern, kt = timed(fn, max(10, args.steps), args.warmup, (E.KC_GROUPED,))
+        wall, kern, kt = timed(fn, max(10, args.steps), args.warmup, (E.KC_GROUPED,), warm_ms=WARM_MS)
         kms = kt[E.KC_GROUPED][0]
         flop, nbytes = 2.0 * size ** 3 * count, 3.0 * e * es * count
         u = float(np.finfo(np.float32 if dtype == E.F32 else np.float64).eps) / 2
@@ -452,7 +461,7 @@ def bench_types(E, torch, dev, args):
     B = (torch.rand(n * n, device=dev) * 2 - 1).to(torch.bfloat16)
     Cm = torch.zeros(n * n, device=dev)
     fn = lambda: E.gemm_bf16(0, 0, n, n, n, 1.0, A.data_ptr(), n, B.data_ptr(), n, 0.0, Cm, n)
-    wall, kern, kt = timed(fn, max(3, args.steps // 2), args.warmup, (E.KC_PACKED_PRODUCT,))
+    wall, kern, kt = timed(fn, max(3, args.steps // 2), args.warmup, (E.KC_PACKED_PRODUCT,), warm_ms=WARM_MS)
     tf = 2.0 * n ** 3 / (wall * 1e-3) / 1e12
     out["bf16_8192"] = {"ms": wall, "tflops": tf, "frac_of_bf16_peak": tf / PEAK["bf16"],
                         "product_kernel_ms": kt[E.KC_PACKED_PRODUCT][0],
@@ -463,7 +472,7 @@ def bench_types(E, torch, dev, args):
     B = (torch.rand(n * n, device=dev) * 2 - 1).to(torch.float16)
     Cm = torch.zeros(n * n, device=dev)
     fn = lambda: E.gemm_f16(0, 0, n, n, n, 1.0, A.data_ptr(), n, B.data_ptr(), n, 0.0, Cm, n)
-    wall, kern, kt = timed(fn, max(3, args.steps // 2), args.warmup, (E.KC_PACKED_PRODUCT,))
+    wall, kern, kt = timed(fn, max(3, args.steps // 2), args.warmup, (E.KC_PACKED_PRODUCT,), warm_ms=WARM_MS)
     tf = 2.0 * n ** 3 / (wall * 1e-3) / 1e12
     out["f16_8192"] = {"ms": wall, "tflops": tf, "frac_of_fp16_peak": tf / PEAK["fp16"],
                        "product_kernel_ms": kt[E.KC_PACKED_PRODUCT][0],
@@ -477,7 +486,7 @@ def bench_types(E, torch, dev, args):
         B = torch.rand(2 * n * n, dtype=tdt, device=dev) * 2 - 1
         Cm = torch.rand(2 * n * n, dtype=tdt, device=dev) * 2 - 1
         fn = lambda: E.gemm_complex(dtype, 0, 0, n, n, n, 1.0 + 0.5j, A, n, B, n, 0.5, Cm, n, prec=E.PREC_PLAIN)
-        wall, kern = timed(fn, max(3, args.steps // 2), args.warmup)
+        wall, kern = timed(fn, max(3, args.steps // 2), args.warmup, warm_ms=WARM_MS)
         tf = 8.0 * n ** 3 / (wall * 1e-3) / 1e12
         out[name] = {"ms": wall, "tflops": tf, f"frac_of_{peak}_peak": tf / PEAK[peak]}
         # error of one more product from a fresh C
@@ -519,7 +528,7 @@ def bench_gemv(E, torch, dev, args):
         u = float(np.finfo(np.float32 if dtype == E.F32 else np.float64).eps) / 2
         for ta in (0, 1):
             fn = lambda: E.gemm(dtype, ta, 0, n, 1, n, 1.0, A, n, x, n, 0.0, y, n)
-            wall, kern, kt = timed(fn, max(10, args.steps), args.warmup, (E.KC_GEMV,))
+            wall, kern, kt = timed(fn, max(10, args.steps), args.warmup, (E.KC_GEMV,), warm_ms=WARM_MS)
             kms = kt[E.KC_GEMV][0]
             gbs = n * n * es / (kms * 1e-3) / 1e9
             ref = exact_product(Ah.T if ta else Ah, xh)[:, 0]
@@ -539,7 +548,7 @@ def bench_batched(E, torch, args):
     from eigen_amd.batched import operand, uneven_batch
     b = uneven_batch(20000, seed=0)
     g = E.Grouped(b.problems)
-    wall_g, kern_g = timed(g.graph_launch, max(5, args.steps), args.warmup)
+    wall_g, kern_g = timed(g.graph_launch, max(5, args.steps), args.warmup, warm_ms=WARM_MS)
     wall_r, _, kt = timed(g.run, max(5, args.steps // 2), 1, (E.KC_GROUPED,))
     wall_p, _ = timed(g.run_per_product, 2, 1)
     g.graph_launch()
