@@ -29,7 +29,9 @@ one pass over UTIL_COUNTERS) under the given root and
 prints per-kernel duration tables and counter readings as markdown, plus the
 HBM bytes per launch of each workload's dominant kernel: each counter
 reading scaled by bytes-moved / reading of the copy kernel (the copy reads
-and writes exactly 2^30 bytes).
+and writes exactly 2^30 bytes).  With a traffic.json path those entries are
+written into that file, replacing the same workloads' entries and keeping
+the others.
 """
 from __future__ import annotations
 
@@ -279,9 +281,14 @@ def main(argv=None) -> int:
             skip = int(argv[i + 1])
             argv = argv[:i] + argv[i + 2:]
         traffic = summarize(argv[1], skip)
-        if len(argv) > 2:
+        if len(argv) > 2:  # merged into the file: workloads profiled in other sessions keep their entries
+            merged = {}
+            if os.path.exists(argv[2]):
+                with open(argv[2]) as f:
+                    merged = json.load(f)
+            merged.update(traffic)
             with open(argv[2], "w") as f:
-                json.dump(traffic, f, indent=1)
+                json.dump(dict(sorted(merged.items())), f, indent=1)
                 f.write("\n")
         return 0
     print(__doc__)
