@@ -47,14 +47,14 @@ def _unpack(buf, p, rows, cols, ld, stride):
     return np.stack([buf[p * stride + j * ld: p * stride + j * ld + rows] for j in range(cols)], axis=1)
 
 
-# every op combination on ragged shapes: through the grouped kernel, one
-# thread per product (up to 4x4x4) and one wave per product (up to 16x16,
-# depth 16 or 64); the 20,000-product 4x4 batch (fixed-size matrices) with
-# op N, N
+# every op combination on ragged shapes: through the grouped kernel (also
+# products up to 16x16 deeper than 64), one thread per product (up to
+# 4x4x4) and one wave per product (up to 16x16, depth 16 or 64); the
+# 20,000-product 4x4 batch (fixed-size matrices) with op N, N
 CASES = [(ta, tb, m, n, k, count) for ta, tb in itertools.product((0, 1), (0, 1))
          for (m, n, k, count) in ((33, 17, 50, 100), (1, 1, 1, 5), (70, 40, 3, 7), (3, 2, 4, 1000),
                                   (16, 16, 16, 2000), (9, 13, 40, 300), (16, 5, 64, 50), (7, 16, 17, 200),
-                                  (5, 1, 3, 100))] + \
+                                  (5, 1, 3, 100), (16, 16, 256, 300), (9, 5, 130, 100))] + \
     [(0, 0, 4, 4, 4, 20000)]
 
 
@@ -128,7 +128,7 @@ def test_batched_strided_alpha_beta_zero(gpu, dt, m, n, k, count):
         assert np.array_equal(_unpack(got, p, m, n, m, m * n), (am[p].astype(np.int64) @ bm[p].astype(np.int64)).astype(dt))
 
 
-@pytest.mark.parametrize("m,n,k,count", [(100, 90, 500, 40), (16, 16, 64, 300)])  # grouped kernel; one wave each
+@pytest.mark.parametrize("m,n,k,count", [(100, 90, 500, 40), (16, 16, 64, 300), (16, 12, 300, 200)])  # grouped; wave
 def test_batched_strided_random_bound(gpu, m, n, k, count):
     rng = np.random.default_rng(4)
     A, am = _batch(rng, count, m, k, m, m * k, np.float32, ints=False)
