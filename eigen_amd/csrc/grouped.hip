@@ -46,7 +46,7 @@
 // eh_gemm_batched_strided (arrays of one shape) writes the same work list on
 // the device, except for small shapes, which take kernels of their own: up
 // to 4 x 4 x 4 one thread per product (16-byte column accesses for dense
-// 4 x 4 matrices), up to 16 x 16 with depth up to 64 (FP64) or 16 (FP32)
+// 4 x 4 matrices), up to 16 x 16 with depth up to 64 (FP64) or 32 (FP32)
 // one wave per product.
 #include <hip/hip_runtime.h>
 
@@ -602,7 +602,7 @@ __global__ void __launch_bounds__(256) dense4_batched_kernel(const T* __restrict
   }
 }
 
-// Products of at most 16 x 16 with depth at most 4 * KS (KS = 4 or 16),
+// Products of at most 16 x 16 with depth at most 4 * KS (KS = 4, 8 or 16),
 // one wave each: a single 16 x 16 MFMA block (v_mfma_f32_16x16x4_f32 /
 // v_mfma_f64_16x16x4_f64) holds C^T = op(B)^T op(A)^T, so consecutive lanes
 // store consecutive rows of C.  Lane l feeds row / column l % 16 at depth
@@ -615,7 +615,7 @@ __global__ void __launch_bounds__(256) dense4_batched_kernel(const T* __restrict
 // also saves the work-list kernel), FP32 18.7 against 24.9 us, 50,000 FP64
 // 8^3 30.2 against 47.1 us.  Deeper products stay on the grouped kernel,
 // whose 16-byte loads of k-contiguous slabs win there (FP32 beyond depth
-// 16, FP64 beyond 64): a loop over passes of 64 here measured 48% slower on
+// 32, FP64 beyond 64): a loop over passes of 64 here measured 48% slower on
 // FP32 16x16x256 (225 against 152 us), 4% faster on FP64.
 template <typename T>
 struct Small16;
@@ -757,15 +757,20 @@ extern "C" eh_status eh_gemm_batched_strided(eh_dtype dtype, eh_op opa, eh_op op
   const double be = dtype == EH_F64 ? *static_cast<const double*>(beta) : *static_cast<const float*>(beta);
   const bool ta = opa != EH_OP_N, tb = opb != EH_OP_N;
   const bool reads = k > 0 && al != 0.0;
-  // one wave per product up to 16 x 16: FP64 to depth 64, FP32 to depth 16
-  // (FP32 16x16x32: 49 us here against 33 us through the grouped kernel,
-  // 16x16x64: 50 against 46 us; 8x8x48 76 against 81 us)
-  if (reads && m <= 16 && n <= 16 && k <= (dtype == EH_F64 ? 64 : 16)) {
+  // one wave per product up to 16 x 16: FP64 to depth 64, FP32 to depth 32,
+  // in one pass of 4, 8 or 16 four-deep steps (a pass with steps past the
+  // depth still loads them: FP64 16x16x32 took 65.7 us with 16 steps, 37.5
+  // with 8; FP32 16x16x32 29.5 us against 33.4 through the grouped kernel,
+  // 16x16x64 with 16 steps 50 against 46 us there)
+  if (reads && m <= 16 && n <= 16 && k <= (dtype == EH_F64 ? 64 : 32)) {
     KernelTimer t(EH_KC_GROUPED, s);
     if (dtype == EH_F64) {
       const double a = *static_cast<const double*>(alpha), b = *static_cast<const double*>(beta);
       if (k <= 16)
         launch_small_t<double, 4>(ta, tb, A, lda, stride_a, B, ldb, stride_b, C, ldc, stride_c, (int)m, (int)n, (int)k,
+                                  a, b, count, s);
+      else if (k <= 32)
+        launch_small_t<double, 8>(ta, tb, A, lda, stride_a, B, ldb, stride_b, C, ldc, stride_c, (int)m, (int)n, (int)k,
                                   a, b, count, s);
       else
         launch_small_t<double, 16>(ta, tb, A, lda, stride_a, B, ldb, stride_b, C, ldc, stride_c, (int)m, (int)n,
@@ -774,6 +779,9 @@ extern "C" eh_status eh_gemm_batched_strided(eh_dtype dtype, eh_op opa, eh_op op
       const float a = *static_cast<const float*>(alpha), b = *static_cast<const float*>(beta);
       if (k <= 16)
         launch_small_t<float, 4>(ta, tb, A, lda, stride_a, B, ldb, stride_b, C, ldc, stride_c, (int)m, (int)n, (int)k, a,
+                                 b, count, s);
+      else if (k <= 32)
+        launch_small_t<float, 8>(ta, tb, A, lda, stride_a, B, ldb, stride_b, C, ldc, stride_c, (int)m, (int)n, (int)k, a,
                                  b, count, s);
       else
         launch_small_t<float, 16>(ta, tb, A, lda, stride_a, B, ldb, stride_b, C, ldc, stride_c, (int)m, (int)n, (int)k,

@@ -49,7 +49,7 @@ def _unpack(buf, p, rows, cols, ld, stride):
 
 # every op combination on ragged shapes: through the grouped kernel (also
 # products up to 16x16 deeper than 64), one thread per product (up to
-# 4x4x4) and one wave per product (up to 16x16, depth up to 16 or, for
+# 4x4x4) and one wave per product (up to 16x16, depth up to 32 or, for
 # FP64, 64); the
 # 20,000-product 4x4 batch (fixed-size matrices) with op N, N
 CASES = [(ta, tb, m, n, k, count) for ta, tb in itertools.product((0, 1), (0, 1))
