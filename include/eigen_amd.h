@@ -433,13 +433,16 @@ eh_status eh_grouped_destroy(eh_grouped* g);
  * (product p: A + p*stride_a, B + p*stride_b, C + p*stride_c), e.g. arrays of
  * fixed-size matrices: the batched form of Eigen's small and fixed-size
  * products, which Eigen evaluates one at a time (coefficient-based below
- * ProductEvaluators.h:606-692's threshold, GEBP above).  The work list is
- * written on the device (one small kernel: descriptors and the equal-cost
- * split of the count*tiles 32x32 tiles over the resident waves) and the
- * grouped kernel runs it: two launches, no host loop, no allocation beyond
- * the per-stream workspace.  Results must not overlap
- * (stride_c >= (n-1)*ldc + m when count > 1).  Products whose operands span
- * 2 GB or more run one product at a time. */
+ * ProductEvaluators.h:606-692's threshold, GEBP above).  Shapes up to
+ * 4x4x4 run one thread per product (whole 16-byte columns for arrays of
+ * dense 4x4 matrices), shapes up to 16x16 with depth up to 64 (FP64) or 32
+ * (FP32) one wave and one 16x16 MFMA block per product: one launch each.
+ * Larger shapes: the work list is written on the device (one small kernel:
+ * descriptors and the equal-cost split of the count*tiles 32x32 tiles over
+ * the resident waves) and the grouped kernel runs it: two launches, no host
+ * loop, no allocation beyond the per-stream workspace.  Results must not
+ * overlap (stride_c >= (n-1)*ldc + m when count > 1).  Products whose
+ * operands span 2 GB or more run one product at a time. */
 eh_status eh_gemm_batched_strided(eh_dtype dtype, eh_op opa, eh_op opb, int64_t m, int64_t n, int64_t k,
                                   const void* alpha, const void* A, int64_t lda, int64_t stride_a, const void* B,
                                   int64_t ldb, int64_t stride_b, const void* beta, void* C, int64_t ldc,
