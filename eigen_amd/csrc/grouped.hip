@@ -45,8 +45,8 @@
 //
 // eh_gemm_batched_strided (arrays of one shape) writes the same work list on
 // the device, except for small shapes, which take kernels of their own: up
-// to 4 x 4 x 4 one thread per product (16-byte column accesses for dense
-// 4 x 4 matrices), up to 16 x 16 with depth up to 64 (FP64) or 32 (FP32)
+// to 4 x 4 x 4 one thread per product (whole-column accesses for dense
+// 2 x 2 and 4 x 4 matrices), up to 16 x 16 with depth up to 64 (FP64) or 32 (FP32)
 // one wave per product.
 #include <hip/hip_runtime.h>
 
@@ -562,44 +562,58 @@ __global__ void __launch_bounds__(256) tiny_batched_kernel(const T* __restrict__
     }
 }
 
-// Dense 4 x 4 x 4 products (Matrix4f / Matrix4d arrays: op N, leading
-// dimensions 4, 16-byte aligned operands): one thread each, every column of
-// A, B and C moved as one 16-byte (FP32) or two 16-byte (FP64) accesses
-// instead of one access per element.  100,000 FP32 products: 7.1 against
-// 18.6 us (kernel events); FP64: 12.1 against 21.9 us.  The k sum runs in
-// the same order as in tiny_batched_kernel, so the results are identical.
-template <typename T>
-__global__ void __launch_bounds__(256) dense4_batched_kernel(const T* __restrict__ A, int64_t sa,
-                                                             const T* __restrict__ B, int64_t sb, T* __restrict__ C,
-                                                             int64_t sc, T alpha, T beta, int64_t count) {
-  using V = T __attribute__((ext_vector_type(4)));
+// Dense N x N x N products, N = 2 or 4 (Matrix2f/2d/4f/4d arrays: op N,
+// leading dimensions N, operands aligned to a column): one thread each,
+// every column of A, B and C moved as one vector access (two 16-byte ones
+// for FP64 N = 4) instead of one access per element.  100,000 FP32 4x4
+// products: 7.1 against 18.6 us (kernel events); FP64: 12.1 against
+// 21.9 us.  The k sum runs in the same order as in tiny_batched_kernel, so
+// the results are identical.
+template <typename T, int N>
+__global__ void __launch_bounds__(256) dense_batched_kernel(const T* __restrict__ A, int64_t sa,
+                                                            const T* __restrict__ B, int64_t sb, T* __restrict__ C,
+                                                            int64_t sc, T alpha, T beta, int64_t count) {
+  using V = T __attribute__((ext_vector_type(N)));
   const int64_t p = blockIdx.x * 256ll + threadIdx.x;
   if (p >= count) return;
   const V* Ap = reinterpret_cast<const V*>(A + p * sa);
   const V* Bp = reinterpret_cast<const V*>(B + p * sb);
   V* Cp = reinterpret_cast<V*>(C + p * sc);
-  V a[4], b[4];
+  V a[N], b[N];
 #pragma unroll
-  for (int j = 0; j < 4; ++j) a[j] = Ap[j];
+  for (int j = 0; j < N; ++j) a[j] = Ap[j];
 #pragma unroll
-  for (int j = 0; j < 4; ++j) b[j] = Bp[j];
-  V c[4];
+  for (int j = 0; j < N; ++j) b[j] = Bp[j];
+  V c[N];
   if (beta != T(0)) {
 #pragma unroll
-    for (int j = 0; j < 4; ++j) c[j] = Cp[j];
+    for (int j = 0; j < N; ++j) c[j] = Cp[j];
   }
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
+  for (int j = 0; j < N; ++j) {
     V acc;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
+    for (int i = 0; i < N; ++i) {
       T s = T(0);
 #pragma unroll
-      for (int kk = 0; kk < 4; ++kk) s = fma_t(a[kk][i], b[j][kk], s);
+      for (int kk = 0; kk < N; ++kk) s = fma_t(a[kk][i], b[j][kk], s);
       acc[i] = beta == T(0) ? alpha * s : fma_t(alpha, s, beta * c[j][i]);
     }
     Cp[j] = acc;
   }
+}
+
+template <int N>
+static void launch_dense_t(eh_dtype dtype, const void* A, int64_t sa, const void* B, int64_t sb, void* C, int64_t sc,
+                           const void* alpha, const void* beta, int64_t count, unsigned blocks, hipStream_t s) {
+  if (dtype == EH_F64)
+    hipLaunchKernelGGL((dense_batched_kernel<double, N>), dim3(blocks), dim3(256), 0, s, static_cast<const double*>(A),
+                       sa, static_cast<const double*>(B), sb, static_cast<double*>(C), sc,
+                       *static_cast<const double*>(alpha), *static_cast<const double*>(beta), count);
+  else
+    hipLaunchKernelGGL((dense_batched_kernel<float, N>), dim3(blocks), dim3(256), 0, s, static_cast<const float*>(A),
+                       sa, static_cast<const float*>(B), sb, static_cast<float*>(C), sc,
+                       *static_cast<const float*>(alpha), *static_cast<const float*>(beta), count);
 }
 
 // Products of at most 16 x 16 with depth at most 4 * KS (KS = 4, 8 or 16),
@@ -722,22 +736,20 @@ extern "C" eh_status eh_gemm_batched_strided(eh_dtype dtype, eh_op opa, eh_op op
   if (m <= 4 && n <= 4 && k <= 4) {  // fixed-size products up to 4 x 4: one thread each
     const unsigned blocks = (unsigned)((count + 255) / 256);
     KernelTimer t(EH_KC_GROUPED, s);
+    // dense N x N arrays (N = 2, 4) with every column aligned to its size
     const int es = dtype == EH_F64 ? 8 : 4;
-    const bool al16 = ((uintptr_t)A | (uintptr_t)B | (uintptr_t)C) % 16 == 0 &&
-                      (stride_a * es) % 16 == 0 && (stride_b * es) % 16 == 0 && (stride_c * es) % 16 == 0;
+    const int64_t col = m * es;
+    const bool aligned = ((uintptr_t)A % col | (uintptr_t)B % col | (uintptr_t)C % col) == 0 &&
+                         (stride_a * es) % col == 0 && (stride_b * es) % col == 0 && (stride_c * es) % col == 0;
     const double alv = dtype == EH_F64 ? *static_cast<const double*>(alpha) : *static_cast<const float*>(alpha);
-    if (m == 4 && n == 4 && k == 4 && opa == EH_OP_N && opb == EH_OP_N && lda == 4 && ldb == 4 && ldc == 4 && al16 &&
-        alv != 0.0) {
-      if (dtype == EH_F64)
-        hipLaunchKernelGGL(dense4_batched_kernel<double>, dim3(blocks), dim3(256), 0, s, static_cast<const double*>(A),
-                           stride_a, static_cast<const double*>(B), stride_b, static_cast<double*>(C), stride_c,
-                           *static_cast<const double*>(alpha), *static_cast<const double*>(beta), count);
+    if ((m == 2 || m == 4) && n == m && k == m && opa == EH_OP_N && opb == EH_OP_N && lda == m && ldb == m &&
+        ldc == m && aligned && alv != 0.0) {
+      if (m == 4)
+        launch_dense_t<4>(dtype, A, stride_a, B, stride_b, C, stride_c, alpha, beta, count, blocks, s);
       else
-        hipLaunchKernelGGL(dense4_batched_kernel<float>, dim3(blocks), dim3(256), 0, s, static_cast<const float*>(A),
-                           stride_a, static_cast<const float*>(B), stride_b, static_cast<float*>(C), stride_c,
-                           *static_cast<const float*>(alpha), *static_cast<const float*>(beta), count);
+        launch_dense_t<2>(dtype, A, stride_a, B, stride_b, C, stride_c, alpha, beta, count, blocks, s);
       hipError_t e = hipGetLastError();
-      return e == hipSuccess ? EH_SUCCESS : hip_fail(e, "dense 4x4 batched launch");
+      return e == hipSuccess ? EH_SUCCESS : hip_fail(e, "dense batched launch");
     }
     if (dtype == EH_F64)
       hipLaunchKernelGGL(tiny_batched_kernel<double>, dim3(blocks), dim3(256), 0, s, static_cast<const double*>(A), lda,

@@ -82,32 +82,36 @@ def test_batched_strided_exact(gpu, dt, ta, tb, m, n, k, count):
 
 
 @pytest.mark.parametrize("dt", [np.float32, np.float64])
-@pytest.mark.parametrize("stride,beta", [(16, 0.0), (16, -1.0), (20, 0.5), (17, -1.0)])
-def test_batched_strided_dense4_exact(gpu, dt, stride, beta):
-    """Arrays of dense 4x4 matrices (leading dimension 4): packed (stride 16)
-    and spaced (20) arrays of 16-byte aligned matrices, and stride 17, whose
-    matrices are not all 16-byte aligned; beta == 0 with NaN in C."""
-    rng = np.random.default_rng(stride)
+@pytest.mark.parametrize("size,stride,beta", [(4, 16, 0.0), (4, 16, -1.0), (4, 20, 0.5), (4, 17, -1.0), (4, 18, 0.5),
+                                              (2, 4, 0.0), (2, 4, -1.0), (2, 6, 0.5), (2, 5, -1.0)])
+def test_batched_strided_dense_exact(gpu, dt, size, stride, beta):
+    """Arrays of dense 4x4 and 2x2 matrices (leading dimension = size):
+    packed (stride size^2) and spaced arrays whose every column is aligned to
+    its byte size (read as whole columns), and strides that leave columns
+    misaligned (17, 18 for 4x4 FP64, 5), read element by element; beta == 0
+    with NaN in C."""
+    rng = np.random.default_rng(stride + 100 * size)
     count = 30000
-    A, am = _batch(rng, count, 4, 4, 4, stride, dt)
-    B, bm = _batch(rng, count, 4, 4, 4, stride, dt)
-    C0, cm = _batch(rng, count, 4, 4, 4, stride, dt)
+    e = size * size
+    A, am = _batch(rng, count, size, size, size, stride, dt)
+    B, bm = _batch(rng, count, size, size, size, stride, dt)
+    C0, cm = _batch(rng, count, size, size, size, stride, dt)
     if beta == 0.0:
         for p in range(count):
-            C0[p * stride: p * stride + 16] = np.nan
+            C0[p * stride: p * stride + e] = np.nan
     dC = _dev(C0)
-    E.gemm_batched_strided(ED[dt], 0, 0, 4, 4, 4, 3.0, _dev(A), 4, stride, _dev(B), 4, stride, beta, dC, 4, stride,
-                           count)
+    E.gemm_batched_strided(ED[dt], 0, 0, size, size, size, 3.0, _dev(A), size, stride, _dev(B), size, stride, beta, dC,
+                           size, stride, count)
     got = _host(dC)
     gaps = np.ones(len(C0), bool)
     for p in range(count):
-        gaps[p * stride: p * stride + 16] = False
+        gaps[p * stride: p * stride + e] = False
     assert np.array_equal(got[gaps], C0[gaps], equal_nan=True), "elements between the products written"
     for p in range(0, count, 97):
         want = 3 * (am[p].astype(np.int64) @ bm[p].astype(np.int64))
         if beta != 0.0:
             want = want + beta * cm[p].astype(np.float64)
-        assert np.array_equal(_unpack(got, p, 4, 4, 4, stride), want.astype(dt)), p
+        assert np.array_equal(_unpack(got, p, size, size, size, stride), want.astype(dt)), p
 
 
 @pytest.mark.parametrize("dt", [np.float32, np.float64])
