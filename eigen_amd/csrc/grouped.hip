@@ -46,7 +46,7 @@
 // eh_gemm_batched_strided (arrays of one shape) writes the same work list on
 // the device, except for small shapes, which take kernels of their own: up
 // to 4 x 4 x 4 one thread per product (whole-column accesses for dense
-// 2 x 2 and 4 x 4 matrices), up to 16 x 16 with depth up to 64 (FP64) or 32 (FP32)
+// 2 x 2 and 4 x 4 matrices, nine-element runs for dense 3 x 3 ones), up to 16 x 16 with depth up to 64 (FP64) or 32 (FP32)
 // one wave per product.
 #include <hip/hip_runtime.h>
 
@@ -603,6 +603,43 @@ __global__ void __launch_bounds__(256) dense_batched_kernel(const T* __restrict_
   }
 }
 
+// Dense 3 x 3 x 3 products (Matrix3f / Matrix3d arrays: op N, leading
+// dimensions 3, any alignment): one thread each, the nine elements of each
+// operand loaded as one contiguous run (the compiler merges them into wide
+// loads) instead of sixteen clamped loads (100,000 FP32 products: 6.5
+// against 9.4 us, FP64 7.7 against 11.5 us); same k order as
+// tiny_batched_kernel.
+template <typename T, int N>
+__global__ void __launch_bounds__(256) dense_run_batched_kernel(const T* __restrict__ A, int64_t sa,
+                                                                const T* __restrict__ B, int64_t sb,
+                                                                T* __restrict__ C, int64_t sc, T alpha, T beta,
+                                                                int64_t count) {
+  const int64_t p = blockIdx.x * 256ll + threadIdx.x;
+  if (p >= count) return;
+  const T* Ap = A + p * sa;
+  const T* Bp = B + p * sb;
+  T* Cp = C + p * sc;
+  T a[N * N], b[N * N];
+#pragma unroll
+  for (int e = 0; e < N * N; ++e) a[e] = Ap[e];
+#pragma unroll
+  for (int e = 0; e < N * N; ++e) b[e] = Bp[e];
+  T c[N * N];
+  if (beta != T(0)) {
+#pragma unroll
+    for (int e = 0; e < N * N; ++e) c[e] = Cp[e];
+  }
+#pragma unroll
+  for (int j = 0; j < N; ++j)
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      T s = T(0);
+#pragma unroll
+      for (int kk = 0; kk < N; ++kk) s = fma_t(a[kk * N + i], b[j * N + kk], s);
+      Cp[j * N + i] = beta == T(0) ? alpha * s : fma_t(alpha, s, beta * c[j * N + i]);
+    }
+}
+
 template <int N>
 static void launch_dense_t(eh_dtype dtype, const void* A, int64_t sa, const void* B, int64_t sb, void* C, int64_t sc,
                            const void* alpha, const void* beta, int64_t count, unsigned blocks, hipStream_t s) {
@@ -750,6 +787,21 @@ extern "C" eh_status eh_gemm_batched_strided(eh_dtype dtype, eh_op opa, eh_op op
         launch_dense_t<2>(dtype, A, stride_a, B, stride_b, C, stride_c, alpha, beta, count, blocks, s);
       hipError_t e = hipGetLastError();
       return e == hipSuccess ? EH_SUCCESS : hip_fail(e, "dense batched launch");
+    }
+    if (m == 3 && n == 3 && k == 3 && opa == EH_OP_N && opb == EH_OP_N && lda == 3 && ldb == 3 && ldc == 3 &&
+        alv != 0.0) {
+      if (dtype == EH_F64)
+        hipLaunchKernelGGL((dense_run_batched_kernel<double, 3>), dim3(blocks), dim3(256), 0, s,
+                           static_cast<const double*>(A), stride_a, static_cast<const double*>(B), stride_b,
+                           static_cast<double*>(C), stride_c, *static_cast<const double*>(alpha),
+                           *static_cast<const double*>(beta), count);
+      else
+        hipLaunchKernelGGL((dense_run_batched_kernel<float, 3>), dim3(blocks), dim3(256), 0, s,
+                           static_cast<const float*>(A), stride_a, static_cast<const float*>(B), stride_b,
+                           static_cast<float*>(C), stride_c, *static_cast<const float*>(alpha),
+                           *static_cast<const float*>(beta), count);
+      hipError_t e = hipGetLastError();
+      return e == hipSuccess ? EH_SUCCESS : hip_fail(e, "dense 3x3 batched launch");
     }
     if (dtype == EH_F64)
       hipLaunchKernelGGL(tiny_batched_kernel<double>, dim3(blocks), dim3(256), 0, s, static_cast<const double*>(A), lda,

@@ -435,7 +435,7 @@ eh_status eh_grouped_destroy(eh_grouped* g);
  * products, which Eigen evaluates one at a time (coefficient-based below
  * ProductEvaluators.h:606-692's threshold, GEBP above).  Shapes up to
  * 4x4x4 run one thread per product (whole columns for arrays of dense 2x2
- * and 4x4 matrices), shapes up to 16x16 with depth up to 64 (FP64) or 32
+ * and 4x4 matrices, nine-element runs for dense 3x3 ones), shapes up to 16x16 with depth up to 64 (FP64) or 32
  * (FP32) one wave and one 16x16 MFMA block per product: one launch each.
  * Larger shapes: the work list is written on the device (one small kernel:
  * descriptors and the equal-cost split of the count*tiles 32x32 tiles over

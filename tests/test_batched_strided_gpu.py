@@ -83,12 +83,14 @@ def test_batched_strided_exact(gpu, dt, ta, tb, m, n, k, count):
 
 @pytest.mark.parametrize("dt", [np.float32, np.float64])
 @pytest.mark.parametrize("size,stride,beta", [(4, 16, 0.0), (4, 16, -1.0), (4, 20, 0.5), (4, 17, -1.0), (4, 18, 0.5),
-                                              (2, 4, 0.0), (2, 4, -1.0), (2, 6, 0.5), (2, 5, -1.0)])
+                                              (2, 4, 0.0), (2, 4, -1.0), (2, 6, 0.5), (2, 5, -1.0),
+                                              (3, 9, 0.0), (3, 9, -1.0), (3, 10, 0.5)])
 def test_batched_strided_dense_exact(gpu, dt, size, stride, beta):
-    """Arrays of dense 4x4 and 2x2 matrices (leading dimension = size):
-    packed (stride size^2) and spaced arrays whose every column is aligned to
-    its byte size (read as whole columns), and strides that leave columns
-    misaligned (17, 18 for 4x4 FP64, 5), read element by element; beta == 0
+    """Arrays of dense 4x4, 2x2 and 3x3 matrices (leading dimension = size):
+    packed (stride size^2) and spaced arrays; 4x4 and 2x2 ones whose every
+    column is aligned to its byte size are read as whole columns, strides
+    that leave columns misaligned (17, 18 for 4x4 FP64, 5) element by
+    element; 3x3 ones as runs of nine elements at any alignment; beta == 0
     with NaN in C."""
     rng = np.random.default_rng(stride + 100 * size)
     count = 30000
