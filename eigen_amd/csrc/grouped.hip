@@ -844,12 +844,9 @@ extern "C" eh_status eh_gemm_batched_strided(eh_dtype dtype, eh_op opa, eh_op op
       if (k <= 16)
         launch_small_t<float, 4>(ta, tb, A, lda, stride_a, B, ldb, stride_b, C, ldc, stride_c, (int)m, (int)n, (int)k, a,
                                  b, count, s);
-      else if (k <= 32)
+      else  // k <= 32
         launch_small_t<float, 8>(ta, tb, A, lda, stride_a, B, ldb, stride_b, C, ldc, stride_c, (int)m, (int)n, (int)k, a,
                                  b, count, s);
-      else
-        launch_small_t<float, 16>(ta, tb, A, lda, stride_a, B, ldb, stride_b, C, ldc, stride_c, (int)m, (int)n, (int)k,
-                                  a, b, count, s);
     }
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? EH_SUCCESS : hip_fail(e, "small batched launch");
