@@ -5,15 +5,14 @@ before reading a stage, waits until only the stages issued after it may
 still be loading.  Near the end of the k loop fewer stages are issued after
 the one being read, so the wait must shrink with them; a wait that assumes a
 full pipeline there lets a wave read a stage that has not landed.  (The
-software-pipelined loop of the bf16 / half tiles and of two split tiles reloads the last
-stage past the end, so its waits stay constant.)  These cases run every
-configuration of the four split tiles (the pipelined 128x128 and 8-wave
-256x128 ones, the plain 4-wave 256x128 one, and 256x256 with 16-deep
-stages) and of the pipelined bf16 / half tiles over k of one to seven
-32-deep slices, so every position of each loop's
-tail is hit, on integer-valued inputs whose products are exact in
-every configuration: the result must equal the exact product bit for bit,
-on every repetition."""
+software-pipelined loop of the bf16 / half tiles and of two split tiles
+reloads the last stage past the end, so its waits stay constant.)  These
+cases run every configuration of the four split tiles (the pipelined
+128x128 and 8-wave 256x128 ones, the plain 4-wave 256x128 one, and 256x256
+with 16-deep stages) and of the pipelined bf16 / half tiles over k of one
+to seven 32-deep slices, so every position of each loop's tail is hit, on
+integer-valued inputs whose products are exact in every configuration: the
+result must equal the exact product bit for bit, on every repetition."""
 import numpy as np
 import pytest
 
