@@ -362,8 +362,8 @@ static eh_status launch_split_product(const GemmParams<float>& g, bool ta, bool 
 // bf16 (or, with f16, IEEE half) inputs, FP32 accumulation and result:
 // the operands are repacked into the one-term plane layout (or read in
 // place, below), then the packed BF16 (FP16) MFMA product runs with a single
-// product per k slice (and the split-K reduction when K is cut).  Operand extents must stay below 4 GB
-// (32-bit buffer offsets).
+// product per k slice (and the split-K reduction when K is cut).  Operand
+// extents must stay below 4 GB (32-bit buffer offsets).
 static eh_status launch_bf16_product(bool ta, bool tb, int m, int n, int k, float alpha, const unsigned short* A,
                                      int64_t lda, const unsigned short* B, int64_t ldb, float beta, float* C,
                                      int64_t ldc, hipStream_t stream, bool f16 = false, int config = -1) {
