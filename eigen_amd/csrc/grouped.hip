@@ -46,8 +46,8 @@
 // eh_gemm_batched_strided (arrays of one shape) writes the same work list on
 // the device, except for small shapes, which take kernels of their own: up
 // to 4 x 4 x 4 one thread per product (whole-column accesses for dense
-// 2 x 2 and 4 x 4 matrices, nine-element runs for dense 3 x 3 ones), up to 16 x 16 with depth up to 64 (FP64) or 32 (FP32)
-// one wave per product.
+// 2 x 2 and 4 x 4 matrices, nine-element runs for dense 3 x 3 ones), up to
+// 16 x 16 with depth up to 64 (FP64) or 32 (FP32) one wave per product.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -673,13 +673,17 @@ struct Small16;
 template <>
 struct Small16<float> {
   using acc_t = floatx4;
-  __device__ static acc_t mma(float a, float b, acc_t c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+  __device__ static acc_t mma(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
   __device__ static int col(int r, int q) { return 4 * q + r; }  // D row (a column of C) of accumulator r
 };
 template <>
 struct Small16<double> {
   using acc_t = doublex4;
-  __device__ static acc_t mma(double a, double b, acc_t c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+  __device__ static acc_t mma(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
   __device__ static int col(int r, int q) { return q + 4 * r; }
 };
 
@@ -842,11 +846,11 @@ extern "C" eh_status eh_gemm_batched_strided(eh_dtype dtype, eh_op opa, eh_op op
     } else {
       const float a = *static_cast<const float*>(alpha), b = *static_cast<const float*>(beta);
       if (k <= 16)
-        launch_small_t<float, 4>(ta, tb, A, lda, stride_a, B, ldb, stride_b, C, ldc, stride_c, (int)m, (int)n, (int)k, a,
-                                 b, count, s);
+        launch_small_t<float, 4>(ta, tb, A, lda, stride_a, B, ldb, stride_b, C, ldc, stride_c, (int)m, (int)n, (int)k,
+                                 a, b, count, s);
       else  // k <= 32
-        launch_small_t<float, 8>(ta, tb, A, lda, stride_a, B, ldb, stride_b, C, ldc, stride_c, (int)m, (int)n, (int)k, a,
-                                 b, count, s);
+        launch_small_t<float, 8>(ta, tb, A, lda, stride_a, B, ldb, stride_b, C, ldc, stride_c, (int)m, (int)n, (int)k,
+                                 a, b, count, s);
     }
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? EH_SUCCESS : hip_fail(e, "small batched launch");

@@ -132,7 +132,8 @@ def test_batched_strided_alpha_beta_zero(gpu, dt, m, n, k, count):
     E.gemm_batched_strided(ED[dt], 0, 0, m, n, k, 1.0, _dev(A), m, m * k, _dev(B), k, k * n, 0.0, dC, m, m * n, count)
     got = _host(dC)
     for p in range(count):
-        assert np.array_equal(_unpack(got, p, m, n, m, m * n), (am[p].astype(np.int64) @ bm[p].astype(np.int64)).astype(dt))
+        want = (am[p].astype(np.int64) @ bm[p].astype(np.int64)).astype(dt)
+        assert np.array_equal(_unpack(got, p, m, n, m, m * n), want)
 
 
 @pytest.mark.parametrize("m,n,k,count", [(100, 90, 500, 40), (16, 16, 64, 300), (16, 12, 300, 200)])  # grouped; wave
